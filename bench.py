@@ -1,0 +1,183 @@
+#!/usr/bin/env python3
+"""Throughput of the MPCD time step on one MI355X.
+
+    python bench.py [--steps K] [--warmup W] [--config c2|c3] [--no-cpu]
+
+Default workload (c2): plain SRD, 1000x1000x20 fluid cells with walls in z
+(1000x1000x22 stored cells), n=20 -> 4e8 particles, T=1, dt=0.02, g=1e-3,
+rotation 120 deg, counting sort into cell order every step; seeded uniform
+positions and Maxwell-Boltzmann velocities (synthetic).  c3: extended MPC,
+500x500x40 fluid cells, n=10 (1e8 particles).
+
+A step is one shift + stream + sort + collide pass over all particles, with
+the particle state resident in HBM.  Prints one JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0
+# reference optimized step, README.md:35-41 (94 ms/step, 1000x1000x20, n=20), counted over 4e8 fluid particles
+REFERENCE_PSTEPS = 4.0e8 / 0.094
+
+CONFIGS = {
+    "c2": dict(fluid=(1000, 1000, 20), n=20, algorithm="srd",
+               workload="SRD Poiseuille channel 1000x1000x20 cells, n=20, walls in z, sort every step"),
+    "c3": dict(fluid=(500, 500, 40), n=10, algorithm="extended",
+               workload="extended MPC 500x500x40 cells, n=10, walls in z, sort every step"),
+}
+
+
+def algorithmic_bytes(n_particles: int, n_cells: int) -> dict:
+    """Compulsory HBM bytes of each kernel of one step ('Kernels')."""
+    collide = 60 * n_particles + 12 * n_cells   # perm 4 + gather 24 + write 24 + next key/rank 8; offsets 4 + counts 8
+    scan = 16 * n_cells                         # counts read twice, offsets written, counts cleared
+    perm = 12 * n_particles + 4 * n_cells       # key/rank 8 + perm write 4; offsets
+    return {"collide": collide, "scan": scan, "perm": perm, "step": collide + scan + perm}
+
+
+def cpu_baseline(n: int, seconds_target: float = 15.0) -> dict:
+    """Reference CPU backend (oracle/_ref, compiled from the reference sources) on a
+    bounded slice of the same workload, one host core."""
+    sys.path.insert(0, str(ROOT / "tests"))
+    from oracle_lib import Oracle, Reference, ref_available  # checker libraries, timed here only
+
+    fluid = (100, 100, 20)
+    volume = (fluid[0], fluid[1], fluid[2] + 2)
+    kind = "reference" if ref_available() else "port"
+    if kind == "reference":
+        sim = Reference(volume, (1, 1, 0), n=n, drag=0.001, delta_t=0.02, algorithm="srd", seed=(11, 13))
+    else:
+        sim = Oracle(volume, (1, 1, 0), n=n, drag=0.001, delta_t=0.02, algorithm="srd", rng="reference", seed=(11, 13))
+    sim.step(1)
+    t0 = time.perf_counter()
+    steps = 0
+    while True:
+        sim.step(1)
+        steps += 1
+        if time.perf_counter() - t0 >= seconds_target:
+            break
+    dt = time.perf_counter() - t0
+    value = sim.N * steps / dt
+    return {"value": value, "unit": "particle-steps/s", "cores": 1, "kind": kind,
+            "sample": f"{'reference CPU backend (cpu_backend.cpp + simulation_functions.cpp, g++ -O3)' if kind == 'reference' else 'oracle restatement'}: "
+                      f"SRD {fluid[0]}x{fluid[1]}x{fluid[2]} cells, n={n}, walls in z ({sim.N} particles), "
+                      f"{steps} steps after 1 warm-up, single thread"}
+
+
+def load_traffic(name: str):
+    f = ROOT / "profiles" / f"{name}_traffic.json"
+    if f.exists():
+        try:
+            return json.loads(f.read_text())
+        except ValueError:
+            return None
+    return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--config", default="c2", choices=sorted(CONFIGS))
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--seed", type=int, default=0x5EED0002)
+    args = ap.parse_args()
+
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(the step would shard as x-slabs with particle halo exchange, see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import cudampcd_amd as pympcd
+    from cudampcd_amd.capi import Handle
+
+    cfg = CONFIGS[args.config]
+    p = pympcd.Params()
+    p.n = cfg["n"]
+    p.temperature = 1.0
+    p.periodicity = (1, 1, 0)
+    p.volume_size = cfg["fluid"]
+    p.drag = 0.001
+    p.delta_t = 0.02
+    p.experiment = "standard"
+    p.algorithm = cfg["algorithm"]
+    p.seed = args.seed
+    print(f"[bench] creating {cfg['workload']} ({p.N} particles)", file=sys.stderr, flush=True)
+    sim = pympcd.Simulation(p, "hip")
+    h = Handle(sim)
+    N, C = h.N, h.C
+
+    print(f"[bench] warmup {args.warmup} steps", file=sys.stderr, flush=True)
+    sim.step(args.warmup)
+    h.enable_timing(True)
+    print(f"[bench] timing {args.steps} steps", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    sim.step(args.steps)            # returns after the device finished
+    wall = time.perf_counter() - t0
+    collide_ms, total_ms, launches = h.last_timing()
+
+    ms_per_step = wall * 1e3 / args.steps
+    value = N * args.steps / wall
+    ab = algorithmic_bytes(N, C)
+    collide_avg_ms = collide_ms / args.steps
+    achieved = ab["collide"] / (collide_avg_ms * 1e-3) / 1e9
+    traffic = load_traffic(f"collide_{args.config}")
+    roofline = {
+        "bound": "hbm",
+        "kernel": "k_collide (gather + stream + sort-in-cell + collide + next-step histogram)",
+        "achieved": round(achieved, 1),
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBS, 4),
+        "traffic": traffic.get("bytes_per_launch") if traffic and traffic.get("particles") == N else None,
+        "algorithmic_bytes_per_launch": ab["collide"],
+        "avg_launch_ms": round(collide_avg_ms, 4),
+    }
+    per_step = 120 + 20 / cfg["n"]
+    step_roofline = {
+        "basis": f"{per_step:.0f} B/particle-step pipeline figure (stream+histogram 24, scatter 48, collision 48, cells 20/n)",
+        "achieved": round(value * per_step / 1e9, 1),
+        "frac": round(value * per_step / 1e9 / HBM_PEAK_GBS, 4),
+        "this_build_bytes_per_particle_step": round(ab["step"] / N, 2),
+        "this_build_GBps": round(ab["step"] * args.steps / wall / 1e9, 1),
+    }
+    out = {
+        "metric": "particle-steps/s",
+        "value": value,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": round(value / REFERENCE_PSTEPS, 3) if args.config == "c2" else None,
+        "dtype": "f32",
+        "data": "synthetic (seeded uniform positions, Maxwell-Boltzmann velocities)",
+        "config": {"workload": cfg["workload"], "cells": [int(v) for v in p.volume_size], "particles": N,
+                   "algorithm": cfg["algorithm"], "n": cfg["n"], "delta_t": 0.02, "drag": 0.001,
+                   "tiling": list(h.tiling()), "parallelism": "single GPU"},
+        "roofline": roofline,
+        "step_roofline": step_roofline,
+        "device_ms": {"collide_total": round(collide_ms, 3), "step_total": round(total_ms, 3), "launches": launches},
+    }
+    if not args.no_cpu:
+        print("[bench] cpu baseline", file=sys.stderr, flush=True)
+        out["cpu_baseline"] = cpu_baseline(cfg["n"])
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

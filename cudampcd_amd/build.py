@@ -1,0 +1,92 @@
+"""In-tree build of the native libraries.
+
+    libmpcd_hip.so   HIP kernels + C-ABI (hipcc, gfx950)
+    _pympcd*.so      C++ API (Backend, HipBackend, SimulationHandle) + pybind11
+                     module, linked against libmpcd_hip.so
+
+Run ``python -m cudampcd_amd.build`` or call ``build()``.
+"""
+from __future__ import annotations
+
+import os
+import shutil
+import subprocess
+import sys
+import sysconfig
+from pathlib import Path
+
+PKG = Path(__file__).resolve().parent
+ROOT = PKG.parent
+CSRC = PKG / "csrc"
+INCLUDE = ROOT / "include"
+HIP_LIB = PKG / "libmpcd_hip.so"
+EXT = PKG / ("_pympcd" + sysconfig.get_config_var("EXT_SUFFIX"))
+
+ARCH = os.environ.get("MPCD_OFFLOAD_ARCH", "gfx950")
+
+# Floating-point contraction stays off so per-particle arithmetic matches the
+# CPU restatement bit for bit; packed FP32 instructions are disabled.
+HIPCC_FLAGS = [
+    "-O3",
+    f"--offload-arch={ARCH}",
+    "-std=c++17",
+    "-fPIC",
+    "-shared",
+    "-ffp-contract=off",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wall", "-Wno-unused-function",
+]
+
+
+def _hipcc() -> str:
+    for cand in (os.environ.get("HIPCC"), shutil.which("hipcc"), "/opt/rocm/bin/hipcc"):
+        if cand and Path(cand).exists():
+            return cand
+    raise RuntimeError("hipcc not found")
+
+
+def _run(cmd, verbose):
+    if verbose:
+        print("+", " ".join(str(c) for c in cmd), flush=True)
+    res = subprocess.run([str(c) for c in cmd], capture_output=True, text=True)
+    lines = [ln for ln in (res.stdout + res.stderr).splitlines()
+             if "packed-fp32-ops' is not a recognized feature" not in ln]
+    if res.returncode != 0:
+        raise RuntimeError("build failed: " + " ".join(str(c) for c in cmd) + "\n" + "\n".join(lines))
+    if verbose and lines:
+        print("\n".join(lines))
+
+
+def build_hip(verbose: bool = True) -> Path:
+    src = CSRC / "mpcd_hip.hip"
+    tmp = HIP_LIB.with_suffix(".so.tmp")
+    _run([_hipcc(), *HIPCC_FLAGS, f"-I{INCLUDE}", f"-I{CSRC}", src, "-o", tmp], verbose)
+    os.replace(tmp, HIP_LIB)
+    return HIP_LIB
+
+
+def build_python(verbose: bool = True) -> Path:
+    import pybind11
+
+    srcs = [CSRC / "bindings.cpp", CSRC / "api" / "hip_backend.cpp", CSRC / "api" / "simulation_handle.cpp"]
+    tmp = EXT.with_name(EXT.name + ".tmp")
+    cmd = [
+        os.environ.get("CXX", "g++"), "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall",
+        f"-I{INCLUDE}", f"-I{CSRC}", f"-I{CSRC / 'api'}",
+        f"-I{pybind11.get_include()}", f"-I{sysconfig.get_paths()['include']}",
+        *srcs,
+        f"-L{PKG}", "-lmpcd_hip", "-Wl,-rpath,$ORIGIN",
+        "-o", tmp,
+    ]
+    _run(cmd, verbose)
+    os.replace(tmp, EXT)
+    return EXT
+
+
+def build(verbose: bool = True) -> None:
+    build_hip(verbose)
+    build_python(verbose)
+
+
+if __name__ == "__main__":
+    build(verbose="-q" not in sys.argv)

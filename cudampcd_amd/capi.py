@@ -1,0 +1,160 @@
+"""ctypes binding of the C-ABI in include/mpcd_hip.h (libmpcd_hip.so).
+
+The pybind11 module covers the reference's Python surface; this module exposes
+the remaining entry points (parity hooks, timing, tiling, device self-tests) on
+the same handle, obtained with ``Simulation._handle()``.
+"""
+from __future__ import annotations
+
+import ctypes
+import re
+from pathlib import Path
+
+import numpy as np
+
+PKG = Path(__file__).resolve().parent
+LIB_PATH = PKG / "libmpcd_hip.so"
+HEADER = PKG.parent / "include" / "mpcd_hip.h"
+
+_u32p = ctypes.POINTER(ctypes.c_uint32)
+_f32p = ctypes.POINTER(ctypes.c_float)
+_f64p = ctypes.POINTER(ctypes.c_double)
+_u64p = ctypes.POINTER(ctypes.c_uint64)
+_vp = ctypes.c_void_p
+
+_lib = None
+
+
+def declared_symbols() -> list[str]:
+    """Entry points declared in include/mpcd_hip.h."""
+    text = HEADER.read_text()
+    return sorted(set(re.findall(r"\b(mpcd_hip_[a-z0-9_]+)\s*\(", text)))
+
+
+def lib() -> ctypes.CDLL:
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise ImportError(f"{LIB_PATH} is not built; run python -m cudampcd_amd.build")
+        L = ctypes.CDLL(str(LIB_PATH))
+        L.mpcd_hip_last_error.restype = ctypes.c_char_p
+        L.mpcd_hip_last_error.argtypes = [_vp]
+        sigs = {
+            "mpcd_hip_step": [_vp, ctypes.c_int],
+            "mpcd_hip_step_and_sample": [_vp, ctypes.c_int],
+            "mpcd_hip_get_sample_mean": [_vp, _f32p, _f32p, ctypes.c_size_t],
+            "mpcd_hip_get_sample_mean_f64": [_vp, _f64p, _f64p, ctypes.c_size_t],
+            "mpcd_hip_n_particles": [_vp, _u64p],
+            "mpcd_hip_n_cells": [_vp, _u64p],
+            "mpcd_hip_get_positions": [_vp, _f32p, ctypes.c_size_t],
+            "mpcd_hip_get_velocities": [_vp, _f32p, ctypes.c_size_t],
+            "mpcd_hip_set_positions": [_vp, _f32p, ctypes.c_size_t],
+            "mpcd_hip_set_velocities": [_vp, _f32p, ctypes.c_size_t],
+            "mpcd_hip_set_particles": [_vp, _f32p, _f32p, ctypes.c_size_t],
+            "mpcd_hip_get_step_index": [_vp, _u64p],
+            "mpcd_hip_set_step_index": [_vp, ctypes.c_uint64],
+            "mpcd_hip_grid_shift": [_vp, ctypes.c_uint64, _f32p],
+            "mpcd_hip_peek_sort": [_vp, _u32p, _u32p, _u32p],
+            "mpcd_hip_record_permutation": [_vp, ctypes.c_int],
+            "mpcd_hip_last_permutation": [_vp, _u32p, ctypes.c_size_t],
+            "mpcd_hip_enable_timing": [_vp, ctypes.c_int],
+            "mpcd_hip_last_timing": [_vp, _f64p, _f64p, ctypes.POINTER(ctypes.c_int)],
+            "mpcd_hip_set_tiling": [_vp, ctypes.c_uint32, ctypes.c_uint32],
+            "mpcd_hip_get_tiling": [_vp, _u32p, _u32p],
+            "mpcd_hip_selftest_philox": [ctypes.c_uint32, ctypes.c_uint32, _u32p, _u32p, ctypes.c_size_t],
+            "mpcd_hip_selftest_math": [_f32p, _f32p, ctypes.c_size_t, ctypes.c_int],
+        }
+        for name, args in sigs.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = ctypes.c_int
+        _lib = L
+    return _lib
+
+
+def _p(a, t):
+    return a.ctypes.data_as(t)
+
+
+class Handle:
+    """C-ABI calls on the handle of a live Simulation."""
+
+    def __init__(self, sim):
+        self.sim = sim  # keeps the simulation alive
+        self.h = ctypes.c_void_p(sim._handle())
+        self.L = lib()
+        n = ctypes.c_uint64()
+        self._check(self.L.mpcd_hip_n_particles(self.h, ctypes.byref(n)))
+        self.N = int(n.value)
+        self._check(self.L.mpcd_hip_n_cells(self.h, ctypes.byref(n)))
+        self.C = int(n.value)
+
+    def _check(self, rc):
+        if rc != 0:
+            raise RuntimeError(self.L.mpcd_hip_last_error(self.h).decode())
+
+    @property
+    def step_index(self) -> int:
+        v = ctypes.c_uint64()
+        self._check(self.L.mpcd_hip_get_step_index(self.h, ctypes.byref(v)))
+        return int(v.value)
+
+    @step_index.setter
+    def step_index(self, s: int):
+        self._check(self.L.mpcd_hip_set_step_index(self.h, s))
+
+    def grid_shift(self, step: int) -> np.ndarray:
+        out = np.empty(3, np.float32)
+        self._check(self.L.mpcd_hip_grid_shift(self.h, step, _p(out, _f32p)))
+        return out
+
+    def peek_sort(self):
+        cell = np.empty(self.N, np.uint32)
+        cnt = np.empty(self.C, np.uint32)
+        perm = np.empty(self.N, np.uint32)
+        self._check(self.L.mpcd_hip_peek_sort(self.h, _p(cell, _u32p), _p(cnt, _u32p), _p(perm, _u32p)))
+        return cell, cnt, perm
+
+    def record_permutation(self, on: bool = True):
+        self._check(self.L.mpcd_hip_record_permutation(self.h, int(on)))
+
+    def last_permutation(self) -> np.ndarray:
+        out = np.empty(self.N, np.uint32)
+        self._check(self.L.mpcd_hip_last_permutation(self.h, _p(out, _u32p), self.N))
+        return out
+
+    def enable_timing(self, on: bool = True):
+        self._check(self.L.mpcd_hip_enable_timing(self.h, int(on)))
+
+    def last_timing(self):
+        a = ctypes.c_double()
+        b = ctypes.c_double()
+        n = ctypes.c_int()
+        self._check(self.L.mpcd_hip_last_timing(self.h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(n)))
+        return a.value, b.value, n.value
+
+    def set_tiling(self, cells_per_tile: int = 0, lds_capacity: int = 0):
+        self._check(self.L.mpcd_hip_set_tiling(self.h, cells_per_tile, lds_capacity))
+
+    def tiling(self):
+        a = ctypes.c_uint32()
+        b = ctypes.c_uint32()
+        self._check(self.L.mpcd_hip_get_tiling(self.h, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
+
+
+def selftest_philox(k0: int, k1: int, ctr: np.ndarray) -> np.ndarray:
+    ctr = np.ascontiguousarray(ctr, dtype=np.uint32).reshape(-1, 4)
+    out = np.empty_like(ctr)
+    if lib().mpcd_hip_selftest_philox(k0, k1, _p(ctr, _u32p), _p(out, _u32p), len(ctr)) != 0:
+        raise RuntimeError("device self-test failed")
+    return out
+
+
+def selftest_math(x: np.ndarray, which: int) -> np.ndarray:
+    """which: 0 log, 1 sincospi, 2 exp, 3 sqrt, 4 fmod/wrap (L=7), 5 reciprocal / divide by 3."""
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    out = np.empty((len(x), 2), np.float32)
+    if lib().mpcd_hip_selftest_math(_p(x, _f32p), _p(out, _f32p), len(x), which) != 0:
+        raise RuntimeError("device self-test failed")
+    return out

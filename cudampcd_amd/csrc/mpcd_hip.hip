@@ -1,0 +1,1403 @@
+// MPCD time step for MI355X (gfx950, wave64): device kernels, host
+// orchestration and the C-ABI declared in include/mpcd_hip.h.
+//
+// One step (reference order: shift -> stream -> collide, cuda_backend_impl.cu:111-131):
+//
+//   scan      offsets[c] = exclusive prefix of the per-cell counts of this
+//             step (counts were accumulated by the previous collide launch);
+//             counts are cleared for the next step in the same pass.
+//   perm      perm[offsets[key_j] + rank_j] = j: cell-ordered permutation of
+//             the particle array (rank from wave-aggregated atomics).
+//   collide   one workgroup per tile of consecutive cells: gathers its cells'
+//             particles through perm into LDS, orders each cell by source
+//             index (stable counting-sort order, identical to the reference's
+//             cell list, simulation_functions.cpp:84-94), re-applies the
+//             streaming of this step, runs the SRD or extended collision, writes
+//             the particles out in cell order (SoA, coalesced), and already
+//             streams them with the next step's shift to count the next step's
+//             cells (fused histogram) and, when sampling, accumulates the mean
+//             fields in fixed point.
+//
+// Particle state is structure-of-arrays FP32 (x, y, z, vx, vy, vz planes,
+// 24 B per particle), double-buffered; the collide launch reads one buffer
+// and writes the other.  All floating-point code is compiled without
+// contraction so that per-particle arithmetic matches the CPU restatement
+// bit for bit.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#pragma clang fp contract(off)
+
+#include "mpcd_hip.h"
+#include "rng.h"
+
+namespace mpcd_hip {
+
+constexpr int WG = 256;          // threads per workgroup (4 wave64)
+constexpr int TC_MAX = 256;      // cells per tile upper bound
+constexpr double FIX_SCALE = 1099511627776.0;   // 2^40, fixed-point velocity sums
+
+struct KParams {
+    float L[3];          // volume edges (wall layers included)
+    float half[3];       // L / 2
+    uint32_t dim[3];     // X, Y, Z
+    uint32_t stride[3];  // internal cell-key stride of axis x, y, z
+    uint32_t D0, D1;     // internal fastest / middle extent
+    uint32_t ord[3];     // axis at internal position 0 (fastest), 1, 2
+    uint32_t C;          // cells
+    int perio_z;
+    float dt, drag_dt, tv, sin_a, cos_a;
+    uint32_t n, cap;     // particles per cell, cell table capacity 4n
+    Stream rs;
+    float axes[13][3];   // extended collision directions (simulation_functions.cpp:272-290)
+    // per launch
+    float shift[3], shift_next[3];
+    uint32_t step;
+    uint32_t tile_cells, lds_cap, ntiles;
+    uint64_t N;
+};
+
+struct SampleBufs {
+    uint32_t* cnt;                  // per reference cell
+    unsigned long long* sum;        // 3 per reference cell, fixed point 2^40, two's complement
+};
+
+// ------------------------------------------------------------ helpers ---
+__device__ __forceinline__ float wrap1(float r, float L) { return fmodf(r + 1.5f * L, L) - L * 0.5f; }
+
+// streaming with z-wall bounce-back (simulation_functions.cpp:56-74), then
+// shift and periodic wrap (:84)
+__device__ __forceinline__ void stream_shift(KParams const& P, float const* sh, float& px, float& py, float& pz,
+                                             float& vx, float& vy, float& vz) {
+    float const dt = P.dt;
+    if (!P.perio_z) {
+        float const z_wall = (0.5f * P.L[2] - 1.0f);
+        float const next_z = pz + vz * dt;
+        if (fabsf(next_z) > z_wall) {
+            float const t = (next_z > 0 ? z_wall - pz : -z_wall - pz) / vz;
+            px = px + vx * t;
+            py = py + vy * t;
+            pz = pz + vz * t;
+            vx = -vx;
+            vy = -vy;
+            vz = -vz;
+            float const rem = dt - t;
+            px = px + vx * rem;
+            py = py + vy * rem;
+            pz = pz + vz * rem;
+        } else {
+            px = px + vx * dt;
+            py = py + vy * dt;
+            pz = pz + vz * dt;
+        }
+    } else {
+        px = px + vx * dt;
+        py = py + vy * dt;
+        pz = pz + vz * dt;
+    }
+    px = wrap1(px + sh[0], P.L[0]);
+    py = wrap1(py + sh[1], P.L[1]);
+    pz = wrap1(pz + sh[2], P.L[2]);
+}
+
+// cell coordinates (volume_container.hpp:69-75)
+__device__ __forceinline__ void cell_of(KParams const& P, float px, float py, float pz, uint32_t& ix, uint32_t& iy,
+                                        uint32_t& iz) {
+    ix = static_cast<uint32_t>(floorf(px + P.half[0])) % P.dim[0];
+    iy = static_cast<uint32_t>(floorf(py + P.half[1])) % P.dim[1];
+    iz = static_cast<uint32_t>(floorf(pz + P.half[2])) % P.dim[2];
+}
+__device__ __forceinline__ uint32_t key_of(KParams const& P, uint32_t ix, uint32_t iy, uint32_t iz) {
+    return ix * P.stride[0] + iy * P.stride[1] + iz * P.stride[2];
+}
+__device__ __forceinline__ uint32_t ref_of(KParams const& P, uint32_t ix, uint32_t iy, uint32_t iz) {
+    return ix + P.dim[0] * (iy + P.dim[1] * iz);
+}
+__device__ __forceinline__ void decode_key(KParams const& P, uint32_t key, uint32_t& ix, uint32_t& iy, uint32_t& iz) {
+    uint32_t const q0 = key % P.D0;
+    uint32_t const k1 = key / P.D0;
+    uint32_t const q1 = k1 % P.D1;
+    uint32_t const q2 = k1 / P.D1;
+    ix = P.ord[0] == 0 ? q0 : (P.ord[1] == 0 ? q1 : q2);
+    iy = P.ord[0] == 1 ? q0 : (P.ord[1] == 1 ? q1 : q2);
+    iz = P.ord[0] == 2 ? q0 : (P.ord[1] == 2 ? q1 : q2);
+}
+
+// Wave-aggregated histogram: lanes holding the same key share one atomic;
+// returns each lane's rank among the lanes of its key plus the count before.
+// Must be called by whole waves in uniform control flow.
+__device__ __forceinline__ uint32_t wave_hist(uint32_t* count, uint32_t key, bool valid) {
+    uint32_t const lane = __lane_id();
+    uint64_t const below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    uint32_t rank = 0;
+    bool active = valid;
+    for (;;) {
+        uint64_t const m = __ballot(active);
+        if (m == 0) break;
+        int const leader = __builtin_ctzll(m);
+        uint32_t const lk = __builtin_amdgcn_readlane(key, leader);
+        bool const mine = active && key == lk;
+        uint64_t const grp = __ballot(mine);
+        uint32_t base = 0;
+        if (static_cast<int>(lane) == leader) base = atomicAdd(count + lk, static_cast<uint32_t>(__popcll(grp)));
+        base = __builtin_amdgcn_readlane(base, leader);
+        if (mine) {
+            rank = base + static_cast<uint32_t>(__popcll(grp & below));
+            active = false;
+        }
+    }
+    return rank;
+}
+
+__device__ __forceinline__ uint32_t xcd_tile(uint32_t b, uint32_t G) {
+    // workgroups are dispatched round-robin over the 8 XCDs; give each XCD a
+    // contiguous run of tiles so neighbouring tiles share its L2
+    uint32_t const q = G / 8, r = G % 8;
+    uint32_t const x = b % 8, i = b / 8;
+    return x * q + (x < r ? x : r) + i;
+}
+
+// ------------------------------------------------------------- kernels ---
+
+// seeded initial distribution (simulation_functions.cpp:7-33): positions
+// uniform in the fluid region, Maxwell-Boltzmann velocities
+__global__ __launch_bounds__(WG) void k_init(KParams P, float* __restrict__ st, float sx, float sy, float sz) {
+    uint64_t const N = P.N;
+    for (uint64_t i = blockIdx.x * static_cast<uint64_t>(WG) + threadIdx.x; i < N;
+         i += static_cast<uint64_t>(gridDim.x) * WG) {
+        uint32_t const lo = static_cast<uint32_t>(i), hi = static_cast<uint32_t>(i >> 32);
+        Words const a = P.rs.at(P_INIT_POS, lo, hi, 0);
+        Words const b = P.rs.at(P_INIT_VEL, lo, hi, 0);
+        float g0, g1, g2, g3;
+        box_muller(b.w[0], b.w[1], &g0, &g1);
+        box_muller(b.w[2], b.w[3], &g2, &g3);
+        st[i] = (uniform(a.w[0]) - 0.5f) * sx;
+        st[N + i] = (uniform(a.w[1]) - 0.5f) * sy;
+        st[2 * N + i] = (uniform(a.w[2]) - 0.5f) * sz;
+        st[3 * N + i] = g0 * P.tv;
+        st[4 * N + i] = g1 * P.tv;
+        st[5 * N + i] = g2 * P.tv;
+    }
+}
+
+// streaming half-step of an unsorted state: cell key and rank per particle
+__global__ __launch_bounds__(WG) void k_keys(KParams P, const float* __restrict__ st, uint32_t* __restrict__ count,
+                                             uint2* __restrict__ keyrank) {
+    uint64_t const N = P.N;
+    uint64_t const j = blockIdx.x * static_cast<uint64_t>(WG) + threadIdx.x;
+    bool const valid = j < N;
+    uint32_t key = 0;
+    if (valid) {
+        float px = st[j], py = st[N + j], pz = st[2 * N + j];
+        float vx = st[3 * N + j], vy = st[4 * N + j], vz = st[5 * N + j];
+        stream_shift(P, P.shift, px, py, pz, vx, vy, vz);
+        uint32_t ix, iy, iz;
+        cell_of(P, px, py, pz, ix, iy, iz);
+        key = key_of(P, ix, iy, iz);
+    }
+    uint32_t const rank = wave_hist(count, key, valid);
+    if (valid) keyrank[j] = make_uint2(key, rank);
+}
+
+// ---- exclusive scan of the per-cell counts (3 passes) ----
+constexpr int SCAN_ITEMS = 16;
+constexpr int SCAN_CHUNK = WG * SCAN_ITEMS;
+
+template <int NT>
+__device__ __forceinline__ uint32_t block_exclusive(uint32_t v, uint32_t* s_w, uint32_t* total) {
+    uint32_t const lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    uint32_t x = v;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        uint32_t const y = __shfl_up(x, d, 64);
+        if (lane >= static_cast<uint32_t>(d)) x += y;
+    }
+    if (lane == 63) s_w[w] = x;
+    __syncthreads();
+    uint32_t pre = 0, tot = 0;
+#pragma unroll
+    for (int k = 0; k < NT / 64; ++k) {
+        uint32_t const t = s_w[k];
+        if (static_cast<uint32_t>(k) < w) pre += t;
+        tot += t;
+    }
+    __syncthreads();
+    *total = tot;
+    return pre + x - v;
+}
+
+__global__ __launch_bounds__(WG) void k_scan_reduce(const uint32_t* __restrict__ count, uint32_t C,
+                                                    uint32_t* __restrict__ partial) {
+    __shared__ uint32_t s_w[WG / 64];
+    uint64_t const base = static_cast<uint64_t>(blockIdx.x) * SCAN_CHUNK;
+    uint32_t s = 0;
+#pragma unroll
+    for (int j = 0; j < SCAN_ITEMS; ++j) {
+        uint64_t const idx = base + j * WG + threadIdx.x;
+        if (idx < C) s += count[idx];
+    }
+    uint32_t tot;
+    block_exclusive<WG>(s, s_w, &tot);
+    if (threadIdx.x == 0) partial[blockIdx.x] = tot;
+}
+
+__global__ __launch_bounds__(1024) void k_scan_partials(uint32_t* __restrict__ partial, uint32_t nb) {
+    __shared__ uint32_t s_w[1024 / 64];
+    uint32_t carry = 0;
+    for (uint32_t base = 0; base < nb; base += 1024) {
+        uint32_t const i = base + threadIdx.x;
+        uint32_t const v = i < nb ? partial[i] : 0u;
+        uint32_t tot;
+        uint32_t const ex = block_exclusive<1024>(v, s_w, &tot);
+        if (i < nb) partial[i] = carry + ex;
+        carry += tot;
+    }
+}
+
+__global__ __launch_bounds__(WG) void k_scan_down(uint32_t* __restrict__ count, uint32_t C,
+                                                  const uint32_t* __restrict__ partial, uint32_t* __restrict__ offset,
+                                                  uint64_t N) {
+    __shared__ uint32_t s_v[SCAN_CHUNK + SCAN_CHUNK / 32];
+    __shared__ uint32_t s_w[WG / 64];
+    uint64_t const base = static_cast<uint64_t>(blockIdx.x) * SCAN_CHUNK;
+    auto pad = [](uint32_t i) { return i + (i >> 5); };
+#pragma unroll
+    for (int j = 0; j < SCAN_ITEMS; ++j) {
+        uint32_t const li = j * WG + threadIdx.x;
+        uint64_t const idx = base + li;
+        s_v[pad(li)] = idx < C ? count[idx] : 0u;
+    }
+    __syncthreads();
+    uint32_t run = 0;
+#pragma unroll
+    for (int j = 0; j < SCAN_ITEMS; ++j) run += s_v[pad(threadIdx.x * SCAN_ITEMS + j)];
+    uint32_t tot;
+    uint32_t pre = block_exclusive<WG>(run, s_w, &tot);
+    pre += partial[blockIdx.x];
+#pragma unroll
+    for (int j = 0; j < SCAN_ITEMS; ++j) {
+        uint32_t const li = pad(threadIdx.x * SCAN_ITEMS + j);
+        uint32_t const x = s_v[li];
+        s_v[li] = pre;
+        pre += x;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < SCAN_ITEMS; ++j) {
+        uint32_t const li = j * WG + threadIdx.x;
+        uint64_t const idx = base + li;
+        if (idx < C) {
+            offset[idx] = s_v[pad(li)];
+            count[idx] = 0u;
+        }
+    }
+    if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) offset[C] = static_cast<uint32_t>(N);
+}
+
+__global__ __launch_bounds__(WG) void k_perm(const uint2* __restrict__ keyrank, const uint32_t* __restrict__ offset,
+                                             uint32_t* __restrict__ perm, uint64_t N) {
+    for (uint64_t j = blockIdx.x * static_cast<uint64_t>(WG) + threadIdx.x; j < N;
+         j += static_cast<uint64_t>(gridDim.x) * WG) {
+        uint2 const kr = keyrank[j];
+        perm[offset[kr.x] + kr.y] = static_cast<uint32_t>(j);
+    }
+}
+
+// per-cell ghost count in a wall layer (simulation_functions.cpp:137-146)
+__device__ __forceinline__ uint32_t ghost_count(KParams const& P, uint32_t ref, float sh, bool layer, bool sign) {
+    uint32_t added = 0;
+    for (uint32_t i = 0; i < P.n; i += 4) {
+        Words const r = P.rs.at(P_GHOST_COUNT, i >> 2, ref, P.step);
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            if (i + k < P.n && (((uniform(r.w[k]) > sh) ^ layer) ^ sign)) ++added;
+    }
+    return added;
+}
+
+// Marsaglia unit vector, attempts keyed by index (random.hpp:237-253)
+__device__ __forceinline__ void srd_axis(KParams const& P, uint32_t ref, float& ax, float& ay, float& az) {
+    for (uint32_t a = 0;; ++a) {
+        Words const r = P.rs.at(P_AXIS, a, ref, P.step);
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            float const rd1 = 1.0f - 2.0f * uniform(r.w[2 * h]);
+            float const rd2 = 1.0f - 2.0f * uniform(r.w[2 * h + 1]);
+            float const rsq = rd1 * rd1 + rd2 * rd2;
+            if (!(rsq > 1.0f)) {
+                float const rdh = 2.0f * sqrtf(1.0f - rsq);
+                ax = rd1 * rdh;
+                ay = rd2 * rdh;
+                az = (1.0f - 2.0f * rsq);
+                return;
+            }
+        }
+    }
+}
+
+struct Sym {
+    float xx, yy, zz, xy, xz, yz;
+};
+
+// mechanic.hpp:26, 52-79
+__device__ __forceinline__ Sym sym_inverse(Sym const& m, float tol) {
+    float det = m.xx * (m.yy * m.zz - m.yz * m.yz) + m.xz * (2.0f * m.xy * m.yz - m.yy * m.xz) - (m.xy * m.xy * m.zz);
+    if (det < 0) det = 0;
+    Sym o = {0, 0, 0, 0, 0, 0};
+    if (det > tol) {
+        float const id = 1.0f / det;
+        o.xx = (m.yy * m.zz - m.yz * m.yz) * id;
+        o.yy = (m.xx * m.zz - m.xz * m.xz) * id;
+        o.zz = (m.xx * m.yy - m.xy * m.xy) * id;
+        o.xy = (m.xz * m.yz - m.zz * m.xy) * id;
+        o.xz = (m.xy * m.yz - m.yy * m.xz) * id;
+        o.yz = (m.xy * m.xz - m.xx * m.yz) * id;
+        return o;
+    }
+    float const tr = m.xx + m.yy + m.zz;
+    if (tr > tol) {
+        float const s = 4.0f / (tr * tr);
+        o.xx = m.xx * s;
+        o.yy = m.yy * s;
+        o.zz = m.zz * s;
+        o.xy = m.xy * s;
+        o.xz = m.xz * s;
+        o.yz = m.yz * s;
+    }
+    return o;
+}
+
+// LDS carve-up of the collide kernel
+struct Lds {
+    uint32_t* perm;   // [cap] source index, arrival order
+    uint32_t* src;    // [cap] source index, cell order
+    uint16_t* cell;   // [cap] tile-local cell of each slot
+    float* p[6];      // [cap] x y z vx vy vz, cell order
+    uint32_t* off;    // [tile_cells + 1]
+    float* cm[6];     // [tile_cells] mean velocity xyz, axis xyz
+    uint32_t* go;     // [tile_cells] collision happened
+    float* g[6];      // [cap/2] ghosts of wall cells (extended), slot k*n + i
+    uint32_t* bc;     // [4] broadcast
+};
+
+__host__ __device__ inline size_t lds_bytes(uint32_t cap, uint32_t tc, int algo) {
+    size_t b = 0;
+    b += 4ull * cap * 2 + 2ull * cap + 4ull * 6 * cap;
+    b += 4ull * (tc + 1) + 4ull * 6 * tc + 4ull * tc;
+    if (algo == 1) b += 4ull * 6 * (cap / 2);
+    b += 16;
+    return (b + 15) & ~size_t(15);
+}
+
+__device__ __forceinline__ Lds carve(unsigned char* base, uint32_t cap, uint32_t tc, int algo) {
+    Lds s;
+    unsigned char* q = base;
+    s.perm = reinterpret_cast<uint32_t*>(q); q += 4ull * cap;
+    s.src = reinterpret_cast<uint32_t*>(q); q += 4ull * cap;
+    for (int k = 0; k < 6; ++k) { s.p[k] = reinterpret_cast<float*>(q); q += 4ull * cap; }
+    s.off = reinterpret_cast<uint32_t*>(q); q += 4ull * (tc + 1);
+    for (int k = 0; k < 6; ++k) { s.cm[k] = reinterpret_cast<float*>(q); q += 4ull * tc; }
+    s.go = reinterpret_cast<uint32_t*>(q); q += 4ull * tc;
+    if (algo == 1) {
+        for (int k = 0; k < 6; ++k) { s.g[k] = reinterpret_cast<float*>(q); q += 4ull * (cap / 2); }
+    } else {
+        for (int k = 0; k < 6; ++k) s.g[k] = nullptr;
+    }
+    s.bc = reinterpret_cast<uint32_t*>(q); q += 16;
+    s.cell = reinterpret_cast<uint16_t*>(q);
+    return s;
+}
+
+// extended collision of one cell by one lane (simulation_functions.cpp:207-406)
+__device__ void extended_cell(KParams const& P, Lds const& s, uint32_t k, uint32_t a, uint32_t mc, uint32_t ref,
+                              uint32_t ix, uint32_t iy, uint32_t iz) {
+    float const sh = fabsf(P.shift[2]);
+    bool const sign = P.shift[2] > 0;
+    float const zf = static_cast<float>(iz);
+    bool const layer = (zf == (P.L[2] - (sign ? 1.0f : 2.0f)));
+    bool const ghosts = (!P.perio_z) && ((zf == static_cast<float>(sign)) || layer);
+    uint32_t const reals_in = mc < P.cap ? mc : P.cap;
+    uint32_t const added = ghosts ? ghost_count(P, ref, sh, layer, sign) : 0u;
+    uint32_t total = reals_in + added;
+    if (total > P.cap) total = P.cap;
+    uint32_t const rr = total - added;
+    s.go[k] = total > 1;
+    // cell centre (volume_container.hpp:86-91)
+    float const ox = (0.5f + static_cast<float>(ix)) - P.half[0];
+    float const oy = (0.5f + static_cast<float>(iy)) - P.half[1];
+    float const oz = (0.5f + static_cast<float>(iz)) - P.half[2];
+    uint32_t const gb = k * P.n;
+    if (ghosts) {
+        for (uint32_t i = 0; i < added; ++i) {
+            Words const r = P.rs.at(P_GHOST_POS, i, ref, P.step);
+            float const uz = uniform(r.w[2]);
+            float const z = (layer ^ sign) ? sh * uz : sh + (1.0f - sh) * uz;
+            s.g[0][gb + i] = (uniform(r.w[0]) - 0.5f) + ox;
+            s.g[1][gb + i] = (uniform(r.w[1]) - 0.5f) + oy;
+            s.g[2][gb + i] = (z - 0.5f) + oz;
+            Words const v = P.rs.at(P_GHOST_VEL, i, ref, P.step);
+            float g0, g1, g2, g3;
+            box_muller(v.w[0], v.w[1], &g0, &g1);
+            box_muller(v.w[2], v.w[3], &g2, &g3);
+            s.g[3][gb + i] = g0 * P.tv;
+            s.g[4][gb + i] = g1 * P.tv;
+            s.g[5][gb + i] = g2 * P.tv;
+        }
+    }
+    if (total <= 1) return;
+    // particle i < rr lives at LDS slot a+i, ghost i >= rr at ghost slot gb+i-rr
+    auto X = [&](int c, uint32_t i) -> float& { return i < rr ? s.p[c][a + i] : s.g[c][gb + (i - rr)]; };
+
+    Words const e = P.rs.at(P_EXT, 0, ref, P.step);
+    int const select = static_cast<int>(e.w[0] / 2) % 13;
+    int const flip = static_cast<int>(e.w[1] / 2) % 2;
+    float ax = P.axes[select][0], ay = P.axes[select][1], az = P.axes[select][2];
+    if (flip) { ax = -ax; ay = -ay; az = -az; }
+
+    float cx = 0, cy = 0, cz = 0, mx = 0, my = 0, mz = 0;
+    for (uint32_t i = 0; i < total; ++i) {
+        cx = cx + (X(0, i) - ox);
+        cy = cy + (X(1, i) - oy);
+        cz = cz + (X(2, i) - oz);
+        mx = mx + X(3, i);
+        my = my + X(4, i);
+        mz = mz + X(5, i);
+    }
+    float const inv = 1.0f / static_cast<float>(total);
+    cx = cx * inv + ox;
+    cy = cy * inv + oy;
+    cz = cz * inv + oz;
+    mx = mx * inv;
+    my = my * inv;
+    mz = mz * inv;
+
+    float proj0 = 0.0f;
+    int size0 = 0;
+    for (uint32_t i = 0; i < total; ++i) {
+        float const rx = X(0, i) - cx, ry = X(1, i) - cy, rz = X(2, i) - cz;
+        float const ux = X(3, i) - mx, uy = X(4, i) - my, uz = X(5, i) - mz;
+        X(0, i) = rx; X(1, i) = ry; X(2, i) = rz;
+        X(3, i) = ux; X(4, i) = uy; X(5, i) = uz;
+        if (rx * ax + ry * ay + rz * az < 0) {
+            proj0 += ux * ax + uy * ay + uz * az;
+            size0 += 1;
+        }
+    }
+    int const size1 = static_cast<int>(total) - size0;
+    float const proj1 = -proj0 / static_cast<float>(size1);
+    proj0 = proj0 / static_cast<float>(size0);
+    float const arg = 0.01f * (proj1 - proj0) * static_cast<float>(size0) * static_cast<float>(size1);
+    float const probability = 1.0f - exp_f(arg);
+    bool const collide = uniform(e.w[2]) < probability;
+
+    float Lx = 0, Ly = 0, Lz = 0;
+    if (collide) {
+        float const t0 = proj1 * (static_cast<float>(size1) / static_cast<float>(size0));
+        float const t1 = proj0 * (static_cast<float>(size0) / static_cast<float>(size1));
+        float mean0 = 0.0f, mean1 = 0.0f;
+        Sym I = {0, 0, 0, 0, 0, 0};
+        float gq[4] = {0, 0, 0, 0};
+        for (uint32_t i = 0; i < total; ++i) {
+            if ((i & 3u) == 0) {
+                Words const r = P.rs.at(P_THERM, i >> 2, ref, P.step);
+                box_muller(r.w[0], r.w[1], &gq[0], &gq[1]);
+                box_muller(r.w[2], r.w[3], &gq[2], &gq[3]);
+            }
+            float const rx = X(0, i), ry = X(1, i), rz = X(2, i);
+            float ux = X(3, i), uy = X(4, i), uz = X(5, i);
+            bool const side = rx * ax + ry * ay + rz * az < 0;
+            float const vr = gq[i & 3u] * P.tv;
+            if (side) mean0 += vr; else mean1 += vr;
+            Lx = Lx + (ry * uz - rz * uy);
+            Ly = Ly + (rz * ux - rx * uz);
+            Lz = Lz + (rx * uy - ry * ux);
+            float const sc = vr - (ux * ax + uy * ay + uz * az) + (side ? t0 : t1);
+            ux = ux + ax * sc;
+            uy = uy + ay * sc;
+            uz = uz + az * sc;
+            X(3, i) = ux; X(4, i) = uy; X(5, i) = uz;
+            float const qx = rx * rx, qy = ry * ry, qz = rz * rz;
+            I.xx += qy + qz;
+            I.yy += qx + qz;
+            I.zz += qx + qy;
+            I.xy += -rx * ry;
+            I.xz += -rx * rz;
+            I.yz += -ry * rz;
+        }
+        float const a0 = mean0 / static_cast<float>(size0), a1 = mean1 / static_cast<float>(size1);
+        float const v0x = ax * a0, v0y = ay * a0, v0z = az * a0;
+        float const v1x = ax * a1, v1y = ay * a1, v1z = az * a1;
+        for (uint32_t i = 0; i < total; ++i) {
+            float const rx = X(0, i), ry = X(1, i), rz = X(2, i);
+            bool const side = rx * ax + ry * ay + rz * az < 0;
+            float const ux = X(3, i) - (side ? v0x : v1x);
+            float const uy = X(4, i) - (side ? v0y : v1y);
+            float const uz = X(5, i) - (side ? v0z : v1z);
+            X(3, i) = ux; X(4, i) = uy; X(5, i) = uz;
+            Lx = Lx - (ry * uz - rz * uy);
+            Ly = Ly - (rz * ux - rx * uz);
+            Lz = Lz - (rx * uy - ry * ux);
+        }
+        Sym const J = sym_inverse(I, 1e-5f);
+        float const nx = (J.xx * Lx) + (J.xy * Ly) + (J.xz * Lz);
+        float const ny = (J.xy * Lx) + (J.yy * Ly) + (J.yz * Lz);
+        float const nz = (J.xz * Lx) + (J.yz * Ly) + (J.zz * Lz);
+        Lx = nx; Ly = ny; Lz = nz;
+    }
+    for (uint32_t i = 0; i < rr; ++i) {
+        float const rx = s.p[0][a + i], ry = s.p[1][a + i], rz = s.p[2][a + i];
+        float ux = s.p[3][a + i] + mx, uy = s.p[4][a + i] + my, uz = s.p[5][a + i] + mz;
+        ux += P.drag_dt;
+        ux = ux - (ry * Lz - rz * Ly);
+        uy = uy - (rz * Lx - rx * Lz);
+        uz = uz - (rx * Ly - ry * Lx);
+        s.p[3][a + i] = ux; s.p[4][a + i] = uy; s.p[5][a + i] = uz;
+        s.p[0][a + i] = wrap1((cx + rx) - P.shift[0], P.L[0]);
+        s.p[1][a + i] = wrap1((cy + ry) - P.shift[1], P.L[1]);
+        s.p[2][a + i] = wrap1((cz + rz) - P.shift[2], P.L[2]);
+    }
+}
+
+template <int ALGO, bool SAMPLE>
+__global__ __launch_bounds__(WG) void k_collide(KParams P, const float* __restrict__ in, float* __restrict__ out,
+                                                const uint32_t* __restrict__ perm, const uint32_t* __restrict__ offset,
+                                                uint32_t* __restrict__ count_next, uint2* __restrict__ keyrank,
+                                                uint32_t* __restrict__ perm_out, SampleBufs S,
+                                                uint32_t* __restrict__ status) {
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    uint32_t const cap = P.lds_cap, TC = P.tile_cells;
+    Lds const s = carve(lds_raw, cap, TC, ALGO);
+    uint32_t const tid = threadIdx.x;
+    uint64_t const N = P.N;
+    uint32_t const tile = xcd_tile(blockIdx.x, gridDim.x);
+    uint32_t const c0 = tile * TC;
+    if (c0 >= P.C) return;
+    uint32_t const nc = min(TC, P.C - c0);
+    for (uint32_t k = tid; k <= nc; k += WG) s.off[k] = offset[c0 + k];
+    __syncthreads();
+
+    uint32_t kb = 0;
+    while (kb < nc) {
+        if (tid == 0) {
+            uint32_t ke = kb + 1;
+            while (ke < nc && s.off[ke + 1] - s.off[kb] <= cap) ++ke;
+            s.bc[0] = ke;
+        }
+        __syncthreads();
+        uint32_t const ke = s.bc[0];
+        uint32_t const gb = s.off[kb];
+        uint32_t const m = s.off[ke] - gb;
+        __syncthreads();
+        if (m > cap) {
+            // A single cell larger than the LDS tile: its particles are moved
+            // through unchanged (streamed and un-shifted, no collision) so the
+            // state and the next step's cell counts stay consistent, and the
+            // step reports the condition as an error.
+            if (tid == 0) atomicOr(status, 1u);
+            for (uint32_t i0 = 0; i0 < m; i0 += WG) {
+                uint32_t const i = i0 + tid;
+                bool const valid = i < m;
+                uint32_t knext = 0;
+                if (valid) {
+                    uint64_t const o = static_cast<uint64_t>(gb) + i;
+                    uint32_t const src = perm[o];
+                    float px = in[src], py = in[N + src], pz = in[2 * N + src];
+                    float vx = in[3 * N + src], vy = in[4 * N + src], vz = in[5 * N + src];
+                    stream_shift(P, P.shift, px, py, pz, vx, vy, vz);
+                    px = wrap1(px - P.shift[0], P.L[0]);
+                    py = wrap1(py - P.shift[1], P.L[1]);
+                    pz = wrap1(pz - P.shift[2], P.L[2]);
+                    out[o] = px; out[N + o] = py; out[2 * N + o] = pz;
+                    out[3 * N + o] = vx; out[4 * N + o] = vy; out[5 * N + o] = vz;
+                    if (perm_out) perm_out[o] = src;
+                    stream_shift(P, P.shift_next, px, py, pz, vx, vy, vz);
+                    uint32_t ix, iy, iz;
+                    cell_of(P, px, py, pz, ix, iy, iz);
+                    knext = key_of(P, ix, iy, iz);
+                }
+                uint32_t const rank = wave_hist(count_next, knext, valid);
+                if (valid) keyrank[static_cast<uint64_t>(gb) + i] = make_uint2(knext, rank);
+            }
+            __syncthreads();
+            kb = ke;
+            continue;
+        }
+        uint32_t const ncell = ke - kb;
+        for (uint32_t i = tid; i < m; i += WG) s.perm[i] = perm[gb + i];
+        for (uint32_t k = tid; k < ncell; k += WG) {
+            uint32_t const a = s.off[kb + k] - gb, e = s.off[kb + k + 1] - gb;
+            for (uint32_t i = a; i < e; ++i) s.cell[i] = static_cast<uint16_t>(k);
+        }
+        __syncthreads();
+        // order each cell by source index, gather, re-apply this step's streaming
+        for (uint32_t i = tid; i < m; i += WG) {
+            uint32_t const k = s.cell[i];
+            uint32_t const a = s.off[kb + k] - gb, e = s.off[kb + k + 1] - gb;
+            uint32_t const src = s.perm[i];
+            uint32_t r = 0;
+            for (uint32_t j = a; j < e; ++j) r += s.perm[j] < src;
+            uint32_t const d = a + r;
+            float px = in[src], py = in[N + src], pz = in[2 * N + src];
+            float vx = in[3 * N + src], vy = in[4 * N + src], vz = in[5 * N + src];
+            stream_shift(P, P.shift, px, py, pz, vx, vy, vz);
+            s.p[0][d] = px; s.p[1][d] = py; s.p[2][d] = pz;
+            s.p[3][d] = vx; s.p[4][d] = vy; s.p[5][d] = vz;
+            s.src[d] = src;
+        }
+        __syncthreads();
+        // per-cell stage, one lane per cell
+        for (uint32_t k = tid; k < ncell; k += WG) {
+            uint32_t const key = c0 + kb + k;
+            uint32_t ix, iy, iz;
+            decode_key(P, key, ix, iy, iz);
+            uint32_t const ref = ref_of(P, ix, iy, iz);
+            uint32_t const a = s.off[kb + k] - gb, mc = s.off[kb + k + 1] - gb - a;
+            if (ALGO == 0) {
+                float const sh = fabsf(P.shift[2]);
+                bool const sign = P.shift[2] > 0;
+                float const zf = static_cast<float>(iz);
+                bool const layer = (zf == (P.L[2] - (sign ? 1.0f : 2.0f)));
+                bool const ghosts = (!P.perio_z) && ((zf == static_cast<float>(sign)) || layer);
+                uint32_t const reals = mc < P.cap ? mc : P.cap;
+                uint32_t const added = ghosts ? ghost_count(P, ref, sh, layer, sign) : 0u;
+                uint32_t const total = reals + added;
+                s.go[k] = total > 1;
+                if (total > 1) {
+                    float sx = 0, sy = 0, sz = 0;
+                    for (uint32_t i = 0; i < reals; ++i) {
+                        sx = sx + s.p[3][a + i];
+                        sy = sy + s.p[4][a + i];
+                        sz = sz + s.p[5][a + i];
+                    }
+                    for (uint32_t g = 0; g < added; ++g) {
+                        Words const v = P.rs.at(P_GHOST_VEL, g, ref, P.step);
+                        float g0, g1, g2, g3;
+                        box_muller(v.w[0], v.w[1], &g0, &g1);
+                        box_muller(v.w[2], v.w[3], &g2, &g3);
+                        sx = sx + g0 * P.tv;
+                        sy = sy + g1 * P.tv;
+                        sz = sz + g2 * P.tv;
+                    }
+                    float const inv = 1.0f / static_cast<float>(total);
+                    s.cm[0][k] = sx * inv;
+                    s.cm[1][k] = sy * inv;
+                    s.cm[2][k] = sz * inv;
+                    float ax, ay, az;
+                    srd_axis(P, ref, ax, ay, az);
+                    s.cm[3][k] = ax;
+                    s.cm[4][k] = ay;
+                    s.cm[5][k] = az;
+                }
+            } else {
+                extended_cell(P, s, k, a, mc, ref, ix, iy, iz);
+            }
+        }
+        __syncthreads();
+        // per-particle finish, write-out, next-step cell count, sampling
+        for (uint32_t i0 = 0; i0 < m; i0 += WG) {
+            uint32_t const i = i0 + tid;
+            bool const valid = i < m;
+            uint32_t knext = 0;
+            if (valid) {
+                uint32_t const k = s.cell[i];
+                uint32_t const slot = i - (s.off[kb + k] - gb);
+                float px = s.p[0][i], py = s.p[1][i], pz = s.p[2][i];
+                float vx = s.p[3][i], vy = s.p[4][i], vz = s.p[5][i];
+                if (slot >= P.cap) {
+                    // table overflow: un-shifted, no collision (simulation_functions.cpp:90-93)
+                    px = wrap1(px - P.shift[0], P.L[0]);
+                    py = wrap1(py - P.shift[1], P.L[1]);
+                    pz = wrap1(pz - P.shift[2], P.L[2]);
+                } else if (ALGO == 0 && s.go[k]) {
+                    // SRD rotation about the cell axis (simulation_functions.cpp:176-188)
+                    float const mx = s.cm[0][k], my = s.cm[1][k], mz = s.cm[2][k];
+                    float const ax = s.cm[3][k], ay = s.cm[4][k], az = s.cm[5][k];
+                    float const ux = vx - mx, uy = vy - my, uz = vz - mz;
+                    float const d = ux * ax + uy * ay + uz * az;
+                    float const wx = ax * d, wy = ay * d, wz = az * d;
+                    float const qx = ux - wx, qy = uy - wy, qz = uz - wz;
+                    float const cxp = qy * az - qz * ay, cyp = qz * ax - qx * az, czp = qx * ay - qy * ax;
+                    vx = ((wx + qx * P.cos_a) + cxp * P.sin_a) + mx;
+                    vy = ((wy + qy * P.cos_a) + cyp * P.sin_a) + my;
+                    vz = ((wz + qz * P.cos_a) + czp * P.sin_a) + mz;
+                    px = wrap1(px - P.shift[0], P.L[0]);
+                    py = wrap1(py - P.shift[1], P.L[1]);
+                    pz = wrap1(pz - P.shift[2], P.L[2]);
+                    vx += P.drag_dt;
+                }
+                uint64_t const o = static_cast<uint64_t>(gb) + i;
+                out[o] = px; out[N + o] = py; out[2 * N + o] = pz;
+                out[3 * N + o] = vx; out[4 * N + o] = vy; out[5 * N + o] = vz;
+                if (perm_out) perm_out[o] = s.src[i];
+                if (SAMPLE) {
+                    uint32_t ix, iy, iz;
+                    cell_of(P, px, py, pz, ix, iy, iz);
+                    uint32_t const rc = ref_of(P, ix, iy, iz);
+                    atomicAdd(S.cnt + rc, 1u);
+                    atomicAdd(S.sum + 3ull * rc + 0, static_cast<unsigned long long>(static_cast<long long>(rint(static_cast<double>(vx) * FIX_SCALE))));
+                    atomicAdd(S.sum + 3ull * rc + 1, static_cast<unsigned long long>(static_cast<long long>(rint(static_cast<double>(vy) * FIX_SCALE))));
+                    atomicAdd(S.sum + 3ull * rc + 2, static_cast<unsigned long long>(static_cast<long long>(rint(static_cast<double>(vz) * FIX_SCALE))));
+                }
+                stream_shift(P, P.shift_next, px, py, pz, vx, vy, vz);
+                uint32_t ix, iy, iz;
+                cell_of(P, px, py, pz, ix, iy, iz);
+                knext = key_of(P, ix, iy, iz);
+            }
+            uint32_t const rank = wave_hist(count_next, knext, valid);
+            if (valid) keyrank[static_cast<uint64_t>(gb) + i] = make_uint2(knext, rank);
+        }
+        __syncthreads();
+        kb = ke;
+    }
+}
+
+// fold one sampled step into the FP64 accumulators (per reference cell)
+__global__ __launch_bounds__(WG) void k_sample_fold(uint32_t C, uint32_t* __restrict__ cnt,
+                                                    unsigned long long* __restrict__ sum, double* __restrict__ acc,
+                                                    int first) {
+    for (uint64_t c = blockIdx.x * static_cast<uint64_t>(WG) + threadIdx.x; c < C;
+         c += static_cast<uint64_t>(gridDim.x) * WG) {
+        uint32_t const n = cnt[c];
+        double const d = static_cast<double>(n);
+        double m[3];
+        for (int k = 0; k < 3; ++k) {
+            long long const v = static_cast<long long>(sum[3 * c + k]);
+            m[k] = n > 0 ? (static_cast<double>(v) / FIX_SCALE) / d : 0.0;
+            sum[3 * c + k] = 0ull;
+        }
+        cnt[c] = 0u;
+        if (first) {
+            acc[4 * c] = d;
+            for (int k = 0; k < 3; ++k) acc[4 * c + 1 + k] = m[k];
+        } else {
+            acc[4 * c] += d;
+            for (int k = 0; k < 3; ++k) acc[4 * c + 1 + k] += m[k];
+        }
+    }
+}
+
+__global__ void k_selftest_philox(uint32_t k0, uint32_t k1, const uint32_t* ctr, uint32_t* out, size_t n) {
+    size_t const i = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x;
+    if (i < n) {
+        Words const r = philox(k0, k1, ctr[4 * i], ctr[4 * i + 1], ctr[4 * i + 2], ctr[4 * i + 3]);
+        for (int k = 0; k < 4; ++k) out[4 * i + k] = r.w[k];
+    }
+}
+
+__global__ void k_selftest_math(const float* in, float* out, size_t n, int which) {
+    size_t const i = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x;
+    if (i >= n) return;
+    float const x = in[i];
+    float r0 = 0, r1 = 0;
+    switch (which) {
+        case 0: r0 = log_pos(x); break;
+        case 1: sincospi(x, &r0, &r1); break;
+        case 2: r0 = exp_f(x); break;
+        case 3: r0 = sqrtf(x); break;
+        case 4: r0 = fmodf(x, 7.0f); r1 = fmodf(x + 1.5f * 7.0f, 7.0f) - 7.0f * 0.5f; break;
+        case 5: r0 = 1.0f / x; r1 = x / 3.0f; break;
+        default: break;
+    }
+    out[2 * i] = r0;
+    out[2 * i + 1] = r1;
+}
+
+}  // namespace mpcd_hip
+
+// ============================================================== host side ===
+
+using namespace mpcd_hip;
+
+namespace {
+
+std::mutex g_err_mu;
+std::string g_create_err;
+
+struct HipError {
+    std::string msg;
+};
+
+inline void ck(hipError_t e, const char* what) {
+    if (e != hipSuccess) throw HipError{std::string(what) + ": " + hipGetErrorString(e)};
+}
+
+}  // namespace
+
+struct mpcd_hip_s {
+    mpcd_params_t prm{};
+    std::string err;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    KParams kp{};
+    uint64_t N = 0, C = 0;
+    float* st[2] = {nullptr, nullptr};
+    int cur = 0;
+    uint32_t* perm = nullptr;
+    uint2* keyrank = nullptr;
+    uint32_t* count = nullptr;
+    uint32_t* offset = nullptr;
+    uint32_t* partial = nullptr;
+    uint32_t nb = 0;
+    uint32_t* status = nullptr;
+    uint32_t* perm_rec = nullptr;
+    bool record = false, recorded = false;
+    // sampling
+    uint32_t* scnt = nullptr;
+    unsigned long long* ssum = nullptr;
+    double* acc = nullptr;
+    bool sampling_done = true;
+    uint64_t sample_counter = 0;
+    uint64_t step_index = 0;
+    bool keys_valid = false, sorted_valid = false;
+    uint32_t tile_cells = 0, lds_cap = 0;
+    // timing
+    bool timing = false;
+    std::vector<hipEvent_t> ev;
+    hipEvent_t ev_begin = nullptr, ev_end = nullptr;
+    double t_collide = 0, t_total = 0;
+    int launches = 0;
+};
+
+namespace {
+
+void fill_geometry(mpcd_hip_s* h) {
+    mpcd_params_t const& p = h->prm;
+    KParams& k = h->kp;
+    for (int a = 0; a < 3; ++a) {
+        k.L[a] = p.volume_size[a];
+        k.half[a] = p.volume_size[a] * 0.5f;
+        k.dim[a] = static_cast<uint32_t>(p.volume_size[a]);
+    }
+    k.C = k.dim[0] * k.dim[1] * k.dim[2];
+    // internal cell order: smallest extent fastest, so the cells a particle
+    // can move between lie as close together in memory as possible
+    uint32_t ord[3] = {0, 1, 2};
+    std::stable_sort(ord, ord + 3, [&](uint32_t x, uint32_t y) { return k.dim[x] < k.dim[y]; });
+    for (int a = 0; a < 3; ++a) k.ord[a] = ord[a];
+    k.D0 = k.dim[ord[0]];
+    k.D1 = k.dim[ord[1]];
+    k.stride[ord[0]] = 1;
+    k.stride[ord[1]] = k.D0;
+    k.stride[ord[2]] = k.D0 * k.D1;
+    k.perio_z = p.periodicity[2];
+    k.dt = p.delta_t;
+    k.drag_dt = p.drag * p.delta_t;
+    k.tv = p.thermal_velocity;
+    k.sin_a = sinf((M_PI * static_cast<double>(p.rotation_angle)) / 180);
+    k.cos_a = cosf((M_PI * static_cast<double>(p.rotation_angle)) / 180);
+    k.n = p.n;
+    k.cap = 4 * p.n;
+    k.rs.k0 = static_cast<uint32_t>(p.seed);
+    k.rs.k1 = static_cast<uint32_t>(p.seed >> 32);
+    k.rs.sys = p.system_id;
+    // the 13 directions of the extended rule (simulation_functions.cpp:273-290),
+    // evaluated with the same float expressions on the host
+    for (int sel = 0; sel < 13; ++sel) {
+        int const steps = 4;
+        int const phi_i = sel % steps + 1, theta_i = sel / steps + 1;
+        float theta;
+        if ((theta_i % 2) && (phi_i % 2))
+            theta = theta_i == 1 ? 0.95531661812450927816f : float(M_PI) - 0.95531661812450927816f;
+        else
+            theta = theta_i * (float(M_PI) / steps);
+        float const phi = phi_i * (float(M_PI) / steps);
+        k.axes[sel][0] = sinf(theta) * cosf(phi);
+        k.axes[sel][1] = sinf(theta) * sinf(phi);
+        k.axes[sel][2] = cosf(theta);
+    }
+    k.N = h->N;
+}
+
+constexpr size_t LDS_BUDGET = 64 * 1024;  // dynamic LDS per workgroup used by k_collide
+
+void choose_tiling(mpcd_hip_s* h, uint32_t tc, uint32_t cap) {
+    uint32_t const n = std::max<uint32_t>(1, h->prm.n);
+    int const algo = h->prm.algorithm;
+    if (cap == 0) cap = h->lds_cap ? h->lds_cap : (algo == 1 ? 1024u : 1536u);
+    cap = std::max<uint32_t>(cap, 64);
+    if (tc == 0) tc = h->tile_cells ? h->tile_cells : std::max<uint32_t>(1, (cap / 2) / n);
+    // ghosts of a batch (at most n per cell) must fit the cap/2 ghost region
+    tc = std::min<uint32_t>(tc, std::max<uint32_t>(1, (cap / 2) / n));
+    tc = std::min<uint32_t>(tc, TC_MAX);
+    h->tile_cells = tc;
+    h->lds_cap = cap;
+}
+
+void set_err(mpcd_hip_s* h, std::string const& m) {
+    if (h) h->err = m;
+}
+
+template <typename F>
+int guarded(mpcd_hip_s* h, F&& f) {
+    if (!h) return 1;
+    try {
+        f();
+        return 0;
+    } catch (HipError const& e) {
+        set_err(h, e.msg);
+    } catch (std::exception const& e) {
+        set_err(h, e.what());
+    }
+    return 2;
+}
+
+void launch_keys(mpcd_hip_s* h) {
+    KParams k = h->kp;
+    grid_shift(k.rs, static_cast<uint32_t>(h->step_index), k.shift);
+    k.step = static_cast<uint32_t>(h->step_index);
+    ck(hipMemsetAsync(h->count, 0, 4 * h->C, h->stream), "clear counts");
+    uint64_t const blocks = (h->N + WG - 1) / WG;
+    if (h->N) hipLaunchKernelGGL(k_keys, dim3(static_cast<uint32_t>(blocks)), dim3(WG), 0, h->stream, k, h->st[h->cur], h->count, h->keyrank);
+    ck(hipGetLastError(), "k_keys");
+    h->keys_valid = true;
+    h->sorted_valid = false;
+}
+
+void launch_sort(mpcd_hip_s* h) {
+    uint32_t const C = static_cast<uint32_t>(h->C);
+    hipLaunchKernelGGL(k_scan_reduce, dim3(h->nb), dim3(WG), 0, h->stream, h->count, C, h->partial);
+    hipLaunchKernelGGL(k_scan_partials, dim3(1), dim3(1024), 0, h->stream, h->partial, h->nb);
+    hipLaunchKernelGGL(k_scan_down, dim3(h->nb), dim3(WG), 0, h->stream, h->count, C, h->partial, h->offset, h->N);
+    uint64_t blocks = std::min<uint64_t>((h->N + WG - 1) / WG, 1u << 20);
+    if (blocks == 0) blocks = 1;
+    hipLaunchKernelGGL(k_perm, dim3(static_cast<uint32_t>(blocks)), dim3(WG), 0, h->stream, h->keyrank, h->offset, h->perm, h->N);
+    ck(hipGetLastError(), "sort");
+    h->sorted_valid = true;
+    h->launches += 4;
+}
+
+template <int ALGO, bool SAMPLE>
+void launch_collide_t(mpcd_hip_s* h, KParams const& k, size_t lds) {
+    uint32_t* rec = h->record ? h->perm_rec : nullptr;
+    SampleBufs S{h->scnt, h->ssum};
+    hipLaunchKernelGGL((k_collide<ALGO, SAMPLE>), dim3(k.ntiles), dim3(WG), lds, h->stream, k, h->st[h->cur],
+                       h->st[1 - h->cur], h->perm, h->offset, h->count, h->keyrank, rec, S, h->status);
+}
+
+void ensure_sampling(mpcd_hip_s* h) {
+    if (h->acc) return;
+    ck(hipMalloc(&h->scnt, 4 * h->C), "alloc sample counts");
+    ck(hipMalloc(&h->ssum, 8 * 3 * h->C), "alloc sample sums");
+    ck(hipMalloc(&h->acc, 8 * 4 * h->C), "alloc accumulators");
+    ck(hipMemsetAsync(h->scnt, 0, 4 * h->C, h->stream), "clear");
+    ck(hipMemsetAsync(h->ssum, 0, 8 * 3 * h->C, h->stream), "clear");
+    ck(hipMemsetAsync(h->acc, 0, 8 * 4 * h->C, h->stream), "clear");
+}
+
+void run_steps(mpcd_hip_s* h, int n_steps, bool sample) {
+    if (n_steps <= 0) return;
+    if (sample) ensure_sampling(h);
+    if (h->record && !h->perm_rec) ck(hipMalloc(&h->perm_rec, 4 * std::max<uint64_t>(h->N, 1)), "alloc perm record");
+    int const algo = h->prm.algorithm;
+    size_t const lds = lds_bytes(h->lds_cap, h->tile_cells, algo);
+    uint32_t const ntiles = static_cast<uint32_t>((h->C + h->tile_cells - 1) / h->tile_cells);
+    if (h->timing) {
+        if (h->ev.size() < 2 * static_cast<size_t>(n_steps)) {
+            size_t const old = h->ev.size();
+            h->ev.resize(2 * static_cast<size_t>(n_steps));
+            for (size_t i = old; i < h->ev.size(); ++i) ck(hipEventCreate(&h->ev[i]), "event");
+        }
+        if (!h->ev_begin) {
+            ck(hipEventCreate(&h->ev_begin), "event");
+            ck(hipEventCreate(&h->ev_end), "event");
+        }
+        ck(hipEventRecord(h->ev_begin, h->stream), "event");
+    }
+    h->launches = 0;
+    for (int s = 0; s < n_steps; ++s) {
+        if (!h->keys_valid) { launch_keys(h); h->launches += 2; }
+        if (!h->sorted_valid) launch_sort(h);
+        KParams k = h->kp;
+        k.step = static_cast<uint32_t>(h->step_index);
+        grid_shift(k.rs, k.step, k.shift);
+        grid_shift(k.rs, k.step + 1, k.shift_next);
+        k.tile_cells = h->tile_cells;
+        k.lds_cap = h->lds_cap;
+        k.ntiles = ntiles;
+        if (h->timing) ck(hipEventRecord(h->ev[2 * s], h->stream), "event");
+        if (algo == 0) {
+            if (sample) launch_collide_t<0, true>(h, k, lds); else launch_collide_t<0, false>(h, k, lds);
+        } else {
+            if (sample) launch_collide_t<1, true>(h, k, lds); else launch_collide_t<1, false>(h, k, lds);
+        }
+        ck(hipGetLastError(), "k_collide");
+        if (h->timing) ck(hipEventRecord(h->ev[2 * s + 1], h->stream), "event");
+        h->launches += 1;
+        if (sample) {
+            bool const first = h->sampling_done;
+            uint64_t blocks = std::min<uint64_t>((h->C + WG - 1) / WG, 1u << 16);
+            hipLaunchKernelGGL(k_sample_fold, dim3(static_cast<uint32_t>(blocks)), dim3(WG), 0, h->stream,
+                               static_cast<uint32_t>(h->C), h->scnt, h->ssum, h->acc, first ? 1 : 0);
+            ck(hipGetLastError(), "k_sample_fold");
+            h->launches += 1;
+            if (first) {
+                h->sampling_done = false;
+                h->sample_counter = 0;
+            } else {
+                h->sample_counter++;
+            }
+        }
+        h->cur = 1 - h->cur;
+        h->step_index++;
+        h->keys_valid = true;
+        h->sorted_valid = false;
+        h->recorded = h->record;
+    }
+    if (h->timing) ck(hipEventRecord(h->ev_end, h->stream), "event");
+    ck(hipStreamSynchronize(h->stream), "step");
+    uint32_t st = 0;
+    ck(hipMemcpy(&st, h->status, 4, hipMemcpyDeviceToHost), "status");
+    if (h->timing) {
+        float ms = 0;
+        h->t_collide = 0;
+        for (int s = 0; s < n_steps; ++s) {
+            ck(hipEventElapsedTime(&ms, h->ev[2 * s], h->ev[2 * s + 1]), "elapsed");
+            h->t_collide += ms;
+        }
+        ck(hipEventElapsedTime(&ms, h->ev_begin, h->ev_end), "elapsed");
+        h->t_total = ms;
+    }
+    if (st & 1u) {
+        ck(hipMemsetAsync(h->status, 0, 4, h->stream), "status");
+        throw HipError{"a cell holds more particles than one LDS tile (" + std::to_string(h->lds_cap) +
+                       "); raise the tile capacity with mpcd_hip_set_tiling"};
+    }
+}
+
+void check_particles(mpcd_hip_s* h, size_t n) {
+    if (n != h->N)
+        throw HipError{"particle array holds " + std::to_string(n) + " particles, the system has " + std::to_string(h->N)};
+}
+
+void put_component(mpcd_hip_s* h, const float* src, int first_plane) {
+    std::vector<float> tmp(3 * h->N);
+    for (uint64_t i = 0; i < h->N; ++i)
+        for (int a = 0; a < 3; ++a) tmp[a * h->N + i] = src[3 * i + a];
+    ck(hipMemcpy(h->st[h->cur] + first_plane * h->N, tmp.data(), 4 * 3 * h->N, hipMemcpyHostToDevice), "upload");
+    h->keys_valid = false;
+    h->sorted_valid = false;
+}
+
+void get_component(mpcd_hip_s* h, float* dst, int first_plane) {
+    std::vector<float> tmp(3 * h->N);
+    ck(hipStreamSynchronize(h->stream), "sync");
+    ck(hipMemcpy(tmp.data(), h->st[h->cur] + first_plane * h->N, 4 * 3 * h->N, hipMemcpyDeviceToHost), "download");
+    for (uint64_t i = 0; i < h->N; ++i)
+        for (int a = 0; a < 3; ++a) dst[3 * i + a] = tmp[a * h->N + i];
+}
+
+void destroy(mpcd_hip_s* h) {
+    if (!h) return;
+    if (h->stream) (void)hipStreamSynchronize(h->stream);
+    for (auto* p : {h->st[0], h->st[1]}) if (p) (void)hipFree(p);
+    for (void* p : {static_cast<void*>(h->perm), static_cast<void*>(h->keyrank), static_cast<void*>(h->count),
+                    static_cast<void*>(h->offset), static_cast<void*>(h->partial), static_cast<void*>(h->status),
+                    static_cast<void*>(h->perm_rec), static_cast<void*>(h->scnt), static_cast<void*>(h->ssum),
+                    static_cast<void*>(h->acc)})
+        if (p) (void)hipFree(p);
+    for (auto e : h->ev) (void)hipEventDestroy(e);
+    if (h->ev_begin) (void)hipEventDestroy(h->ev_begin);
+    if (h->ev_end) (void)hipEventDestroy(h->ev_end);
+    if (h->stream) (void)hipStreamDestroy(h->stream);
+    delete h;
+}
+
+}  // namespace
+
+// ================================================================== C-ABI ===
+extern "C" {
+
+const char* mpcd_hip_last_error(const mpcd_hip_t* h) {
+    if (h) return h->err.c_str();
+    std::lock_guard<std::mutex> g(g_err_mu);
+    return g_create_err.c_str();
+}
+
+int mpcd_hip_create(const mpcd_params_t* params, mpcd_hip_t** out) {
+    auto fail = [](std::string const& m) {
+        std::lock_guard<std::mutex> g(g_err_mu);
+        g_create_err = m;
+        return 1;
+    };
+    if (!params || !out) return fail("null argument");
+    *out = nullptr;
+    mpcd_params_t const& p = *params;
+    if (p.experiment != 0) return fail("the hip backend implements the standard geometry only (experiment = \"standard\")");
+    if (p.algorithm != 0 && p.algorithm != 1) return fail("Unsupported MPCD algorithm");
+    for (int a = 0; a < 3; ++a)
+        if (!(p.volume_size[a] >= 1.0f) || p.volume_size[a] != floorf(p.volume_size[a]))
+            return fail("volume_size must hold positive whole numbers of cells");
+    if (p.periodicity[0] == 0 || p.periodicity[1] == 0)
+        return fail("the hip backend supports walls in z only (periodicity (1, 1, 0) or (1, 1, 1))");
+    if (p.periodicity[2] == 0 && p.volume_size[2] < 3.0f) return fail("walls in z need at least 3 cell layers");
+    if (p.n == 0) return fail("n must be positive");
+    if (p.N >= 0xffffffffull) return fail("particle count must be below 2^32");
+    double const cells = static_cast<double>(p.volume_size[0]) * p.volume_size[1] * p.volume_size[2];
+    if (cells >= 4294967295.0) return fail("cell count must be below 2^32");
+    auto* h = new mpcd_hip_s();
+    h->prm = p;
+    h->N = p.N;
+    h->C = static_cast<uint64_t>(cells);
+    try {
+        int ndev = 0;
+        ck(hipGetDeviceCount(&ndev), "hipGetDeviceCount");
+        if (ndev <= 0) throw HipError{"No HIP device found"};
+        if (static_cast<int>(p.device_id) >= ndev) throw HipError{"device_id out of range"};
+        h->device = static_cast<int>(p.device_id);
+        ck(hipSetDevice(h->device), "hipSetDevice");
+        ck(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking), "stream");
+        fill_geometry(h);
+        choose_tiling(h, 0, 0);
+        uint64_t const N = std::max<uint64_t>(h->N, 1);
+        ck(hipMalloc(&h->st[0], 24 * N), "alloc state");
+        ck(hipMalloc(&h->st[1], 24 * N), "alloc state");
+        ck(hipMalloc(&h->perm, 4 * N), "alloc perm");
+        ck(hipMalloc(&h->keyrank, 8 * N), "alloc keys");
+        ck(hipMalloc(&h->count, 4 * h->C), "alloc counts");
+        ck(hipMalloc(&h->offset, 4 * (h->C + 1)), "alloc offsets");
+        h->nb = static_cast<uint32_t>((h->C + SCAN_CHUNK - 1) / SCAN_CHUNK);
+        ck(hipMalloc(&h->partial, 4 * std::max<uint32_t>(h->nb, 1)), "alloc scan partials");
+        ck(hipMalloc(&h->status, 4), "alloc status");
+        ck(hipMemsetAsync(h->status, 0, 4, h->stream), "clear status");
+        ck(hipMemsetAsync(h->count, 0, 4 * h->C, h->stream), "clear counts");
+        // distribute_particles: scale = volume - 2 (1 - periodic)  (simulation_functions.cpp:8-10)
+        float const sx = p.volume_size[0] - static_cast<float>(2 * (1 - p.periodicity[0]));
+        float const sy = p.volume_size[1] - static_cast<float>(2 * (1 - p.periodicity[1]));
+        float const sz = p.volume_size[2] - static_cast<float>(2 * (1 - p.periodicity[2]));
+        uint64_t const blocks = std::min<uint64_t>((h->N + WG - 1) / WG, 1u << 20);
+        if (h->N) hipLaunchKernelGGL(k_init, dim3(static_cast<uint32_t>(blocks)), dim3(WG), 0, h->stream, h->kp, h->st[0], sx, sy, sz);
+        ck(hipGetLastError(), "k_init");
+        ck(hipStreamSynchronize(h->stream), "init");
+    } catch (HipError const& e) {
+        destroy(h);
+        return fail(e.msg);
+    }
+    *out = h;
+    return 0;
+}
+
+int mpcd_hip_destroy(mpcd_hip_t* h) {
+    destroy(h);
+    return 0;
+}
+
+int mpcd_hip_step(mpcd_hip_t* h, int n_steps) {
+    return guarded(h, [&] {
+        ck(hipSetDevice(h->device), "hipSetDevice");
+        run_steps(h, n_steps, false);
+    });
+}
+
+int mpcd_hip_step_and_sample(mpcd_hip_t* h, int n_steps) {
+    return guarded(h, [&] {
+        ck(hipSetDevice(h->device), "hipSetDevice");
+        run_steps(h, n_steps, true);
+    });
+}
+
+static void sample_mean_impl(mpcd_hip_t* h, float* d32, float* v32, double* d64, double* v64, size_t n_cells) {
+    if (n_cells != h->C) throw HipError{"cell array size does not match the grid"};
+    std::vector<double> acc(4 * h->C, 0.0);
+    if (h->acc) {
+        ck(hipStreamSynchronize(h->stream), "sync");
+        ck(hipMemcpy(acc.data(), h->acc, 8 * 4 * h->C, hipMemcpyDeviceToHost), "download accumulators");
+    }
+    // the reference divides the sum of S samples by S-1 (cpu_backend.cpp:85-114)
+    double const inverse = 1.0 / static_cast<double>(h->sample_counter);
+    for (uint64_t c = 0; c < h->C; ++c) {
+        if (d32) {
+            d32[c] = static_cast<float>(acc[4 * c] * inverse);
+            for (int k = 0; k < 3; ++k) v32[3 * c + k] = static_cast<float>(acc[4 * c + 1 + k] * inverse);
+        }
+        if (d64) {
+            d64[c] = acc[4 * c] * inverse;
+            for (int k = 0; k < 3; ++k) v64[3 * c + k] = acc[4 * c + 1 + k] * inverse;
+        }
+    }
+}
+
+int mpcd_hip_get_sample_mean(mpcd_hip_t* h, float* density, float* velocity, size_t n_cells) {
+    return guarded(h, [&] {
+        sample_mean_impl(h, density, velocity, nullptr, nullptr, n_cells);
+        h->sampling_done = true;
+    });
+}
+
+int mpcd_hip_get_sample_mean_f64(mpcd_hip_t* h, double* density, double* velocity, size_t n_cells) {
+    return guarded(h, [&] { sample_mean_impl(h, nullptr, nullptr, density, velocity, n_cells); });
+}
+
+int mpcd_hip_n_particles(const mpcd_hip_t* h, uint64_t* out) {
+    if (!h || !out) return 1;
+    *out = h->N;
+    return 0;
+}
+
+int mpcd_hip_n_cells(const mpcd_hip_t* h, uint64_t* out) {
+    if (!h || !out) return 1;
+    *out = h->C;
+    return 0;
+}
+
+int mpcd_hip_get_positions(mpcd_hip_t* h, float* out, size_t n) {
+    return guarded(h, [&] { check_particles(h, n); get_component(h, out, 0); });
+}
+int mpcd_hip_get_velocities(mpcd_hip_t* h, float* out, size_t n) {
+    return guarded(h, [&] { check_particles(h, n); get_component(h, out, 3); });
+}
+int mpcd_hip_set_positions(mpcd_hip_t* h, const float* pos, size_t n) {
+    return guarded(h, [&] { check_particles(h, n); put_component(h, pos, 0); });
+}
+int mpcd_hip_set_velocities(mpcd_hip_t* h, const float* vel, size_t n) {
+    return guarded(h, [&] { check_particles(h, n); put_component(h, vel, 3); });
+}
+int mpcd_hip_set_particles(mpcd_hip_t* h, const float* pos, const float* vel, size_t n) {
+    return guarded(h, [&] {
+        check_particles(h, n);
+        put_component(h, pos, 0);
+        put_component(h, vel, 3);
+    });
+}
+
+int mpcd_hip_get_step_index(const mpcd_hip_t* h, uint64_t* out) {
+    if (!h || !out) return 1;
+    *out = h->step_index;
+    return 0;
+}
+int mpcd_hip_set_step_index(mpcd_hip_t* h, uint64_t step) {
+    return guarded(h, [&] {
+        h->step_index = step;
+        h->keys_valid = false;
+        h->sorted_valid = false;
+    });
+}
+
+int mpcd_hip_grid_shift(const mpcd_hip_t* h, uint64_t step, float* out3) {
+    if (!h || !out3) return 1;
+    grid_shift(h->kp.rs, static_cast<uint32_t>(step), out3);
+    return 0;
+}
+
+int mpcd_hip_peek_sort(mpcd_hip_t* h, uint32_t* cell_of_particle, uint32_t* counts, uint32_t* permutation) {
+    return guarded(h, [&] {
+        ck(hipSetDevice(h->device), "hipSetDevice");
+        if (!h->keys_valid) launch_keys(h);
+        if (!h->sorted_valid) launch_sort(h);
+        ck(hipStreamSynchronize(h->stream), "sync");
+        std::vector<uint2> kr(h->N);
+        std::vector<uint32_t> off(h->C + 1), pm(h->N);
+        if (h->N) {
+            ck(hipMemcpy(kr.data(), h->keyrank, 8 * h->N, hipMemcpyDeviceToHost), "download keys");
+            ck(hipMemcpy(pm.data(), h->perm, 4 * h->N, hipMemcpyDeviceToHost), "download perm");
+        }
+        ck(hipMemcpy(off.data(), h->offset, 4 * (h->C + 1), hipMemcpyDeviceToHost), "download offsets");
+        KParams const& k = h->kp;
+        auto to_ref = [&](uint32_t key) {
+            uint32_t q[3];
+            q[0] = key % k.D0;
+            q[1] = (key / k.D0) % k.D1;
+            q[2] = (key / k.D0) / k.D1;
+            uint32_t i[3];
+            for (int a = 0; a < 3; ++a) i[k.ord[a]] = q[a];
+            return i[0] + k.dim[0] * (i[1] + k.dim[1] * i[2]);
+        };
+        std::vector<uint32_t> key_of_ref(h->C);
+        for (uint32_t key = 0; key < h->C; ++key) key_of_ref[to_ref(key)] = key;
+        if (cell_of_particle)
+            for (uint64_t j = 0; j < h->N; ++j) cell_of_particle[j] = to_ref(kr[j].x);
+        size_t w = 0;
+        for (uint32_t c = 0; c < h->C; ++c) {
+            uint32_t const key = key_of_ref[c];
+            uint32_t const a = off[key], e = off[key + 1];
+            if (counts) counts[c] = e - a;
+            if (permutation) {
+                std::vector<uint32_t> seg(pm.begin() + a, pm.begin() + e);
+                std::sort(seg.begin(), seg.end());
+                for (uint32_t v : seg) permutation[w++] = v;
+            }
+        }
+    });
+}
+
+int mpcd_hip_record_permutation(mpcd_hip_t* h, int enable) {
+    return guarded(h, [&] { h->record = enable != 0; });
+}
+
+int mpcd_hip_last_permutation(mpcd_hip_t* h, uint32_t* out, size_t n) {
+    return guarded(h, [&] {
+        check_particles(h, n);
+        if (!h->recorded || !h->perm_rec) throw HipError{"no recorded permutation (enable recording before stepping)"};
+        ck(hipStreamSynchronize(h->stream), "sync");
+        ck(hipMemcpy(out, h->perm_rec, 4 * h->N, hipMemcpyDeviceToHost), "download permutation");
+    });
+}
+
+int mpcd_hip_enable_timing(mpcd_hip_t* h, int enable) {
+    return guarded(h, [&] { h->timing = enable != 0; });
+}
+
+int mpcd_hip_last_timing(const mpcd_hip_t* h, double* collide_ms, double* total_ms, int* launches) {
+    if (!h) return 1;
+    if (collide_ms) *collide_ms = h->t_collide;
+    if (total_ms) *total_ms = h->t_total;
+    if (launches) *launches = h->launches;
+    return 0;
+}
+
+int mpcd_hip_set_tiling(mpcd_hip_t* h, uint32_t cells_per_tile, uint32_t lds_capacity) {
+    return guarded(h, [&] {
+        uint32_t const old_tc = h->tile_cells, old_cap = h->lds_cap;
+        h->tile_cells = 0;
+        choose_tiling(h, cells_per_tile, lds_capacity ? lds_capacity : old_cap);
+        if (lds_bytes(h->lds_cap, h->tile_cells, h->prm.algorithm) > LDS_BUDGET) {
+            h->tile_cells = old_tc;
+            h->lds_cap = old_cap;
+            throw HipError{"tile needs more than 64 KiB of LDS"};
+        }
+    });
+}
+
+int mpcd_hip_get_tiling(const mpcd_hip_t* h, uint32_t* cells_per_tile, uint32_t* lds_capacity) {
+    if (!h) return 1;
+    if (cells_per_tile) *cells_per_tile = h->tile_cells;
+    if (lds_capacity) *lds_capacity = h->lds_cap;
+    return 0;
+}
+
+int mpcd_hip_selftest_philox(uint32_t k0, uint32_t k1, const uint32_t* ctr, uint32_t* out, size_t n) {
+    uint32_t *dc = nullptr, *dout = nullptr;
+    int rc = 0;
+    if (hipMalloc(&dc, 16 * std::max<size_t>(n, 1)) != hipSuccess) return 1;
+    if (hipMalloc(&dout, 16 * std::max<size_t>(n, 1)) != hipSuccess) { (void)hipFree(dc); return 1; }
+    if (hipMemcpy(dc, ctr, 16 * n, hipMemcpyHostToDevice) != hipSuccess) rc = 1;
+    if (!rc && n) hipLaunchKernelGGL(k_selftest_philox, dim3(static_cast<uint32_t>((n + 255) / 256)), dim3(256), 0, 0, k0, k1, dc, dout, n);
+    if (!rc && hipDeviceSynchronize() != hipSuccess) rc = 1;
+    if (!rc && hipMemcpy(out, dout, 16 * n, hipMemcpyDeviceToHost) != hipSuccess) rc = 1;
+    (void)hipFree(dc);
+    (void)hipFree(dout);
+    return rc;
+}
+
+int mpcd_hip_selftest_math(const float* in, float* out, size_t n, int which) {
+    float *di = nullptr, *dout = nullptr;
+    int rc = 0;
+    if (hipMalloc(&di, 4 * std::max<size_t>(n, 1)) != hipSuccess) return 1;
+    if (hipMalloc(&dout, 8 * std::max<size_t>(n, 1)) != hipSuccess) { (void)hipFree(di); return 1; }
+    if (hipMemcpy(di, in, 4 * n, hipMemcpyHostToDevice) != hipSuccess) rc = 1;
+    if (!rc && n) hipLaunchKernelGGL(k_selftest_math, dim3(static_cast<uint32_t>((n + 255) / 256)), dim3(256), 0, 0, di, dout, n, which);
+    if (!rc && hipDeviceSynchronize() != hipSuccess) rc = 1;
+    if (!rc && hipMemcpy(out, dout, 8 * n, hipMemcpyDeviceToHost) != hipSuccess) rc = 1;
+    (void)hipFree(di);
+    (void)hipFree(dout);
+    return rc;
+}
+
+}  // extern "C"

@@ -1,0 +1,122 @@
+/* C-ABI of the MI355X (gfx950) MPCD backend: libmpcd_hip.so.
+ *
+ * Every entry point returns 0 on success and a non-zero status on failure;
+ * mpcd_hip_last_error() then holds the message.  No entry point takes or
+ * returns C++ or torch types: plain pointers, sizes and a POD parameter block.
+ * Host arrays are caller-allocated.  Particle arrays are interleaved xyz
+ * (3 floats per particle), cell arrays follow the reference cell index
+ * c = x + X*(y + Y*z) (volume_container.hpp:69-75).
+ *
+ * Each entry point names the reference interface it replaces.  The reference
+ * reaches its backends through the abstract mpcd::Backend
+ * (src/libmpcd/common/backend.hpp:11-31) selected by create_backend
+ * (src/libmpcd/common/backend_factory.hpp:15-27); HipBackend (C++) wraps this
+ * C-ABI behind that same interface.
+ */
+#ifndef MPCD_HIP_H
+#define MPCD_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Mirrors mpcd::SimulationParameters (include/mpcd/api/simulation_parameters.hpp:23-55),
+ * numeric fields only, plus the options the reference fixes in code. */
+typedef struct mpcd_params_t {
+    int experiment;          /* 0 standard, 1 channel (channel: rejected by this backend) */
+    int algorithm;           /* 0 srd, 1 extended */
+    int collision_kernel;    /* accepted for compatibility; this backend always runs its fused kernels */
+    int periodicity[3];
+    float volume_size[3];    /* cells per axis, wall layers included (bindings.cpp:108-123) */
+    uint64_t N;              /* particles */
+    uint32_t n;              /* particles per cell */
+    uint32_t device_id;
+    float delta_t;
+    float temperature;
+    float thermal_velocity;
+    float drag;
+    float rotation_angle;    /* SRD rotation angle in degrees; the reference fixes 120 (simulation_functions.cpp:118-119) */
+    uint64_t seed;           /* Philox key of the counter-based random stream */
+    uint32_t system_id;      /* ensemble member id folded into the random stream */
+    uint32_t reserved;
+} mpcd_params_t;
+
+typedef struct mpcd_hip_s mpcd_hip_t;
+
+/* Message of the last failure on this handle (or of the last failed create when h is NULL). */
+const char* mpcd_hip_last_error(const mpcd_hip_t* h);
+
+/* Replaces CudaBackend::CudaBackend -> BackendContext::BackendContext
+ * (cuda_backend.cu:7-8, backend_context.cu:17-127) and CPUBackend::CPUBackend
+ * (cpu_backend.cpp:8-21): allocates device state and distributes particles
+ * uniformly in the fluid region with Maxwell-Boltzmann velocities
+ * (simulation_functions.cpp:7-33), seeded from params->seed. */
+int mpcd_hip_create(const mpcd_params_t* params, mpcd_hip_t** out);
+/* Replaces the backend destructor (cuda_backend_impl.cu:19-21). */
+int mpcd_hip_destroy(mpcd_hip_t* h);
+
+/* Replaces Backend::step (backend.hpp:12; cuda_backend_impl.cu:111-131,
+ * cpu_backend.cpp:65-70): n_steps of grid shift + streaming + cell sort +
+ * collision.  Returns after the steps finished on the device. */
+int mpcd_hip_step(mpcd_hip_t* h, int n_steps);
+/* Replaces Backend::stepAndAccumulateSample (backend.hpp:14;
+ * cuda_backend_impl.cu:182-204, cpu_backend.cpp:72-109). Accumulates in FP64. */
+int mpcd_hip_step_and_sample(mpcd_hip_t* h, int n_steps);
+/* Replaces Backend::getSampleMean (backend.hpp:15; cpu_backend.cpp:111-122):
+ * density[n_cells], velocity[3*n_cells], scaled by 1/sample_counter as the
+ * reference does, and ends the current average. */
+int mpcd_hip_get_sample_mean(mpcd_hip_t* h, float* density, float* velocity, size_t n_cells);
+/* Same fields in FP64 (no reference counterpart; the FP64 accumulators themselves). */
+int mpcd_hip_get_sample_mean_f64(mpcd_hip_t* h, double* density, double* velocity, size_t n_cells);
+
+/* Replaces Backend::getNParticles (backend.hpp:17). */
+int mpcd_hip_n_particles(const mpcd_hip_t* h, uint64_t* out);
+/* Number of cells X*Y*Z (volume_container size). */
+int mpcd_hip_n_cells(const mpcd_hip_t* h, uint64_t* out);
+/* Replace Backend::getParticlePositions / getParticleVelocities (backend.hpp:18-19). */
+int mpcd_hip_get_positions(mpcd_hip_t* h, float* out, size_t n_particles);
+int mpcd_hip_get_velocities(mpcd_hip_t* h, float* out, size_t n_particles);
+/* Replace Backend::setParticlePositions / setParticleVelocities / setParticles
+ * (backend.hpp:20-22).  n_particles must equal the particle count. */
+int mpcd_hip_set_positions(mpcd_hip_t* h, const float* pos, size_t n_particles);
+int mpcd_hip_set_velocities(mpcd_hip_t* h, const float* vel, size_t n_particles);
+int mpcd_hip_set_particles(mpcd_hip_t* h, const float* pos, const float* vel, size_t n_particles);
+
+/* ---- parity and measurement hooks (no reference counterpart) ---- */
+
+/* Step index that keys the counter-based random stream of the next step. */
+int mpcd_hip_get_step_index(const mpcd_hip_t* h, uint64_t* out);
+int mpcd_hip_set_step_index(mpcd_hip_t* h, uint64_t step);
+/* The grid shift the given step draws (the reference draws it on the host, cpu_backend.cpp:23-26). */
+int mpcd_hip_grid_shift(const mpcd_hip_t* h, uint64_t step, float* out3);
+/* Streaming half-step of the next step without collision, as translate_particles
+ * (simulation_functions.cpp:38-103) leaves it: shifted cell index per particle,
+ * per-cell counts (uniform_counter), and the stable cell-ordered permutation
+ * (uniform_list flattened cell by cell, all entries of each cell).  Leaves the
+ * state unchanged; the next step reuses the sort. */
+int mpcd_hip_peek_sort(mpcd_hip_t* h, uint32_t* cell_of_particle, uint32_t* counts, uint32_t* permutation);
+/* When enabled, each step records for every output slot the input slot it came
+ * from; mpcd_hip_last_permutation returns the record of the last step. */
+int mpcd_hip_record_permutation(mpcd_hip_t* h, int enable);
+int mpcd_hip_last_permutation(mpcd_hip_t* h, uint32_t* out, size_t n_particles);
+/* Record HIP events around every collide launch of later step() calls. */
+int mpcd_hip_enable_timing(mpcd_hip_t* h, int enable);
+/* Device-side timing of the last step() call: milliseconds of the collision
+ * kernels and of the whole step sequence, from HIP events on the handle's stream. */
+int mpcd_hip_last_timing(const mpcd_hip_t* h, double* collide_ms, double* total_ms, int* launches);
+/* Kernel launch shape: cells per tile and LDS particle capacity per tile
+ * (0 = keep).  Results do not depend on it. */
+int mpcd_hip_set_tiling(mpcd_hip_t* h, uint32_t cells_per_tile, uint32_t lds_capacity);
+int mpcd_hip_get_tiling(const mpcd_hip_t* h, uint32_t* cells_per_tile, uint32_t* lds_capacity);
+/* Device math of the random stream, for known-answer tests: Philox4x32-10 of
+ * n counters (4 words each), and the float transforms. */
+int mpcd_hip_selftest_philox(uint32_t k0, uint32_t k1, const uint32_t* ctr, uint32_t* out, size_t n);
+int mpcd_hip_selftest_math(const float* in, float* out, size_t n, int which);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MPCD_HIP_H */

@@ -1,0 +1,912 @@
+// CPU oracle for the MPCD time step.
+//
+// TEST INFRASTRUCTURE ONLY.  This file is a single-threaded C++ restatement of
+// the reference CPU backend (azantop/cudaMPCD, src/libmpcd/cpu_backend/).  Only
+// tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it,
+// and only as the checker.  The product (cudampcd_amd) never links or calls it.
+//
+// What it restates (reference file:line):
+//   distribute_particles      simulation_functions.cpp:7-33
+//   translate_particles       simulation_functions.cpp:38-103  (standard geometry)
+//   srd_collision             simulation_functions.cpp:108-202
+//   extended_collision        simulation_functions.cpp:207-406
+//   SymmetricMatrix::inverse  common/mechanic.hpp:52-79
+//   VolumeContainer::getIndex / getPosition / getZIdx   cpu_backend/volume_container.hpp:69-95
+//   MPCCell::unlockedAdd / average / averageReduceOnly  common/mpc_cell.hpp:67-86
+//   CPUBackend step / stepAndAccumulateSample / getSampleMean  cpu_backend.cpp:23-122
+//   Xoshiro128Plus (seed, next, uniform, int, gaussianf, unit vector)  common/random.hpp:24-258
+//
+// Randomness is supplied by one of two providers that are called in exactly
+// the order the reference draws from its single Xoshiro128+ stream:
+//   * mode 0 "reference": a restated Xoshiro128+ stream with the reference's
+//     transforms (libm logf/sinf/cosf), so that, given the same seed, the
+//     oracle reproduces the reference CPU trajectory bit for bit;
+//   * mode 1 "counter": stateless Philox4x32-10 draws keyed by
+//     (seed, step, cell, index, purpose) with float transforms built from
+//     +,-,*,/,sqrt and integer bit operations only, so that a GPU using the
+//     same key layout produces the same bits.
+//
+// Parity status of each function (how it is pinned) is listed in DESIGN.md.
+//
+// Every floating-point expression keeps the operation order of the reference
+// (vector_3d.hpp operator definitions), and the file is compiled with
+// -ffp-contract=off so no fused multiply-add changes a rounding.
+
+#include <cfloat>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+#include <algorithm>
+
+namespace oracle {
+
+// ---------------------------------------------------------------- vectors ---
+struct V3 {
+    float x, y, z;
+};
+static inline V3 add(V3 a, V3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+static inline V3 sub(V3 a, V3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+static inline V3 mul(V3 a, float s) { return {a.x * s, a.y * s, a.z * s}; }
+static inline V3 neg(V3 a) { return {-a.x, -a.y, -a.z}; }
+// vector_3d.hpp:223-225: x*o.x + y*o.y + z*o.z (left to right)
+static inline float dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+// vector_3d.hpp:233-235
+static inline V3 cross(V3 a, V3 o) {
+    return {a.y * o.z - a.z * o.y, a.z * o.x - a.x * o.z, a.x * o.y - a.y * o.x};
+}
+
+// ------------------------------------------------------ Xoshiro128+ stream ---
+// Restated from Blackman & Vigna's xoshiro128+ (random.hpp:80-92) with the
+// reference's seeding (random.hpp:66-74) and float transforms (:126-164, :237-253).
+struct Xoshiro {
+    uint32_t s[4] = {0, 0, 0, 0};
+    bool gen_f = false;
+    float z1_f = 0.f;
+
+    void seed(uint64_t a, uint64_t b) {
+        s[0] = static_cast<uint32_t>(a >> 2);
+        s[1] = static_cast<uint32_t>(b);
+        s[2] = static_cast<uint32_t>(b >> 2);
+        s[3] = static_cast<uint32_t>(a);
+        z1_f = 0.f;
+        gen_f = false;
+    }
+    static inline uint32_t rotl(uint32_t x, int k) { return (x << k) | (x >> (32 - k)); }
+    uint32_t next() {
+        uint32_t const result = s[0] + s[3];
+        uint32_t const t = s[1] << 9;
+        s[2] ^= s[0];
+        s[3] ^= s[1];
+        s[1] ^= s[2];
+        s[0] ^= s[3];
+        s[2] ^= t;
+        s[3] = rotl(s[3], 11);
+        return result;
+    }
+    float uniform() { return static_cast<float>(next()) * (1.0f / static_cast<float>(UINT32_MAX)); }
+    int uniform_int(int lo, int hi) { return static_cast<int>(next() / 2) % (hi - lo + 1) + lo; }
+    float gaussian() {
+        gen_f = !gen_f;
+        if (!gen_f) return z1_f;
+        float u1 = logf(uniform() + FLT_MIN);
+        float u2 = static_cast<float>(next()) * (2.0f / static_cast<float>(UINT32_MAX));
+        z1_f = sinf(u2 * float(M_PI));
+        u2 = cosf(u2 * float(M_PI));
+        z1_f *= sqrtf(-2 * u1);
+        return u2 * sqrtf(-2 * u1);
+    }
+    V3 unit_vector() {
+        float rsq, rd1, rd2;
+        do {
+            rd1 = 1.0f - 2.0f * uniform();
+            rd2 = 1.0f - 2.0f * uniform();
+            rsq = rd1 * rd1 + rd2 * rd2;
+        } while (rsq > 1.0f);
+        float rdh = 2.0f * sqrtf(1.0f - rsq);
+        return {rd1 * rdh, rd2 * rdh, (1.0f - 2.0f * rsq)};
+    }
+};
+
+// ------------------------------------------------------- Philox4x32-10 ---
+// Salmon, Moraes, Dror, Shaw, "Parallel random numbers: as easy as 1, 2, 3",
+// SC'11.  Round: (c0,c1,c2,c3) -> (hi(B*c2)^c1^k0, lo(B*c2), hi(A*c0)^c3^k1, lo(A*c0))
+// with A = 0xD2511F53, B = 0xCD9E8D57, Weyl key bumps 0x9E3779B9 / 0xBB67AE85.
+struct U4 {
+    uint32_t v[4];
+};
+static inline U4 philox(uint32_t k0, uint32_t k1, uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3) {
+    for (int r = 0; r < 10; ++r) {
+        uint64_t const p0 = static_cast<uint64_t>(0xD2511F53u) * c0;
+        uint64_t const p1 = static_cast<uint64_t>(0xCD9E8D57u) * c2;
+        uint32_t const n0 = static_cast<uint32_t>(p1 >> 32) ^ c1 ^ k0;
+        uint32_t const n1 = static_cast<uint32_t>(p1);
+        uint32_t const n2 = static_cast<uint32_t>(p0 >> 32) ^ c3 ^ k1;
+        uint32_t const n3 = static_cast<uint32_t>(p0);
+        c0 = n0; c1 = n1; c2 = n2; c3 = n3;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    return {{c0, c1, c2, c3}};
+}
+
+// ------------------------------------------- portable float transforms ---
+// Built only from IEEE-exact operations (+,-,*,/,sqrt, floor, bit moves) so the
+// same source order gives the same bits on any IEEE-754 binary32 machine.
+static inline uint32_t f2u(float f) { uint32_t u; std::memcpy(&u, &f, 4); return u; }
+static inline float u2f(uint32_t u) { float f; std::memcpy(&f, &u, 4); return f; }
+
+// uniform in [0,1]: same transform as the reference genUniformFloat (random.hpp:126-129)
+static inline float uf(uint32_t u) { return static_cast<float>(u) * (1.0f / 4294967296.0f); }
+
+// natural log for positive normal x: x = m*2^e, m in (sqrt(1/2), sqrt(2)],
+// log(m) = 2 atanh((m-1)/(m+1)) by its odd series (fdlibm-style ln2 split).
+static inline float p_logf(float x) {
+    uint32_t b = f2u(x);
+    int e = static_cast<int>((b >> 23) & 0xffu) - 127;
+    float m = u2f((b & 0x7fffffu) | 0x3f800000u);
+    if (m > 1.41421356f) { m = m * 0.5f; e = e + 1; }
+    float f = (m - 1.0f) / (m + 1.0f);
+    float s = f * f;
+    float p = f * (2.0f + s * (0.666666667f + s * (0.4f + s * (0.285714286f + s * 0.222222222f))));
+    float fe = static_cast<float>(e);
+    return fe * 0.693145751953125f + (fe * 1.42860682e-06f + p);
+}
+
+// sin(pi t), cos(pi t) for t in [0, 2]: quadrant reduction then Taylor series.
+static inline void p_sincospi(float t, float* s_out, float* c_out) {
+    float q = floorf(2.0f * t + 0.5f);
+    float r = t - 0.5f * q;                // exact, |r| <= 1/4
+    float x = r * 3.14159265f;
+    float x2 = x * x;
+    float s = x + x * x2 * (-0.166666667f + x2 * (0.00833333333f + x2 * (-0.000198412698f + x2 * 2.75573192e-06f)));
+    float c = 1.0f + x2 * (-0.5f + x2 * (0.0416666667f + x2 * (-0.00138888889f + x2 * (2.48015873e-05f + x2 * -2.75573192e-07f))));
+    int qi = static_cast<int>(q) & 3;
+    float so, co;
+    if (qi == 0) { so = s; co = c; }
+    else if (qi == 1) { so = c; co = -s; }
+    else if (qi == 2) { so = -s; co = -c; }
+    else { so = -c; co = s; }
+    *s_out = so;
+    *c_out = co;
+}
+
+// exp(x): x = k ln2 + r, |r| <= ln2/2, degree-7 Taylor polynomial, scale by 2^k.
+static inline float p_expf(float x) {
+    if (x != x) return x;
+    if (x < -104.0f) return 0.0f;
+    if (x > 89.0f) return INFINITY;
+    float k = floorf(x * 1.44269504f + 0.5f);
+    float r = (x - k * 0.693145751953125f) - k * 1.42860682e-06f;
+    float p = 1.0f + r * (1.0f + r * (0.5f + r * (0.166666667f + r * (0.0416666667f + r * (0.00833333333f + r * (0.00138888889f + r * 0.000198412698f))))));
+    int ki = static_cast<int>(k);
+    if (ki >= -126) {
+        if (ki > 127) return INFINITY;
+        return p * u2f(static_cast<uint32_t>(ki + 127) << 23);
+    }
+    return (p * u2f(static_cast<uint32_t>(ki + 127 + 64) << 23)) * u2f(static_cast<uint32_t>(127 - 64) << 23);
+}
+
+// Box-Muller pair from two words: (cos part, sin part), same structure as the
+// reference gaussianf (random.hpp:141-164).
+static inline void box_muller(uint32_t a, uint32_t b, float* g0, float* g1) {
+    float u1 = p_logf(uf(a) + FLT_MIN);
+    float t = static_cast<float>(b) * (2.0f / 4294967296.0f);
+    float s, c;
+    p_sincospi(t, &s, &c);
+    float rad = sqrtf(-2.0f * u1);
+    *g0 = c * rad;
+    *g1 = s * rad;
+}
+
+enum Purpose : uint32_t {
+    P_SHIFT = 1,
+    P_GHOST_COUNT = 2,
+    P_GHOST_POS = 3,
+    P_GHOST_VEL = 4,
+    P_AXIS = 5,
+    P_EXT = 6,
+    P_THERM = 7,
+    P_INIT_POS = 8,
+    P_INIT_VEL = 9,
+};
+
+// ------------------------------------------------------ randomness source ---
+struct Draws {
+    int mode = 0;              // 0 reference stream, 1 counter
+    Xoshiro xo;                // mode 0
+    uint32_t k0 = 0, k1 = 0;   // mode 1 key
+    uint32_t system = 0;       // mode 1: ensemble member id
+    int ghost_xy_order = 0;    // mode 0: order of the two ghost x/y uniforms
+
+    U4 ctr(uint32_t purpose, uint32_t c0, uint32_t cell, uint32_t step) const {
+        return philox(k0, k1, c0, cell, step, (purpose << 24) | (system & 0xffffffu));
+    }
+
+    V3 shift(uint32_t step) {
+        if (mode == 0) {
+            float a = xo.uniform() - 0.5f;
+            float b = xo.uniform() - 0.5f;
+            float c = xo.uniform() - 0.5f;
+            return {a, b, c};
+        }
+        U4 r = ctr(P_SHIFT, 0, 0, step);
+        return {uf(r.v[0]) - 0.5f, uf(r.v[1]) - 0.5f, uf(r.v[2]) - 0.5f};
+    }
+    float ghost_count_u(uint32_t step, uint32_t cell, uint32_t i) {
+        if (mode == 0) return xo.uniform();
+        U4 r = ctr(P_GHOST_COUNT, i >> 2, cell, step);
+        return uf(r.v[i & 3]);
+    }
+    // ghost position inside the cell, each coordinate in [0,1]; z is drawn
+    // from the region the reference's rejection loop accepts
+    // (simulation_functions.cpp:159-161).
+    V3 ghost_pos(uint32_t step, uint32_t cell, uint32_t k, float shift, bool layer, bool sign) {
+        if (mode == 0) {
+            float z;
+            do {
+                z = xo.uniform();
+            } while (((z < shift) ^ layer) ^ sign);
+            float a = xo.uniform();
+            float b = xo.uniform();
+            if (ghost_xy_order == 0) return {b, a, z};  // g++ evaluates constructor arguments right to left
+            return {a, b, z};
+        }
+        U4 r = ctr(P_GHOST_POS, k, cell, step);
+        float uz = uf(r.v[2]);
+        float z = (layer ^ sign) ? shift * uz : shift + (1.0f - shift) * uz;
+        return {uf(r.v[0]), uf(r.v[1]), z};
+    }
+    V3 ghost_vel(uint32_t step, uint32_t cell, uint32_t k) {
+        if (mode == 0) {
+            float a = xo.gaussian();
+            float b = xo.gaussian();
+            float c = xo.gaussian();
+            return {a, b, c};
+        }
+        U4 r = ctr(P_GHOST_VEL, k, cell, step);
+        float g0, g1, g2, g3;
+        box_muller(r.v[0], r.v[1], &g0, &g1);
+        box_muller(r.v[2], r.v[3], &g2, &g3);
+        return {g0, g1, g2};
+    }
+    V3 srd_axis(uint32_t step, uint32_t cell) {
+        if (mode == 0) return xo.unit_vector();
+        for (uint32_t a = 0;; ++a) {
+            U4 r = ctr(P_AXIS, a, cell, step);
+            for (int h = 0; h < 2; ++h) {
+                float rd1 = 1.0f - 2.0f * uf(r.v[2 * h]);
+                float rd2 = 1.0f - 2.0f * uf(r.v[2 * h + 1]);
+                float rsq = rd1 * rd1 + rd2 * rd2;
+                if (!(rsq > 1.0f)) {
+                    float rdh = 2.0f * sqrtf(1.0f - rsq);
+                    return {rd1 * rdh, rd2 * rdh, (1.0f - 2.0f * rsq)};
+                }
+            }
+        }
+    }
+    // extended: (discrete direction 0..12, sign bit) then gate uniform
+    void ext_axis(uint32_t step, uint32_t cell, int* select, int* flip) {
+        if (mode == 0) {
+            *select = xo.uniform_int(0, 12);
+            *flip = xo.uniform_int(0, 1);
+            return;
+        }
+        U4 r = ctr(P_EXT, 0, cell, step);
+        *select = static_cast<int>(r.v[0] / 2) % 13;
+        *flip = static_cast<int>(r.v[1] / 2) % 2;
+    }
+    float ext_gate(uint32_t step, uint32_t cell) {
+        if (mode == 0) return xo.uniform();
+        U4 r = ctr(P_EXT, 0, cell, step);
+        return uf(r.v[2]);
+    }
+    float ext_gauss(uint32_t step, uint32_t cell, uint32_t i) {
+        if (mode == 0) return xo.gaussian();
+        U4 r = ctr(P_THERM, i >> 2, cell, step);
+        float g[4];
+        box_muller(r.v[0], r.v[1], &g[0], &g[1]);
+        box_muller(r.v[2], r.v[3], &g[2], &g[3]);
+        return g[i & 3];
+    }
+    V3 init_pos_u(uint64_t i) {
+        if (mode == 0) {
+            float a = xo.uniform();
+            float b = xo.uniform();
+            float c = xo.uniform();
+            return {a, b, c};
+        }
+        U4 r = ctr(P_INIT_POS, static_cast<uint32_t>(i), static_cast<uint32_t>(i >> 32), 0);
+        return {uf(r.v[0]), uf(r.v[1]), uf(r.v[2])};
+    }
+    V3 init_vel(uint64_t i) {
+        if (mode == 0) {
+            float a = xo.gaussian();
+            float b = xo.gaussian();
+            float c = xo.gaussian();
+            return {a, b, c};
+        }
+        U4 r = ctr(P_INIT_VEL, static_cast<uint32_t>(i), static_cast<uint32_t>(i >> 32), 0);
+        float g0, g1, g2, g3;
+        box_muller(r.v[0], r.v[1], &g0, &g1);
+        box_muller(r.v[2], r.v[3], &g2, &g3);
+        return {g0, g1, g2};
+    }
+};
+
+// ------------------------------------------------------------- params ---
+}  // namespace oracle
+
+extern "C" {
+struct orc_params {
+    float volume[3];        // cells per axis, wall layers included
+    int periodic[3];
+    unsigned n;             // particles per cell
+    unsigned long long N;   // particles
+    float delta_t;
+    float drag;
+    float thermal_velocity;
+    float rotation_deg;     // SRD rotation angle in degrees (reference: 120)
+    int algorithm;          // 0 srd, 1 extended
+    int rng_mode;           // 0 reference Xoshiro128+, 1 counter (Philox)
+    unsigned long long seed_a, seed_b;  // mode 0: Xoshiro seed(a,b); mode 1: Philox key = seed_a
+    unsigned system;        // mode 1: ensemble member id
+    int ghost_xy_order;     // mode 0 only
+};
+}
+
+namespace oracle {
+
+struct Particle {
+    V3 pos, vel;
+    uint32_t cell;
+};
+
+struct Sample {
+    float density;
+    V3 mean;
+};
+
+struct Oracle {
+    orc_params p;
+    V3 L;                    // volume edges
+    uint32_t X, Y, Z, XY, C;
+    V3 half;                 // L * 0.5
+    std::vector<Particle> parts;
+    std::vector<uint32_t> counter, list;
+    uint32_t cap;            // 4 n table slots per cell
+    Draws rng;
+    uint64_t step_index = 0;  // counter-mode step key
+    V3 shift = {0, 0, 0};
+    bool fixed_shift = false;  // known-answer tests: use `forced` instead of a drawn shift
+    V3 forced = {0, 0, 0};
+    float sin_a, cos_a;
+
+    // sampling, reference semantics (cpu_backend.cpp:72-122) in FP32 Kahan
+    bool sampling_done = true;
+    size_t sample_counter = 0;
+    std::vector<Sample> states, kahan;
+    // sampling in FP64 (sum of per-step means, same S-1 normalisation)
+    std::vector<double> acc_d, acc_v;
+
+    explicit Oracle(orc_params const& q) : p(q) {
+        L = {p.volume[0], p.volume[1], p.volume[2]};
+        X = static_cast<uint32_t>(L.x);
+        Y = static_cast<uint32_t>(L.y);
+        Z = static_cast<uint32_t>(L.z);
+        XY = X * Y;
+        C = XY * Z;
+        half = mul(L, 0.5f);
+        cap = 4 * p.n;
+        parts.resize(p.N);
+        counter.assign(C, 0);
+        list.assign(static_cast<size_t>(C) * cap, 0);
+        states.assign(C, {0, {0, 0, 0}});
+        kahan.assign(C, {0, {0, 0, 0}});
+        acc_d.assign(C, 0.0);
+        acc_v.assign(3 * static_cast<size_t>(C), 0.0);
+        rng.mode = p.rng_mode;
+        rng.ghost_xy_order = p.ghost_xy_order;
+        if (p.rng_mode == 0) {
+            rng.xo.seed(p.seed_a, p.seed_b);
+        } else {
+            rng.k0 = static_cast<uint32_t>(p.seed_a);
+            rng.k1 = static_cast<uint32_t>(p.seed_a >> 32);
+            rng.system = p.system;
+        }
+        // simulation_functions.cpp:118-119 (angle fixed to 120 degrees there)
+        sin_a = sinf((M_PI * static_cast<double>(p.rotation_deg)) / 180);
+        cos_a = cosf((M_PI * static_cast<double>(p.rotation_deg)) / 180);
+    }
+
+    // volume_container.hpp:69-75
+    uint32_t index(V3 r) const {
+        uint32_t x = static_cast<uint32_t>(floorf(r.x + half.x)) % X;
+        uint32_t y = static_cast<uint32_t>(floorf(r.y + half.y)) % Y;
+        uint32_t z = static_cast<uint32_t>(floorf(r.z + half.z)) % Z;
+        return x + (y * X) + (z * XY);
+    }
+    // volume_container.hpp:86-91
+    V3 cell_centre(uint32_t idx) const {
+        V3 v = {0.5f + static_cast<float>(idx % X), 0.5f + static_cast<float>((idx % XY) / X),
+                0.5f + static_cast<float>(idx / XY)};
+        return sub(v, half);
+    }
+    float zidx(uint32_t idx) const { return static_cast<float>(idx / XY); }
+
+    // simulation_functions.cpp:46-51
+    V3 wrap(V3 r) const {
+        r.x = fmodf(r.x + 1.5f * L.x, L.x) - L.x * 0.5f;
+        r.y = fmodf(r.y + 1.5f * L.y, L.y) - L.y * 0.5f;
+        r.z = fmodf(r.z + 1.5f * L.z, L.z) - L.z * 0.5f;
+        return r;
+    }
+
+    // simulation_functions.cpp:7-33 (standard geometry)
+    void distribute() {
+        V3 scale = {L.x - static_cast<float>(2 * (1 - p.periodic[0])),
+                    L.y - static_cast<float>(2 * (1 - p.periodic[1])),
+                    L.z - static_cast<float>(2 * (1 - p.periodic[2]))};
+        for (size_t i = 0; i < parts.size(); ++i) {
+            Particle q = {};
+            V3 u = rng.init_pos_u(i);
+            V3 r = sub(u, V3{0.5f, 0.5f, 0.5f});
+            q.pos = {r.x * scale.x, r.y * scale.y, r.z * scale.z};
+            q.vel = mul(rng.init_vel(i), p.thermal_velocity);
+            q.cell = index(q.pos);
+            parts[i] = q;
+        }
+    }
+
+    // cpu_backend.cpp:23-33 + simulation_functions.cpp:38-103
+    void translate() {
+        shift = fixed_shift ? forced : rng.shift(static_cast<uint32_t>(step_index));
+        std::fill(counter.begin(), counter.end(), 0u);
+        float const dt = p.delta_t;
+        for (size_t idx = 0; idx < parts.size(); ++idx) {
+            Particle q = parts[idx];
+            if (!p.periodic[2]) {
+                float z_wall = (0.5f * L.z - 1);
+                float next_z = q.pos.z + q.vel.z * dt;
+                if (fabsf(next_z) > z_wall) {
+                    float t = (next_z > 0 ? z_wall - q.pos.z : -z_wall - q.pos.z) / q.vel.z;
+                    q.pos = add(q.pos, mul(q.vel, t));
+                    q.vel = neg(q.vel);
+                    q.pos = add(q.pos, mul(q.vel, dt - t));
+                } else {
+                    q.pos = add(q.pos, mul(q.vel, dt));
+                }
+            } else {
+                q.pos = add(q.pos, mul(q.vel, dt));
+            }
+            q.pos = wrap(add(q.pos, shift));
+            q.cell = index(q.pos);
+            uint32_t slot = counter[q.cell];
+            counter[q.cell] += 1;
+            if (slot < cap)
+                list[q.cell + static_cast<size_t>(slot) * C] = static_cast<uint32_t>(idx);
+            else
+                q.pos = wrap(sub(q.pos, shift));
+            parts[idx] = q;
+        }
+    }
+
+    // wall-layer test shared by both collision rules (simulation_functions.cpp:137-138)
+    void wall_flags(uint32_t c, bool sign, bool* layer, bool* ghosts) const {
+        *layer = (zidx(c) == (L.z - (sign ? 1 : 2)));
+        *ghosts = (!p.periodic[2]) && ((zidx(c) == static_cast<float>(sign)) || *layer);
+    }
+
+    // simulation_functions.cpp:108-202
+    void srd() {
+        float const sh = fabsf(shift.z);
+        bool const sign = shift.z > 0;
+        float const drag = p.drag * p.delta_t;
+        uint32_t const step = static_cast<uint32_t>(step_index);
+        std::vector<V3> bp(cap + p.n + 1), bv(cap + p.n + 1);
+        std::vector<int64_t> bidx(cap + p.n + 1);
+        for (uint32_t c = 0; c < C; ++c) {
+            uint32_t m = std::min(cap, counter[c]);
+            bool layer, ghosts;
+            wall_flags(c, sign, &layer, &ghosts);
+            uint32_t added = 0;
+            if (ghosts)
+                for (uint32_t i = 0; i < p.n; ++i)
+                    if ((rng.ghost_count_u(step, c, i) > sh) ^ layer ^ sign) ++added;
+            uint32_t total = m + added;
+            for (uint32_t i = 0; i < m; ++i) {
+                bidx[i] = list[c + static_cast<size_t>(i) * C];
+                bp[i] = parts[bidx[i]].pos;
+                bv[i] = parts[bidx[i]].vel;
+            }
+            if (ghosts) {
+                V3 centre = cell_centre(c);
+                for (uint32_t i = m; i < total; ++i) {
+                    V3 g = rng.ghost_pos(step, c, i - m, sh, layer, sign);
+                    bidx[i] = -1;
+                    bp[i] = add(V3{g.x - 0.5f, g.y - 0.5f, g.z - 0.5f}, centre);
+                    bv[i] = mul(rng.ghost_vel(step, c, i - m), p.thermal_velocity);
+                }
+            }
+            if (total > 1) {
+                V3 sum = {0, 0, 0};
+                for (uint32_t i = 0; i < total; ++i) sum = add(sum, bv[i]);
+                float inv = 1.0f / static_cast<float>(total);
+                V3 mean = mul(sum, inv);
+                V3 axis = rng.srd_axis(step, c);
+                for (uint32_t i = 0; i < total; ++i) {
+                    V3 v = sub(bv[i], mean);
+                    V3 vpar = mul(axis, dot(v, axis));
+                    V3 vperp = sub(v, vpar);
+                    bv[i] = add(add(vpar, mul(vperp, cos_a)), mul(cross(vperp, axis), sin_a));
+                }
+                for (uint32_t i = 0; i < total; ++i) {
+                    bv[i] = add(bv[i], mean);
+                    bp[i] = wrap(sub(bp[i], shift));
+                    bv[i].x += drag;
+                }
+            }
+            for (uint32_t i = 0; i < m; ++i) {
+                Particle& q = parts[bidx[i]];
+                q.pos = bp[i];
+                q.vel = bv[i];
+                q.cell = index(bp[i]);
+            }
+        }
+    }
+
+    // the 13 discrete directions of simulation_functions.cpp:272-293
+    static V3 ext_direction(int select) {
+        int const steps = 4;
+        float theta, phi;
+        int phi_i = select % steps + 1, theta_i = select / steps + 1;
+        if ((theta_i % 2) && (phi_i % 2))
+            theta = theta_i == 1 ? 0.95531661812450927816f : float(M_PI) - 0.95531661812450927816f;
+        else
+            theta = theta_i * (float(M_PI) / steps);
+        phi = phi_i * (float(M_PI) / steps);
+        return {sinf(theta) * cosf(phi), sinf(theta) * sinf(phi), cosf(theta)};
+    }
+
+    struct Sym {
+        float xx = 0, yy = 0, zz = 0, xy = 0, xz = 0, yz = 0;
+    };
+    // mechanic.hpp:26 and :52-79
+    static Sym inverse(Sym const& m, float tol) {
+        float det = m.xx * (m.yy * m.zz - m.yz * m.yz) + m.xz * (2 * m.xy * m.yz - m.yy * m.xz) - (m.xy * m.xy * m.zz);
+        if (det < 0) det = 0;
+        Sym o;
+        if (det > tol) {
+            float id = 1 / det;
+            o.xx = (m.yy * m.zz - m.yz * m.yz) * id;
+            o.yy = (m.xx * m.zz - m.xz * m.xz) * id;
+            o.zz = (m.xx * m.yy - m.xy * m.xy) * id;
+            o.xy = (m.xz * m.yz - m.zz * m.xy) * id;
+            o.xz = (m.xy * m.yz - m.yy * m.xz) * id;
+            o.yz = (m.xy * m.xz - m.xx * m.yz) * id;
+            return o;
+        }
+        float tr = m.xx + m.yy + m.zz;
+        if (tr > tol) {
+            float s = 4 / (tr * tr);
+            o.xx = m.xx * s; o.yy = m.yy * s; o.zz = m.zz * s;
+            o.xy = m.xy * s; o.xz = m.xz * s; o.yz = m.yz * s;
+        }
+        return o;
+    }
+    static V3 apply(Sym const& m, V3 v) {
+        return {(m.xx * v.x) + (m.xy * v.y) + (m.xz * v.z), (m.xy * v.x) + (m.yy * v.y) + (m.yz * v.z),
+                (m.xz * v.x) + (m.yz * v.y) + (m.zz * v.z)};
+    }
+
+    // simulation_functions.cpp:207-406
+    void extended() {
+        float const sh = fabsf(shift.z);
+        bool const sign = shift.z > 0;
+        float const scale = 0.01f;
+        float const drag = p.drag * p.delta_t;
+        float const tv = p.thermal_velocity;
+        uint32_t const step = static_cast<uint32_t>(step_index);
+        std::vector<V3> bp(cap + p.n + 1), bv(cap + p.n + 1);
+        std::vector<int64_t> bidx(cap + p.n + 1);
+        std::vector<uint8_t> side(cap + p.n + 1);
+        for (uint32_t c = 0; c < C; ++c) {
+            uint32_t m = std::min(cap, counter[c]);
+            bool layer, ghosts;
+            wall_flags(c, sign, &layer, &ghosts);
+            uint32_t added = 0;
+            if (ghosts)
+                for (uint32_t i = 0; i < p.n; ++i)
+                    if ((rng.ghost_count_u(step, c, i) > sh) ^ layer ^ sign) ++added;
+            uint32_t total = std::min(m + added, cap);
+            uint32_t reals = total - added;   // may drop table entries when ghosts fill the cap
+            for (uint32_t i = 0; i < reals; ++i) {
+                bidx[i] = list[c + static_cast<size_t>(i) * C];
+                bp[i] = parts[bidx[i]].pos;
+                bv[i] = parts[bidx[i]].vel;
+            }
+            V3 offset = cell_centre(c);
+            if (ghosts) {
+                for (uint32_t i = reals; i < total; ++i) {
+                    V3 g = rng.ghost_pos(step, c, i - reals, sh, layer, sign);
+                    bidx[i] = -1;
+                    bp[i] = add(V3{g.x - 0.5f, g.y - 0.5f, g.z - 0.5f}, offset);
+                    bv[i] = mul(rng.ghost_vel(step, c, i - reals), tv);
+                }
+            }
+            if (total > 1) {
+                int select, flip;
+                rng.ext_axis(step, c, &select, &flip);
+                V3 axis = ext_direction(select);
+                if (flip) axis = neg(axis);
+
+                V3 com = {0, 0, 0}, mean = {0, 0, 0};
+                for (uint32_t i = 0; i < total; ++i) {
+                    com = add(com, sub(bp[i], offset));
+                    mean = add(mean, bv[i]);
+                }
+                com = add(mul(com, 1.0f / total), offset);
+                mean = mul(mean, 1.0f / total);
+
+                float proj0 = 0.0f, proj1 = 0.0f;
+                int size0 = 0;
+                for (uint32_t i = 0; i < total; ++i) {
+                    bp[i] = sub(bp[i], com);
+                    bv[i] = sub(bv[i], mean);
+                    side[i] = dot(bp[i], axis) < 0;
+                    if (side[i]) {
+                        proj0 += dot(bv[i], axis);
+                        size0 += 1;
+                    }
+                }
+                int size1 = static_cast<int>(total) - size0;
+                proj1 = -proj0 / size1;
+                proj0 = proj0 / size0;
+                float arg = scale * (proj1 - proj0) * size0 * size1;
+                float probability = 1 - (p.rng_mode == 0 ? std::exp(arg) : p_expf(arg));
+
+                V3 dL = {0, 0, 0};
+                bool collide = rng.ext_gate(step, c) < probability;
+                if (collide) {
+                    float t0 = proj1 * (float(size1) / size0), t1 = proj0 * (float(size0) / size1);
+                    float mean0 = 0.0f, mean1 = 0.0f;
+                    Sym I;
+                    for (uint32_t i = 0; i < total; ++i) {
+                        bool s = side[i];
+                        float vr = rng.ext_gauss(step, c, i) * tv;
+                        (s ? mean0 : mean1) += vr;
+                        dL = add(dL, cross(bp[i], bv[i]));
+                        bv[i] = add(bv[i], mul(axis, vr - dot(bv[i], axis) + (s ? t0 : t1)));
+                        V3 q = {bp[i].x * bp[i].x, bp[i].y * bp[i].y, bp[i].z * bp[i].z};
+                        I.xx += q.y + q.z;
+                        I.yy += q.x + q.z;
+                        I.zz += q.x + q.y;
+                        I.xy += -bp[i].x * bp[i].y;
+                        I.xz += -bp[i].x * bp[i].z;
+                        I.yz += -bp[i].y * bp[i].z;
+                    }
+                    V3 vm0 = mul(axis, mean0 / size0), vm1 = mul(axis, mean1 / size1);
+                    for (uint32_t i = 0; i < total; ++i) {
+                        bv[i] = sub(bv[i], side[i] ? vm0 : vm1);
+                        dL = sub(dL, cross(bp[i], bv[i]));
+                    }
+                    I = inverse(I, 1e-5f);
+                    dL = apply(I, dL);
+                }
+                for (uint32_t i = 0; i < reals; ++i) {
+                    bv[i] = add(bv[i], mean);
+                    bv[i].x += drag;
+                    bv[i] = sub(bv[i], cross(bp[i], dL));
+                    bp[i] = wrap(sub(add(com, bp[i]), shift));
+                }
+            }
+            for (uint32_t i = 0; i < reals; ++i) {
+                Particle& q = parts[bidx[i]];
+                q.pos = bp[i];
+                q.vel = bv[i];
+                q.cell = index(bp[i]);
+            }
+        }
+    }
+
+    void collide() {
+        if (p.algorithm == 0) srd(); else extended();
+    }
+
+    void step(int n) {
+        for (int i = 0; i < n; ++i) {
+            translate();
+            collide();
+            ++step_index;
+        }
+    }
+
+    // cpu_backend.cpp:72-109 plus an FP64 accumulator of the same per-step fields
+    void step_and_sample(int n) {
+        std::vector<uint32_t> dens(C);
+        std::vector<V3> sum(C);
+        std::vector<double> sd(3 * static_cast<size_t>(C));
+        for (int s = 0; s < n; ++s) {
+            step(1);
+            std::fill(dens.begin(), dens.end(), 0u);
+            std::fill(sum.begin(), sum.end(), V3{0, 0, 0});
+            std::fill(sd.begin(), sd.end(), 0.0);
+            for (auto const& q : parts) {
+                uint32_t c = index(q.pos);
+                dens[c] += 1;
+                sum[c] = add(sum[c], q.vel);
+                sd[3 * c + 0] += q.vel.x;
+                sd[3 * c + 1] += q.vel.y;
+                sd[3 * c + 2] += q.vel.z;
+            }
+            std::vector<Sample> cur(C);
+            for (uint32_t c = 0; c < C; ++c) {
+                V3 mv = sum[c];
+                if (dens[c] > 0) mv = mul(mv, 1.0f / static_cast<float>(dens[c]));  // mpc_cell.hpp:84
+                cur[c] = {static_cast<float>(dens[c]), mv};
+            }
+            bool first = sampling_done;
+            if (sampling_done) {
+                sampling_done = false;
+                sample_counter = 0;
+                std::fill(kahan.begin(), kahan.end(), Sample{0, {0, 0, 0}});
+                states = cur;
+            } else {
+                sample_counter++;
+                for (uint32_t c = 0; c < C; ++c) {
+                    {
+                        float y = cur[c].density - kahan[c].density;
+                        float t = states[c].density + y;
+                        kahan[c].density = (t - states[c].density) - y;
+                        states[c].density = t;
+                    }
+                    V3 y = sub(cur[c].mean, kahan[c].mean);
+                    V3 t = add(states[c].mean, y);
+                    kahan[c].mean = sub(sub(t, states[c].mean), y);
+                    states[c].mean = t;
+                }
+            }
+            for (uint32_t c = 0; c < C; ++c) {
+                double d = static_cast<double>(dens[c]);
+                double inv = dens[c] > 0 ? 1.0 / d : 0.0;
+                if (first) {
+                    acc_d[c] = d;
+                    for (int k = 0; k < 3; ++k) acc_v[3 * c + k] = sd[3 * c + k] * inv;
+                } else {
+                    acc_d[c] += d;
+                    for (int k = 0; k < 3; ++k) acc_v[3 * c + k] += sd[3 * c + k] * inv;
+                }
+            }
+        }
+    }
+
+    // cpu_backend.cpp:111-122 (sum over S samples times 1/(S-1))
+    void sample_mean(float* dens, float* vel) {
+        double inverse = 1.0 / sample_counter;
+        for (uint32_t c = 0; c < C; ++c) {
+            dens[c] = states[c].density * inverse;
+            vel[3 * c + 0] = states[c].mean.x * inverse;
+            vel[3 * c + 1] = states[c].mean.y * inverse;
+            vel[3 * c + 2] = states[c].mean.z * inverse;
+        }
+        sampling_done = true;
+    }
+    void sample_mean_f64(double* dens, double* vel) const {
+        double inverse = 1.0 / static_cast<double>(sample_counter);
+        for (uint32_t c = 0; c < C; ++c) {
+            dens[c] = acc_d[c] * inverse;
+            for (int k = 0; k < 3; ++k) vel[3 * c + k] = acc_v[3 * c + k] * inverse;
+        }
+    }
+};
+
+}  // namespace oracle
+
+using oracle::Oracle;
+
+extern "C" {
+
+void* orc_create(const orc_params* p) { return new Oracle(*p); }
+void orc_destroy(void* h) { delete static_cast<Oracle*>(h); }
+void orc_distribute(void* h) { static_cast<Oracle*>(h)->distribute(); }
+unsigned long long orc_n_particles(void* h) { return static_cast<Oracle*>(h)->parts.size(); }
+unsigned orc_n_cells(void* h) { return static_cast<Oracle*>(h)->C; }
+void orc_set_step(void* h, unsigned long long s) { static_cast<Oracle*>(h)->step_index = s; }
+unsigned long long orc_get_step(void* h) { return static_cast<Oracle*>(h)->step_index; }
+
+void orc_set_particles(void* h, const float* pos, const float* vel, unsigned long long n) {
+    auto* o = static_cast<Oracle*>(h);
+    o->parts.resize(n);
+    for (size_t i = 0; i < n; ++i) {
+        o->parts[i].pos = {pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]};
+        o->parts[i].vel = {vel[3 * i], vel[3 * i + 1], vel[3 * i + 2]};
+        o->parts[i].cell = o->index(o->parts[i].pos);
+    }
+}
+void orc_get_particles(void* h, float* pos, float* vel) {
+    auto* o = static_cast<Oracle*>(h);
+    for (size_t i = 0; i < o->parts.size(); ++i) {
+        auto const& q = o->parts[i];
+        if (pos) { pos[3 * i] = q.pos.x; pos[3 * i + 1] = q.pos.y; pos[3 * i + 2] = q.pos.z; }
+        if (vel) { vel[3 * i] = q.vel.x; vel[3 * i + 1] = q.vel.y; vel[3 * i + 2] = q.vel.z; }
+    }
+}
+
+// streaming half-step: draws the shift, streams, fills counter/list (no collision)
+void orc_translate(void* h, float* shift_out) {
+    auto* o = static_cast<Oracle*>(h);
+    o->translate();
+    if (shift_out) { shift_out[0] = o->shift.x; shift_out[1] = o->shift.y; shift_out[2] = o->shift.z; }
+}
+void orc_force_shift(void* h, const float* s) {
+    auto* o = static_cast<Oracle*>(h);
+    o->fixed_shift = s != nullptr;
+    if (s) o->forced = {s[0], s[1], s[2]};
+}
+void orc_collide(void* h) {
+    auto* o = static_cast<Oracle*>(h);
+    o->collide();
+    ++o->step_index;
+}
+// per-particle cell index, per-cell counter, and the cell list flattened in
+// cell order (first min(counter, 4n) entries of each cell); returns entries written
+unsigned long long orc_get_cells(void* h, unsigned* cell_of_particle, unsigned* counter, unsigned* flat_list) {
+    auto* o = static_cast<Oracle*>(h);
+    for (size_t i = 0; i < o->parts.size(); ++i) cell_of_particle[i] = o->parts[i].cell;
+    std::memcpy(counter, o->counter.data(), sizeof(unsigned) * o->C);
+    size_t k = 0;
+    for (uint32_t c = 0; c < o->C; ++c) {
+        uint32_t m = std::min(o->cap, o->counter[c]);
+        for (uint32_t i = 0; i < m; ++i) flat_list[k++] = o->list[c + static_cast<size_t>(i) * o->C];
+    }
+    return k;
+}
+void orc_step(void* h, int n) { static_cast<Oracle*>(h)->step(n); }
+void orc_step_and_sample(void* h, int n) { static_cast<Oracle*>(h)->step_and_sample(n); }
+void orc_get_sample_mean(void* h, float* dens, float* vel) { static_cast<Oracle*>(h)->sample_mean(dens, vel); }
+void orc_get_sample_mean_f64(void* h, double* dens, double* vel) { static_cast<Oracle*>(h)->sample_mean_f64(dens, vel); }
+unsigned long long orc_sample_count(void* h) { return static_cast<Oracle*>(h)->sample_counter; }
+
+// building blocks, exported for the known-answer tests
+void orc_philox(unsigned k0, unsigned k1, const unsigned* ctr, unsigned* out) {
+    oracle::U4 r = oracle::philox(k0, k1, ctr[0], ctr[1], ctr[2], ctr[3]);
+    for (int i = 0; i < 4; ++i) out[i] = r.v[i];
+}
+float orc_logf(float x) { return oracle::p_logf(x); }
+float orc_expf(float x) { return oracle::p_expf(x); }
+void orc_sincospi(float t, float* s, float* c) { oracle::p_sincospi(t, s, c); }
+void orc_box_muller(unsigned a, unsigned b, float* g0, float* g1) { oracle::box_muller(a, b, g0, g1); }
+void orc_xoshiro_uniform(unsigned long long a, unsigned long long b, unsigned long long n, float* out) {
+    oracle::Xoshiro x;
+    x.seed(a, b);
+    for (size_t i = 0; i < n; ++i) out[i] = x.uniform();
+}
+void orc_xoshiro_gaussian(unsigned long long a, unsigned long long b, unsigned long long n, float* out) {
+    oracle::Xoshiro x;
+    x.seed(a, b);
+    for (size_t i = 0; i < n; ++i) out[i] = x.gaussian();
+}
+// counter-mode draws, for statistics tests
+void orc_counter_draws(void* h, int purpose, unsigned step, unsigned cell, unsigned count, float* out) {
+    auto* o = static_cast<Oracle*>(h);
+    for (unsigned i = 0; i < count; ++i) {
+        if (purpose == oracle::P_THERM) out[i] = o->rng.ext_gauss(step, cell, i);
+        else if (purpose == oracle::P_GHOST_COUNT) out[i] = o->rng.ghost_count_u(step, cell, i);
+        else if (purpose == oracle::P_AXIS) {
+            oracle::V3 a = o->rng.srd_axis(step, cell + i);
+            out[3 * i] = a.x; out[3 * i + 1] = a.y; out[3 * i + 2] = a.z;
+        }
+    }
+}
+void orc_ext_direction(int select, float* out) {
+    oracle::V3 a = Oracle::ext_direction(select);
+    out[0] = a.x; out[1] = a.y; out[2] = a.z;
+}
+void orc_sym_inverse(const float* m6, float tol, float* out6) {
+    Oracle::Sym m;
+    m.xx = m6[0]; m.yy = m6[1]; m.zz = m6[2]; m.xy = m6[3]; m.xz = m6[4]; m.yz = m6[5];
+    Oracle::Sym o = Oracle::inverse(m, tol);
+    out6[0] = o.xx; out6[1] = o.yy; out6[2] = o.zz; out6[3] = o.xy; out6[4] = o.xz; out6[5] = o.yz;
+}
+
+}  // extern "C"

@@ -1,0 +1,240 @@
+"""Parity of the HIP step with the CPU oracle (same state, same counter-based draws).
+
+(a) cell index per particle, per-cell counts and the stable cell-ordered
+    permutation: bit-exact against the oracle's translate_particles table
+    (simulation_functions.cpp:84-94).
+(b) per-particle positions and velocities after one step: expected bit-exact
+    (same IEEE operations in the same order, no contraction); the asserted bound
+    is the norm-wise relative 1e-5 of the specification.
+Also: device Philox known answers and float transforms bitwise against the
+oracle, seeded initial state, sampling accumulators against the oracle's FP64
+accumulation.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from cudampcd_amd import capi
+from cudampcd_amd.capi import Handle
+from helpers import make_params, min_image, stored_volume, vel_error
+from oracle_lib import Oracle, oracle_lib
+
+pytestmark = pytest.mark.gpu
+
+CONFIGS = [
+    # fluid extents, periodicity, n, algorithm
+    ((6, 5, 8), (1, 1, 0), 10, "srd"),
+    ((6, 5, 9), (1, 1, 1), 10, "srd"),
+    ((5, 4, 7), (1, 1, 0), 3, "srd"),
+    ((4, 4, 6), (1, 1, 0), 1, "srd"),
+    ((7, 5, 9), (1, 1, 0), 10, "srd"),
+    ((32, 24, 18), (1, 1, 0), 20, "srd"),
+    ((6, 5, 8), (1, 1, 0), 10, "extended"),
+    ((6, 5, 9), (1, 1, 1), 10, "extended"),
+    ((5, 4, 7), (1, 1, 0), 3, "extended"),
+    ((4, 4, 6), (1, 1, 0), 1, "extended"),
+    ((20, 16, 12), (1, 1, 0), 10, "extended"),
+]
+IDS = [f"{a}-{'x'.join(map(str, f))}-p{''.join(map(str, p))}-n{n}" for f, p, n, a in CONFIGS]
+
+
+def _sim(pympcd, cfg, seed=4242, drag=0.001):
+    fluid, perio, n, algo = cfg
+    p = make_params(pympcd, fluid, perio, n=n, algorithm=algo, seed=seed, drag=drag)
+    sim = pympcd.Simulation(p, "hip")
+    return p, sim, Handle(sim)
+
+
+def _oracle(p, distribute=False):
+    return Oracle(stored_volume(p), tuple(p.periodicity), n=p.n, drag=p.drag, delta_t=p.delta_t,
+                  thermal_velocity=float(np.sqrt(p.temperature)), rotation_deg=p.rotation_angle,
+                  algorithm=p.algorithm, rng="counter", seed=(p.seed, 0), distribute=distribute)
+
+
+def test_device_philox_known_answers():
+    ctr = np.array([[0, 0, 0, 0], [0xffffffff] * 4, [0x243f6a88, 0x85a308d3, 0x13198a2e, 0x03707344]], np.uint32)
+    keys = [(0, 0), (0xffffffff, 0xffffffff), (0xa4093822, 0x299f31d0)]
+    want = [(0x6627e8d5, 0xe169c58d, 0xbc57ac4c, 0x9b00dbd8), (0x408f276d, 0x41c83b0e, 0xa20bc7c6, 0x6d5451fd),
+            (0xd16cfe09, 0x94fdcceb, 0x5001e420, 0x24126ea1)]
+    for (k0, k1), c, w in zip(keys, ctr, want):
+        assert tuple(capi.selftest_philox(k0, k1, c[None])[0]) == w
+    # a bulk of random counters against the oracle's Philox
+    import ctypes
+    rng = np.random.default_rng(0)
+    many = rng.integers(0, 2 ** 32, size=(4096, 4), dtype=np.uint64).astype(np.uint32)
+    dev = capi.selftest_philox(0x1234, 0x5678, many)
+    lib = oracle_lib()
+    U = ctypes.c_uint * 4
+    for i in range(0, 4096, 97):
+        out = U()
+        lib.orc_philox(0x1234, 0x5678, U(*map(int, many[i])), out)
+        assert tuple(out) == tuple(dev[i])
+
+
+def test_device_transforms_bitwise():
+    import ctypes
+    lib = oracle_lib()
+    rng = np.random.default_rng(1)
+    u = rng.uniform(0, 1, 20000).astype(np.float32) + np.float32(1.17549435e-38)
+    got = capi.selftest_math(u, 0)[:, 0]
+    want = np.array([lib.orc_logf(float(x)) for x in u], np.float32)
+    assert np.array_equal(got, want)
+    t = rng.uniform(0, 2, 20000).astype(np.float32)
+    got = capi.selftest_math(t, 1)
+    s = ctypes.c_float()
+    c = ctypes.c_float()
+    want = []
+    for x in t:
+        lib.orc_sincospi(float(x), ctypes.byref(s), ctypes.byref(c))
+        want.append((s.value, c.value))
+    assert np.array_equal(got, np.array(want, np.float32))
+    x = rng.uniform(-90, 90, 20000).astype(np.float32)
+    got = capi.selftest_math(x, 2)[:, 0]
+    want = np.array([lib.orc_expf(float(v)) for v in x], np.float32)
+    assert np.array_equal(got, want)
+    y = rng.uniform(0, 4, 20000).astype(np.float32)
+    assert np.array_equal(capi.selftest_math(y, 3)[:, 0], np.sqrt(y))
+    w = rng.uniform(-12, 12, 20000).astype(np.float32)
+    got = capi.selftest_math(w, 4)
+    L = np.float32(7.0)
+    assert np.array_equal(got[:, 0], np.fmod(w, L))
+    assert np.array_equal(got[:, 1], np.fmod(w + np.float32(1.5) * L, L) - L * np.float32(0.5))
+    got = capi.selftest_math(y + np.float32(0.5), 5)
+    assert np.array_equal(got[:, 0], np.float32(1) / (y + np.float32(0.5)))
+    assert np.array_equal(got[:, 1], (y + np.float32(0.5)) / np.float32(3))
+
+
+@pytest.mark.parametrize("cfg", CONFIGS[:6], ids=IDS[:6])
+def test_initial_state_matches_oracle(pympcd, cfg):
+    p, sim, h = _sim(pympcd, cfg)
+    o = _oracle(p, distribute=True)
+    pos, vel = o.particles()
+    assert np.array_equal(sim.get_particle_positions(), pos)
+    assert np.array_equal(sim.get_particle_velocities(), vel)
+
+
+@pytest.mark.parametrize("cfg", CONFIGS, ids=IDS)
+def test_cell_keys_counts_permutation_bitexact(pympcd, cfg):
+    p, sim, h = _sim(pympcd, cfg)
+    sim.step(3)                      # a sorted, collided state
+    pos, vel = sim.get_particle_positions(), sim.get_particle_velocities()
+    o = _oracle(p)
+    o.set_particles(pos, vel)
+    o.set_step(h.step_index)
+    cell, cnt, perm = h.peek_sort()
+    o.translate()
+    ocell, ocnt, olist = o.cells()
+    assert np.array_equal(cell, ocell)
+    assert np.array_equal(cnt, ocnt)
+    cap = 4 * p.n
+    starts = np.concatenate([[0], np.cumsum(cnt)[:-1]])
+    kept = np.concatenate([perm[s:s + min(c, cap)] for s, c in zip(starts, cnt)]) if len(cnt) else perm
+    assert np.array_equal(kept, olist)
+    # the permutation is a permutation
+    assert np.array_equal(np.sort(perm), np.arange(len(perm), dtype=np.uint32))
+
+
+@pytest.mark.parametrize("cfg", CONFIGS, ids=IDS)
+def test_step_matches_oracle(pympcd, cfg):
+    p, sim, h = _sim(pympcd, cfg)
+    sim.step(2)
+    L = np.array(stored_volume(p), np.float64)
+    h.record_permutation(True)
+    bitwise = 0
+    total = 0
+    for _ in range(4):
+        pos, vel = sim.get_particle_positions(), sim.get_particle_velocities()
+        o = _oracle(p)
+        o.set_particles(pos, vel)
+        o.set_step(h.step_index)
+        sim.step(1)
+        o.step(1)
+        perm = h.last_permutation()
+        gpos, gvel = sim.get_particle_positions(), sim.get_particle_velocities()
+        opos, ovel = o.particles()
+        opos, ovel = opos[perm], ovel[perm]
+        assert vel_error(gvel, ovel) <= 1e-5
+        dpos = np.abs(min_image(gpos.astype(np.float64) - opos, L)).max()
+        assert dpos <= 1e-5 * L.max()
+        bitwise += int((np.all(gpos == opos, axis=1) & np.all(gvel == ovel, axis=1)).sum())
+        total += len(gpos)
+    print(f"bitwise-identical particles: {bitwise}/{total}")
+    assert bitwise == total
+
+
+@pytest.mark.parametrize("algorithm", ["srd", "extended"])
+def test_sampling_matches_oracle(pympcd, algorithm):
+    cfg = ((6, 5, 8), (1, 1, 0), 10, algorithm)
+    p, sim, h = _sim(pympcd, cfg, drag=0.01)
+    sim.step(2)
+    o = _oracle(p)
+    for _ in range(7):
+        o.set_particles(sim.get_particle_positions(), sim.get_particle_velocities())
+        o.set_step(h.step_index)
+        sim.step_and_sample(1)
+        o.step_and_sample(1)
+    X, Y, Z = stored_volume(p)
+    d64, v64 = sim.get_mean_fields_f64()
+    od, ov = o.sample_mean_f64()
+    od = od.reshape(Z, Y, X).transpose(2, 1, 0)
+    ov = ov.reshape(Z, Y, X, 3).transpose(2, 1, 0, 3)
+    assert np.array_equal(d64, od)
+    assert np.abs(v64 - ov).max() < 1e-9
+    d32, v32 = sim.get_mean_fields()
+    fd, fv = o.sample_mean()
+    fd = fd.reshape(Z, Y, X).transpose(2, 1, 0)
+    fv = fv.reshape(Z, Y, X, 3).transpose(2, 1, 0, 3)
+    assert np.abs(d32 - fd).max() <= 1e-5 * max(1.0, np.abs(fd).max())
+    assert np.abs(v32 - fv).max() <= 1e-5
+    # get_mean_fields ended the average: the next sample starts a new one
+    sim.step_and_sample(2)
+    d, v = sim.get_mean_fields()
+    assert np.isfinite(v).all()
+
+
+def test_determinism_and_tiling_independence(pympcd):
+    cfg = ((24, 20, 14), (1, 1, 0), 10, "srd")
+    states = []
+    for tiling in [(0, 0), (0, 0), (7, 512), (64, 1024), (1, 256)]:
+        p, sim, h = _sim(pympcd, cfg, seed=99)
+        if tiling != (0, 0):
+            h.set_tiling(*tiling)
+        sim.step(8)
+        states.append((sim.get_particle_positions(), sim.get_particle_velocities()))
+    for s in states[1:]:
+        assert np.array_equal(s[0], states[0][0]) and np.array_equal(s[1], states[0][1])
+
+
+def test_set_get_roundtrip_and_errors(pympcd):
+    p, sim, h = _sim(pympcd, ((5, 5, 6), (1, 1, 0), 4, "srd"))
+    rng = np.random.default_rng(3)
+    pos = sim.get_particle_positions()
+    vel = rng.normal(size=pos.shape).astype(np.float32)
+    sim.set_particle_velocities(vel)
+    assert np.array_equal(sim.get_particle_velocities(), vel)
+    pos2 = (pos * np.float32(0.5)).astype(np.float32)
+    sim.set_particles(pos2, vel)
+    assert np.array_equal(sim.get_particle_positions(), pos2)
+    with pytest.raises(RuntimeError, match="particles"):
+        sim.set_particle_positions(pos[:-1])
+    sim.step(2)
+    assert np.isfinite(sim.get_particle_positions()).all()
+
+
+def test_oversized_cell_is_reported(pympcd):
+    """A cell beyond one LDS tile is streamed without collision and reported."""
+    p, sim, h = _sim(pympcd, ((10, 10, 10), (1, 1, 1), 4, "srd"))
+    cap = h.tiling()[1]
+    n = sim.n_particles
+    assert n > cap
+    pos = np.zeros((n, 3), np.float32)
+    vel = np.zeros((n, 3), np.float32)
+    sim.set_particles(pos, vel)
+    with pytest.raises(RuntimeError, match="LDS"):
+        sim.step(1)
+    # the state stays consistent: a normal configuration steps again
+    rng = np.random.default_rng(5)
+    sim.set_particles(rng.uniform(-5, 5, size=(n, 3)).astype(np.float32), rng.normal(size=(n, 3)).astype(np.float32))
+    sim.step(3)
+    assert np.isfinite(sim.get_particle_velocities()).all()
