@@ -131,13 +131,19 @@ __device__ __forceinline__ void decode_key(KParams const& P, uint32_t key, uint3
     iz = P.ord[0] == 2 ? q0 : (P.ord[1] == 2 ? q1 : q2);
 }
 
-// Wave-aggregated histogram: lanes holding the same key share one atomic;
-// returns each lane's rank among the lanes of its key plus the count before.
-// Must be called by whole waves in uniform control flow.
-__device__ __forceinline__ uint32_t wave_hist(uint32_t* count, uint32_t key, bool valid) {
+// Groups the lanes of a wave by key without touching memory: every lane
+// learns the leader lane of its group and its position inside the group, the
+// leader learns the group size.  Must be called by whole waves in uniform
+// control flow; invalid lanes join no group.
+struct WaveGroup {
+    int leader;
+    uint32_t pos;
+    uint32_t size;  // valid on the leader
+};
+__device__ __forceinline__ WaveGroup wave_group(uint32_t key, bool valid) {
     uint32_t const lane = __lane_id();
     uint64_t const below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
-    uint32_t rank = 0;
+    WaveGroup g = {static_cast<int>(lane), 0u, 0u};
     bool active = valid;
     for (;;) {
         uint64_t const m = __ballot(active);
@@ -146,15 +152,20 @@ __device__ __forceinline__ uint32_t wave_hist(uint32_t* count, uint32_t key, boo
         uint32_t const lk = __builtin_amdgcn_readlane(key, leader);
         bool const mine = active && key == lk;
         uint64_t const grp = __ballot(mine);
-        uint32_t base = 0;
-        if (static_cast<int>(lane) == leader) base = atomicAdd(count + lk, static_cast<uint32_t>(__popcll(grp)));
-        base = __builtin_amdgcn_readlane(base, leader);
         if (mine) {
-            rank = base + static_cast<uint32_t>(__popcll(grp & below));
+            g.leader = leader;
+            g.pos = static_cast<uint32_t>(__popcll(grp & below));
             active = false;
         }
+        if (static_cast<int>(lane) == leader) g.size = static_cast<uint32_t>(__popcll(grp));
     }
-    return rank;
+    return g;
+}
+
+// histogram update: one non-returning atomic per distinct key of the wave
+__device__ __forceinline__ void wave_count(uint32_t* count, uint32_t key, bool valid) {
+    WaveGroup const g = wave_group(key, valid);
+    if (valid && g.leader == static_cast<int>(__lane_id())) atomicAdd(count + key, g.size);
 }
 
 __device__ __forceinline__ uint32_t xcd_tile(uint32_t b, uint32_t G) {
@@ -188,9 +199,9 @@ __global__ __launch_bounds__(WG) void k_init(KParams P, float* __restrict__ st, 
     }
 }
 
-// streaming half-step of an unsorted state: cell key and rank per particle
+// streaming half-step of an unsorted state: cell key per particle and the histogram
 __global__ __launch_bounds__(WG) void k_keys(KParams P, const float* __restrict__ st, uint32_t* __restrict__ count,
-                                             uint2* __restrict__ keyrank) {
+                                             uint32_t* __restrict__ keys) {
     uint64_t const N = P.N;
     uint64_t const j = blockIdx.x * static_cast<uint64_t>(WG) + threadIdx.x;
     bool const valid = j < N;
@@ -203,8 +214,8 @@ __global__ __launch_bounds__(WG) void k_keys(KParams P, const float* __restrict_
         cell_of(P, px, py, pz, ix, iy, iz);
         key = key_of(P, ix, iy, iz);
     }
-    uint32_t const rank = wave_hist(count, key, valid);
-    if (valid) keyrank[j] = make_uint2(key, rank);
+    wave_count(count, key, valid);
+    if (valid) keys[j] = key;
 }
 
 // ---- exclusive scan of the per-cell counts (3 passes) ----
@@ -264,7 +275,7 @@ __global__ __launch_bounds__(1024) void k_scan_partials(uint32_t* __restrict__ p
 
 __global__ __launch_bounds__(WG) void k_scan_down(uint32_t* __restrict__ count, uint32_t C,
                                                   const uint32_t* __restrict__ partial, uint32_t* __restrict__ offset,
-                                                  uint64_t N) {
+                                                  uint32_t* __restrict__ cursor, uint64_t N) {
     __shared__ uint32_t s_v[SCAN_CHUNK + SCAN_CHUNK / 32];
     __shared__ uint32_t s_w[WG / 64];
     uint64_t const base = static_cast<uint64_t>(blockIdx.x) * SCAN_CHUNK;
@@ -295,19 +306,29 @@ __global__ __launch_bounds__(WG) void k_scan_down(uint32_t* __restrict__ count, 
         uint32_t const li = j * WG + threadIdx.x;
         uint64_t const idx = base + li;
         if (idx < C) {
-            offset[idx] = s_v[pad(li)];
+            uint32_t const o = s_v[pad(li)];
+            offset[idx] = o;
+            cursor[idx] = o;
             count[idx] = 0u;
         }
     }
     if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) offset[C] = static_cast<uint32_t>(N);
 }
 
-__global__ __launch_bounds__(WG) void k_perm(const uint2* __restrict__ keyrank, const uint32_t* __restrict__ offset,
+// cell-ordered permutation: each particle takes the next free slot of its
+// cell (one returning atomic per distinct key of the wave, issued together)
+__global__ __launch_bounds__(WG) void k_perm(const uint32_t* __restrict__ keys, uint32_t* __restrict__ cursor,
                                              uint32_t* __restrict__ perm, uint64_t N) {
-    for (uint64_t j = blockIdx.x * static_cast<uint64_t>(WG) + threadIdx.x; j < N;
-         j += static_cast<uint64_t>(gridDim.x) * WG) {
-        uint2 const kr = keyrank[j];
-        perm[offset[kr.x] + kr.y] = static_cast<uint32_t>(j);
+    uint64_t const stride = static_cast<uint64_t>(gridDim.x) * WG;
+    uint64_t const span = (N + stride - 1) / stride * stride;
+    for (uint64_t j = blockIdx.x * static_cast<uint64_t>(WG) + threadIdx.x; j < span; j += stride) {
+        bool const valid = j < N;
+        uint32_t const key = valid ? keys[j] : 0u;
+        WaveGroup const g = wave_group(key, valid);
+        uint32_t base = 0;
+        if (valid && g.leader == static_cast<int>(__lane_id())) base = atomicAdd(cursor + key, g.size);
+        base = __shfl(base, g.leader, 64);
+        if (valid) perm[base + g.pos] = static_cast<uint32_t>(j);
     }
 }
 
@@ -378,7 +399,6 @@ __device__ __forceinline__ Sym sym_inverse(Sym const& m, float tol) {
 // LDS carve-up of the collide kernel
 struct Lds {
     uint32_t* perm;   // [cap] source index, arrival order
-    uint32_t* src;    // [cap] source index, cell order
     uint16_t* cell;   // [cap] tile-local cell of each slot
     float* p[6];      // [cap] x y z vx vy vz, cell order
     uint32_t* off;    // [tile_cells + 1]
@@ -390,7 +410,7 @@ struct Lds {
 
 __host__ __device__ inline size_t lds_bytes(uint32_t cap, uint32_t tc, int algo) {
     size_t b = 0;
-    b += 4ull * cap * 2 + 2ull * cap + 4ull * 6 * cap;
+    b += 4ull * cap + 2ull * cap + 4ull * 6 * cap;
     b += 4ull * (tc + 1) + 4ull * 6 * tc + 4ull * tc;
     if (algo == 1) b += 4ull * 6 * (cap / 2);
     b += 16;
@@ -401,7 +421,6 @@ __device__ __forceinline__ Lds carve(unsigned char* base, uint32_t cap, uint32_t
     Lds s;
     unsigned char* q = base;
     s.perm = reinterpret_cast<uint32_t*>(q); q += 4ull * cap;
-    s.src = reinterpret_cast<uint32_t*>(q); q += 4ull * cap;
     for (int k = 0; k < 6; ++k) { s.p[k] = reinterpret_cast<float*>(q); q += 4ull * cap; }
     s.off = reinterpret_cast<uint32_t*>(q); q += 4ull * (tc + 1);
     for (int k = 0; k < 6; ++k) { s.cm[k] = reinterpret_cast<float*>(q); q += 4ull * tc; }
@@ -566,10 +585,13 @@ __device__ void extended_cell(KParams const& P, Lds const& s, uint32_t k, uint32
     }
 }
 
+constexpr int MAXP = 8;             // lds_cap <= MAXP * WG
+constexpr int GQ = 4;               // particles gathered per thread per round
+
 template <int ALGO, bool SAMPLE>
 __global__ __launch_bounds__(WG) void k_collide(KParams P, const float* __restrict__ in, float* __restrict__ out,
                                                 const uint32_t* __restrict__ perm, const uint32_t* __restrict__ offset,
-                                                uint32_t* __restrict__ count_next, uint2* __restrict__ keyrank,
+                                                uint32_t* __restrict__ count_next, uint32_t* __restrict__ keys,
                                                 uint32_t* __restrict__ perm_out, SampleBufs S,
                                                 uint32_t* __restrict__ status) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
@@ -586,16 +608,13 @@ __global__ __launch_bounds__(WG) void k_collide(KParams P, const float* __restri
 
     uint32_t kb = 0;
     while (kb < nc) {
-        if (tid == 0) {
-            uint32_t ke = kb + 1;
-            while (ke < nc && s.off[ke + 1] - s.off[kb] <= cap) ++ke;
-            s.bc[0] = ke;
-        }
-        __syncthreads();
-        uint32_t const ke = s.bc[0];
+        // batch = longest run of whole cells from kb that fits the LDS tile
+        // (offsets are non-decreasing, so the fitting cells form a prefix)
+        bool const fits = (kb + 1 + tid <= nc) && (s.off[kb + 1 + tid] - s.off[kb] <= cap);
+        uint32_t const nfit = static_cast<uint32_t>(__syncthreads_count(fits));
+        uint32_t const ke = kb + (nfit > 0 ? nfit : 1u);
         uint32_t const gb = s.off[kb];
         uint32_t const m = s.off[ke] - gb;
-        __syncthreads();
         if (m > cap) {
             // A single cell larger than the LDS tile: its particles are moved
             // through unchanged (streamed and un-shifted, no collision) so the
@@ -623,8 +642,8 @@ __global__ __launch_bounds__(WG) void k_collide(KParams P, const float* __restri
                     cell_of(P, px, py, pz, ix, iy, iz);
                     knext = key_of(P, ix, iy, iz);
                 }
-                uint32_t const rank = wave_hist(count_next, knext, valid);
-                if (valid) keyrank[static_cast<uint64_t>(gb) + i] = make_uint2(knext, rank);
+                wave_count(count_next, knext, valid);
+                if (valid) keys[static_cast<uint64_t>(gb) + i] = knext;
             }
             __syncthreads();
             kb = ke;
@@ -637,20 +656,41 @@ __global__ __launch_bounds__(WG) void k_collide(KParams P, const float* __restri
             for (uint32_t i = a; i < e; ++i) s.cell[i] = static_cast<uint16_t>(k);
         }
         __syncthreads();
-        // order each cell by source index, gather, re-apply this step's streaming
-        for (uint32_t i = tid; i < m; i += WG) {
-            uint32_t const k = s.cell[i];
-            uint32_t const a = s.off[kb + k] - gb, e = s.off[kb + k + 1] - gb;
-            uint32_t const src = s.perm[i];
-            uint32_t r = 0;
-            for (uint32_t j = a; j < e; ++j) r += s.perm[j] < src;
-            uint32_t const d = a + r;
-            float px = in[src], py = in[N + src], pz = in[2 * N + src];
-            float vx = in[3 * N + src], vy = in[4 * N + src], vz = in[5 * N + src];
-            stream_shift(P, P.shift, px, py, pz, vx, vy, vz);
-            s.p[0][d] = px; s.p[1][d] = py; s.p[2][d] = pz;
-            s.p[3][d] = vx; s.p[4][d] = vy; s.p[5][d] = vz;
-            s.src[d] = src;
+        // gather this thread's particles, GQ at a time with all their loads in
+        // flight together, then order each cell by source slot and re-apply
+        // this step's streaming
+        for (uint32_t i0 = 0; i0 < m; i0 += GQ * WG) {
+            float g[GQ][6];
+            uint32_t srcs[GQ];
+#pragma unroll
+            for (int q = 0; q < GQ; ++q) {
+                uint32_t const i = i0 + tid + q * WG;
+                srcs[q] = 0;
+                if (i < m) {
+                    uint32_t const src = s.perm[i];
+                    srcs[q] = src;
+#pragma unroll
+                    for (int c = 0; c < 6; ++c) g[q][c] = in[c * N + src];
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < GQ; ++q) {
+                uint32_t const i = i0 + tid + q * WG;
+                if (i < m) {
+                    uint32_t const k = s.cell[i];
+                    uint32_t const a = s.off[kb + k] - gb, e = s.off[kb + k + 1] - gb;
+                    uint32_t const src = srcs[q];
+                    uint32_t r = 0;
+                    for (uint32_t j = a; j < e; ++j) r += s.perm[j] < src;
+                    uint32_t const d = a + r;
+                    float px = g[q][0], py = g[q][1], pz = g[q][2];
+                    float vx = g[q][3], vy = g[q][4], vz = g[q][5];
+                    stream_shift(P, P.shift, px, py, pz, vx, vy, vz);
+                    s.p[0][d] = px; s.p[1][d] = py; s.p[2][d] = pz;
+                    s.p[3][d] = vx; s.p[4][d] = vy; s.p[5][d] = vz;
+                    if (perm_out) perm_out[static_cast<uint64_t>(gb) + d] = src;
+                }
+            }
         }
         __syncthreads();
         // per-cell stage, one lane per cell
@@ -677,8 +717,8 @@ __global__ __launch_bounds__(WG) void k_collide(KParams P, const float* __restri
                         sy = sy + s.p[4][a + i];
                         sz = sz + s.p[5][a + i];
                     }
-                    for (uint32_t g = 0; g < added; ++g) {
-                        Words const v = P.rs.at(P_GHOST_VEL, g, ref, P.step);
+                    for (uint32_t gi = 0; gi < added; ++gi) {
+                        Words const v = P.rs.at(P_GHOST_VEL, gi, ref, P.step);
                         float g0, g1, g2, g3;
                         box_muller(v.w[0], v.w[1], &g0, &g1);
                         box_muller(v.w[2], v.w[3], &g2, &g3);
@@ -736,7 +776,6 @@ __global__ __launch_bounds__(WG) void k_collide(KParams P, const float* __restri
                 uint64_t const o = static_cast<uint64_t>(gb) + i;
                 out[o] = px; out[N + o] = py; out[2 * N + o] = pz;
                 out[3 * N + o] = vx; out[4 * N + o] = vy; out[5 * N + o] = vz;
-                if (perm_out) perm_out[o] = s.src[i];
                 if (SAMPLE) {
                     uint32_t ix, iy, iz;
                     cell_of(P, px, py, pz, ix, iy, iz);
@@ -751,8 +790,8 @@ __global__ __launch_bounds__(WG) void k_collide(KParams P, const float* __restri
                 cell_of(P, px, py, pz, ix, iy, iz);
                 knext = key_of(P, ix, iy, iz);
             }
-            uint32_t const rank = wave_hist(count_next, knext, valid);
-            if (valid) keyrank[static_cast<uint64_t>(gb) + i] = make_uint2(knext, rank);
+            wave_count(count_next, knext, valid);
+            if (valid) keys[static_cast<uint64_t>(gb) + i] = knext;
         }
         __syncthreads();
         kb = ke;
@@ -841,7 +880,8 @@ struct mpcd_hip_s {
     float* st[2] = {nullptr, nullptr};
     int cur = 0;
     uint32_t* perm = nullptr;
-    uint2* keyrank = nullptr;
+    uint32_t* keys = nullptr;
+    uint32_t* cursor = nullptr;
     uint32_t* count = nullptr;
     uint32_t* offset = nullptr;
     uint32_t* partial = nullptr;
@@ -921,8 +961,9 @@ constexpr size_t LDS_BUDGET = 64 * 1024;  // dynamic LDS per workgroup used by k
 void choose_tiling(mpcd_hip_s* h, uint32_t tc, uint32_t cap) {
     uint32_t const n = std::max<uint32_t>(1, h->prm.n);
     int const algo = h->prm.algorithm;
-    if (cap == 0) cap = h->lds_cap ? h->lds_cap : (algo == 1 ? 1024u : 1536u);
+    if (cap == 0) cap = h->lds_cap ? h->lds_cap : (algo == 1 ? 1024u : 1280u);
     cap = std::max<uint32_t>(cap, 64);
+    cap = std::min<uint32_t>(cap, MAXP * WG);
     if (tc == 0) tc = h->tile_cells ? h->tile_cells : std::max<uint32_t>(1, (cap / 2) / n);
     // ghosts of a batch (at most n per cell) must fit the cap/2 ghost region
     tc = std::min<uint32_t>(tc, std::max<uint32_t>(1, (cap / 2) / n));
@@ -955,7 +996,7 @@ void launch_keys(mpcd_hip_s* h) {
     k.step = static_cast<uint32_t>(h->step_index);
     ck(hipMemsetAsync(h->count, 0, 4 * h->C, h->stream), "clear counts");
     uint64_t const blocks = (h->N + WG - 1) / WG;
-    if (h->N) hipLaunchKernelGGL(k_keys, dim3(static_cast<uint32_t>(blocks)), dim3(WG), 0, h->stream, k, h->st[h->cur], h->count, h->keyrank);
+    if (h->N) hipLaunchKernelGGL(k_keys, dim3(static_cast<uint32_t>(blocks)), dim3(WG), 0, h->stream, k, h->st[h->cur], h->count, h->keys);
     ck(hipGetLastError(), "k_keys");
     h->keys_valid = true;
     h->sorted_valid = false;
@@ -965,10 +1006,10 @@ void launch_sort(mpcd_hip_s* h) {
     uint32_t const C = static_cast<uint32_t>(h->C);
     hipLaunchKernelGGL(k_scan_reduce, dim3(h->nb), dim3(WG), 0, h->stream, h->count, C, h->partial);
     hipLaunchKernelGGL(k_scan_partials, dim3(1), dim3(1024), 0, h->stream, h->partial, h->nb);
-    hipLaunchKernelGGL(k_scan_down, dim3(h->nb), dim3(WG), 0, h->stream, h->count, C, h->partial, h->offset, h->N);
+    hipLaunchKernelGGL(k_scan_down, dim3(h->nb), dim3(WG), 0, h->stream, h->count, C, h->partial, h->offset, h->cursor, h->N);
     uint64_t blocks = std::min<uint64_t>((h->N + WG - 1) / WG, 1u << 20);
     if (blocks == 0) blocks = 1;
-    hipLaunchKernelGGL(k_perm, dim3(static_cast<uint32_t>(blocks)), dim3(WG), 0, h->stream, h->keyrank, h->offset, h->perm, h->N);
+    hipLaunchKernelGGL(k_perm, dim3(static_cast<uint32_t>(blocks)), dim3(WG), 0, h->stream, h->keys, h->cursor, h->perm, h->N);
     ck(hipGetLastError(), "sort");
     h->sorted_valid = true;
     h->launches += 4;
@@ -979,7 +1020,7 @@ void launch_collide_t(mpcd_hip_s* h, KParams const& k, size_t lds) {
     uint32_t* rec = h->record ? h->perm_rec : nullptr;
     SampleBufs S{h->scnt, h->ssum};
     hipLaunchKernelGGL((k_collide<ALGO, SAMPLE>), dim3(k.ntiles), dim3(WG), lds, h->stream, k, h->st[h->cur],
-                       h->st[1 - h->cur], h->perm, h->offset, h->count, h->keyrank, rec, S, h->status);
+                       h->st[1 - h->cur], h->perm, h->offset, h->count, h->keys, rec, S, h->status);
 }
 
 void ensure_sampling(mpcd_hip_s* h) {
@@ -1098,7 +1139,7 @@ void destroy(mpcd_hip_s* h) {
     if (!h) return;
     if (h->stream) (void)hipStreamSynchronize(h->stream);
     for (auto* p : {h->st[0], h->st[1]}) if (p) (void)hipFree(p);
-    for (void* p : {static_cast<void*>(h->perm), static_cast<void*>(h->keyrank), static_cast<void*>(h->count),
+    for (void* p : {static_cast<void*>(h->perm), static_cast<void*>(h->keys), static_cast<void*>(h->cursor), static_cast<void*>(h->count),
                     static_cast<void*>(h->offset), static_cast<void*>(h->partial), static_cast<void*>(h->status),
                     static_cast<void*>(h->perm_rec), static_cast<void*>(h->scnt), static_cast<void*>(h->ssum),
                     static_cast<void*>(h->acc)})
@@ -1160,9 +1201,10 @@ int mpcd_hip_create(const mpcd_params_t* params, mpcd_hip_t** out) {
         ck(hipMalloc(&h->st[0], 24 * N), "alloc state");
         ck(hipMalloc(&h->st[1], 24 * N), "alloc state");
         ck(hipMalloc(&h->perm, 4 * N), "alloc perm");
-        ck(hipMalloc(&h->keyrank, 8 * N), "alloc keys");
+        ck(hipMalloc(&h->keys, 4 * N), "alloc keys");
         ck(hipMalloc(&h->count, 4 * h->C), "alloc counts");
         ck(hipMalloc(&h->offset, 4 * (h->C + 1)), "alloc offsets");
+        ck(hipMalloc(&h->cursor, 4 * h->C), "alloc cursors");
         h->nb = static_cast<uint32_t>((h->C + SCAN_CHUNK - 1) / SCAN_CHUNK);
         ck(hipMalloc(&h->partial, 4 * std::max<uint32_t>(h->nb, 1)), "alloc scan partials");
         ck(hipMalloc(&h->status, 4), "alloc status");
@@ -1292,10 +1334,10 @@ int mpcd_hip_peek_sort(mpcd_hip_t* h, uint32_t* cell_of_particle, uint32_t* coun
         if (!h->keys_valid) launch_keys(h);
         if (!h->sorted_valid) launch_sort(h);
         ck(hipStreamSynchronize(h->stream), "sync");
-        std::vector<uint2> kr(h->N);
+        std::vector<uint32_t> kr(h->N);
         std::vector<uint32_t> off(h->C + 1), pm(h->N);
         if (h->N) {
-            ck(hipMemcpy(kr.data(), h->keyrank, 8 * h->N, hipMemcpyDeviceToHost), "download keys");
+            ck(hipMemcpy(kr.data(), h->keys, 4 * h->N, hipMemcpyDeviceToHost), "download keys");
             ck(hipMemcpy(pm.data(), h->perm, 4 * h->N, hipMemcpyDeviceToHost), "download perm");
         }
         ck(hipMemcpy(off.data(), h->offset, 4 * (h->C + 1), hipMemcpyDeviceToHost), "download offsets");
@@ -1312,7 +1354,7 @@ int mpcd_hip_peek_sort(mpcd_hip_t* h, uint32_t* cell_of_particle, uint32_t* coun
         std::vector<uint32_t> key_of_ref(h->C);
         for (uint32_t key = 0; key < h->C; ++key) key_of_ref[to_ref(key)] = key;
         if (cell_of_particle)
-            for (uint64_t j = 0; j < h->N; ++j) cell_of_particle[j] = to_ref(kr[j].x);
+            for (uint64_t j = 0; j < h->N; ++j) cell_of_particle[j] = to_ref(kr[j]);
         size_t w = 0;
         for (uint32_t c = 0; c < h->C; ++c) {
             uint32_t const key = key_of_ref[c];

@@ -128,8 +128,8 @@ def test_cell_keys_counts_permutation_bitexact(pympcd, cfg):
     assert np.array_equal(cell, ocell)
     assert np.array_equal(cnt, ocnt)
     cap = 4 * p.n
-    starts = np.concatenate([[0], np.cumsum(cnt)[:-1]])
-    kept = np.concatenate([perm[s:s + min(c, cap)] for s, c in zip(starts, cnt)]) if len(cnt) else perm
+    starts = np.concatenate([[0], np.cumsum(cnt.astype(np.int64))[:-1]]).astype(np.int64)
+    kept = np.concatenate([perm[s:s + min(int(c), cap)] for s, c in zip(starts, cnt)])
     assert np.array_equal(kept, olist)
     # the permutation is a permutation
     assert np.array_equal(np.sort(perm), np.arange(len(perm), dtype=np.uint32))
