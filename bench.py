@@ -38,10 +38,10 @@ CONFIGS = {
 
 
 def algorithmic_bytes(n_particles: int, n_cells: int) -> dict:
-    """Compulsory HBM bytes of each kernel of one step ('Kernels')."""
-    collide = 60 * n_particles + 12 * n_cells   # perm 4 + gather 24 + write 24 + next key/rank 8; offsets 4 + counts 8
-    scan = 16 * n_cells                         # counts read twice, offsets written, counts cleared
-    perm = 12 * n_particles + 4 * n_cells       # key/rank 8 + perm write 4; offsets
+    """Compulsory HBM bytes of each kernel of one step."""
+    collide = 56 * n_particles + 12 * n_cells   # perm 4 + gathered state 24 + state out 24 + next key 4; offsets 4 + count RMW 8
+    scan = 20 * n_cells                         # counts read twice, offsets + cursors written, counts cleared
+    perm = 8 * n_particles + 8 * n_cells        # key 4 + perm slot 4; cursor RMW 8
     return {"collide": collide, "scan": scan, "perm": perm, "step": collide + scan + perm}
 
 
@@ -92,6 +92,8 @@ def main():
     ap.add_argument("--config", default="c2", choices=sorted(CONFIGS))
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--seed", type=int, default=0x5EED0002)
+    ap.add_argument("--copy-gb", type=float, default=0.0,
+                    help="only run the plain copy kernel over this many GB (profiling reference) and exit")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -101,7 +103,14 @@ def main():
         sys.exit(2)
 
     import cudampcd_amd as pympcd
+    from cudampcd_amd import capi
     from cudampcd_amd.capi import Handle
+
+    if args.copy_gb > 0:
+        nbytes = int(args.copy_gb * 1e9) // 16 * 16
+        ms = capi.bench_copy(nbytes, 5)
+        print(json.dumps({"copy_bytes": nbytes, "ms_per_copy": ms, "GBps": 2 * nbytes / ms / 1e6}), flush=True)
+        return
 
     cfg = CONFIGS[args.config]
     p = pympcd.Params()
@@ -128,6 +137,10 @@ def main():
     wall = time.perf_counter() - t0
     collide_ms, total_ms, launches = h.last_timing()
 
+    copy_bytes = 4 * 10 ** 9
+    copy_ms = capi.bench_copy(copy_bytes, 5)
+    copy_gbps = 2 * copy_bytes / copy_ms / 1e6
+
     ms_per_step = wall * 1e3 / args.steps
     value = N * args.steps / wall
     ab = algorithmic_bytes(N, C)
@@ -144,6 +157,8 @@ def main():
         "traffic": traffic.get("bytes_per_launch") if traffic and traffic.get("particles") == N else None,
         "algorithmic_bytes_per_launch": ab["collide"],
         "avg_launch_ms": round(collide_avg_ms, 4),
+        "copy_kernel_GBps": round(copy_gbps, 1),
+        "frac_of_copy": round(achieved / copy_gbps, 4),
     }
     per_step = 120 + 20 / cfg["n"]
     step_roofline = {

@@ -61,6 +61,7 @@ def lib() -> ctypes.CDLL:
             "mpcd_hip_last_timing": [_vp, _f64p, _f64p, ctypes.POINTER(ctypes.c_int)],
             "mpcd_hip_set_tiling": [_vp, ctypes.c_uint32, ctypes.c_uint32],
             "mpcd_hip_get_tiling": [_vp, _u32p, _u32p],
+            "mpcd_hip_bench_copy": [ctypes.c_size_t, ctypes.c_int, _f64p],
             "mpcd_hip_selftest_philox": [ctypes.c_uint32, ctypes.c_uint32, _u32p, _u32p, ctypes.c_size_t],
             "mpcd_hip_selftest_math": [_f32p, _f32p, ctypes.c_size_t, ctypes.c_int],
         }
@@ -141,6 +142,14 @@ class Handle:
         b = ctypes.c_uint32()
         self._check(self.L.mpcd_hip_get_tiling(self.h, ctypes.byref(a), ctypes.byref(b)))
         return a.value, b.value
+
+
+def bench_copy(nbytes: int, reps: int = 10) -> float:
+    """Milliseconds per device-to-device copy of nbytes (read nbytes + write nbytes)."""
+    ms = ctypes.c_double()
+    if lib().mpcd_hip_bench_copy(nbytes, reps, ctypes.byref(ms)) != 0:
+        raise RuntimeError("copy benchmark failed")
+    return ms.value
 
 
 def selftest_philox(k0: int, k1: int, ctr: np.ndarray) -> np.ndarray:

@@ -49,15 +49,16 @@ struct KParams {
     float L[3];          // volume edges (wall layers included)
     float half[3];       // L / 2
     uint32_t dim[3];     // X, Y, Z
-    uint32_t stride[3];  // internal cell-key stride of axis x, y, z
-    uint32_t D0, D1;     // internal fastest / middle extent
     uint32_t ord[3];     // axis at internal position 0 (fastest), 1, 2
+    uint32_t D0, D1, D2; // extents of the internal axes
+    uint32_t Bsh;        // rows of the slowest axis per block = 1 << Bsh
+    uint32_t nblk, Blast, BLK;  // blocks, rows of the last block, cells per full block
+    uint32_t mBLK[2], mD0[2], mBlast[2];  // division magics (multiplier, shift)
     uint32_t C;          // cells
     int perio_z;
     float dt, drag_dt, tv, sin_a, cos_a;
     uint32_t n, cap;     // particles per cell, cell table capacity 4n
     Stream rs;
-    float axes[13][3];   // extended collision directions (simulation_functions.cpp:272-290)
     // per launch
     float shift[3], shift_next[3];
     uint32_t step;
@@ -65,13 +66,42 @@ struct KParams {
     uint64_t N;
 };
 
+// the 13 directions of the extended rule (simulation_functions.cpp:272-290),
+// evaluated on the host with the reference's float expressions
+__constant__ float c_axes[13][3];
+
 struct SampleBufs {
     uint32_t* cnt;                  // per reference cell
     unsigned long long* sum;        // 3 per reference cell, fixed point 2^40, two's complement
 };
 
 // ------------------------------------------------------------ helpers ---
-__device__ __forceinline__ float wrap1(float r, float L) { return fmodf(r + 1.5f * L, L) - L * 0.5f; }
+
+// periodic wrap fmodf(r + 1.5 L, L) - L/2 (simulation_functions.cpp:46-51).
+// fmod is exact; for x = r + 1.5 L in [0, 3L) it equals x, x - L or x - 2L,
+// and those subtractions are exact (Sterbenz), so the fast path returns the
+// same bits as fmodf; anything else takes fmodf itself.
+__device__ __forceinline__ float wrap1(float r, float L) {
+    float const x = r + 1.5f * L;
+    float m;
+    if (x >= 0.0f && x < L) {
+        m = x;
+    } else if (x >= L && x < 2.0f * L) {
+        m = x - L;
+    } else if (x >= 2.0f * L && x - 2.0f * L < L) {
+        m = x - 2.0f * L;
+    } else {
+        m = fmodf(x, L);
+    }
+    return m - L * 0.5f;
+}
+
+// u % D with the common cases as a compare and a subtraction
+__device__ __forceinline__ uint32_t mod_dim(uint32_t u, uint32_t D) {
+    if (u < D) return u;
+    if (u - D < D) return u - D;
+    return u % D;
+}
 
 // streaming with z-wall bounce-back (simulation_functions.cpp:56-74), then
 // shift and periodic wrap (:84)
@@ -111,21 +141,61 @@ __device__ __forceinline__ void stream_shift(KParams const& P, float const* sh, 
 // cell coordinates (volume_container.hpp:69-75)
 __device__ __forceinline__ void cell_of(KParams const& P, float px, float py, float pz, uint32_t& ix, uint32_t& iy,
                                         uint32_t& iz) {
-    ix = static_cast<uint32_t>(floorf(px + P.half[0])) % P.dim[0];
-    iy = static_cast<uint32_t>(floorf(py + P.half[1])) % P.dim[1];
-    iz = static_cast<uint32_t>(floorf(pz + P.half[2])) % P.dim[2];
+    ix = mod_dim(static_cast<uint32_t>(floorf(px + P.half[0])), P.dim[0]);
+    iy = mod_dim(static_cast<uint32_t>(floorf(py + P.half[1])), P.dim[1]);
+    iz = mod_dim(static_cast<uint32_t>(floorf(pz + P.half[2])), P.dim[2]);
 }
-__device__ __forceinline__ uint32_t key_of(KParams const& P, uint32_t ix, uint32_t iy, uint32_t iz) {
-    return ix * P.stride[0] + iy * P.stride[1] + iz * P.stride[2];
+
+// Internal cell key: axes ordered by extent (i0 fastest); the slowest axis is
+// cut into blocks of 2^Bsh rows, and inside a block the order is (i1, row, i0),
+// so a cell's neighbours along all three axes lie within one block.
+__host__ __device__ __forceinline__ uint32_t encode_key(KParams const& P, uint32_t i0, uint32_t i1, uint32_t i2) {
+    uint32_t const b = i2 >> P.Bsh;
+    uint32_t const r2 = i2 - (b << P.Bsh);
+    uint32_t const Bb = b + 1 == P.nblk ? P.Blast : (1u << P.Bsh);
+    return b * P.BLK + (i1 * Bb + r2) * P.D0 + i0;
 }
-__device__ __forceinline__ uint32_t ref_of(KParams const& P, uint32_t ix, uint32_t iy, uint32_t iz) {
+// a / d for 32-bit a by a precomputed (multiplier, shift) pair
+// (Granlund & Montgomery, "Division by invariant integers using
+// multiplication", PLDI 1994: m = floor(2^32 (2^l - d) / d) + 1, l = ceil(log2 d)).
+__host__ __device__ __forceinline__ uint32_t udiv(uint32_t a, uint32_t const* mg) {
+    uint32_t const t = static_cast<uint32_t>((static_cast<uint64_t>(a) * mg[0]) >> 32);
+    return (t + ((a - t) >> 1)) >> mg[1];
+}
+inline void udiv_magic(uint32_t d, uint32_t* mg) {
+    if (d <= 1) {  // not representable; callers divide by 1 without udiv
+        mg[0] = 0;
+        mg[1] = 0;
+        return;
+    }
+    uint32_t l = 0;
+    while ((1ull << l) < d) ++l;
+    mg[0] = static_cast<uint32_t>(((1ull << 32) * ((1ull << l) - d)) / d + 1);
+    mg[1] = l - 1;
+}
+
+__host__ __device__ __forceinline__ void decode_key3(KParams const& P, uint32_t key, uint32_t& i0, uint32_t& i1,
+                                                     uint32_t& i2) {
+    uint32_t const b = P.mBLK[0] || P.mBLK[1] ? udiv(key, P.mBLK) : key / P.BLK;
+    uint32_t const rem = key - b * P.BLK;
+    bool const last = b + 1 == P.nblk;
+    uint32_t const Bb = last ? P.Blast : (1u << P.Bsh);
+    uint32_t const t = P.D0 > 1 ? udiv(rem, P.mD0) : rem;
+    i0 = rem - t * P.D0;
+    i1 = last ? (P.Blast > 1 ? udiv(t, P.mBlast) : t) : (t >> P.Bsh);
+    i2 = (b << P.Bsh) + (t - i1 * Bb);
+}
+__host__ __device__ __forceinline__ uint32_t key_of(KParams const& P, uint32_t ix, uint32_t iy, uint32_t iz) {
+    uint32_t const i[3] = {ix, iy, iz};
+    return encode_key(P, i[P.ord[0]], i[P.ord[1]], i[P.ord[2]]);
+}
+__host__ __device__ __forceinline__ uint32_t ref_of(KParams const& P, uint32_t ix, uint32_t iy, uint32_t iz) {
     return ix + P.dim[0] * (iy + P.dim[1] * iz);
 }
-__device__ __forceinline__ void decode_key(KParams const& P, uint32_t key, uint32_t& ix, uint32_t& iy, uint32_t& iz) {
-    uint32_t const q0 = key % P.D0;
-    uint32_t const k1 = key / P.D0;
-    uint32_t const q1 = k1 % P.D1;
-    uint32_t const q2 = k1 / P.D1;
+__host__ __device__ __forceinline__ void decode_key(KParams const& P, uint32_t key, uint32_t& ix, uint32_t& iy,
+                                                    uint32_t& iz) {
+    uint32_t q0, q1, q2;
+    decode_key3(P, key, q0, q1, q2);
     ix = P.ord[0] == 0 ? q0 : (P.ord[1] == 0 ? q1 : q2);
     iy = P.ord[0] == 1 ? q0 : (P.ord[1] == 1 ? q1 : q2);
     iz = P.ord[0] == 2 ? q0 : (P.ord[1] == 2 ? q1 : q2);
@@ -396,47 +466,143 @@ __device__ __forceinline__ Sym sym_inverse(Sym const& m, float tol) {
     return o;
 }
 
-// LDS carve-up of the collide kernel
-struct Lds {
-    uint32_t* perm;   // [cap] source index, arrival order
-    uint16_t* cell;   // [cap] tile-local cell of each slot
-    float* p[6];      // [cap] x y z vx vy vz, cell order
-    uint32_t* off;    // [tile_cells + 1]
-    float* cm[6];     // [tile_cells] mean velocity xyz, axis xyz
-    uint32_t* go;     // [tile_cells] collision happened
-    float* g[6];      // [cap/2] ghosts of wall cells (extended), slot k*n + i
-    uint32_t* bc;     // [4] broadcast
+// LDS tile of the collide kernel: up to CAP particles of whole cells, up to
+// TCM cells.  Sizes are compile-time so every LDS access is an immediate offset.
+constexpr int TCM = 64;
+
+template <int CAP, int ALGO>
+struct Tile {
+    float p[6][CAP];                 // x y z vx vy vz, cell order
+    uint32_t perm[CAP];              // source slot, arrival order; later scratch
+    float cm[6][TCM];                // per cell: mean velocity xyz, axis xyz
+    uint32_t off[TCM + 1];           // cell offsets of the tile
+    uint32_t go[TCM];                // collision happened
+    uint32_t cref[TCM];              // reference cell index
+    uint32_t cflag[TCM];             // bit 0 wall cell with ghosts, bit 1 upper wall layer
+    uint32_t gcnt[TCM];              // ghost count
+    uint32_t goff[TCM];              // first ghost slot
+    uint32_t sw[WG / 64];            // scan scratch
+    uint16_t cell[CAP];              // tile-local cell of each slot
+    float g[ALGO == 1 ? 6 : 1][ALGO == 1 ? CAP / 2 : 1];  // extended: ghosts, slot k*n + i
 };
 
-__host__ __device__ inline size_t lds_bytes(uint32_t cap, uint32_t tc, int algo) {
-    size_t b = 0;
-    b += 4ull * cap + 2ull * cap + 4ull * 6 * cap;
-    b += 4ull * (tc + 1) + 4ull * 6 * tc + 4ull * tc;
-    if (algo == 1) b += 4ull * 6 * (cap / 2);
-    b += 16;
-    return (b + 15) & ~size_t(15);
-}
-
-__device__ __forceinline__ Lds carve(unsigned char* base, uint32_t cap, uint32_t tc, int algo) {
-    Lds s;
-    unsigned char* q = base;
-    s.perm = reinterpret_cast<uint32_t*>(q); q += 4ull * cap;
-    for (int k = 0; k < 6; ++k) { s.p[k] = reinterpret_cast<float*>(q); q += 4ull * cap; }
-    s.off = reinterpret_cast<uint32_t*>(q); q += 4ull * (tc + 1);
-    for (int k = 0; k < 6; ++k) { s.cm[k] = reinterpret_cast<float*>(q); q += 4ull * tc; }
-    s.go = reinterpret_cast<uint32_t*>(q); q += 4ull * tc;
-    if (algo == 1) {
-        for (int k = 0; k < 6; ++k) { s.g[k] = reinterpret_cast<float*>(q); q += 4ull * (cap / 2); }
-    } else {
-        for (int k = 0; k < 6; ++k) s.g[k] = nullptr;
+// SRD per-cell stage of one batch (simulation_functions.cpp:133-182), called by
+// the whole workgroup: wall-layer ghost counts and ghost velocities are drawn
+// in parallel, then one lane per cell sums its particles in slot order and
+// its ghosts in ghost order (the reference's order) and draws the axis.
+template <int CAP>
+__device__ void srd_cells(KParams const& P, Tile<CAP, 0>& s, uint32_t key0, uint32_t ncell, uint32_t const* off,
+                          uint32_t gb) {
+    uint32_t const tid = threadIdx.x;
+    float const sh = fabsf(P.shift[2]);
+    bool const sign = P.shift[2] > 0;
+    for (uint32_t k = tid; k < ncell; k += WG) {
+        uint32_t ix, iy, iz;
+        decode_key(P, key0 + k, ix, iy, iz);
+        s.cref[k] = ref_of(P, ix, iy, iz);
+        float const zf = static_cast<float>(iz);
+        bool const layer = (zf == (P.L[2] - (sign ? 1.0f : 2.0f)));
+        bool const ghosts = (!P.perio_z) && ((zf == static_cast<float>(sign)) || layer);
+        s.cflag[k] = (ghosts ? 1u : 0u) | (layer ? 2u : 0u);
+        s.gcnt[k] = 0u;
     }
-    s.bc = reinterpret_cast<uint32_t*>(q); q += 16;
-    s.cell = reinterpret_cast<uint16_t*>(q);
-    return s;
+    __syncthreads();
+    // ghost counts (simulation_functions.cpp:141-144): one wave per wall cell,
+    // each lane draws four of the cell's n uniforms
+    uint32_t const wave = tid >> 6, lane = tid & 63u;
+    for (uint32_t k = wave; k < ncell; k += WG / 64) {
+        uint32_t const f = s.cflag[k];
+        if (f & 1u) {
+            bool const layer = (f & 2u) != 0;
+            uint32_t const ref = s.cref[k];
+            for (uint32_t j = lane; 4 * j < P.n; j += 64) {
+                Words const r = P.rs.at(P_GHOST_COUNT, j, ref, P.step);
+                uint32_t c = 0;
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+                    if (4 * j + q < P.n && (((uniform(r.w[q]) > sh) ^ layer) ^ sign)) ++c;
+                if (c) atomicAdd(s.gcnt + k, c);
+            }
+        }
+    }
+    __syncthreads();
+    uint32_t tot;
+    uint32_t const pre = block_exclusive<WG>(tid < ncell ? s.gcnt[tid] : 0u, s.sw, &tot);
+    if (tid < ncell) s.goff[tid] = pre;
+    __syncthreads();
+    // ghost velocities (Maxwell-Boltzmann times sqrt(T)) into the permutation
+    // scratch, which is free once the particles are gathered: one wave per
+    // wall cell, one lane per ghost
+    float* const gv = reinterpret_cast<float*>(s.perm);
+    bool const stored = 3 * tot <= static_cast<uint32_t>(CAP);
+    if (stored) {
+        for (uint32_t k = wave; k < ncell; k += WG / 64) {
+            uint32_t const ng = s.gcnt[k];
+            uint32_t const ref = s.cref[k];
+            uint32_t const base = 3 * s.goff[k];
+            for (uint32_t g = lane; g < ng; g += 64) {
+                Words const v = P.rs.at(P_GHOST_VEL, g, ref, P.step);
+                float g0, g1, g2, g3;
+                box_muller(v.w[0], v.w[1], &g0, &g1);
+                box_muller(v.w[2], v.w[3], &g2, &g3);
+                uint32_t const o = base + 3 * g;
+                gv[o] = g0 * P.tv;
+                gv[o + 1] = g1 * P.tv;
+                gv[o + 2] = g2 * P.tv;
+            }
+        }
+        __syncthreads();
+    }
+    for (uint32_t k = tid; k < ncell; k += WG) {
+        uint32_t const a = off[k] - gb, mc = off[k + 1] - gb - a;
+        uint32_t const reals = mc < P.cap ? mc : P.cap;
+        uint32_t const added = s.gcnt[k];
+        uint32_t const total = reals + added;
+        uint32_t const ref = s.cref[k];
+        s.go[k] = total > 1;
+        if (total > 1) {
+            float sx = 0, sy = 0, sz = 0;
+            for (uint32_t i = 0; i < reals; ++i) {
+                sx = sx + s.p[3][a + i];
+                sy = sy + s.p[4][a + i];
+                sz = sz + s.p[5][a + i];
+            }
+            for (uint32_t g = 0; g < added; ++g) {
+                float g0, g1, g2;
+                if (stored) {
+                    uint32_t const o = 3 * (s.goff[k] + g);
+                    g0 = gv[o];
+                    g1 = gv[o + 1];
+                    g2 = gv[o + 2];
+                } else {
+                    Words const v = P.rs.at(P_GHOST_VEL, g, ref, P.step);
+                    float h0, h1, h2, h3;
+                    box_muller(v.w[0], v.w[1], &h0, &h1);
+                    box_muller(v.w[2], v.w[3], &h2, &h3);
+                    g0 = h0 * P.tv;
+                    g1 = h1 * P.tv;
+                    g2 = h2 * P.tv;
+                }
+                sx = sx + g0;
+                sy = sy + g1;
+                sz = sz + g2;
+            }
+            float const inv = 1.0f / static_cast<float>(total);
+            s.cm[0][k] = sx * inv;
+            s.cm[1][k] = sy * inv;
+            s.cm[2][k] = sz * inv;
+            float ax, ay, az;
+            srd_axis(P, ref, ax, ay, az);
+            s.cm[3][k] = ax;
+            s.cm[4][k] = ay;
+            s.cm[5][k] = az;
+        }
+    }
 }
 
 // extended collision of one cell by one lane (simulation_functions.cpp:207-406)
-__device__ void extended_cell(KParams const& P, Lds const& s, uint32_t k, uint32_t a, uint32_t mc, uint32_t ref,
+template <int CAP>
+__device__ void extended_cell(KParams const& P, Tile<CAP, 1>& s, uint32_t k, uint32_t a, uint32_t mc, uint32_t ref,
                               uint32_t ix, uint32_t iy, uint32_t iz) {
     float const sh = fabsf(P.shift[2]);
     bool const sign = P.shift[2] > 0;
@@ -478,7 +644,7 @@ __device__ void extended_cell(KParams const& P, Lds const& s, uint32_t k, uint32
     Words const e = P.rs.at(P_EXT, 0, ref, P.step);
     int const select = static_cast<int>(e.w[0] / 2) % 13;
     int const flip = static_cast<int>(e.w[1] / 2) % 2;
-    float ax = P.axes[select][0], ay = P.axes[select][1], az = P.axes[select][2];
+    float ax = c_axes[select][0], ay = c_axes[select][1], az = c_axes[select][2];
     if (flip) { ax = -ax; ay = -ay; az = -az; }
 
     float cx = 0, cy = 0, cz = 0, mx = 0, my = 0, mz = 0;
@@ -585,18 +751,16 @@ __device__ void extended_cell(KParams const& P, Lds const& s, uint32_t k, uint32
     }
 }
 
-constexpr int MAXP = 8;             // lds_cap <= MAXP * WG
 constexpr int GQ = 4;               // particles gathered per thread per round
 
-template <int ALGO, bool SAMPLE>
-__global__ __launch_bounds__(WG) void k_collide(KParams P, const float* __restrict__ in, float* __restrict__ out,
+template <int ALGO, bool SAMPLE, int CAP>
+__global__ __launch_bounds__(WG, (ALGO == 0 ? 4 : 3)) void k_collide(KParams P, const float* __restrict__ in, float* __restrict__ out,
                                                 const uint32_t* __restrict__ perm, const uint32_t* __restrict__ offset,
                                                 uint32_t* __restrict__ count_next, uint32_t* __restrict__ keys,
                                                 uint32_t* __restrict__ perm_out, SampleBufs S,
                                                 uint32_t* __restrict__ status) {
-    extern __shared__ __align__(16) unsigned char lds_raw[];
-    uint32_t const cap = P.lds_cap, TC = P.tile_cells;
-    Lds const s = carve(lds_raw, cap, TC, ALGO);
+    __shared__ Tile<CAP, ALGO> s;
+    uint32_t const cap = CAP, TC = P.tile_cells;
     uint32_t const tid = threadIdx.x;
     uint64_t const N = P.N;
     uint32_t const tile = xcd_tile(blockIdx.x, gridDim.x);
@@ -680,8 +844,12 @@ __global__ __launch_bounds__(WG) void k_collide(KParams P, const float* __restri
                     uint32_t const k = s.cell[i];
                     uint32_t const a = s.off[kb + k] - gb, e = s.off[kb + k + 1] - gb;
                     uint32_t const src = srcs[q];
-                    uint32_t r = 0;
-                    for (uint32_t j = a; j < e; ++j) r += s.perm[j] < src;
+                    uint32_t r = 0, j = a;
+                    for (; j + 4 <= e; j += 4) {
+                        uint32_t const p0 = s.perm[j], p1 = s.perm[j + 1], p2 = s.perm[j + 2], p3 = s.perm[j + 3];
+                        r += (p0 < src) + (p1 < src) + (p2 < src) + (p3 < src);
+                    }
+                    for (; j < e; ++j) r += s.perm[j] < src;
                     uint32_t const d = a + r;
                     float px = g[q][0], py = g[q][1], pz = g[q][2];
                     float vx = g[q][3], vy = g[q][4], vz = g[q][5];
@@ -693,53 +861,26 @@ __global__ __launch_bounds__(WG) void k_collide(KParams P, const float* __restri
             }
         }
         __syncthreads();
-        // per-cell stage, one lane per cell
-        for (uint32_t k = tid; k < ncell; k += WG) {
-            uint32_t const key = c0 + kb + k;
-            uint32_t ix, iy, iz;
-            decode_key(P, key, ix, iy, iz);
-            uint32_t const ref = ref_of(P, ix, iy, iz);
-            uint32_t const a = s.off[kb + k] - gb, mc = s.off[kb + k + 1] - gb - a;
-            if (ALGO == 0) {
-                float const sh = fabsf(P.shift[2]);
-                bool const sign = P.shift[2] > 0;
-                float const zf = static_cast<float>(iz);
-                bool const layer = (zf == (P.L[2] - (sign ? 1.0f : 2.0f)));
-                bool const ghosts = (!P.perio_z) && ((zf == static_cast<float>(sign)) || layer);
-                uint32_t const reals = mc < P.cap ? mc : P.cap;
-                uint32_t const added = ghosts ? ghost_count(P, ref, sh, layer, sign) : 0u;
-                uint32_t const total = reals + added;
-                s.go[k] = total > 1;
-                if (total > 1) {
-                    float sx = 0, sy = 0, sz = 0;
-                    for (uint32_t i = 0; i < reals; ++i) {
-                        sx = sx + s.p[3][a + i];
-                        sy = sy + s.p[4][a + i];
-                        sz = sz + s.p[5][a + i];
-                    }
-                    for (uint32_t gi = 0; gi < added; ++gi) {
-                        Words const v = P.rs.at(P_GHOST_VEL, gi, ref, P.step);
-                        float g0, g1, g2, g3;
-                        box_muller(v.w[0], v.w[1], &g0, &g1);
-                        box_muller(v.w[2], v.w[3], &g2, &g3);
-                        sx = sx + g0 * P.tv;
-                        sy = sy + g1 * P.tv;
-                        sz = sz + g2 * P.tv;
-                    }
-                    float const inv = 1.0f / static_cast<float>(total);
-                    s.cm[0][k] = sx * inv;
-                    s.cm[1][k] = sy * inv;
-                    s.cm[2][k] = sz * inv;
-                    float ax, ay, az;
-                    srd_axis(P, ref, ax, ay, az);
-                    s.cm[3][k] = ax;
-                    s.cm[4][k] = ay;
-                    s.cm[5][k] = az;
-                }
-            } else {
-                extended_cell(P, s, k, a, mc, ref, ix, iy, iz);
+        // per-cell stage
+        if constexpr (ALGO == 0) {
+            srd_cells<CAP>(P, s, c0 + kb, ncell, s.off + kb, gb);
+        } else {
+            for (uint32_t k = tid; k < ncell; k += WG) {
+                uint32_t ix, iy, iz;
+                decode_key(P, c0 + kb + k, ix, iy, iz);
+                uint32_t const ref = ref_of(P, ix, iy, iz);
+                uint32_t const a = s.off[kb + k] - gb, mc = s.off[kb + k + 1] - gb - a;
+                extended_cell<CAP>(P, s, k, a, mc, ref, ix, iy, iz);
             }
         }
+        __syncthreads();
+        // next step's cell counts: keys near this batch's cells go to an LDS
+        // window (the free permutation scratch), the rest straight to memory
+        uint32_t* const hist = s.perm;
+        uint32_t const span = ke - kb;
+        uint32_t const wlo = (CAP - span) / 2;
+        uint32_t const wb = c0 + kb > wlo ? c0 + kb - wlo : 0u;
+        for (uint32_t i = tid; i < static_cast<uint32_t>(CAP); i += WG) hist[i] = 0u;
         __syncthreads();
         // per-particle finish, write-out, next-step cell count, sampling
         for (uint32_t i0 = 0; i0 < m; i0 += WG) {
@@ -790,8 +931,17 @@ __global__ __launch_bounds__(WG) void k_collide(KParams P, const float* __restri
                 cell_of(P, px, py, pz, ix, iy, iz);
                 knext = key_of(P, ix, iy, iz);
             }
-            wave_count(count_next, knext, valid);
-            if (valid) keys[static_cast<uint64_t>(gb) + i] = knext;
+            if (valid) {
+                keys[static_cast<uint64_t>(gb) + i] = knext;
+                uint32_t const w = knext - wb;
+                if (w < static_cast<uint32_t>(CAP)) atomicAdd(hist + w, 1u);
+                else atomicAdd(count_next + knext, 1u);
+            }
+        }
+        __syncthreads();
+        for (uint32_t i = tid; i < static_cast<uint32_t>(CAP); i += WG) {
+            uint32_t const c = hist[i];
+            if (c) atomicAdd(count_next + wb + i, c);
         }
         __syncthreads();
         kb = ke;
@@ -821,6 +971,13 @@ __global__ __launch_bounds__(WG) void k_sample_fold(uint32_t C, uint32_t* __rest
             for (int k = 0; k < 3; ++k) acc[4 * c + 1 + k] += m[k];
         }
     }
+}
+
+// plain device copy, the practical HBM reference point for the roofline
+__global__ __launch_bounds__(WG) void k_copy(const float4* __restrict__ src, float4* __restrict__ dst, uint64_t n) {
+    for (uint64_t i = blockIdx.x * static_cast<uint64_t>(WG) + threadIdx.x; i < n;
+         i += static_cast<uint64_t>(gridDim.x) * WG)
+        dst[i] = src[i];
 }
 
 __global__ void k_selftest_philox(uint32_t k0, uint32_t k1, const uint32_t* ctr, uint32_t* out, size_t n) {
@@ -917,16 +1074,21 @@ void fill_geometry(mpcd_hip_s* h) {
         k.dim[a] = static_cast<uint32_t>(p.volume_size[a]);
     }
     k.C = k.dim[0] * k.dim[1] * k.dim[2];
-    // internal cell order: smallest extent fastest, so the cells a particle
-    // can move between lie as close together in memory as possible
+    // internal cell order: smallest extent fastest, slowest axis in blocks of
+    // 16 rows, so the cells a particle can move between lie close in memory
     uint32_t ord[3] = {0, 1, 2};
     std::stable_sort(ord, ord + 3, [&](uint32_t x, uint32_t y) { return k.dim[x] < k.dim[y]; });
     for (int a = 0; a < 3; ++a) k.ord[a] = ord[a];
     k.D0 = k.dim[ord[0]];
     k.D1 = k.dim[ord[1]];
-    k.stride[ord[0]] = 1;
-    k.stride[ord[1]] = k.D0;
-    k.stride[ord[2]] = k.D0 * k.D1;
+    k.D2 = k.dim[ord[2]];
+    k.Bsh = 4;
+    k.nblk = (k.D2 + (1u << k.Bsh) - 1) >> k.Bsh;
+    k.Blast = k.D2 - ((k.nblk - 1) << k.Bsh);
+    k.BLK = (1u << k.Bsh) * k.D1 * k.D0;
+    udiv_magic(k.BLK, k.mBLK);
+    udiv_magic(k.D0, k.mD0);
+    udiv_magic(k.Blast, k.mBlast);
     k.perio_z = p.periodicity[2];
     k.dt = p.delta_t;
     k.drag_dt = p.drag * p.delta_t;
@@ -938,8 +1100,16 @@ void fill_geometry(mpcd_hip_s* h) {
     k.rs.k0 = static_cast<uint32_t>(p.seed);
     k.rs.k1 = static_cast<uint32_t>(p.seed >> 32);
     k.rs.sys = p.system_id;
-    // the 13 directions of the extended rule (simulation_functions.cpp:273-290),
-    // evaluated with the same float expressions on the host
+    k.N = h->N;
+}
+
+// fills c_axes on the current device (once per device)
+void upload_axes(int device) {
+    static std::mutex mu;
+    static std::vector<int> done;
+    std::lock_guard<std::mutex> g(mu);
+    if (std::find(done.begin(), done.end(), device) != done.end()) return;
+    float axes[13][3];
     for (int sel = 0; sel < 13; ++sel) {
         int const steps = 4;
         int const phi_i = sel % steps + 1, theta_i = sel / steps + 1;
@@ -949,25 +1119,29 @@ void fill_geometry(mpcd_hip_s* h) {
         else
             theta = theta_i * (float(M_PI) / steps);
         float const phi = phi_i * (float(M_PI) / steps);
-        k.axes[sel][0] = sinf(theta) * cosf(phi);
-        k.axes[sel][1] = sinf(theta) * sinf(phi);
-        k.axes[sel][2] = cosf(theta);
+        axes[sel][0] = sinf(theta) * cosf(phi);
+        axes[sel][1] = sinf(theta) * sinf(phi);
+        axes[sel][2] = cosf(theta);
     }
-    k.N = h->N;
+    ck(hipMemcpyToSymbol(HIP_SYMBOL(c_axes), axes, sizeof(axes)), "upload direction table");
+    done.push_back(device);
 }
 
-constexpr size_t LDS_BUDGET = 64 * 1024;  // dynamic LDS per workgroup used by k_collide
+// LDS tile capacities compiled into k_collide (particles per tile)
+constexpr uint32_t TILE_CAPS[3] = {512, 1152, 2048};
 
 void choose_tiling(mpcd_hip_s* h, uint32_t tc, uint32_t cap) {
     uint32_t const n = std::max<uint32_t>(1, h->prm.n);
-    int const algo = h->prm.algorithm;
-    if (cap == 0) cap = h->lds_cap ? h->lds_cap : (algo == 1 ? 1024u : 1280u);
-    cap = std::max<uint32_t>(cap, 64);
-    cap = std::min<uint32_t>(cap, MAXP * WG);
+    if (cap == 0) cap = h->lds_cap ? h->lds_cap : 1152u;
+    // the smallest compiled capacity that holds the request
+    uint32_t chosen = TILE_CAPS[2];
+    for (uint32_t c : TILE_CAPS)
+        if (c >= cap) { chosen = c; break; }
+    cap = chosen;
     if (tc == 0) tc = h->tile_cells ? h->tile_cells : std::max<uint32_t>(1, (cap / 2) / n);
     // ghosts of a batch (at most n per cell) must fit the cap/2 ghost region
     tc = std::min<uint32_t>(tc, std::max<uint32_t>(1, (cap / 2) / n));
-    tc = std::min<uint32_t>(tc, TC_MAX);
+    tc = std::min<uint32_t>(tc, static_cast<uint32_t>(TCM));
     h->tile_cells = tc;
     h->lds_cap = cap;
 }
@@ -1015,12 +1189,21 @@ void launch_sort(mpcd_hip_s* h) {
     h->launches += 4;
 }
 
-template <int ALGO, bool SAMPLE>
-void launch_collide_t(mpcd_hip_s* h, KParams const& k, size_t lds) {
+template <int ALGO, bool SAMPLE, int CAP>
+void launch_collide_t(mpcd_hip_s* h, KParams const& k) {
     uint32_t* rec = h->record ? h->perm_rec : nullptr;
     SampleBufs S{h->scnt, h->ssum};
-    hipLaunchKernelGGL((k_collide<ALGO, SAMPLE>), dim3(k.ntiles), dim3(WG), lds, h->stream, k, h->st[h->cur],
+    hipLaunchKernelGGL((k_collide<ALGO, SAMPLE, CAP>), dim3(k.ntiles), dim3(WG), 0, h->stream, k, h->st[h->cur],
                        h->st[1 - h->cur], h->perm, h->offset, h->count, h->keys, rec, S, h->status);
+}
+
+template <int ALGO, bool SAMPLE>
+void launch_collide_cap(mpcd_hip_s* h, KParams const& k) {
+    switch (h->lds_cap) {
+        case 512: launch_collide_t<ALGO, SAMPLE, 512>(h, k); break;
+        case 1152: launch_collide_t<ALGO, SAMPLE, 1152>(h, k); break;
+        default: launch_collide_t<ALGO, SAMPLE, 2048>(h, k); break;
+    }
 }
 
 void ensure_sampling(mpcd_hip_s* h) {
@@ -1038,7 +1221,6 @@ void run_steps(mpcd_hip_s* h, int n_steps, bool sample) {
     if (sample) ensure_sampling(h);
     if (h->record && !h->perm_rec) ck(hipMalloc(&h->perm_rec, 4 * std::max<uint64_t>(h->N, 1)), "alloc perm record");
     int const algo = h->prm.algorithm;
-    size_t const lds = lds_bytes(h->lds_cap, h->tile_cells, algo);
     uint32_t const ntiles = static_cast<uint32_t>((h->C + h->tile_cells - 1) / h->tile_cells);
     if (h->timing) {
         if (h->ev.size() < 2 * static_cast<size_t>(n_steps)) {
@@ -1065,9 +1247,9 @@ void run_steps(mpcd_hip_s* h, int n_steps, bool sample) {
         k.ntiles = ntiles;
         if (h->timing) ck(hipEventRecord(h->ev[2 * s], h->stream), "event");
         if (algo == 0) {
-            if (sample) launch_collide_t<0, true>(h, k, lds); else launch_collide_t<0, false>(h, k, lds);
+            if (sample) launch_collide_cap<0, true>(h, k); else launch_collide_cap<0, false>(h, k);
         } else {
-            if (sample) launch_collide_t<1, true>(h, k, lds); else launch_collide_t<1, false>(h, k, lds);
+            if (sample) launch_collide_cap<1, true>(h, k); else launch_collide_cap<1, false>(h, k);
         }
         ck(hipGetLastError(), "k_collide");
         if (h->timing) ck(hipEventRecord(h->ev[2 * s + 1], h->stream), "event");
@@ -1196,6 +1378,7 @@ int mpcd_hip_create(const mpcd_params_t* params, mpcd_hip_t** out) {
         ck(hipSetDevice(h->device), "hipSetDevice");
         ck(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking), "stream");
         fill_geometry(h);
+        upload_axes(h->device);
         choose_tiling(h, 0, 0);
         uint64_t const N = std::max<uint64_t>(h->N, 1);
         ck(hipMalloc(&h->st[0], 24 * N), "alloc state");
@@ -1343,13 +1526,9 @@ int mpcd_hip_peek_sort(mpcd_hip_t* h, uint32_t* cell_of_particle, uint32_t* coun
         ck(hipMemcpy(off.data(), h->offset, 4 * (h->C + 1), hipMemcpyDeviceToHost), "download offsets");
         KParams const& k = h->kp;
         auto to_ref = [&](uint32_t key) {
-            uint32_t q[3];
-            q[0] = key % k.D0;
-            q[1] = (key / k.D0) % k.D1;
-            q[2] = (key / k.D0) / k.D1;
-            uint32_t i[3];
-            for (int a = 0; a < 3; ++a) i[k.ord[a]] = q[a];
-            return i[0] + k.dim[0] * (i[1] + k.dim[1] * i[2]);
+            uint32_t ix, iy, iz;
+            decode_key(k, key, ix, iy, iz);
+            return ref_of(k, ix, iy, iz);
         };
         std::vector<uint32_t> key_of_ref(h->C);
         for (uint32_t key = 0; key < h->C; ++key) key_of_ref[to_ref(key)] = key;
@@ -1396,14 +1575,10 @@ int mpcd_hip_last_timing(const mpcd_hip_t* h, double* collide_ms, double* total_
 
 int mpcd_hip_set_tiling(mpcd_hip_t* h, uint32_t cells_per_tile, uint32_t lds_capacity) {
     return guarded(h, [&] {
-        uint32_t const old_tc = h->tile_cells, old_cap = h->lds_cap;
+        if (lds_capacity > TILE_CAPS[2])
+            throw HipError{"tile capacity above " + std::to_string(TILE_CAPS[2]) + " particles"};
         h->tile_cells = 0;
-        choose_tiling(h, cells_per_tile, lds_capacity ? lds_capacity : old_cap);
-        if (lds_bytes(h->lds_cap, h->tile_cells, h->prm.algorithm) > LDS_BUDGET) {
-            h->tile_cells = old_tc;
-            h->lds_cap = old_cap;
-            throw HipError{"tile needs more than 64 KiB of LDS"};
-        }
+        choose_tiling(h, cells_per_tile, lds_capacity ? lds_capacity : h->lds_cap);
     });
 }
 
@@ -1412,6 +1587,35 @@ int mpcd_hip_get_tiling(const mpcd_hip_t* h, uint32_t* cells_per_tile, uint32_t*
     if (cells_per_tile) *cells_per_tile = h->tile_cells;
     if (lds_capacity) *lds_capacity = h->lds_cap;
     return 0;
+}
+
+int mpcd_hip_bench_copy(size_t bytes, int reps, double* ms_per_copy) {
+    if (!ms_per_copy || reps <= 0 || bytes < 16) return 1;
+    uint64_t const n = bytes / 16;
+    float4 *a = nullptr, *b = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = 0;
+    if (hipMalloc(&a, 16 * n) != hipSuccess) return 1;
+    if (hipMalloc(&b, 16 * n) != hipSuccess) { (void)hipFree(a); return 1; }
+    rc |= hipMemset(a, 0, 16 * n) != hipSuccess;
+    rc |= hipEventCreate(&e0) != hipSuccess;
+    rc |= hipEventCreate(&e1) != hipSuccess;
+    if (!rc) {
+        uint32_t const blocks = static_cast<uint32_t>(std::min<uint64_t>((n + WG - 1) / WG, 256ull * 64));
+        hipLaunchKernelGGL(k_copy, dim3(blocks), dim3(WG), 0, 0, a, b, n);
+        rc |= hipEventRecord(e0, 0) != hipSuccess;
+        for (int r = 0; r < reps; ++r) hipLaunchKernelGGL(k_copy, dim3(blocks), dim3(WG), 0, 0, a, b, n);
+        rc |= hipEventRecord(e1, 0) != hipSuccess;
+        rc |= hipEventSynchronize(e1) != hipSuccess;
+        float ms = 0;
+        rc |= hipEventElapsedTime(&ms, e0, e1) != hipSuccess;
+        *ms_per_copy = ms / reps;
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipFree(a);
+    (void)hipFree(b);
+    return rc;
 }
 
 int mpcd_hip_selftest_philox(uint32_t k0, uint32_t k1, const uint32_t* ctr, uint32_t* out, size_t n) {

@@ -107,10 +107,14 @@ int mpcd_hip_enable_timing(mpcd_hip_t* h, int enable);
 /* Device-side timing of the last step() call: milliseconds of the collision
  * kernels and of the whole step sequence, from HIP events on the handle's stream. */
 int mpcd_hip_last_timing(const mpcd_hip_t* h, double* collide_ms, double* total_ms, int* launches);
-/* Kernel launch shape: cells per tile and LDS particle capacity per tile
- * (0 = keep).  Results do not depend on it. */
+/* Kernel launch shape: cells per tile (at most 64) and LDS particle capacity
+ * per tile (rounded up to 512, 1152 or 2048; 0 = keep).  Results do not
+ * depend on it. */
 int mpcd_hip_set_tiling(mpcd_hip_t* h, uint32_t cells_per_tile, uint32_t lds_capacity);
 int mpcd_hip_get_tiling(const mpcd_hip_t* h, uint32_t* cells_per_tile, uint32_t* lds_capacity);
+/* Copies `bytes` between two device buffers `reps` times with a plain float4
+ * kernel and returns the mean time per copy: the practical HBM reference. */
+int mpcd_hip_bench_copy(size_t bytes, int reps, double* ms_per_copy);
 /* Device math of the random stream, for known-answer tests: Philox4x32-10 of
  * n counters (4 words each), and the float transforms. */
 int mpcd_hip_selftest_philox(uint32_t k0, uint32_t k1, const uint32_t* ctr, uint32_t* out, size_t n);
