@@ -92,6 +92,8 @@ def main():
     ap.add_argument("--config", default="c2", choices=sorted(CONFIGS))
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--seed", type=int, default=0x5EED0002)
+    ap.add_argument("--tiling", default="",
+                    help="cells_per_tile,lds_capacity for k_collide (results do not depend on it)")
     ap.add_argument("--copy-gb", type=float, default=0.0,
                     help="only run the plain copy kernel over this many GB (profiling reference) and exit")
     args = ap.parse_args()
@@ -127,6 +129,9 @@ def main():
     sim = pympcd.Simulation(p, "hip")
     h = Handle(sim)
     N, C = h.N, h.C
+    if args.tiling:
+        tc, cap = (int(x) for x in args.tiling.split(","))
+        h.set_tiling(tc, cap)
 
     print(f"[bench] warmup {args.warmup} steps", file=sys.stderr, flush=True)
     sim.step(args.warmup)

@@ -466,6 +466,51 @@ __device__ __forceinline__ Sym sym_inverse(Sym const& m, float tol) {
     return o;
 }
 
+// Per-cell draws of the SRD rule, computed once per step by fully populated
+// waves instead of inside the particle tiles:
+//   axis[a][key]   random rotation axis of every cell (random.hpp:237-253)
+//   gcnt[w]        ghost count of wall cell w (simulation_functions.cpp:141-146)
+//   gvel[w][g][a]  ghost velocities, Maxwell-Boltzmann times sqrt(T) (:153-166)
+// Wall cells: the two z layers next to the walls for this step's shift,
+// w = (upper ? X*Y : 0) + x + X*y.
+__global__ __launch_bounds__(WG) void k_cell_axes(KParams P, float* __restrict__ axis) {
+    uint32_t const C = P.C;
+    for (uint32_t key = blockIdx.x * WG + threadIdx.x; key < C; key += gridDim.x * WG) {
+        uint32_t ix, iy, iz;
+        decode_key(P, key, ix, iy, iz);
+        float ax, ay, az;
+        srd_axis(P, ref_of(P, ix, iy, iz), ax, ay, az);
+        axis[key] = ax;
+        axis[C + key] = ay;
+        axis[2ull * C + key] = az;
+    }
+}
+
+__global__ __launch_bounds__(WG) void k_wall_ghosts(KParams P, uint32_t* __restrict__ gcnt, float* __restrict__ gvel) {
+    uint32_t const XY = P.dim[0] * P.dim[1];
+    float const sh = fabsf(P.shift[2]);
+    bool const sign = P.shift[2] > 0;
+    uint32_t const iz_layer = static_cast<uint32_t>(P.L[2] - (sign ? 1.0f : 2.0f));
+    for (uint32_t w = blockIdx.x * WG + threadIdx.x; w < 2 * XY; w += gridDim.x * WG) {
+        bool const layer = w >= XY;
+        uint32_t const xy = layer ? w - XY : w;
+        uint32_t const iz = layer ? iz_layer : static_cast<uint32_t>(sign);
+        uint32_t const ref = xy + XY * iz;
+        uint32_t const added = ghost_count(P, ref, sh, layer, sign);
+        gcnt[w] = added;
+        float* const out = gvel + 3ull * P.n * w;
+        for (uint32_t g = 0; g < added; ++g) {
+            Words const v = P.rs.at(P_GHOST_VEL, g, ref, P.step);
+            float g0, g1, g2, g3;
+            box_muller(v.w[0], v.w[1], &g0, &g1);
+            box_muller(v.w[2], v.w[3], &g2, &g3);
+            out[3 * g] = g0 * P.tv;
+            out[3 * g + 1] = g1 * P.tv;
+            out[3 * g + 2] = g2 * P.tv;
+        }
+    }
+}
+
 // LDS tile of the collide kernel: up to CAP particles of whole cells, up to
 // TCM cells.  Sizes are compile-time so every LDS access is an immediate offset.
 constexpr int TCM = 64;
@@ -483,82 +528,25 @@ struct Tile {
     uint32_t goff[TCM];              // first ghost slot
     uint32_t sw[WG / 64];            // scan scratch
     uint16_t cell[CAP];              // tile-local cell of each slot
-    float g[ALGO == 1 ? 6 : 1][ALGO == 1 ? CAP / 2 : 1];  // extended: ghosts, slot k*n + i
+    float g[ALGO == 1 ? 6 : 1][ALGO == 1 ? CAP / 2 : CAP / 2];  // extended: ghosts, slot k*n + i;
+                                                               // SRD: staged ghost velocities
+    uint32_t nwall;                  // SRD: wall cells of the batch staged so far
 };
 
-// SRD per-cell stage of one batch (simulation_functions.cpp:133-182), called by
-// the whole workgroup: wall-layer ghost counts and ghost velocities are drawn
-// in parallel, then one lane per cell sums its particles in slot order and
-// its ghosts in ghost order (the reference's order) and draws the axis.
+// SRD per-cell stage of one batch (simulation_functions.cpp:133-182): one lane
+// per cell sums its particles in slot order and its ghosts in ghost order (the
+// reference's order).  Ghost counts and velocities of wall cells (drawn by
+// k_wall_ghosts) were staged into LDS while the particles were gathered:
+// s.gcnt[k] ghosts at s.g[0][s.goff[k] ...], or, when the staging region was
+// full (s.goff[k] == ~0u), read from memory at gvel + 3 n s.cref[k].
 template <int CAP>
-__device__ void srd_cells(KParams const& P, Tile<CAP, 0>& s, uint32_t key0, uint32_t ncell, uint32_t const* off,
-                          uint32_t gb) {
-    uint32_t const tid = threadIdx.x;
-    float const sh = fabsf(P.shift[2]);
-    bool const sign = P.shift[2] > 0;
-    for (uint32_t k = tid; k < ncell; k += WG) {
-        uint32_t ix, iy, iz;
-        decode_key(P, key0 + k, ix, iy, iz);
-        s.cref[k] = ref_of(P, ix, iy, iz);
-        float const zf = static_cast<float>(iz);
-        bool const layer = (zf == (P.L[2] - (sign ? 1.0f : 2.0f)));
-        bool const ghosts = (!P.perio_z) && ((zf == static_cast<float>(sign)) || layer);
-        s.cflag[k] = (ghosts ? 1u : 0u) | (layer ? 2u : 0u);
-        s.gcnt[k] = 0u;
-    }
-    __syncthreads();
-    // ghost counts (simulation_functions.cpp:141-144): one wave per wall cell,
-    // each lane draws four of the cell's n uniforms
-    uint32_t const wave = tid >> 6, lane = tid & 63u;
-    for (uint32_t k = wave; k < ncell; k += WG / 64) {
-        uint32_t const f = s.cflag[k];
-        if (f & 1u) {
-            bool const layer = (f & 2u) != 0;
-            uint32_t const ref = s.cref[k];
-            for (uint32_t j = lane; 4 * j < P.n; j += 64) {
-                Words const r = P.rs.at(P_GHOST_COUNT, j, ref, P.step);
-                uint32_t c = 0;
-#pragma unroll
-                for (int q = 0; q < 4; ++q)
-                    if (4 * j + q < P.n && (((uniform(r.w[q]) > sh) ^ layer) ^ sign)) ++c;
-                if (c) atomicAdd(s.gcnt + k, c);
-            }
-        }
-    }
-    __syncthreads();
-    uint32_t tot;
-    uint32_t const pre = block_exclusive<WG>(tid < ncell ? s.gcnt[tid] : 0u, s.sw, &tot);
-    if (tid < ncell) s.goff[tid] = pre;
-    __syncthreads();
-    // ghost velocities (Maxwell-Boltzmann times sqrt(T)) into the permutation
-    // scratch, which is free once the particles are gathered: one wave per
-    // wall cell, one lane per ghost
-    float* const gv = reinterpret_cast<float*>(s.perm);
-    bool const stored = 3 * tot <= static_cast<uint32_t>(CAP);
-    if (stored) {
-        for (uint32_t k = wave; k < ncell; k += WG / 64) {
-            uint32_t const ng = s.gcnt[k];
-            uint32_t const ref = s.cref[k];
-            uint32_t const base = 3 * s.goff[k];
-            for (uint32_t g = lane; g < ng; g += 64) {
-                Words const v = P.rs.at(P_GHOST_VEL, g, ref, P.step);
-                float g0, g1, g2, g3;
-                box_muller(v.w[0], v.w[1], &g0, &g1);
-                box_muller(v.w[2], v.w[3], &g2, &g3);
-                uint32_t const o = base + 3 * g;
-                gv[o] = g0 * P.tv;
-                gv[o + 1] = g1 * P.tv;
-                gv[o + 2] = g2 * P.tv;
-            }
-        }
-        __syncthreads();
-    }
-    for (uint32_t k = tid; k < ncell; k += WG) {
+__device__ void srd_cells(KParams const& P, Tile<CAP, 0>& s, uint32_t ncell, uint32_t const* off, uint32_t gb,
+                          float const* __restrict__ gvel) {
+    for (uint32_t k = threadIdx.x; k < ncell; k += WG) {
         uint32_t const a = off[k] - gb, mc = off[k + 1] - gb - a;
         uint32_t const reals = mc < P.cap ? mc : P.cap;
         uint32_t const added = s.gcnt[k];
         uint32_t const total = reals + added;
-        uint32_t const ref = s.cref[k];
         s.go[k] = total > 1;
         if (total > 1) {
             float sx = 0, sy = 0, sz = 0;
@@ -567,35 +555,19 @@ __device__ void srd_cells(KParams const& P, Tile<CAP, 0>& s, uint32_t key0, uint
                 sy = sy + s.p[4][a + i];
                 sz = sz + s.p[5][a + i];
             }
-            for (uint32_t g = 0; g < added; ++g) {
-                float g0, g1, g2;
-                if (stored) {
-                    uint32_t const o = 3 * (s.goff[k] + g);
-                    g0 = gv[o];
-                    g1 = gv[o + 1];
-                    g2 = gv[o + 2];
-                } else {
-                    Words const v = P.rs.at(P_GHOST_VEL, g, ref, P.step);
-                    float h0, h1, h2, h3;
-                    box_muller(v.w[0], v.w[1], &h0, &h1);
-                    box_muller(v.w[2], v.w[3], &h2, &h3);
-                    g0 = h0 * P.tv;
-                    g1 = h1 * P.tv;
-                    g2 = h2 * P.tv;
+            if (added) {
+                uint32_t const go = s.goff[k];
+                float const* const g = go != ~0u ? &s.g[0][go] : gvel + 3ull * P.n * s.cref[k];
+                for (uint32_t i = 0; i < added; ++i) {
+                    sx = sx + g[3 * i];
+                    sy = sy + g[3 * i + 1];
+                    sz = sz + g[3 * i + 2];
                 }
-                sx = sx + g0;
-                sy = sy + g1;
-                sz = sz + g2;
             }
             float const inv = 1.0f / static_cast<float>(total);
             s.cm[0][k] = sx * inv;
             s.cm[1][k] = sy * inv;
             s.cm[2][k] = sz * inv;
-            float ax, ay, az;
-            srd_axis(P, ref, ax, ay, az);
-            s.cm[3][k] = ax;
-            s.cm[4][k] = ay;
-            s.cm[5][k] = az;
         }
     }
 }
@@ -758,7 +730,8 @@ __global__ __launch_bounds__(WG, (ALGO == 0 ? 4 : 3)) void k_collide(KParams P, 
                                                 const uint32_t* __restrict__ perm, const uint32_t* __restrict__ offset,
                                                 uint32_t* __restrict__ count_next, uint32_t* __restrict__ keys,
                                                 uint32_t* __restrict__ perm_out, SampleBufs S,
-                                                uint32_t* __restrict__ status) {
+                                                uint32_t* __restrict__ status, float const* __restrict__ axis,
+                                                uint32_t const* __restrict__ gcnt, float const* __restrict__ gvel) {
     __shared__ Tile<CAP, ALGO> s;
     uint32_t const cap = CAP, TC = P.tile_cells;
     uint32_t const tid = threadIdx.x;
@@ -815,6 +788,35 @@ __global__ __launch_bounds__(WG, (ALGO == 0 ? 4 : 3)) void k_collide(KParams P, 
         }
         uint32_t const ncell = ke - kb;
         for (uint32_t i = tid; i < m; i += WG) s.perm[i] = perm[gb + i];
+        if constexpr (ALGO == 0) {
+            // per-cell draws: axis, and for wall cells the ghost count and a
+            // slot in the staging region (simulation_functions.cpp:137-146)
+            if (tid == 0) s.nwall = 0u;
+            __syncthreads();
+            bool const sign = P.shift[2] > 0;
+            uint32_t const XY = P.dim[0] * P.dim[1];
+            for (uint32_t k = tid; k < ncell; k += WG) {
+                uint32_t const key = c0 + kb + k;
+                s.cm[3][k] = axis[key];
+                s.cm[4][k] = axis[P.C + key];
+                s.cm[5][k] = axis[2ull * P.C + key];
+                uint32_t ix, iy, iz;
+                decode_key(P, key, ix, iy, iz);
+                float const zf = static_cast<float>(iz);
+                bool const layer = (zf == (P.L[2] - (sign ? 1.0f : 2.0f)));
+                bool const ghosts = (!P.perio_z) && ((zf == static_cast<float>(sign)) || layer);
+                uint32_t gc = 0u, go = ~0u;
+                if (ghosts) {
+                    uint32_t const w = (layer ? XY : 0u) + ix + P.dim[0] * iy;
+                    s.cref[k] = w;
+                    gc = gcnt[w];
+                    uint32_t const slot = atomicAdd(&s.nwall, 1u);
+                    if ((slot + 1) * 3 * P.n <= static_cast<uint32_t>(CAP / 2)) go = slot * 3 * P.n;
+                }
+                s.gcnt[k] = gc;
+                s.goff[k] = go;
+            }
+        }
         for (uint32_t k = tid; k < ncell; k += WG) {
             uint32_t const a = s.off[kb + k] - gb, e = s.off[kb + k + 1] - gb;
             for (uint32_t i = a; i < e; ++i) s.cell[i] = static_cast<uint16_t>(k);
@@ -860,10 +862,22 @@ __global__ __launch_bounds__(WG, (ALGO == 0 ? 4 : 3)) void k_collide(KParams P, 
                 }
             }
         }
+        if constexpr (ALGO == 0) {
+            // stage the wall cells' ghost velocities, one wave per cell
+            uint32_t const wave = tid >> 6, lane = tid & 63u;
+            for (uint32_t k = wave; k < ncell; k += WG / 64) {
+                uint32_t const go = s.goff[k];
+                if (go != ~0u) {
+                    uint32_t const nf = 3 * s.gcnt[k];
+                    float const* const src = gvel + 3ull * P.n * s.cref[k];
+                    for (uint32_t f = lane; f < nf; f += 64) s.g[0][go + f] = src[f];
+                }
+            }
+        }
         __syncthreads();
         // per-cell stage
         if constexpr (ALGO == 0) {
-            srd_cells<CAP>(P, s, c0 + kb, ncell, s.off + kb, gb);
+            srd_cells<CAP>(P, s, ncell, s.off + kb, gb, gvel);
         } else {
             for (uint32_t k = tid; k < ncell; k += WG) {
                 uint32_t ix, iy, iz;
@@ -1046,6 +1060,9 @@ struct mpcd_hip_s {
     uint32_t* status = nullptr;
     uint32_t* perm_rec = nullptr;
     bool record = false, recorded = false;
+    float* axis = nullptr;           // SRD per-cell axes of the current step
+    uint32_t* wgcnt = nullptr;       // SRD wall-cell ghost counts
+    float* wgvel = nullptr;          // SRD wall-cell ghost velocities
     // sampling
     uint32_t* scnt = nullptr;
     unsigned long long* ssum = nullptr;
@@ -1138,9 +1155,12 @@ void choose_tiling(mpcd_hip_s* h, uint32_t tc, uint32_t cap) {
     for (uint32_t c : TILE_CAPS)
         if (c >= cap) { chosen = c; break; }
     cap = chosen;
-    if (tc == 0) tc = h->tile_cells ? h->tile_cells : std::max<uint32_t>(1, (cap / 2) / n);
-    // ghosts of a batch (at most n per cell) must fit the cap/2 ghost region
-    tc = std::min<uint32_t>(tc, std::max<uint32_t>(1, (cap / 2) / n));
+    // extended: ghosts of a batch (at most n per cell) must fit the cap/2 ghost
+    // region; SRD: about 80% of the tile's particle capacity on average
+    uint32_t const tc_max = h->prm.algorithm == 1 ? std::max<uint32_t>(1, (cap / 2) / n)
+                                                  : std::max<uint32_t>(1, (cap * 4 / 5) / n);
+    if (tc == 0) tc = h->tile_cells ? h->tile_cells : tc_max;
+    tc = std::min<uint32_t>(tc, tc_max);
     tc = std::min<uint32_t>(tc, static_cast<uint32_t>(TCM));
     h->tile_cells = tc;
     h->lds_cap = cap;
@@ -1194,7 +1214,21 @@ void launch_collide_t(mpcd_hip_s* h, KParams const& k) {
     uint32_t* rec = h->record ? h->perm_rec : nullptr;
     SampleBufs S{h->scnt, h->ssum};
     hipLaunchKernelGGL((k_collide<ALGO, SAMPLE, CAP>), dim3(k.ntiles), dim3(WG), 0, h->stream, k, h->st[h->cur],
-                       h->st[1 - h->cur], h->perm, h->offset, h->count, h->keys, rec, S, h->status);
+                       h->st[1 - h->cur], h->perm, h->offset, h->count, h->keys, rec, S, h->status, h->axis,
+                       h->wgcnt, h->wgvel);
+}
+
+// per-cell draws of this step (SRD)
+void launch_cell_draws(mpcd_hip_s* h, KParams const& k) {
+    uint32_t const C = static_cast<uint32_t>(h->C);
+    uint32_t const blocks = std::min<uint32_t>((C + WG - 1) / WG, 1u << 18);
+    hipLaunchKernelGGL(k_cell_axes, dim3(blocks), dim3(WG), 0, h->stream, k, h->axis);
+    if (!h->prm.periodicity[2]) {
+        uint32_t const nw = 2 * k.dim[0] * k.dim[1];
+        hipLaunchKernelGGL(k_wall_ghosts, dim3((nw + WG - 1) / WG), dim3(WG), 0, h->stream, k, h->wgcnt, h->wgvel);
+    }
+    ck(hipGetLastError(), "cell draws");
+    h->launches += h->prm.periodicity[2] ? 1 : 2;
 }
 
 template <int ALGO, bool SAMPLE>
@@ -1245,6 +1279,7 @@ void run_steps(mpcd_hip_s* h, int n_steps, bool sample) {
         k.tile_cells = h->tile_cells;
         k.lds_cap = h->lds_cap;
         k.ntiles = ntiles;
+        if (algo == 0) launch_cell_draws(h, k);
         if (h->timing) ck(hipEventRecord(h->ev[2 * s], h->stream), "event");
         if (algo == 0) {
             if (sample) launch_collide_cap<0, true>(h, k); else launch_collide_cap<0, false>(h, k);
@@ -1324,6 +1359,7 @@ void destroy(mpcd_hip_s* h) {
     for (void* p : {static_cast<void*>(h->perm), static_cast<void*>(h->keys), static_cast<void*>(h->cursor), static_cast<void*>(h->count),
                     static_cast<void*>(h->offset), static_cast<void*>(h->partial), static_cast<void*>(h->status),
                     static_cast<void*>(h->perm_rec), static_cast<void*>(h->scnt), static_cast<void*>(h->ssum),
+                    static_cast<void*>(h->axis), static_cast<void*>(h->wgcnt), static_cast<void*>(h->wgvel),
                     static_cast<void*>(h->acc)})
         if (p) (void)hipFree(p);
     for (auto e : h->ev) (void)hipEventDestroy(e);
@@ -1388,6 +1424,12 @@ int mpcd_hip_create(const mpcd_params_t* params, mpcd_hip_t** out) {
         ck(hipMalloc(&h->count, 4 * h->C), "alloc counts");
         ck(hipMalloc(&h->offset, 4 * (h->C + 1)), "alloc offsets");
         ck(hipMalloc(&h->cursor, 4 * h->C), "alloc cursors");
+        if (p.algorithm == 0) {
+            uint64_t const XY = static_cast<uint64_t>(h->kp.dim[0]) * h->kp.dim[1];
+            ck(hipMalloc(&h->axis, 12 * h->C), "alloc axes");
+            ck(hipMalloc(&h->wgcnt, 4 * 2 * XY), "alloc ghost counts");
+            ck(hipMalloc(&h->wgvel, 12ull * p.n * 2 * XY), "alloc ghost velocities");
+        }
         h->nb = static_cast<uint32_t>((h->C + SCAN_CHUNK - 1) / SCAN_CHUNK);
         ck(hipMalloc(&h->partial, 4 * std::max<uint32_t>(h->nb, 1)), "alloc scan partials");
         ck(hipMalloc(&h->status, 4), "alloc status");
