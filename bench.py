@@ -74,14 +74,18 @@ def cpu_baseline(n: int, seconds_target: float = 15.0) -> dict:
                       f"{steps} steps after 1 warm-up, single thread"}
 
 
-def load_traffic(name: str):
-    f = ROOT / "profiles" / f"{name}_traffic.json"
-    if f.exists():
-        try:
-            return json.loads(f.read_text())
-        except ValueError:
-            return None
-    return None
+def load_traffic(config: str):
+    """HBM bytes per k_collide launch measured by rocprofv3 FETCH_SIZE / WRITE_SIZE
+    (calibrated against a copy kernel), from the committed profile summary."""
+    f = ROOT / "profiles" / f"k_collide_{config}_traffic.md"
+    if not f.exists():
+        return None
+    out = {}
+    for line in f.read_text().splitlines():
+        key, _, val = line.strip().partition(":")
+        if key in ("particles", "bytes_per_launch") and val.strip().isdigit():
+            out[key] = int(val)
+    return out if len(out) == 2 else None
 
 
 def main():
@@ -151,7 +155,7 @@ def main():
     ab = algorithmic_bytes(N, C)
     collide_avg_ms = collide_ms / args.steps
     achieved = ab["collide"] / (collide_avg_ms * 1e-3) / 1e9
-    traffic = load_traffic(f"collide_{args.config}")
+    traffic = load_traffic(args.config)
     roofline = {
         "bound": "hbm",
         "kernel": "k_collide (gather + stream + sort-in-cell + collide + next-step histogram)",
