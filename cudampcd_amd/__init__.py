@@ -12,11 +12,11 @@
     sim.step_and_sample(10000)
     density, velocity = sim.get_mean_fields()
 
-The native extension must be built (python -m cudampcd_amd.build); importing
+The native extension must be built (python cudampcd_amd/build.py); importing
 without it raises ImportError rather than falling back to anything else.
 """
-from ._pympcd import Params, Simulation, __version__  # noqa: F401
+from ._pympcd import Ensemble, Params, Simulation, __version__  # noqa: F401
 
 from .physics import srd_viscosity, poiseuille_viscosity  # noqa: F401,E402
 
-__all__ = ["Params", "Simulation", "srd_viscosity", "poiseuille_viscosity", "__version__"]
+__all__ = ["Ensemble", "Params", "Simulation", "srd_viscosity", "poiseuille_viscosity", "__version__"]

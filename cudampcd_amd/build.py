@@ -4,7 +4,8 @@
     _pympcd*.so      C++ API (Backend, HipBackend, SimulationHandle) + pybind11
                      module, linked against libmpcd_hip.so
 
-Run ``python -m cudampcd_amd.build`` or call ``build()``.
+Run ``python cudampcd_amd/build.py`` (by path: importing the package needs the
+extension) or call ``build()``.
 """
 from __future__ import annotations
 

@@ -35,7 +35,7 @@ def lib() -> ctypes.CDLL:
     global _lib
     if _lib is None:
         if not LIB_PATH.exists():
-            raise ImportError(f"{LIB_PATH} is not built; run python -m cudampcd_amd.build")
+            raise ImportError(f"{LIB_PATH} is not built; run python cudampcd_amd/build.py")
         L = ctypes.CDLL(str(LIB_PATH))
         L.mpcd_hip_last_error.restype = ctypes.c_char_p
         L.mpcd_hip_last_error.argtypes = [_vp]
@@ -61,6 +61,14 @@ def lib() -> ctypes.CDLL:
             "mpcd_hip_last_timing": [_vp, _f64p, _f64p, ctypes.POINTER(ctypes.c_int)],
             "mpcd_hip_set_tiling": [_vp, ctypes.c_uint32, ctypes.c_uint32],
             "mpcd_hip_get_tiling": [_vp, _u32p, _u32p],
+            "mpcd_hip_n_systems": [_vp, _u32p],
+            "mpcd_hip_system_info": [_vp, ctypes.c_uint32, _u64p, _u64p, _u64p, _u64p],
+            "mpcd_hip_system_get_particles": [_vp, ctypes.c_uint32, _f32p, _f32p, ctypes.c_size_t],
+            "mpcd_hip_system_sample_mean": [_vp, ctypes.c_uint32, _f32p, _f32p, ctypes.c_size_t],
+            "mpcd_hip_system_sample_mean_f64": [_vp, ctypes.c_uint32, _f64p, _f64p, ctypes.c_size_t],
+            "mpcd_hip_reset_sampling": [_vp],
+            "mpcd_hip_use_graph": [_vp, ctypes.c_int],
+            "mpcd_hip_autotune": [_vp, ctypes.c_int, _u32p, _u32p, _f64p],
             "mpcd_hip_bench_copy": [ctypes.c_size_t, ctypes.c_int, _f64p],
             "mpcd_hip_selftest_philox": [ctypes.c_uint32, ctypes.c_uint32, _u32p, _u32p, ctypes.c_size_t],
             "mpcd_hip_selftest_math": [_f32p, _f32p, ctypes.c_size_t, ctypes.c_int],
@@ -81,6 +89,7 @@ class Handle:
     """C-ABI calls on the handle of a live Simulation."""
 
     def __init__(self, sim):
+        """sim: a Simulation or an Ensemble of this module."""
         self.sim = sim  # keeps the simulation alive
         self.h = ctypes.c_void_p(sim._handle())
         self.L = lib()
@@ -142,6 +151,28 @@ class Handle:
         b = ctypes.c_uint32()
         self._check(self.L.mpcd_hip_get_tiling(self.h, ctypes.byref(a), ctypes.byref(b)))
         return a.value, b.value
+
+    def use_graph(self, on: bool = True):
+        self._check(self.L.mpcd_hip_use_graph(self.h, int(on)))
+
+    @property
+    def n_systems(self) -> int:
+        v = ctypes.c_uint32()
+        self._check(self.L.mpcd_hip_n_systems(self.h, ctypes.byref(v)))
+        return int(v.value)
+
+    def system_info(self, s: int):
+        """(first particle, particles, first cell, cells) of system s."""
+        v = [ctypes.c_uint64() for _ in range(4)]
+        self._check(self.L.mpcd_hip_system_info(self.h, s, *[ctypes.byref(x) for x in v]))
+        return tuple(int(x.value) for x in v)
+
+    def autotune(self, steps_per_trial: int = 3):
+        a = ctypes.c_uint32()
+        b = ctypes.c_uint32()
+        ms = ctypes.c_double()
+        self._check(self.L.mpcd_hip_autotune(self.h, steps_per_trial, ctypes.byref(a), ctypes.byref(b), ctypes.byref(ms)))
+        return a.value, b.value, ms.value
 
 
 def bench_copy(nbytes: int, reps: int = 10) -> float:

@@ -6,14 +6,12 @@
 
 namespace mpcd::hip {
 
-namespace {
 uint64_t fresh_seed() {
     std::random_device rd;
     return (static_cast<uint64_t>(rd()) << 32) ^ rd();
 }
-}  // namespace
 
-HipBackend::HipBackend(SimulationParameters const& params) : Backend(params) {
+mpcd_params_t to_c_params(SimulationParameters const& params, uint64_t seed) {
     mpcd_params_t p{};
     p.experiment = params.experiment == ExperimentType::standard ? 0 : 1;
     p.algorithm = params.algorithm == MPCDAlgorithm::srd ? 0 : 1;
@@ -30,9 +28,14 @@ HipBackend::HipBackend(SimulationParameters const& params) : Backend(params) {
     p.thermal_velocity = params.thermal_velocity;
     p.drag = params.drag;
     p.rotation_angle = params.rotation_angle;
-    seed_ = params.seed_set ? params.seed : fresh_seed();
-    p.seed = seed_;
+    p.seed = seed;
     p.system_id = params.system_id;
+    return p;
+}
+
+HipBackend::HipBackend(SimulationParameters const& params) : Backend(params) {
+    seed_ = params.seed_set ? params.seed : fresh_seed();
+    mpcd_params_t const p = to_c_params(params, seed_);
     if (mpcd_hip_create(&p, &h_) != 0) throw std::runtime_error(mpcd_hip_last_error(nullptr));
     uint64_t c = 0;
     mpcd_hip_n_cells(h_, &c);
@@ -95,6 +98,79 @@ void HipBackend::setParticleVelocities(std::vector<float> const& velocities) {
 void HipBackend::setParticles(std::vector<float> const& positions, std::vector<float> const& velocities) {
     if (positions.size() != velocities.size()) throw std::runtime_error("positions and velocities differ in length");
     check(mpcd_hip_set_particles(h_, positions.data(), velocities.data(), triples(positions)));
+}
+
+HipEnsemble::HipEnsemble(std::vector<SimulationParameters> const& params) : params_(params) {
+    if (params.empty()) throw std::runtime_error("an ensemble needs at least one system");
+    std::vector<mpcd_params_t> c;
+    for (auto const& p : params) {
+        seeds_.push_back(p.seed_set ? p.seed : fresh_seed());
+        c.push_back(to_c_params(p, seeds_.back()));
+    }
+    if (mpcd_hip_create_ensemble(c.data(), static_cast<uint32_t>(c.size()), &h_) != 0)
+        throw std::runtime_error(mpcd_hip_last_error(nullptr));
+}
+
+HipEnsemble::~HipEnsemble() { mpcd_hip_destroy(h_); }
+
+void HipEnsemble::check(int status) const {
+    if (status != 0) throw std::runtime_error(mpcd_hip_last_error(h_));
+}
+
+void HipEnsemble::step(int n_steps) { check(mpcd_hip_step(h_, n_steps)); }
+
+void HipEnsemble::stepAndAccumulateSample(int n_steps) { check(mpcd_hip_step_and_sample(h_, n_steps)); }
+
+static size_t system_cells(mpcd_hip_t* h, uint32_t i) {
+    uint64_t c = 0;
+    if (mpcd_hip_system_info(h, i, nullptr, nullptr, nullptr, &c) != 0) throw std::runtime_error("system index out of range");
+    return c;
+}
+
+void HipEnsemble::getSampleMean(uint32_t i, std::vector<float>& density, std::vector<float>& velocity) {
+    size_t const c = system_cells(h_, i);
+    density.resize(c);
+    velocity.resize(3 * c);
+    check(mpcd_hip_system_sample_mean(h_, i, density.data(), velocity.data(), c));
+}
+
+void HipEnsemble::getSampleMeanF64(uint32_t i, std::vector<double>& density, std::vector<double>& velocity) {
+    size_t const c = system_cells(h_, i);
+    density.resize(c);
+    velocity.resize(3 * c);
+    check(mpcd_hip_system_sample_mean_f64(h_, i, density.data(), velocity.data(), c));
+}
+
+void HipEnsemble::resetSampling() { check(mpcd_hip_reset_sampling(h_)); }
+
+size_t HipEnsemble::nParticles(uint32_t i) const {
+    uint64_t n = 0;
+    if (mpcd_hip_system_info(h_, i, nullptr, &n, nullptr, nullptr) != 0) throw std::runtime_error("system index out of range");
+    return n;
+}
+
+SimulationParameters const& HipEnsemble::parameters(uint32_t i) const {
+    if (i >= params_.size()) throw std::runtime_error("system index out of range");
+    return params_[i];
+}
+
+uint64_t HipEnsemble::seed(uint32_t i) const {
+    if (i >= seeds_.size()) throw std::runtime_error("system index out of range");
+    return seeds_[i];
+}
+
+void HipEnsemble::getParticles(uint32_t i, std::vector<float>* positions, std::vector<float>* velocities) {
+    size_t const n = nParticles(i);
+    if (positions) positions->resize(3 * n);
+    if (velocities) velocities->resize(3 * n);
+    check(mpcd_hip_system_get_particles(h_, i, positions ? positions->data() : nullptr,
+                                        velocities ? velocities->data() : nullptr, n));
+}
+
+void HipEnsemble::useGraph(bool enable) { check(mpcd_hip_use_graph(h_, enable ? 1 : 0)); }
+
+void HipEnsemble::autotune(int steps_per_trial, uint32_t& cells_per_tile, uint32_t& lds_capacity, double& ms_per_step) {
+    check(mpcd_hip_autotune(h_, steps_per_trial, &cells_per_tile, &lds_capacity, &ms_per_step));
 }
 
 }  // namespace mpcd::hip

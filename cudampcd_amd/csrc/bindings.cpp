@@ -7,7 +7,8 @@
 // periodicity, N recomputed, thermal_velocity = sqrt(temperature)).
 // Additions: Params.rotation_angle / seed / system_id, and Simulation
 // set_particle_* (the reference binds none), get_mean_fields_f64, seed,
-// n_particles and the raw C-ABI handle for the parity hooks.
+// n_particles and the raw C-ABI handle for the parity hooks; Ensemble, a set
+// of independent systems advanced together (no reference counterpart).
 //
 // Deliberate fix: get_mean_fields uses strides (4, 4X, 4XY) for shape (X, Y, Z);
 // the reference uses (4, 4Y, 4XY) (bindings.cpp:214-216), which equals it only
@@ -59,6 +60,15 @@ mpcd::hip::HipBackend& hip_of(SimulationHandle& s) {
     auto* b = dynamic_cast<mpcd::hip::HipBackend*>(&s.backend());
     if (!b) throw std::runtime_error("not a hip backend");
     return *b;
+}
+
+template <typename T>
+py::tuple mean_fields_t(SimulationParameters const& p, std::vector<T>&& density, std::vector<T>&& velocity) {
+    ssize_t const x = p.volume_size[0], y = p.volume_size[1], z = p.volume_size[2];
+    ssize_t const sf = static_cast<ssize_t>(sizeof(T));
+    auto d = to_numpy(std::move(density), {x, y, z}, {sf, x * sf, x * y * sf});
+    auto v = to_numpy(std::move(velocity), {x, y, z, 3}, {3 * sf, 3 * x * sf, 3 * x * y * sf, sf});
+    return py::make_tuple(d, v);
 }
 
 py::tuple mean_fields(SimulationParameters const& p, std::vector<float>&& density, std::vector<float>&& velocity) {
@@ -132,6 +142,60 @@ PYBIND11_MODULE(_pympcd, m) {
         .def_property_readonly("seed", [](SimulationHandle& self) { return hip_of(self).seed(); })
         .def("_handle", [](SimulationHandle& self) { return reinterpret_cast<uintptr_t>(hip_of(self).handle()); },
              "Address of the mpcd_hip_t handle, for calls into the C-ABI.");
+
+    using mpcd::hip::HipEnsemble;
+    py::class_<HipEnsemble>(m, "Ensemble",
+                            "Independent systems (one Params each) advanced together on one device; system i "
+                            "reproduces Simulation(params[i], 'hip') with the same seed bit for bit")
+        .def(py::init<std::vector<SimulationParameters> const&>(), py::arg("params"))
+        .def("step", &HipEnsemble::step, py::call_guard<py::gil_scoped_release>())
+        .def("step_and_sample", &HipEnsemble::stepAndAccumulateSample, py::call_guard<py::gil_scoped_release>())
+        .def_property_readonly("n_systems", &HipEnsemble::nSystems)
+        .def("n_particles", &HipEnsemble::nParticles)
+        .def("seed", &HipEnsemble::seed)
+        .def("get_mean_fields",
+             [](HipEnsemble& self, uint32_t i) {
+                 std::vector<float> d, v;
+                 self.getSampleMean(i, d, v);
+                 return mean_fields_t(self.parameters(i), std::move(d), std::move(v));
+             },
+             "Mean density (X, Y, Z) and velocity (X, Y, Z, 3) of system i; the average continues until reset_sampling().")
+        .def("get_mean_fields_f64",
+             [](HipEnsemble& self, uint32_t i) {
+                 std::vector<double> d, v;
+                 self.getSampleMeanF64(i, d, v);
+                 return mean_fields_t(self.parameters(i), std::move(d), std::move(v));
+             })
+        .def("reset_sampling", &HipEnsemble::resetSampling)
+        .def("get_particle_positions",
+             [](HipEnsemble& self, uint32_t i) {
+                 std::vector<float> pos;
+                 self.getParticles(i, &pos, nullptr);
+                 ssize_t const n = pos.size() / 3;
+                 return to_numpy(std::move(pos), {n, 3}, {12, 4});
+             })
+        .def("get_particle_velocities",
+             [](HipEnsemble& self, uint32_t i) {
+                 std::vector<float> vel;
+                 self.getParticles(i, nullptr, &vel);
+                 ssize_t const n = vel.size() / 3;
+                 return to_numpy(std::move(vel), {n, 3}, {12, 4});
+             })
+        .def("use_graph", &HipEnsemble::useGraph, py::arg("enable") = true)
+        .def("autotune",
+             [](HipEnsemble& self, int steps_per_trial) {
+                 uint32_t tc = 0, cap = 0;
+                 double ms = 0;
+                 {
+                     py::gil_scoped_release release;
+                     self.autotune(steps_per_trial, tc, cap, ms);
+                 }
+                 return py::make_tuple(tc, cap, ms);
+             },
+             py::arg("steps_per_trial") = 3,
+             "Times the compiled tile shapes on real steps (the systems advance) and keeps the fastest; "
+             "returns (cells per tile of system 0, LDS capacity, ms per step).")
+        .def("_handle", [](HipEnsemble& self) { return reinterpret_cast<uintptr_t>(self.handle()); });
 
     py::class_<SimulationParameters>(m, "Params", "Class representing simulation parameters")
         .def(py::init<>())

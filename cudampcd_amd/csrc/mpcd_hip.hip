@@ -42,9 +42,13 @@
 namespace mpcd_hip {
 
 constexpr int WG = 256;          // threads per workgroup (4 wave64)
-constexpr int TC_MAX = 256;      // cells per tile upper bound
 constexpr double FIX_SCALE = 1099511627776.0;   // 2^40, fixed-point velocity sums
 
+// Parameters of one system.  A handle holds a table of them in device memory
+// (one entry per ensemble member; a plain simulation is a table of one), and
+// every kernel reads the entry of the system its block works on.  All systems
+// share flattened arrays: system s owns particle slots [pb, pb + N), cell keys
+// [cb, cb + C) and wall cells [wb, wb + 2 X Y).
 struct KParams {
     float L[3];          // volume edges (wall layers included)
     float half[3];       // L / 2
@@ -54,17 +58,42 @@ struct KParams {
     uint32_t Bsh;        // rows of the slowest axis per block = 1 << Bsh
     uint32_t nblk, Blast, BLK;  // blocks, rows of the last block, cells per full block
     uint32_t mBLK[2], mD0[2], mBlast[2];  // division magics (multiplier, shift)
-    uint32_t C;          // cells
+    uint32_t C;          // cells of this system
     int perio_z;
     float dt, drag_dt, tv, sin_a, cos_a;
     uint32_t n, cap;     // particles per cell, cell table capacity 4n
     Stream rs;
-    // per launch
-    float shift[3], shift_next[3];
+    uint32_t tile_cells; // cells per collide tile
+    uint32_t N;          // particles of this system
+    uint32_t cb, pb, wb; // first cell key, particle slot, wall cell in the shared arrays
+    uint64_t gvb;        // first ghost-velocity float of this system
+    // step state: written by the host at set-up, advanced on the device by k_advance
     uint32_t step;
-    uint32_t tile_cells, lds_cap, ntiles;
-    uint64_t N;
+    float shift[3], shift_next[3];
 };
+
+// The system table and the block -> system maps of one handle.
+struct Sys {
+    KParams* P;                  // [ns]
+    uint32_t const* blk_part;    // [ns + 1] first block of each system in per-particle kernels
+    uint32_t const* blk_cell;    // [ns + 1] ... in per-cell kernels
+    uint32_t const* blk_wall;    // [ns + 1] ... in per-wall-cell kernels
+    uint16_t const* tile_sys;    // [tiles] system of each collide tile (ns > 1)
+    uint32_t const* tile0;       // [ns + 1] first collide tile of each system
+    uint32_t ns;
+    uint32_t CT;                 // cells of all systems (stride of the axis planes)
+    uint64_t NT;                 // particles of all systems (stride of the state planes)
+};
+
+// system whose block range holds block b (base is non-decreasing, base[0] = 0)
+__device__ __forceinline__ uint32_t find_sys(uint32_t const* base, uint32_t ns, uint32_t b) {
+    uint32_t lo = 0, hi = ns;
+    while (hi - lo > 1) {
+        uint32_t const mid = (lo + hi) >> 1;
+        if (base[mid] <= b) lo = mid; else hi = mid;
+    }
+    return lo;
+}
 
 // the 13 directions of the extended rule (simulation_functions.cpp:272-290),
 // evaluated on the host with the reference's float expressions
@@ -249,43 +278,56 @@ __device__ __forceinline__ uint32_t xcd_tile(uint32_t b, uint32_t G) {
 // ------------------------------------------------------------- kernels ---
 
 // seeded initial distribution (simulation_functions.cpp:7-33): positions
-// uniform in the fluid region, Maxwell-Boltzmann velocities
-__global__ __launch_bounds__(WG) void k_init(KParams P, float* __restrict__ st, float sx, float sy, float sz) {
-    uint64_t const N = P.N;
-    for (uint64_t i = blockIdx.x * static_cast<uint64_t>(WG) + threadIdx.x; i < N;
-         i += static_cast<uint64_t>(gridDim.x) * WG) {
-        uint32_t const lo = static_cast<uint32_t>(i), hi = static_cast<uint32_t>(i >> 32);
-        Words const a = P.rs.at(P_INIT_POS, lo, hi, 0);
-        Words const b = P.rs.at(P_INIT_VEL, lo, hi, 0);
+// uniform in the fluid region (edge = volume - 2 (1 - periodic), :8-10),
+// Maxwell-Boltzmann velocities
+__global__ __launch_bounds__(WG) void k_init(Sys sys, float* __restrict__ st) {
+    uint32_t const s = find_sys(sys.blk_part, sys.ns, blockIdx.x);
+    KParams const P = sys.P[s];
+    uint32_t const b0 = sys.blk_part[s], nb = sys.blk_part[s + 1] - b0;
+    uint64_t const NT = sys.NT;
+    float const sx = P.L[0], sy = P.L[1], sz = P.perio_z ? P.L[2] : P.L[2] - 2.0f;
+    for (uint64_t i = static_cast<uint64_t>(blockIdx.x - b0) * WG + threadIdx.x; i < P.N;
+         i += static_cast<uint64_t>(nb) * WG) {
+        uint32_t const lo = static_cast<uint32_t>(i);
+        Words const a = P.rs.at(P_INIT_POS, lo, 0u, 0);
+        Words const b = P.rs.at(P_INIT_VEL, lo, 0u, 0);
         float g0, g1, g2, g3;
         box_muller(b.w[0], b.w[1], &g0, &g1);
         box_muller(b.w[2], b.w[3], &g2, &g3);
-        st[i] = (uniform(a.w[0]) - 0.5f) * sx;
-        st[N + i] = (uniform(a.w[1]) - 0.5f) * sy;
-        st[2 * N + i] = (uniform(a.w[2]) - 0.5f) * sz;
-        st[3 * N + i] = g0 * P.tv;
-        st[4 * N + i] = g1 * P.tv;
-        st[5 * N + i] = g2 * P.tv;
+        uint64_t const j = P.pb + i;
+        st[j] = (uniform(a.w[0]) - 0.5f) * sx;
+        st[NT + j] = (uniform(a.w[1]) - 0.5f) * sy;
+        st[2 * NT + j] = (uniform(a.w[2]) - 0.5f) * sz;
+        st[3 * NT + j] = g0 * P.tv;
+        st[4 * NT + j] = g1 * P.tv;
+        st[5 * NT + j] = g2 * P.tv;
     }
 }
 
 // streaming half-step of an unsorted state: cell key per particle and the histogram
-__global__ __launch_bounds__(WG) void k_keys(KParams P, const float* __restrict__ st, uint32_t* __restrict__ count,
+__global__ __launch_bounds__(WG) void k_keys(Sys sys, const float* __restrict__ st, uint32_t* __restrict__ count,
                                              uint32_t* __restrict__ keys) {
-    uint64_t const N = P.N;
-    uint64_t const j = blockIdx.x * static_cast<uint64_t>(WG) + threadIdx.x;
-    bool const valid = j < N;
-    uint32_t key = 0;
-    if (valid) {
-        float px = st[j], py = st[N + j], pz = st[2 * N + j];
-        float vx = st[3 * N + j], vy = st[4 * N + j], vz = st[5 * N + j];
-        stream_shift(P, P.shift, px, py, pz, vx, vy, vz);
-        uint32_t ix, iy, iz;
-        cell_of(P, px, py, pz, ix, iy, iz);
-        key = key_of(P, ix, iy, iz);
+    uint32_t const s = find_sys(sys.blk_part, sys.ns, blockIdx.x);
+    KParams const P = sys.P[s];
+    uint32_t const b0 = sys.blk_part[s], nb = sys.blk_part[s + 1] - b0;
+    uint64_t const NT = sys.NT;
+    // the loop bound is uniform: wave_count needs whole waves
+    for (uint64_t i0 = static_cast<uint64_t>(blockIdx.x - b0) * WG; i0 < P.N; i0 += static_cast<uint64_t>(nb) * WG) {
+        uint64_t const i = i0 + threadIdx.x;
+        bool const valid = i < P.N;
+        uint64_t const j = P.pb + i;
+        uint32_t key = 0;
+        if (valid) {
+            float px = st[j], py = st[NT + j], pz = st[2 * NT + j];
+            float vx = st[3 * NT + j], vy = st[4 * NT + j], vz = st[5 * NT + j];
+            stream_shift(P, P.shift, px, py, pz, vx, vy, vz);
+            uint32_t ix, iy, iz;
+            cell_of(P, px, py, pz, ix, iy, iz);
+            key = P.cb + key_of(P, ix, iy, iz);
+        }
+        wave_count(count, key, valid);
+        if (valid) keys[j] = key;
     }
-    wave_count(count, key, valid);
-    if (valid) keys[j] = key;
 }
 
 // ---- exclusive scan of the per-cell counts (3 passes) ----
@@ -473,32 +515,39 @@ __device__ __forceinline__ Sym sym_inverse(Sym const& m, float tol) {
 //   gvel[w][g][a]  ghost velocities, Maxwell-Boltzmann times sqrt(T) (:153-166)
 // Wall cells: the two z layers next to the walls for this step's shift,
 // w = (upper ? X*Y : 0) + x + X*y.
-__global__ __launch_bounds__(WG) void k_cell_axes(KParams P, float* __restrict__ axis) {
-    uint32_t const C = P.C;
-    for (uint32_t key = blockIdx.x * WG + threadIdx.x; key < C; key += gridDim.x * WG) {
+__global__ __launch_bounds__(WG) void k_cell_axes(Sys sys, float* __restrict__ axis) {
+    uint32_t const s = find_sys(sys.blk_cell, sys.ns, blockIdx.x);
+    KParams const P = sys.P[s];
+    uint32_t const b0 = sys.blk_cell[s], nb = sys.blk_cell[s + 1] - b0;
+    uint64_t const CT = sys.CT;
+    for (uint32_t key = (blockIdx.x - b0) * WG + threadIdx.x; key < P.C; key += nb * WG) {
         uint32_t ix, iy, iz;
         decode_key(P, key, ix, iy, iz);
         float ax, ay, az;
         srd_axis(P, ref_of(P, ix, iy, iz), ax, ay, az);
-        axis[key] = ax;
-        axis[C + key] = ay;
-        axis[2ull * C + key] = az;
+        uint64_t const g = P.cb + key;
+        axis[g] = ax;
+        axis[CT + g] = ay;
+        axis[2 * CT + g] = az;
     }
 }
 
-__global__ __launch_bounds__(WG) void k_wall_ghosts(KParams P, uint32_t* __restrict__ gcnt, float* __restrict__ gvel) {
+__global__ __launch_bounds__(WG) void k_wall_ghosts(Sys sys, uint32_t* __restrict__ gcnt, float* __restrict__ gvel) {
+    uint32_t const s = find_sys(sys.blk_wall, sys.ns, blockIdx.x);
+    KParams const P = sys.P[s];
+    uint32_t const b0 = sys.blk_wall[s], nb = sys.blk_wall[s + 1] - b0;
     uint32_t const XY = P.dim[0] * P.dim[1];
     float const sh = fabsf(P.shift[2]);
     bool const sign = P.shift[2] > 0;
     uint32_t const iz_layer = static_cast<uint32_t>(P.L[2] - (sign ? 1.0f : 2.0f));
-    for (uint32_t w = blockIdx.x * WG + threadIdx.x; w < 2 * XY; w += gridDim.x * WG) {
+    for (uint32_t w = (blockIdx.x - b0) * WG + threadIdx.x; w < 2 * XY; w += nb * WG) {
         bool const layer = w >= XY;
         uint32_t const xy = layer ? w - XY : w;
         uint32_t const iz = layer ? iz_layer : static_cast<uint32_t>(sign);
         uint32_t const ref = xy + XY * iz;
         uint32_t const added = ghost_count(P, ref, sh, layer, sign);
-        gcnt[w] = added;
-        float* const out = gvel + 3ull * P.n * w;
+        gcnt[P.wb + w] = added;
+        float* const out = gvel + P.gvb + 3ull * P.n * w;
         for (uint32_t g = 0; g < added; ++g) {
             Words const v = P.rs.at(P_GHOST_VEL, g, ref, P.step);
             float g0, g1, g2, g3;
@@ -508,6 +557,22 @@ __global__ __launch_bounds__(WG) void k_wall_ghosts(KParams P, uint32_t* __restr
             out[3 * g + 1] = g1 * P.tv;
             out[3 * g + 2] = g2 * P.tv;
         }
+    }
+}
+
+// one step forward of every system's step state: the shift of step t + 1 and
+// the next shift (the grid shift is a pure function of (seed, system, step))
+__global__ void k_advance(Sys sys) {
+    uint32_t const s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= sys.ns) return;
+    KParams* const P = sys.P + s;
+    uint32_t const step = P->step + 1;
+    float nx[3];
+    grid_shift(P->rs, step + 1, nx);
+    P->step = step;
+    for (int a = 0; a < 3; ++a) {
+        P->shift[a] = P->shift_next[a];
+        P->shift_next[a] = nx[a];
     }
 }
 
@@ -557,7 +622,7 @@ __device__ void srd_cells(KParams const& P, Tile<CAP, 0>& s, uint32_t ncell, uin
             }
             if (added) {
                 uint32_t const go = s.goff[k];
-                float const* const g = go != ~0u ? &s.g[0][go] : gvel + 3ull * P.n * s.cref[k];
+                float const* const g = go != ~0u ? &s.g[0][go] : gvel + P.gvb + 3ull * P.n * s.cref[k];
                 for (uint32_t i = 0; i < added; ++i) {
                     sx = sx + g[3 * i];
                     sy = sy + g[3 * i + 1];
@@ -726,21 +791,26 @@ __device__ void extended_cell(KParams const& P, Tile<CAP, 1>& s, uint32_t k, uin
 constexpr int GQ = 4;               // particles gathered per thread per round
 
 template <int ALGO, bool SAMPLE, int CAP>
-__global__ __launch_bounds__(WG, (ALGO == 0 ? 4 : 3)) void k_collide(KParams P, const float* __restrict__ in, float* __restrict__ out,
+__global__ __launch_bounds__(WG, (ALGO == 0 ? 4 : 3)) void k_collide(Sys sys, const float* __restrict__ in, float* __restrict__ out,
                                                 const uint32_t* __restrict__ perm, const uint32_t* __restrict__ offset,
                                                 uint32_t* __restrict__ count_next, uint32_t* __restrict__ keys,
                                                 uint32_t* __restrict__ perm_out, SampleBufs S,
                                                 uint32_t* __restrict__ status, float const* __restrict__ axis,
                                                 uint32_t const* __restrict__ gcnt, float const* __restrict__ gvel) {
     __shared__ Tile<CAP, ALGO> s;
+    uint32_t const tile = xcd_tile(blockIdx.x, gridDim.x);
+    uint32_t const sy = sys.ns > 1 ? sys.tile_sys[tile] : 0u;
+    KParams const P = sys.P[sy];
     uint32_t const cap = CAP, TC = P.tile_cells;
     uint32_t const tid = threadIdx.x;
-    uint64_t const N = P.N;
-    uint32_t const tile = xcd_tile(blockIdx.x, gridDim.x);
-    uint32_t const c0 = tile * TC;
+    uint64_t const N = sys.NT;     // stride of the state planes
+    uint64_t const CT = sys.CT;    // stride of the axis planes
+    // tile -> cells [c0, c0 + nc) of system sy (local keys); g0 = global key of c0
+    uint32_t const c0 = (tile - (sys.ns > 1 ? sys.tile0[sy] : 0u)) * TC;
     if (c0 >= P.C) return;
     uint32_t const nc = min(TC, P.C - c0);
-    for (uint32_t k = tid; k <= nc; k += WG) s.off[k] = offset[c0 + k];
+    uint32_t const g0 = P.cb + c0;
+    for (uint32_t k = tid; k <= nc; k += WG) s.off[k] = offset[g0 + k];
     __syncthreads();
 
     uint32_t kb = 0;
@@ -777,7 +847,7 @@ __global__ __launch_bounds__(WG, (ALGO == 0 ? 4 : 3)) void k_collide(KParams P, 
                     stream_shift(P, P.shift_next, px, py, pz, vx, vy, vz);
                     uint32_t ix, iy, iz;
                     cell_of(P, px, py, pz, ix, iy, iz);
-                    knext = key_of(P, ix, iy, iz);
+                    knext = P.cb + key_of(P, ix, iy, iz);
                 }
                 wave_count(count_next, knext, valid);
                 if (valid) keys[static_cast<uint64_t>(gb) + i] = knext;
@@ -797,9 +867,10 @@ __global__ __launch_bounds__(WG, (ALGO == 0 ? 4 : 3)) void k_collide(KParams P, 
             uint32_t const XY = P.dim[0] * P.dim[1];
             for (uint32_t k = tid; k < ncell; k += WG) {
                 uint32_t const key = c0 + kb + k;
-                s.cm[3][k] = axis[key];
-                s.cm[4][k] = axis[P.C + key];
-                s.cm[5][k] = axis[2ull * P.C + key];
+                uint64_t const gk = P.cb + key;
+                s.cm[3][k] = axis[gk];
+                s.cm[4][k] = axis[CT + gk];
+                s.cm[5][k] = axis[2 * CT + gk];
                 uint32_t ix, iy, iz;
                 decode_key(P, key, ix, iy, iz);
                 float const zf = static_cast<float>(iz);
@@ -809,7 +880,7 @@ __global__ __launch_bounds__(WG, (ALGO == 0 ? 4 : 3)) void k_collide(KParams P, 
                 if (ghosts) {
                     uint32_t const w = (layer ? XY : 0u) + ix + P.dim[0] * iy;
                     s.cref[k] = w;
-                    gc = gcnt[w];
+                    gc = gcnt[P.wb + w];
                     uint32_t const slot = atomicAdd(&s.nwall, 1u);
                     if ((slot + 1) * 3 * P.n <= static_cast<uint32_t>(CAP / 2)) go = slot * 3 * P.n;
                 }
@@ -869,7 +940,7 @@ __global__ __launch_bounds__(WG, (ALGO == 0 ? 4 : 3)) void k_collide(KParams P, 
                 uint32_t const go = s.goff[k];
                 if (go != ~0u) {
                     uint32_t const nf = 3 * s.gcnt[k];
-                    float const* const src = gvel + 3ull * P.n * s.cref[k];
+                    float const* const src = gvel + P.gvb + 3ull * P.n * s.cref[k];
                     for (uint32_t f = lane; f < nf; f += 64) s.g[0][go + f] = src[f];
                 }
             }
@@ -893,7 +964,7 @@ __global__ __launch_bounds__(WG, (ALGO == 0 ? 4 : 3)) void k_collide(KParams P, 
         uint32_t* const hist = s.perm;
         uint32_t const span = ke - kb;
         uint32_t const wlo = (CAP - span) / 2;
-        uint32_t const wb = c0 + kb > wlo ? c0 + kb - wlo : 0u;
+        uint32_t const wb = g0 + kb > wlo ? g0 + kb - wlo : 0u;
         for (uint32_t i = tid; i < static_cast<uint32_t>(CAP); i += WG) hist[i] = 0u;
         __syncthreads();
         // per-particle finish, write-out, next-step cell count, sampling
@@ -934,7 +1005,7 @@ __global__ __launch_bounds__(WG, (ALGO == 0 ? 4 : 3)) void k_collide(KParams P, 
                 if (SAMPLE) {
                     uint32_t ix, iy, iz;
                     cell_of(P, px, py, pz, ix, iy, iz);
-                    uint32_t const rc = ref_of(P, ix, iy, iz);
+                    uint64_t const rc = P.cb + ref_of(P, ix, iy, iz);
                     atomicAdd(S.cnt + rc, 1u);
                     atomicAdd(S.sum + 3ull * rc + 0, static_cast<unsigned long long>(static_cast<long long>(rint(static_cast<double>(vx) * FIX_SCALE))));
                     atomicAdd(S.sum + 3ull * rc + 1, static_cast<unsigned long long>(static_cast<long long>(rint(static_cast<double>(vy) * FIX_SCALE))));
@@ -943,7 +1014,7 @@ __global__ __launch_bounds__(WG, (ALGO == 0 ? 4 : 3)) void k_collide(KParams P, 
                 stream_shift(P, P.shift_next, px, py, pz, vx, vy, vz);
                 uint32_t ix, iy, iz;
                 cell_of(P, px, py, pz, ix, iy, iz);
-                knext = key_of(P, ix, iy, iz);
+                knext = P.cb + key_of(P, ix, iy, iz);
             }
             if (valid) {
                 keys[static_cast<uint64_t>(gb) + i] = knext;
@@ -1042,12 +1113,15 @@ inline void ck(hipError_t e, const char* what) {
 }  // namespace
 
 struct mpcd_hip_s {
-    mpcd_params_t prm{};
+    std::vector<mpcd_params_t> prm;  // one per system
     std::string err;
     int device = 0;
     hipStream_t stream = nullptr;
-    KParams kp{};
-    uint64_t N = 0, C = 0;
+    std::vector<KParams> kp;         // host copy of the system table
+    uint32_t ns = 0;
+    int algorithm = 0;
+    uint64_t N = 0, C = 0, W = 0;    // particles, cells, wall cells of all systems
+    uint64_t GV = 0;                 // ghost-velocity floats of all systems
     float* st[2] = {nullptr, nullptr};
     int cur = 0;
     uint32_t* perm = nullptr;
@@ -1063,6 +1137,13 @@ struct mpcd_hip_s {
     float* axis = nullptr;           // SRD per-cell axes of the current step
     uint32_t* wgcnt = nullptr;       // SRD wall-cell ghost counts
     float* wgvel = nullptr;          // SRD wall-cell ghost velocities
+    // system table and block -> system maps on the device
+    KParams* d_kp = nullptr;
+    uint32_t* d_blk = nullptr;       // blk_part | blk_cell | blk_wall | tile0, ns + 1 entries each
+    uint16_t* d_tile_sys = nullptr;
+    uint64_t tile_sys_cap = 0;
+    uint32_t blocks_part = 0, blocks_cell = 0, blocks_wall = 0, ntiles = 0;
+    Sys sys{};
     // sampling
     uint32_t* scnt = nullptr;
     unsigned long long* ssum = nullptr;
@@ -1071,7 +1152,11 @@ struct mpcd_hip_s {
     uint64_t sample_counter = 0;
     uint64_t step_index = 0;
     bool keys_valid = false, sorted_valid = false;
-    uint32_t tile_cells = 0, lds_cap = 0;
+    uint32_t tile_req = 0, lds_cap = 0;  // requested cells per tile (0: largest that fits), LDS capacity
+    // captured steps, by input buffer and sampling
+    bool use_graph = true;
+    hipGraphExec_t graph[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
+    int graph_launches[2] = {0, 0};
     // timing
     bool timing = false;
     std::vector<hipEvent_t> ev;
@@ -1082,9 +1167,8 @@ struct mpcd_hip_s {
 
 namespace {
 
-void fill_geometry(mpcd_hip_s* h) {
-    mpcd_params_t const& p = h->prm;
-    KParams& k = h->kp;
+void fill_geometry(mpcd_params_t const& p, KParams& k) {
+    k = KParams{};
     for (int a = 0; a < 3; ++a) {
         k.L[a] = p.volume_size[a];
         k.half[a] = p.volume_size[a] * 0.5f;
@@ -1117,7 +1201,7 @@ void fill_geometry(mpcd_hip_s* h) {
     k.rs.k0 = static_cast<uint32_t>(p.seed);
     k.rs.k1 = static_cast<uint32_t>(p.seed >> 32);
     k.rs.sys = p.system_id;
-    k.N = h->N;
+    k.N = static_cast<uint32_t>(p.N);
 }
 
 // fills c_axes on the current device (once per device)
@@ -1147,23 +1231,99 @@ void upload_axes(int device) {
 // LDS tile capacities compiled into k_collide (particles per tile)
 constexpr uint32_t TILE_CAPS[3] = {512, 1152, 2048};
 
+// Cells per tile of every system for the LDS capacity `cap` (0: keep) and the
+// request `tc` (0: the largest that fits).  SRD: on average about 80% of the
+// tile's particle capacity; extended: the ghosts of a batch (at most n per
+// cell) must fit the cap/2 ghost region.
 void choose_tiling(mpcd_hip_s* h, uint32_t tc, uint32_t cap) {
-    uint32_t const n = std::max<uint32_t>(1, h->prm.n);
     if (cap == 0) cap = h->lds_cap ? h->lds_cap : 1152u;
-    // the smallest compiled capacity that holds the request
     uint32_t chosen = TILE_CAPS[2];
     for (uint32_t c : TILE_CAPS)
         if (c >= cap) { chosen = c; break; }
-    cap = chosen;
-    // extended: ghosts of a batch (at most n per cell) must fit the cap/2 ghost
-    // region; SRD: about 80% of the tile's particle capacity on average
-    uint32_t const tc_max = h->prm.algorithm == 1 ? std::max<uint32_t>(1, (cap / 2) / n)
-                                                  : std::max<uint32_t>(1, (cap * 4 / 5) / n);
-    if (tc == 0) tc = h->tile_cells ? h->tile_cells : tc_max;
-    tc = std::min<uint32_t>(tc, tc_max);
-    tc = std::min<uint32_t>(tc, static_cast<uint32_t>(TCM));
-    h->tile_cells = tc;
-    h->lds_cap = cap;
+    h->lds_cap = chosen;
+    h->tile_req = tc;
+    for (KParams& k : h->kp) {
+        uint32_t const n = std::max<uint32_t>(1, k.n);
+        uint32_t const tc_max = h->algorithm == 1 ? std::max<uint32_t>(1, (chosen / 2) / n)
+                                                  : std::max<uint32_t>(1, (chosen * 4 / 5) / n);
+        uint32_t t = tc ? std::min(tc, tc_max) : tc_max;
+        k.tile_cells = std::min<uint32_t>(t, static_cast<uint32_t>(TCM));
+    }
+}
+
+void drop_graphs(mpcd_hip_s* h) {
+    for (auto& row : h->graph)
+        for (auto& g : row)
+            if (g) {
+                (void)hipGraphExecDestroy(g);
+                g = nullptr;
+            }
+}
+
+// the system table with the step state of h->step_index
+void upload_table(mpcd_hip_s* h) {
+    for (KParams& k : h->kp) {
+        k.step = static_cast<uint32_t>(h->step_index);
+        grid_shift(k.rs, k.step, k.shift);
+        grid_shift(k.rs, k.step + 1, k.shift_next);
+    }
+    ck(hipStreamSynchronize(h->stream), "sync");
+    ck(hipMemcpy(h->d_kp, h->kp.data(), sizeof(KParams) * h->ns, hipMemcpyHostToDevice), "upload system table");
+}
+
+// places every system in the shared arrays and builds the block -> system maps
+void layout(mpcd_hip_s* h) {
+    uint32_t const ns = h->ns;
+    std::vector<uint32_t> blk(4 * (ns + 1), 0u);
+    uint32_t* bp = blk.data();
+    uint32_t* bc = bp + ns + 1;
+    uint32_t* bw = bc + ns + 1;
+    uint32_t* bt = bw + ns + 1;
+    auto blocks = [](uint64_t items) { return static_cast<uint32_t>(std::min<uint64_t>((items + WG - 1) / WG, 65536)); };
+    uint64_t cb = 0, pb = 0, wb = 0, gvb = 0, tiles = 0;
+    for (uint32_t s = 0; s < ns; ++s) {
+        KParams& k = h->kp[s];
+        k.cb = static_cast<uint32_t>(cb);
+        k.pb = static_cast<uint32_t>(pb);
+        k.wb = static_cast<uint32_t>(wb);
+        k.gvb = gvb;
+        uint64_t const walls = k.perio_z ? 0 : 2ull * k.dim[0] * k.dim[1];
+        bp[s + 1] = bp[s] + blocks(k.N);
+        bc[s + 1] = bc[s] + blocks(k.C);
+        bw[s + 1] = bw[s] + blocks(walls);
+        tiles += (k.C + k.tile_cells - 1) / k.tile_cells;
+        if (tiles >= 0xffffffffull) throw HipError{"too many collide tiles"};
+        bt[s + 1] = static_cast<uint32_t>(tiles);
+        cb += k.C;
+        pb += k.N;
+        wb += walls;
+        gvb += 3ull * k.n * walls;
+    }
+    h->blocks_part = bp[ns];
+    h->blocks_cell = bc[ns];
+    h->blocks_wall = bw[ns];
+    h->ntiles = bt[ns];
+    h->W = wb;
+    h->GV = gvb;
+    ck(hipStreamSynchronize(h->stream), "sync");
+    if (!h->d_blk) ck(hipMalloc(&h->d_blk, 4 * blk.size()), "alloc block maps");
+    ck(hipMemcpy(h->d_blk, blk.data(), 4 * blk.size(), hipMemcpyHostToDevice), "upload block maps");
+    if (ns > 1) {
+        std::vector<uint16_t> ts(h->ntiles);
+        for (uint32_t s = 0; s < ns; ++s)
+            for (uint32_t t = bt[s]; t < bt[s + 1]; ++t) ts[t] = static_cast<uint16_t>(s);
+        if (h->tile_sys_cap < ts.size()) {
+            if (h->d_tile_sys) ck(hipFree(h->d_tile_sys), "free tile map");
+            h->d_tile_sys = nullptr;
+            ck(hipMalloc(&h->d_tile_sys, 2 * ts.size()), "alloc tile map");
+            h->tile_sys_cap = ts.size();
+        }
+        ck(hipMemcpy(h->d_tile_sys, ts.data(), 2 * ts.size(), hipMemcpyHostToDevice), "upload tile map");
+    }
+    h->sys = Sys{h->d_kp, h->d_blk, h->d_blk + (ns + 1), h->d_blk + 2 * (ns + 1), h->d_tile_sys,
+                 h->d_blk + 3 * (ns + 1), ns, static_cast<uint32_t>(h->C), h->N};
+    drop_graphs(h);
+    upload_table(h);
 }
 
 void set_err(mpcd_hip_s* h, std::string const& m) {
@@ -1185,18 +1345,15 @@ int guarded(mpcd_hip_s* h, F&& f) {
 }
 
 void launch_keys(mpcd_hip_s* h) {
-    KParams k = h->kp;
-    grid_shift(k.rs, static_cast<uint32_t>(h->step_index), k.shift);
-    k.step = static_cast<uint32_t>(h->step_index);
     ck(hipMemsetAsync(h->count, 0, 4 * h->C, h->stream), "clear counts");
-    uint64_t const blocks = (h->N + WG - 1) / WG;
-    if (h->N) hipLaunchKernelGGL(k_keys, dim3(static_cast<uint32_t>(blocks)), dim3(WG), 0, h->stream, k, h->st[h->cur], h->count, h->keys);
+    if (h->blocks_part)
+        hipLaunchKernelGGL(k_keys, dim3(h->blocks_part), dim3(WG), 0, h->stream, h->sys, h->st[h->cur], h->count, h->keys);
     ck(hipGetLastError(), "k_keys");
     h->keys_valid = true;
     h->sorted_valid = false;
 }
 
-void launch_sort(mpcd_hip_s* h) {
+int launch_sort(mpcd_hip_s* h) {
     uint32_t const C = static_cast<uint32_t>(h->C);
     hipLaunchKernelGGL(k_scan_reduce, dim3(h->nb), dim3(WG), 0, h->stream, h->count, C, h->partial);
     hipLaunchKernelGGL(k_scan_partials, dim3(1), dim3(1024), 0, h->stream, h->partial, h->nb);
@@ -1205,38 +1362,23 @@ void launch_sort(mpcd_hip_s* h) {
     if (blocks == 0) blocks = 1;
     hipLaunchKernelGGL(k_perm, dim3(static_cast<uint32_t>(blocks)), dim3(WG), 0, h->stream, h->keys, h->cursor, h->perm, h->N);
     ck(hipGetLastError(), "sort");
-    h->sorted_valid = true;
-    h->launches += 4;
+    return 4;
 }
 
 template <int ALGO, bool SAMPLE, int CAP>
-void launch_collide_t(mpcd_hip_s* h, KParams const& k) {
-    uint32_t* rec = h->record ? h->perm_rec : nullptr;
+void launch_collide_t(mpcd_hip_s* h, uint32_t* rec) {
     SampleBufs S{h->scnt, h->ssum};
-    hipLaunchKernelGGL((k_collide<ALGO, SAMPLE, CAP>), dim3(k.ntiles), dim3(WG), 0, h->stream, k, h->st[h->cur],
+    hipLaunchKernelGGL((k_collide<ALGO, SAMPLE, CAP>), dim3(h->ntiles), dim3(WG), 0, h->stream, h->sys, h->st[h->cur],
                        h->st[1 - h->cur], h->perm, h->offset, h->count, h->keys, rec, S, h->status, h->axis,
                        h->wgcnt, h->wgvel);
 }
 
-// per-cell draws of this step (SRD)
-void launch_cell_draws(mpcd_hip_s* h, KParams const& k) {
-    uint32_t const C = static_cast<uint32_t>(h->C);
-    uint32_t const blocks = std::min<uint32_t>((C + WG - 1) / WG, 1u << 18);
-    hipLaunchKernelGGL(k_cell_axes, dim3(blocks), dim3(WG), 0, h->stream, k, h->axis);
-    if (!h->prm.periodicity[2]) {
-        uint32_t const nw = 2 * k.dim[0] * k.dim[1];
-        hipLaunchKernelGGL(k_wall_ghosts, dim3((nw + WG - 1) / WG), dim3(WG), 0, h->stream, k, h->wgcnt, h->wgvel);
-    }
-    ck(hipGetLastError(), "cell draws");
-    h->launches += h->prm.periodicity[2] ? 1 : 2;
-}
-
 template <int ALGO, bool SAMPLE>
-void launch_collide_cap(mpcd_hip_s* h, KParams const& k) {
+void launch_collide_cap(mpcd_hip_s* h, uint32_t* rec) {
     switch (h->lds_cap) {
-        case 512: launch_collide_t<ALGO, SAMPLE, 512>(h, k); break;
-        case 1152: launch_collide_t<ALGO, SAMPLE, 1152>(h, k); break;
-        default: launch_collide_t<ALGO, SAMPLE, 2048>(h, k); break;
+        case 512: launch_collide_t<ALGO, SAMPLE, 512>(h, rec); break;
+        case 1152: launch_collide_t<ALGO, SAMPLE, 1152>(h, rec); break;
+        default: launch_collide_t<ALGO, SAMPLE, 2048>(h, rec); break;
     }
 }
 
@@ -1250,12 +1392,66 @@ void ensure_sampling(mpcd_hip_s* h) {
     ck(hipMemsetAsync(h->acc, 0, 8 * 4 * h->C, h->stream), "clear");
 }
 
+// The launches of one step on h->stream (also what a captured step graph
+// holds): sort, per-cell draws, collide, sampling fold, step-state advance.
+// Uses no memset, so the sequence can be captured and replayed.
+int enqueue_step(mpcd_hip_s* h, bool sample, bool first, bool sort, uint32_t* rec, hipEvent_t e0, hipEvent_t e1) {
+    int launches = 0;
+    if (sort) launches += launch_sort(h);
+    if (h->algorithm == 0) {
+        if (h->blocks_cell) hipLaunchKernelGGL(k_cell_axes, dim3(h->blocks_cell), dim3(WG), 0, h->stream, h->sys, h->axis);
+        if (h->blocks_wall)
+            hipLaunchKernelGGL(k_wall_ghosts, dim3(h->blocks_wall), dim3(WG), 0, h->stream, h->sys, h->wgcnt, h->wgvel);
+        ck(hipGetLastError(), "cell draws");
+        launches += (h->blocks_cell ? 1 : 0) + (h->blocks_wall ? 1 : 0);
+    }
+    if (e0) ck(hipEventRecord(e0, h->stream), "event");
+    if (h->algorithm == 0) {
+        if (sample) launch_collide_cap<0, true>(h, rec); else launch_collide_cap<0, false>(h, rec);
+    } else {
+        if (sample) launch_collide_cap<1, true>(h, rec); else launch_collide_cap<1, false>(h, rec);
+    }
+    ck(hipGetLastError(), "k_collide");
+    if (e1) ck(hipEventRecord(e1, h->stream), "event");
+    launches += 1;
+    if (sample) {
+        uint64_t const blocks = std::max<uint64_t>(1, std::min<uint64_t>((h->C + WG - 1) / WG, 1u << 16));
+        hipLaunchKernelGGL(k_sample_fold, dim3(static_cast<uint32_t>(blocks)), dim3(WG), 0, h->stream,
+                           static_cast<uint32_t>(h->C), h->scnt, h->ssum, h->acc, first ? 1 : 0);
+        ck(hipGetLastError(), "k_sample_fold");
+        launches += 1;
+    }
+    hipLaunchKernelGGL(k_advance, dim3((h->ns + 63) / 64), dim3(64), 0, h->stream, h->sys);
+    ck(hipGetLastError(), "k_advance");
+    return launches + 1;
+}
+
+// the captured step reading h->st[h->cur] (captured on first use)
+hipGraphExec_t step_graph(mpcd_hip_s* h, bool sample) {
+    hipGraphExec_t& ge = h->graph[h->cur][sample ? 1 : 0];
+    if (ge) return ge;
+    hipGraph_t g = nullptr;
+    ck(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal), "begin capture");
+    int n = 0;
+    try {
+        n = enqueue_step(h, sample, false, true, nullptr, nullptr, nullptr);
+    } catch (...) {
+        (void)hipStreamEndCapture(h->stream, &g);
+        if (g) (void)hipGraphDestroy(g);
+        throw;
+    }
+    ck(hipStreamEndCapture(h->stream, &g), "end capture");
+    hipError_t const e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    ck(e, "instantiate step graph");
+    h->graph_launches[sample ? 1 : 0] = n;
+    return ge;
+}
+
 void run_steps(mpcd_hip_s* h, int n_steps, bool sample) {
     if (n_steps <= 0) return;
     if (sample) ensure_sampling(h);
     if (h->record && !h->perm_rec) ck(hipMalloc(&h->perm_rec, 4 * std::max<uint64_t>(h->N, 1)), "alloc perm record");
-    int const algo = h->prm.algorithm;
-    uint32_t const ntiles = static_cast<uint32_t>((h->C + h->tile_cells - 1) / h->tile_cells);
     if (h->timing) {
         if (h->ev.size() < 2 * static_cast<size_t>(n_steps)) {
             size_t const old = h->ev.size();
@@ -1270,32 +1466,21 @@ void run_steps(mpcd_hip_s* h, int n_steps, bool sample) {
     }
     h->launches = 0;
     for (int s = 0; s < n_steps; ++s) {
-        if (!h->keys_valid) { launch_keys(h); h->launches += 2; }
-        if (!h->sorted_valid) launch_sort(h);
-        KParams k = h->kp;
-        k.step = static_cast<uint32_t>(h->step_index);
-        grid_shift(k.rs, k.step, k.shift);
-        grid_shift(k.rs, k.step + 1, k.shift_next);
-        k.tile_cells = h->tile_cells;
-        k.lds_cap = h->lds_cap;
-        k.ntiles = ntiles;
-        if (algo == 0) launch_cell_draws(h, k);
-        if (h->timing) ck(hipEventRecord(h->ev[2 * s], h->stream), "event");
-        if (algo == 0) {
-            if (sample) launch_collide_cap<0, true>(h, k); else launch_collide_cap<0, false>(h, k);
-        } else {
-            if (sample) launch_collide_cap<1, true>(h, k); else launch_collide_cap<1, false>(h, k);
+        if (!h->keys_valid) {
+            launch_keys(h);
+            h->launches += 2;
         }
-        ck(hipGetLastError(), "k_collide");
-        if (h->timing) ck(hipEventRecord(h->ev[2 * s + 1], h->stream), "event");
-        h->launches += 1;
+        bool const first = sample && h->sampling_done;
+        bool const graphed = h->use_graph && !h->timing && !h->record && !first && !h->sorted_valid;
+        if (graphed) {
+            ck(hipGraphLaunch(step_graph(h, sample), h->stream), "launch step graph");
+            h->launches += h->graph_launches[sample ? 1 : 0];
+        } else {
+            hipEvent_t const e0 = h->timing ? h->ev[2 * s] : nullptr;
+            hipEvent_t const e1 = h->timing ? h->ev[2 * s + 1] : nullptr;
+            h->launches += enqueue_step(h, sample, first, !h->sorted_valid, h->record ? h->perm_rec : nullptr, e0, e1);
+        }
         if (sample) {
-            bool const first = h->sampling_done;
-            uint64_t blocks = std::min<uint64_t>((h->C + WG - 1) / WG, 1u << 16);
-            hipLaunchKernelGGL(k_sample_fold, dim3(static_cast<uint32_t>(blocks)), dim3(WG), 0, h->stream,
-                               static_cast<uint32_t>(h->C), h->scnt, h->ssum, h->acc, first ? 1 : 0);
-            ck(hipGetLastError(), "k_sample_fold");
-            h->launches += 1;
             if (first) {
                 h->sampling_done = false;
                 h->sample_counter = 0;
@@ -1325,6 +1510,7 @@ void run_steps(mpcd_hip_s* h, int n_steps, bool sample) {
     }
     if (st & 1u) {
         ck(hipMemsetAsync(h->status, 0, 4, h->stream), "status");
+        ck(hipStreamSynchronize(h->stream), "status");
         throw HipError{"a cell holds more particles than one LDS tile (" + std::to_string(h->lds_cap) +
                        "); raise the tile capacity with mpcd_hip_set_tiling"};
     }
@@ -1335,38 +1521,157 @@ void check_particles(mpcd_hip_s* h, size_t n) {
         throw HipError{"particle array holds " + std::to_string(n) + " particles, the system has " + std::to_string(h->N)};
 }
 
+void check_sys(mpcd_hip_s const* h, uint32_t s) {
+    if (s >= h->ns) throw HipError{"system index " + std::to_string(s) + " out of range (" + std::to_string(h->ns) + " systems)"};
+}
+
+// xyz-interleaved host array <-> three SoA planes, slots [first, first + n)
 void put_component(mpcd_hip_s* h, const float* src, int first_plane) {
     std::vector<float> tmp(3 * h->N);
     for (uint64_t i = 0; i < h->N; ++i)
         for (int a = 0; a < 3; ++a) tmp[a * h->N + i] = src[3 * i + a];
+    ck(hipStreamSynchronize(h->stream), "sync");
     ck(hipMemcpy(h->st[h->cur] + first_plane * h->N, tmp.data(), 4 * 3 * h->N, hipMemcpyHostToDevice), "upload");
     h->keys_valid = false;
     h->sorted_valid = false;
 }
 
-void get_component(mpcd_hip_s* h, float* dst, int first_plane) {
-    std::vector<float> tmp(3 * h->N);
+void get_component(mpcd_hip_s* h, float* dst, int first_plane, uint64_t first, uint64_t n) {
+    std::vector<float> tmp(3 * n);
     ck(hipStreamSynchronize(h->stream), "sync");
-    ck(hipMemcpy(tmp.data(), h->st[h->cur] + first_plane * h->N, 4 * 3 * h->N, hipMemcpyDeviceToHost), "download");
-    for (uint64_t i = 0; i < h->N; ++i)
-        for (int a = 0; a < 3; ++a) dst[3 * i + a] = tmp[a * h->N + i];
+    for (int a = 0; a < 3; ++a)
+        if (n)
+            ck(hipMemcpy(tmp.data() + a * n, h->st[h->cur] + (first_plane + a) * h->N + first, 4 * n,
+                         hipMemcpyDeviceToHost), "download");
+    for (uint64_t i = 0; i < n; ++i)
+        for (int a = 0; a < 3; ++a) dst[3 * i + a] = tmp[a * n + i];
 }
 
 void destroy(mpcd_hip_s* h) {
     if (!h) return;
     if (h->stream) (void)hipStreamSynchronize(h->stream);
+    drop_graphs(h);
     for (auto* p : {h->st[0], h->st[1]}) if (p) (void)hipFree(p);
     for (void* p : {static_cast<void*>(h->perm), static_cast<void*>(h->keys), static_cast<void*>(h->cursor), static_cast<void*>(h->count),
                     static_cast<void*>(h->offset), static_cast<void*>(h->partial), static_cast<void*>(h->status),
                     static_cast<void*>(h->perm_rec), static_cast<void*>(h->scnt), static_cast<void*>(h->ssum),
                     static_cast<void*>(h->axis), static_cast<void*>(h->wgcnt), static_cast<void*>(h->wgvel),
-                    static_cast<void*>(h->acc)})
+                    static_cast<void*>(h->acc), static_cast<void*>(h->d_kp), static_cast<void*>(h->d_blk),
+                    static_cast<void*>(h->d_tile_sys)})
         if (p) (void)hipFree(p);
     for (auto e : h->ev) (void)hipEventDestroy(e);
     if (h->ev_begin) (void)hipEventDestroy(h->ev_begin);
     if (h->ev_end) (void)hipEventDestroy(h->ev_end);
     if (h->stream) (void)hipStreamDestroy(h->stream);
     delete h;
+}
+
+std::string validate(mpcd_params_t const& p) {
+    if (p.experiment != 0) return "the hip backend implements the standard geometry only (experiment = \"standard\")";
+    if (p.algorithm != 0 && p.algorithm != 1) return "Unsupported MPCD algorithm";
+    for (int a = 0; a < 3; ++a)
+        if (!(p.volume_size[a] >= 1.0f) || p.volume_size[a] != floorf(p.volume_size[a]))
+            return "volume_size must hold positive whole numbers of cells";
+    if (p.periodicity[0] == 0 || p.periodicity[1] == 0)
+        return "the hip backend supports walls in z only (periodicity (1, 1, 0) or (1, 1, 1))";
+    if (p.periodicity[2] == 0 && p.volume_size[2] < 3.0f) return "walls in z need at least 3 cell layers";
+    if (p.n == 0) return "n must be positive";
+    if (p.N >= 0xffffffffull) return "particle count must be below 2^32";
+    double const cells = static_cast<double>(p.volume_size[0]) * p.volume_size[1] * p.volume_size[2];
+    if (cells >= 4294967295.0) return "cell count must be below 2^32";
+    return "";
+}
+
+int create_impl(const mpcd_params_t* params, uint32_t n_systems, mpcd_hip_t** out) {
+    auto fail = [](std::string const& m) {
+        std::lock_guard<std::mutex> g(g_err_mu);
+        g_create_err = m;
+        return 1;
+    };
+    if (!params || !out) return fail("null argument");
+    *out = nullptr;
+    if (n_systems == 0 || n_systems > 65535) return fail("an ensemble holds 1 to 65535 systems");
+    uint64_t N = 0, C = 0;
+    for (uint32_t s = 0; s < n_systems; ++s) {
+        std::string const e = validate(params[s]);
+        if (!e.empty()) return fail(n_systems > 1 ? "system " + std::to_string(s) + ": " + e : e);
+        if (params[s].algorithm != params[0].algorithm) return fail("all systems of an ensemble use the same algorithm");
+        if (params[s].device_id != params[0].device_id) return fail("all systems of an ensemble use the same device");
+        N += params[s].N;
+        C += static_cast<uint64_t>(params[s].volume_size[0]) * static_cast<uint64_t>(params[s].volume_size[1]) *
+             static_cast<uint64_t>(params[s].volume_size[2]);
+    }
+    if (N >= 0xffffffffull) return fail("particle count of all systems must be below 2^32");
+    if (C >= 0xffffffffull) return fail("cell count of all systems must be below 2^32");
+    auto* h = new mpcd_hip_s();
+    h->prm.assign(params, params + n_systems);
+    h->ns = n_systems;
+    h->algorithm = params[0].algorithm;
+    h->N = N;
+    h->C = C;
+    try {
+        int ndev = 0;
+        ck(hipGetDeviceCount(&ndev), "hipGetDeviceCount");
+        if (ndev <= 0) throw HipError{"No HIP device found"};
+        if (static_cast<int>(params[0].device_id) >= ndev) throw HipError{"device_id out of range"};
+        h->device = static_cast<int>(params[0].device_id);
+        ck(hipSetDevice(h->device), "hipSetDevice");
+        ck(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking), "stream");
+        h->kp.resize(n_systems);
+        for (uint32_t s = 0; s < n_systems; ++s) fill_geometry(params[s], h->kp[s]);
+        upload_axes(h->device);
+        ck(hipMalloc(&h->d_kp, sizeof(KParams) * n_systems), "alloc system table");
+        choose_tiling(h, 0, 0);
+        layout(h);
+        uint64_t const Na = std::max<uint64_t>(h->N, 1);
+        ck(hipMalloc(&h->st[0], 24 * Na), "alloc state");
+        ck(hipMalloc(&h->st[1], 24 * Na), "alloc state");
+        ck(hipMalloc(&h->perm, 4 * Na), "alloc perm");
+        ck(hipMalloc(&h->keys, 4 * Na), "alloc keys");
+        ck(hipMalloc(&h->count, 4 * h->C), "alloc counts");
+        ck(hipMalloc(&h->offset, 4 * (h->C + 1)), "alloc offsets");
+        ck(hipMalloc(&h->cursor, 4 * h->C), "alloc cursors");
+        if (h->algorithm == 0) {
+            ck(hipMalloc(&h->axis, 12 * h->C), "alloc axes");
+            ck(hipMalloc(&h->wgcnt, 4 * std::max<uint64_t>(h->W, 1)), "alloc ghost counts");
+            ck(hipMalloc(&h->wgvel, 4 * std::max<uint64_t>(h->GV, 1)), "alloc ghost velocities");
+        }
+        h->nb = static_cast<uint32_t>((h->C + SCAN_CHUNK - 1) / SCAN_CHUNK);
+        ck(hipMalloc(&h->partial, 4 * std::max<uint32_t>(h->nb, 1)), "alloc scan partials");
+        ck(hipMalloc(&h->status, 4), "alloc status");
+        ck(hipMemsetAsync(h->status, 0, 4, h->stream), "clear status");
+        ck(hipMemsetAsync(h->count, 0, 4 * h->C, h->stream), "clear counts");
+        if (h->blocks_part)
+            hipLaunchKernelGGL(k_init, dim3(h->blocks_part), dim3(WG), 0, h->stream, h->sys, h->st[0]);
+        ck(hipGetLastError(), "k_init");
+        ck(hipStreamSynchronize(h->stream), "init");
+    } catch (HipError const& e) {
+        destroy(h);
+        return fail(e.msg);
+    }
+    *out = h;
+    return 0;
+}
+
+// FP64 accumulators of cells [first, first + n) scaled by 1/sample_counter
+// (the reference divides the sum of S samples by S-1, cpu_backend.cpp:85-114)
+void sample_mean_impl(mpcd_hip_t* h, uint64_t first, uint64_t n, float* d32, float* v32, double* d64, double* v64) {
+    std::vector<double> acc(4 * n, 0.0);
+    if (h->acc) {
+        ck(hipStreamSynchronize(h->stream), "sync");
+        ck(hipMemcpy(acc.data(), h->acc + 4 * first, 8 * 4 * n, hipMemcpyDeviceToHost), "download accumulators");
+    }
+    double const inverse = 1.0 / static_cast<double>(h->sample_counter);
+    for (uint64_t c = 0; c < n; ++c) {
+        if (d32) {
+            d32[c] = static_cast<float>(acc[4 * c] * inverse);
+            for (int k = 0; k < 3; ++k) v32[3 * c + k] = static_cast<float>(acc[4 * c + 1 + k] * inverse);
+        }
+        if (d64) {
+            d64[c] = acc[4 * c] * inverse;
+            for (int k = 0; k < 3; ++k) v64[3 * c + k] = acc[4 * c + 1 + k] * inverse;
+        }
+    }
 }
 
 }  // namespace
@@ -1380,75 +1685,10 @@ const char* mpcd_hip_last_error(const mpcd_hip_t* h) {
     return g_create_err.c_str();
 }
 
-int mpcd_hip_create(const mpcd_params_t* params, mpcd_hip_t** out) {
-    auto fail = [](std::string const& m) {
-        std::lock_guard<std::mutex> g(g_err_mu);
-        g_create_err = m;
-        return 1;
-    };
-    if (!params || !out) return fail("null argument");
-    *out = nullptr;
-    mpcd_params_t const& p = *params;
-    if (p.experiment != 0) return fail("the hip backend implements the standard geometry only (experiment = \"standard\")");
-    if (p.algorithm != 0 && p.algorithm != 1) return fail("Unsupported MPCD algorithm");
-    for (int a = 0; a < 3; ++a)
-        if (!(p.volume_size[a] >= 1.0f) || p.volume_size[a] != floorf(p.volume_size[a]))
-            return fail("volume_size must hold positive whole numbers of cells");
-    if (p.periodicity[0] == 0 || p.periodicity[1] == 0)
-        return fail("the hip backend supports walls in z only (periodicity (1, 1, 0) or (1, 1, 1))");
-    if (p.periodicity[2] == 0 && p.volume_size[2] < 3.0f) return fail("walls in z need at least 3 cell layers");
-    if (p.n == 0) return fail("n must be positive");
-    if (p.N >= 0xffffffffull) return fail("particle count must be below 2^32");
-    double const cells = static_cast<double>(p.volume_size[0]) * p.volume_size[1] * p.volume_size[2];
-    if (cells >= 4294967295.0) return fail("cell count must be below 2^32");
-    auto* h = new mpcd_hip_s();
-    h->prm = p;
-    h->N = p.N;
-    h->C = static_cast<uint64_t>(cells);
-    try {
-        int ndev = 0;
-        ck(hipGetDeviceCount(&ndev), "hipGetDeviceCount");
-        if (ndev <= 0) throw HipError{"No HIP device found"};
-        if (static_cast<int>(p.device_id) >= ndev) throw HipError{"device_id out of range"};
-        h->device = static_cast<int>(p.device_id);
-        ck(hipSetDevice(h->device), "hipSetDevice");
-        ck(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking), "stream");
-        fill_geometry(h);
-        upload_axes(h->device);
-        choose_tiling(h, 0, 0);
-        uint64_t const N = std::max<uint64_t>(h->N, 1);
-        ck(hipMalloc(&h->st[0], 24 * N), "alloc state");
-        ck(hipMalloc(&h->st[1], 24 * N), "alloc state");
-        ck(hipMalloc(&h->perm, 4 * N), "alloc perm");
-        ck(hipMalloc(&h->keys, 4 * N), "alloc keys");
-        ck(hipMalloc(&h->count, 4 * h->C), "alloc counts");
-        ck(hipMalloc(&h->offset, 4 * (h->C + 1)), "alloc offsets");
-        ck(hipMalloc(&h->cursor, 4 * h->C), "alloc cursors");
-        if (p.algorithm == 0) {
-            uint64_t const XY = static_cast<uint64_t>(h->kp.dim[0]) * h->kp.dim[1];
-            ck(hipMalloc(&h->axis, 12 * h->C), "alloc axes");
-            ck(hipMalloc(&h->wgcnt, 4 * 2 * XY), "alloc ghost counts");
-            ck(hipMalloc(&h->wgvel, 12ull * p.n * 2 * XY), "alloc ghost velocities");
-        }
-        h->nb = static_cast<uint32_t>((h->C + SCAN_CHUNK - 1) / SCAN_CHUNK);
-        ck(hipMalloc(&h->partial, 4 * std::max<uint32_t>(h->nb, 1)), "alloc scan partials");
-        ck(hipMalloc(&h->status, 4), "alloc status");
-        ck(hipMemsetAsync(h->status, 0, 4, h->stream), "clear status");
-        ck(hipMemsetAsync(h->count, 0, 4 * h->C, h->stream), "clear counts");
-        // distribute_particles: scale = volume - 2 (1 - periodic)  (simulation_functions.cpp:8-10)
-        float const sx = p.volume_size[0] - static_cast<float>(2 * (1 - p.periodicity[0]));
-        float const sy = p.volume_size[1] - static_cast<float>(2 * (1 - p.periodicity[1]));
-        float const sz = p.volume_size[2] - static_cast<float>(2 * (1 - p.periodicity[2]));
-        uint64_t const blocks = std::min<uint64_t>((h->N + WG - 1) / WG, 1u << 20);
-        if (h->N) hipLaunchKernelGGL(k_init, dim3(static_cast<uint32_t>(blocks)), dim3(WG), 0, h->stream, h->kp, h->st[0], sx, sy, sz);
-        ck(hipGetLastError(), "k_init");
-        ck(hipStreamSynchronize(h->stream), "init");
-    } catch (HipError const& e) {
-        destroy(h);
-        return fail(e.msg);
-    }
-    *out = h;
-    return 0;
+int mpcd_hip_create(const mpcd_params_t* params, mpcd_hip_t** out) { return create_impl(params, 1, out); }
+
+int mpcd_hip_create_ensemble(const mpcd_params_t* params, uint32_t n_systems, mpcd_hip_t** out) {
+    return create_impl(params, n_systems, out);
 }
 
 int mpcd_hip_destroy(mpcd_hip_t* h) {
@@ -1470,36 +1710,19 @@ int mpcd_hip_step_and_sample(mpcd_hip_t* h, int n_steps) {
     });
 }
 
-static void sample_mean_impl(mpcd_hip_t* h, float* d32, float* v32, double* d64, double* v64, size_t n_cells) {
-    if (n_cells != h->C) throw HipError{"cell array size does not match the grid"};
-    std::vector<double> acc(4 * h->C, 0.0);
-    if (h->acc) {
-        ck(hipStreamSynchronize(h->stream), "sync");
-        ck(hipMemcpy(acc.data(), h->acc, 8 * 4 * h->C, hipMemcpyDeviceToHost), "download accumulators");
-    }
-    // the reference divides the sum of S samples by S-1 (cpu_backend.cpp:85-114)
-    double const inverse = 1.0 / static_cast<double>(h->sample_counter);
-    for (uint64_t c = 0; c < h->C; ++c) {
-        if (d32) {
-            d32[c] = static_cast<float>(acc[4 * c] * inverse);
-            for (int k = 0; k < 3; ++k) v32[3 * c + k] = static_cast<float>(acc[4 * c + 1 + k] * inverse);
-        }
-        if (d64) {
-            d64[c] = acc[4 * c] * inverse;
-            for (int k = 0; k < 3; ++k) v64[3 * c + k] = acc[4 * c + 1 + k] * inverse;
-        }
-    }
-}
-
 int mpcd_hip_get_sample_mean(mpcd_hip_t* h, float* density, float* velocity, size_t n_cells) {
     return guarded(h, [&] {
-        sample_mean_impl(h, density, velocity, nullptr, nullptr, n_cells);
+        if (n_cells != h->C) throw HipError{"cell array size does not match the grid"};
+        sample_mean_impl(h, 0, h->C, density, velocity, nullptr, nullptr);
         h->sampling_done = true;
     });
 }
 
 int mpcd_hip_get_sample_mean_f64(mpcd_hip_t* h, double* density, double* velocity, size_t n_cells) {
-    return guarded(h, [&] { sample_mean_impl(h, nullptr, nullptr, density, velocity, n_cells); });
+    return guarded(h, [&] {
+        if (n_cells != h->C) throw HipError{"cell array size does not match the grid"};
+        sample_mean_impl(h, 0, h->C, nullptr, nullptr, density, velocity);
+    });
 }
 
 int mpcd_hip_n_particles(const mpcd_hip_t* h, uint64_t* out) {
@@ -1515,10 +1738,10 @@ int mpcd_hip_n_cells(const mpcd_hip_t* h, uint64_t* out) {
 }
 
 int mpcd_hip_get_positions(mpcd_hip_t* h, float* out, size_t n) {
-    return guarded(h, [&] { check_particles(h, n); get_component(h, out, 0); });
+    return guarded(h, [&] { check_particles(h, n); get_component(h, out, 0, 0, h->N); });
 }
 int mpcd_hip_get_velocities(mpcd_hip_t* h, float* out, size_t n) {
-    return guarded(h, [&] { check_particles(h, n); get_component(h, out, 3); });
+    return guarded(h, [&] { check_particles(h, n); get_component(h, out, 3, 0, h->N); });
 }
 int mpcd_hip_set_positions(mpcd_hip_t* h, const float* pos, size_t n) {
     return guarded(h, [&] { check_particles(h, n); put_component(h, pos, 0); });
@@ -1544,12 +1767,13 @@ int mpcd_hip_set_step_index(mpcd_hip_t* h, uint64_t step) {
         h->step_index = step;
         h->keys_valid = false;
         h->sorted_valid = false;
+        upload_table(h);
     });
 }
 
 int mpcd_hip_grid_shift(const mpcd_hip_t* h, uint64_t step, float* out3) {
     if (!h || !out3) return 1;
-    grid_shift(h->kp.rs, static_cast<uint32_t>(step), out3);
+    grid_shift(h->kp[0].rs, static_cast<uint32_t>(step), out3);
     return 0;
 }
 
@@ -1557,7 +1781,10 @@ int mpcd_hip_peek_sort(mpcd_hip_t* h, uint32_t* cell_of_particle, uint32_t* coun
     return guarded(h, [&] {
         ck(hipSetDevice(h->device), "hipSetDevice");
         if (!h->keys_valid) launch_keys(h);
-        if (!h->sorted_valid) launch_sort(h);
+        if (!h->sorted_valid) {
+            launch_sort(h);
+            h->sorted_valid = true;
+        }
         ck(hipStreamSynchronize(h->stream), "sync");
         std::vector<uint32_t> kr(h->N);
         std::vector<uint32_t> off(h->C + 1), pm(h->N);
@@ -1566,16 +1793,17 @@ int mpcd_hip_peek_sort(mpcd_hip_t* h, uint32_t* cell_of_particle, uint32_t* coun
             ck(hipMemcpy(pm.data(), h->perm, 4 * h->N, hipMemcpyDeviceToHost), "download perm");
         }
         ck(hipMemcpy(off.data(), h->offset, 4 * (h->C + 1), hipMemcpyDeviceToHost), "download offsets");
-        KParams const& k = h->kp;
-        auto to_ref = [&](uint32_t key) {
-            uint32_t ix, iy, iz;
-            decode_key(k, key, ix, iy, iz);
-            return ref_of(k, ix, iy, iz);
-        };
-        std::vector<uint32_t> key_of_ref(h->C);
-        for (uint32_t key = 0; key < h->C; ++key) key_of_ref[to_ref(key)] = key;
+        // global key -> global reference cell index (system's first cell + x + X (y + Y z))
+        std::vector<uint32_t> ref_of_key(h->C), key_of_ref(h->C);
+        for (KParams const& k : h->kp)
+            for (uint32_t key = 0; key < k.C; ++key) {
+                uint32_t ix, iy, iz;
+                decode_key(k, key, ix, iy, iz);
+                ref_of_key[k.cb + key] = k.cb + ref_of(k, ix, iy, iz);
+            }
+        for (uint32_t key = 0; key < h->C; ++key) key_of_ref[ref_of_key[key]] = key;
         if (cell_of_particle)
-            for (uint64_t j = 0; j < h->N; ++j) cell_of_particle[j] = to_ref(kr[j]);
+            for (uint64_t j = 0; j < h->N; ++j) cell_of_particle[j] = ref_of_key[kr[j]];
         size_t w = 0;
         for (uint32_t c = 0; c < h->C; ++c) {
             uint32_t const key = key_of_ref[c];
@@ -1619,16 +1847,130 @@ int mpcd_hip_set_tiling(mpcd_hip_t* h, uint32_t cells_per_tile, uint32_t lds_cap
     return guarded(h, [&] {
         if (lds_capacity > TILE_CAPS[2])
             throw HipError{"tile capacity above " + std::to_string(TILE_CAPS[2]) + " particles"};
-        h->tile_cells = 0;
-        choose_tiling(h, cells_per_tile, lds_capacity ? lds_capacity : h->lds_cap);
+        choose_tiling(h, cells_per_tile, lds_capacity);
+        layout(h);
     });
 }
 
 int mpcd_hip_get_tiling(const mpcd_hip_t* h, uint32_t* cells_per_tile, uint32_t* lds_capacity) {
     if (!h) return 1;
-    if (cells_per_tile) *cells_per_tile = h->tile_cells;
+    if (cells_per_tile) *cells_per_tile = h->kp[0].tile_cells;
     if (lds_capacity) *lds_capacity = h->lds_cap;
     return 0;
+}
+
+/* ---- ensembles ---- */
+
+int mpcd_hip_n_systems(const mpcd_hip_t* h, uint32_t* out) {
+    if (!h || !out) return 1;
+    *out = h->ns;
+    return 0;
+}
+
+int mpcd_hip_system_info(const mpcd_hip_t* h, uint32_t s, uint64_t* first_particle, uint64_t* n_particles,
+                         uint64_t* first_cell, uint64_t* n_cells) {
+    if (!h || s >= h->ns) return 1;
+    KParams const& k = h->kp[s];
+    if (first_particle) *first_particle = k.pb;
+    if (n_particles) *n_particles = k.N;
+    if (first_cell) *first_cell = k.cb;
+    if (n_cells) *n_cells = k.C;
+    return 0;
+}
+
+int mpcd_hip_system_get_particles(mpcd_hip_t* h, uint32_t s, float* pos, float* vel, size_t n) {
+    return guarded(h, [&] {
+        check_sys(h, s);
+        KParams const& k = h->kp[s];
+        if (n != k.N) throw HipError{"particle array size does not match the system"};
+        if (pos) get_component(h, pos, 0, k.pb, k.N);
+        if (vel) get_component(h, vel, 3, k.pb, k.N);
+    });
+}
+
+int mpcd_hip_system_sample_mean(mpcd_hip_t* h, uint32_t s, float* density, float* velocity, size_t n_cells) {
+    return guarded(h, [&] {
+        check_sys(h, s);
+        if (n_cells != h->kp[s].C) throw HipError{"cell array size does not match the system"};
+        sample_mean_impl(h, h->kp[s].cb, h->kp[s].C, density, velocity, nullptr, nullptr);
+    });
+}
+
+int mpcd_hip_system_sample_mean_f64(mpcd_hip_t* h, uint32_t s, double* density, double* velocity, size_t n_cells) {
+    return guarded(h, [&] {
+        check_sys(h, s);
+        if (n_cells != h->kp[s].C) throw HipError{"cell array size does not match the system"};
+        sample_mean_impl(h, h->kp[s].cb, h->kp[s].C, nullptr, nullptr, density, velocity);
+    });
+}
+
+int mpcd_hip_reset_sampling(mpcd_hip_t* h) {
+    return guarded(h, [&] { h->sampling_done = true; });
+}
+
+int mpcd_hip_use_graph(mpcd_hip_t* h, int enable) {
+    return guarded(h, [&] {
+        h->use_graph = enable != 0;
+        if (!h->use_graph) {
+            ck(hipStreamSynchronize(h->stream), "sync");
+            drop_graphs(h);
+        }
+    });
+}
+
+int mpcd_hip_autotune(mpcd_hip_t* h, int steps_per_trial, uint32_t* cells_per_tile, uint32_t* lds_capacity,
+                      double* ms_per_step) {
+    return guarded(h, [&] {
+        ck(hipSetDevice(h->device), "hipSetDevice");
+        if (steps_per_trial < 1) throw HipError{"steps_per_trial must be positive"};
+        struct Cand {
+            uint32_t tc, cap;
+        };
+        std::vector<Cand> cands;
+        for (uint32_t cap : TILE_CAPS)
+            for (uint32_t tc : {0u, 48u, 32u, 16u}) cands.push_back({tc, cap});
+        bool const timing = h->timing;
+        h->timing = false;
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        ck(hipEventCreate(&e0), "event");
+        ck(hipEventCreate(&e1), "event");
+        double best = 1e300;
+        Cand pick = {h->tile_req, h->lds_cap};
+        std::vector<std::pair<uint32_t, uint32_t>> seen;
+        try {
+            for (Cand const& c : cands) {
+                choose_tiling(h, c.tc, c.cap);
+                std::pair<uint32_t, uint32_t> const key{h->kp[0].tile_cells, h->lds_cap};
+                if (c.tc && std::find(seen.begin(), seen.end(), key) != seen.end()) continue;
+                seen.push_back(key);
+                layout(h);
+                run_steps(h, 1, false);  // captures the step graph
+                ck(hipEventRecord(e0, h->stream), "event");
+                run_steps(h, steps_per_trial, false);
+                ck(hipEventRecord(e1, h->stream), "event");
+                ck(hipEventSynchronize(e1), "event");
+                float ms = 0;
+                ck(hipEventElapsedTime(&ms, e0, e1), "elapsed");
+                if (ms < best) {
+                    best = ms;
+                    pick = c;
+                }
+            }
+        } catch (...) {
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+            h->timing = timing;
+            throw;
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        h->timing = timing;
+        choose_tiling(h, pick.tc, pick.cap);
+        layout(h);
+        if (cells_per_tile) *cells_per_tile = h->kp[0].tile_cells;
+        if (lds_capacity) *lds_capacity = h->lds_cap;
+        if (ms_per_step) *ms_per_step = best / steps_per_trial;
+    });
 }
 
 int mpcd_hip_bench_copy(size_t bytes, int reps, double* ms_per_copy) {

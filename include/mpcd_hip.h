@@ -112,6 +112,42 @@ int mpcd_hip_last_timing(const mpcd_hip_t* h, double* collide_ms, double* total_
  * depend on it. */
 int mpcd_hip_set_tiling(mpcd_hip_t* h, uint32_t cells_per_tile, uint32_t lds_capacity);
 int mpcd_hip_get_tiling(const mpcd_hip_t* h, uint32_t* cells_per_tile, uint32_t* lds_capacity);
+/* ---- ensembles (no reference counterpart: the reference runs one system per
+ * process, cpu_backend.cpp / cuda_backend_impl.cu) ---- */
+
+/* n_systems independent systems advanced together: each kernel of a step
+ * covers every system in one launch (per-system parameter table, block ->
+ * system maps).  System s owns particle slots [first_particle, first_particle +
+ * n_particles) of the flattened particle arrays the entry points above read and
+ * write, and cells [first_cell, first_cell + n_cells) of the flattened cell
+ * arrays (get_sample_mean, peek_sort: cell index first_cell + x + X (y + Y z)).
+ * All systems use the same algorithm and device; n, extents, walls, drag,
+ * temperature, rotation angle and seed are per system.  System s reproduces,
+ * bit for bit, a handle created from params[s] alone.  mpcd_hip_create is
+ * mpcd_hip_create_ensemble with one system. */
+int mpcd_hip_create_ensemble(const mpcd_params_t* params, uint32_t n_systems, mpcd_hip_t** out);
+int mpcd_hip_n_systems(const mpcd_hip_t* h, uint32_t* out);
+int mpcd_hip_system_info(const mpcd_hip_t* h, uint32_t system, uint64_t* first_particle, uint64_t* n_particles,
+                         uint64_t* first_cell, uint64_t* n_cells);
+/* Positions and/or velocities (either may be NULL) of one system, interleaved xyz. */
+int mpcd_hip_system_get_particles(mpcd_hip_t* h, uint32_t system, float* pos, float* vel, size_t n_particles);
+/* Mean fields of one system's cells, scaled as mpcd_hip_get_sample_mean; the
+ * current average continues (mpcd_hip_reset_sampling ends it for all systems). */
+int mpcd_hip_system_sample_mean(mpcd_hip_t* h, uint32_t system, float* density, float* velocity, size_t n_cells);
+int mpcd_hip_system_sample_mean_f64(mpcd_hip_t* h, uint32_t system, double* density, double* velocity, size_t n_cells);
+int mpcd_hip_reset_sampling(mpcd_hip_t* h);
+/* Replay each step from a HIP graph captured on first use (default on; steps
+ * with timing, permutation recording or the first sample of an average run
+ * eagerly).  The step state (index, grid shifts) lives in device memory and is
+ * advanced by the last kernel of the step, so one graph serves every step. */
+int mpcd_hip_use_graph(mpcd_hip_t* h, int enable);
+/* Autotuner: runs steps_per_trial real steps (after one untimed step) with each
+ * compiled LDS capacity and several cells-per-tile limits, keeps the fastest
+ * and returns it (cells per tile of system 0) with its time per step.  The
+ * systems advance by these steps; results do not depend on the tile shape. */
+int mpcd_hip_autotune(mpcd_hip_t* h, int steps_per_trial, uint32_t* cells_per_tile, uint32_t* lds_capacity,
+                      double* ms_per_step);
+
 /* Copies `bytes` between two device buffers `reps` times with a plain float4
  * kernel and returns the mean time per copy: the practical HBM reference. */
 int mpcd_hip_bench_copy(size_t bytes, int reps, double* ms_per_copy);

@@ -151,3 +151,30 @@ def test_theory_numbers():
     assert abs(srd_viscosity(5, 0.02) - 16.695) < 5e-3
     assert abs(srd_viscosity(10, 0.02, kinetic=True) - 37.585) < 5e-3
     assert abs(srd_viscosity(20, 0.02, kinetic=True) - 79.318) < 5e-3
+
+
+def test_ensemble_validation(pympcd):
+    a = pympcd.Params()
+    a.n = 2
+    a.volume_size = (4, 4, 4)
+    b = pympcd.Params()
+    b.n = 2
+    b.volume_size = (4, 4, 4)
+    b.algorithm = "extended"
+    with pytest.raises(RuntimeError, match="same algorithm"):
+        pympcd.Ensemble([a, b])
+    with pytest.raises(RuntimeError, match="at least one system"):
+        pympcd.Ensemble([])
+    c = pympcd.Params()
+    c.periodicity = (0, 1, 1)
+    with pytest.raises(RuntimeError, match="system 1: .*walls in z only"):
+        pympcd.Ensemble([a, c])
+
+
+@pytest.mark.skipif(_has_gpu(), reason="a GPU is present")
+def test_ensemble_without_device_is_an_error(pympcd):
+    a = pympcd.Params()
+    a.n = 2
+    a.volume_size = (4, 4, 4)
+    with pytest.raises(RuntimeError, match="device"):
+        pympcd.Ensemble([a, a])

@@ -1,0 +1,113 @@
+"""Ensembles (independent systems advanced together) and captured steps.
+
+Every member of an ensemble must reproduce, bit for bit, a Simulation created
+from the same parameters and seed (the counter-based random stream is keyed by
+(seed, system id) and the cell-local order is stable, so neither the other
+members nor the launch shape can change a result).  Steps replayed from a
+captured HIP graph must equal eagerly launched steps, replay after replay,
+with eager work in between; the autotuner may not change results.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from cudampcd_amd.capi import Handle
+from helpers import make_params
+
+pytestmark = pytest.mark.gpu
+
+MEMBERS = [
+    # fluid extents, periodicity, n, drag, seed, system id, rotation angle
+    ((8, 6, 10), (1, 1, 0), 10, 0.001, 11, 0, 120.0),
+    ((5, 7, 6), (1, 1, 1), 5, 0.0, 12, 3, 120.0),
+    ((12, 12, 20), (1, 1, 0), 20, 0.002, 13, 1, 90.0),
+    ((4, 4, 3), (1, 1, 0), 1, 0.0005, 14, 7, 120.0),
+    ((16, 8, 9), (1, 1, 0), 3, 0.0015, 15, 2, 130.0),
+]
+
+
+def _params(pympcd, m, algorithm):
+    fluid, perio, n, drag, seed, sid, angle = m
+    p = make_params(pympcd, fluid, perio, n=n, algorithm=algorithm, seed=seed, drag=drag, angle=angle)
+    p.system_id = sid
+    return p
+
+
+@pytest.mark.parametrize("algorithm", ["srd", "extended"])
+def test_members_match_separate_simulations_bitwise(pympcd, algorithm):
+    params = [_params(pympcd, m, algorithm) for m in MEMBERS]
+    ens = pympcd.Ensemble(params)
+    sims = [pympcd.Simulation(p, "hip") for p in params]
+    assert ens.n_systems == len(params)
+    for k in range(len(params)):
+        assert ens.n_particles(k) == sims[k].n_particles
+        assert np.array_equal(ens.get_particle_positions(k), sims[k].get_particle_positions())
+    ens.step(7)
+    ens.step_and_sample(9)
+    for s in sims:
+        s.step(7)
+        s.step_and_sample(9)
+    for k, s in enumerate(sims):
+        assert np.array_equal(ens.get_particle_positions(k), s.get_particle_positions()), k
+        assert np.array_equal(ens.get_particle_velocities(k), s.get_particle_velocities()), k
+        d, v = ens.get_mean_fields_f64(k)
+        sd, sv = s.get_mean_fields_f64()
+        assert np.array_equal(d, sd) and np.array_equal(v, sv), k
+        d32, v32 = ens.get_mean_fields(k)
+        s32, sv32 = s.get_mean_fields()
+        assert d32.shape == s32.shape and np.array_equal(d32, s32) and np.array_equal(v32, sv32), k
+    # the flattened C-ABI view places system k at its slot range
+    h = Handle(ens)
+    assert h.n_systems == len(params)
+    first, count, cfirst, ccount = h.system_info(2)
+    assert count == sims[2].n_particles and first == sum(s.n_particles for s in sims[:2])
+
+
+def test_graph_replay_equals_eager_steps(pympcd):
+    cfg = ((20, 16, 14), (1, 1, 0))
+    a = pympcd.Simulation(make_params(pympcd, *cfg, n=10, seed=77), "hip")
+    b = pympcd.Simulation(make_params(pympcd, *cfg, n=10, seed=77), "hip")
+    Handle(a).use_graph(False)
+    for rep in range(5):
+        a.step(3)
+        b.step(3)              # graph replays
+        assert np.array_equal(a.get_particle_positions(), b.get_particle_positions()), rep
+        assert np.array_equal(a.get_particle_velocities(), b.get_particle_velocities()), rep
+        a.step_and_sample(2)   # first sample eager, second from the sampling graph
+        b.step_and_sample(2)
+        da, va = a.get_mean_fields_f64()
+        db, vb = b.get_mean_fields_f64()
+        assert np.array_equal(da, db) and np.array_equal(va, vb), rep
+    assert Handle(a).step_index == Handle(b).step_index == 25
+
+
+def test_autotune_keeps_results(pympcd):
+    params = [_params(pympcd, m, "srd") for m in MEMBERS[:3]]
+    tuned = pympcd.Ensemble(params)
+    plain = pympcd.Ensemble(params)
+    tc, cap, ms = tuned.autotune(2)
+    assert cap in (512, 1152, 2048) and 1 <= tc <= 64 and ms > 0
+    steps = Handle(tuned).step_index
+    plain.step(steps)
+    tuned.step(3)
+    plain.step(3)
+    for k in range(len(params)):
+        assert np.array_equal(tuned.get_particle_positions(k), plain.get_particle_positions(k))
+        assert np.array_equal(tuned.get_particle_velocities(k), plain.get_particle_velocities(k))
+
+
+def test_ensemble_member_set_and_reset(pympcd):
+    params = [_params(pympcd, m, "srd") for m in MEMBERS[:2]]
+    ens = pympcd.Ensemble(params)
+    ens.step_and_sample(3)
+    d1, _ = ens.get_mean_fields_f64(0)
+    ens.step_and_sample(2)
+    d2, _ = ens.get_mean_fields_f64(0)
+    assert d1.sum() > 0 and d2.sum() > 0
+    ens.reset_sampling()
+    ens.step_and_sample(3)
+    d3, v3 = ens.get_mean_fields(1)
+    assert np.isfinite(v3).all()
+    with pytest.raises(RuntimeError, match="out of range"):
+        ens.get_particle_positions(5)
