@@ -15,7 +15,17 @@
 The native extension must be built (python cudampcd_amd/build.py); importing
 without it raises ImportError rather than falling back to anything else.
 """
-from ._pympcd import Ensemble, Params, Simulation, __version__  # noqa: F401
+import sys as _sys
+
+if "torch" in _sys.modules:
+    # A process holding torch's HIP runtime and this library's must start
+    # torch's first, or torch finds no GPU afterwards.
+    try:
+        _sys.modules["torch"].cuda.is_available()
+    except Exception:  # noqa: BLE001
+        pass
+
+from ._pympcd import Ensemble, Params, Simulation, __version__  # noqa: F401,E402
 
 from .physics import srd_viscosity, poiseuille_viscosity  # noqa: F401,E402
 

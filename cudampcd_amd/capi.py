@@ -69,6 +69,10 @@ def lib() -> ctypes.CDLL:
             "mpcd_hip_reset_sampling": [_vp],
             "mpcd_hip_use_graph": [_vp, ctypes.c_int],
             "mpcd_hip_autotune": [_vp, ctypes.c_int, _u32p, _u32p, _f64p],
+            "mpcd_hip_profile_loss": [_vp, ctypes.c_int, ctypes.c_float, ctypes.c_float, _f32p, _f64p,
+                                      ctypes.c_size_t, _f64p, _f64p],
+            "mpcd_hip_profile_loss_grad": [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_float, _f32p,
+                                           _f64p, ctypes.c_size_t, _f64p, _f64p, _f64p, _f64p, _f32p],
             "mpcd_hip_bench_copy": [ctypes.c_size_t, ctypes.c_int, _f64p],
             "mpcd_hip_selftest_philox": [ctypes.c_uint32, ctypes.c_uint32, _u32p, _u32p, ctypes.c_size_t],
             "mpcd_hip_selftest_math": [_f32p, _f32p, ctypes.c_size_t, ctypes.c_int],

@@ -104,6 +104,16 @@ struct SampleBufs {
     unsigned long long* sum;        // 3 per reference cell, fixed point 2^40, two's complement
 };
 
+// Adjoint runs (SRD, one system): the body-force field and what one step
+// records for the reverse pass.  Null pointers record nothing / apply no force.
+struct Tape {
+    float* vp;             // [3][N] streamed velocity of every output slot, before the collision
+    float* u;              // [3][C] mean velocity of every cell (particles + ghosts; 0 if not collided)
+    uint32_t* tot;         // [C] particles in the table + ghosts of every cell
+    float const* force;    // [C] x body force per reference cell, applied as v_x += g dt + f dt
+};
+constexpr uint32_t BOUNCED = 0x80000000u;  // flag in a recorded permutation entry: the particle bounced
+
 // ------------------------------------------------------------ helpers ---
 
 // periodic wrap fmodf(r + 1.5 L, L) - L/2 (simulation_functions.cpp:46-51).
@@ -133,10 +143,11 @@ __device__ __forceinline__ uint32_t mod_dim(uint32_t u, uint32_t D) {
 }
 
 // streaming with z-wall bounce-back (simulation_functions.cpp:56-74), then
-// shift and periodic wrap (:84)
-__device__ __forceinline__ void stream_shift(KParams const& P, float const* sh, float& px, float& py, float& pz,
+// shift and periodic wrap (:84); returns whether the particle bounced
+__device__ __forceinline__ bool stream_shift(KParams const& P, float const* sh, float& px, float& py, float& pz,
                                              float& vx, float& vy, float& vz) {
     float const dt = P.dt;
+    bool bounced = false;
     if (!P.perio_z) {
         float const z_wall = (0.5f * P.L[2] - 1.0f);
         float const next_z = pz + vz * dt;
@@ -148,6 +159,7 @@ __device__ __forceinline__ void stream_shift(KParams const& P, float const* sh, 
             vx = -vx;
             vy = -vy;
             vz = -vz;
+            bounced = true;
             float const rem = dt - t;
             px = px + vx * rem;
             py = py + vy * rem;
@@ -165,6 +177,7 @@ __device__ __forceinline__ void stream_shift(KParams const& P, float const* sh, 
     px = wrap1(px + sh[0], P.L[0]);
     py = wrap1(py + sh[1], P.L[1]);
     pz = wrap1(pz + sh[2], P.L[2]);
+    return bounced;
 }
 
 // cell coordinates (volume_container.hpp:69-75)
@@ -588,7 +601,7 @@ struct Tile {
     uint32_t off[TCM + 1];           // cell offsets of the tile
     uint32_t go[TCM];                // collision happened
     uint32_t cref[TCM];              // reference cell index
-    uint32_t cflag[TCM];             // bit 0 wall cell with ghosts, bit 1 upper wall layer
+    float cforce[TCM];               // adjoint runs: force term f dt of each cell
     uint32_t gcnt[TCM];              // ghost count
     uint32_t goff[TCM];              // first ghost slot
     uint32_t sw[WG / 64];            // scan scratch
@@ -604,9 +617,9 @@ struct Tile {
 // k_wall_ghosts) were staged into LDS while the particles were gathered:
 // s.gcnt[k] ghosts at s.g[0][s.goff[k] ...], or, when the staging region was
 // full (s.goff[k] == ~0u), read from memory at gvel + 3 n s.cref[k].
-template <int CAP>
+template <int CAP, bool ADJ>
 __device__ void srd_cells(KParams const& P, Tile<CAP, 0>& s, uint32_t ncell, uint32_t const* off, uint32_t gb,
-                          float const* __restrict__ gvel) {
+                          float const* __restrict__ gvel, uint32_t key0, Tape const& T, uint64_t CT) {
     for (uint32_t k = threadIdx.x; k < ncell; k += WG) {
         uint32_t const a = off[k] - gb, mc = off[k + 1] - gb - a;
         uint32_t const reals = mc < P.cap ? mc : P.cap;
@@ -633,6 +646,22 @@ __device__ void srd_cells(KParams const& P, Tile<CAP, 0>& s, uint32_t ncell, uin
             s.cm[0][k] = sx * inv;
             s.cm[1][k] = sy * inv;
             s.cm[2][k] = sz * inv;
+        }
+        if constexpr (ADJ) {
+            uint64_t const gk = P.cb + key0 + k;
+            if (T.tot) {
+                T.tot[gk] = total;
+                T.u[gk] = total > 1 ? s.cm[0][k] : 0.0f;
+                T.u[CT + gk] = total > 1 ? s.cm[1][k] : 0.0f;
+                T.u[2 * CT + gk] = total > 1 ? s.cm[2][k] : 0.0f;
+            }
+            float f = 0.0f;
+            if (T.force) {
+                uint32_t ix, iy, iz;
+                decode_key(P, key0 + k, ix, iy, iz);
+                f = T.force[P.cb + ref_of(P, ix, iy, iz)] * P.dt;
+            }
+            s.cforce[k] = f;
         }
     }
 }
@@ -790,13 +819,14 @@ __device__ void extended_cell(KParams const& P, Tile<CAP, 1>& s, uint32_t k, uin
 
 constexpr int GQ = 4;               // particles gathered per thread per round
 
-template <int ALGO, bool SAMPLE, int CAP>
+template <int ALGO, bool SAMPLE, int CAP, bool ADJ = false>
 __global__ __launch_bounds__(WG, (ALGO == 0 ? 4 : 3)) void k_collide(Sys sys, const float* __restrict__ in, float* __restrict__ out,
                                                 const uint32_t* __restrict__ perm, const uint32_t* __restrict__ offset,
                                                 uint32_t* __restrict__ count_next, uint32_t* __restrict__ keys,
                                                 uint32_t* __restrict__ perm_out, SampleBufs S,
                                                 uint32_t* __restrict__ status, float const* __restrict__ axis,
-                                                uint32_t const* __restrict__ gcnt, float const* __restrict__ gvel) {
+                                                uint32_t const* __restrict__ gcnt, float const* __restrict__ gvel,
+                                                Tape T) {
     __shared__ Tile<CAP, ALGO> s;
     uint32_t const tile = xcd_tile(blockIdx.x, gridDim.x);
     uint32_t const sy = sys.ns > 1 ? sys.tile_sys[tile] : 0u;
@@ -926,10 +956,21 @@ __global__ __launch_bounds__(WG, (ALGO == 0 ? 4 : 3)) void k_collide(Sys sys, co
                     uint32_t const d = a + r;
                     float px = g[q][0], py = g[q][1], pz = g[q][2];
                     float vx = g[q][3], vy = g[q][4], vz = g[q][5];
-                    stream_shift(P, P.shift, px, py, pz, vx, vy, vz);
+                    bool const bounced = stream_shift(P, P.shift, px, py, pz, vx, vy, vz);
                     s.p[0][d] = px; s.p[1][d] = py; s.p[2][d] = pz;
                     s.p[3][d] = vx; s.p[4][d] = vy; s.p[5][d] = vz;
-                    if (perm_out) perm_out[static_cast<uint64_t>(gb) + d] = src;
+                    uint64_t const o = static_cast<uint64_t>(gb) + d;
+                    if constexpr (ADJ) {
+                        if (perm_out) perm_out[o] = src | (bounced ? BOUNCED : 0u);
+                        if (T.vp) {
+                            T.vp[o] = vx;
+                            T.vp[N + o] = vy;
+                            T.vp[2 * N + o] = vz;
+                        }
+                    } else {
+                        (void)bounced;
+                        if (perm_out) perm_out[o] = src;
+                    }
                 }
             }
         }
@@ -948,7 +989,7 @@ __global__ __launch_bounds__(WG, (ALGO == 0 ? 4 : 3)) void k_collide(Sys sys, co
         __syncthreads();
         // per-cell stage
         if constexpr (ALGO == 0) {
-            srd_cells<CAP>(P, s, ncell, s.off + kb, gb, gvel);
+            srd_cells<CAP, ADJ>(P, s, ncell, s.off + kb, gb, gvel, c0 + kb, T, CT);
         } else {
             for (uint32_t k = tid; k < ncell; k += WG) {
                 uint32_t ix, iy, iz;
@@ -997,7 +1038,11 @@ __global__ __launch_bounds__(WG, (ALGO == 0 ? 4 : 3)) void k_collide(Sys sys, co
                     px = wrap1(px - P.shift[0], P.L[0]);
                     py = wrap1(py - P.shift[1], P.L[1]);
                     pz = wrap1(pz - P.shift[2], P.L[2]);
-                    vx += P.drag_dt;
+                    if constexpr (ADJ) {
+                        vx += P.drag_dt + s.cforce[k];   // f = 0 adds exactly drag_dt
+                    } else {
+                        vx += P.drag_dt;
+                    }
                 }
                 uint64_t const o = static_cast<uint64_t>(gb) + i;
                 out[o] = px; out[N + o] = py; out[2 * N + o] = pz;
@@ -1030,6 +1075,172 @@ __global__ __launch_bounds__(WG, (ALGO == 0 ? 4 : 3)) void k_collide(Sys sys, co
         }
         __syncthreads();
         kb = ke;
+    }
+}
+
+// ------------------------------------------------------ adjoint kernels ---
+// Loss of an adjoint run: the x velocity averaged over each z layer of the
+// cell grid (the layer of a particle is the z cell index of its stored
+// position, as in sampling), then over the steps of the run.
+
+constexpr int LAYERS_MAX = 1024;
+
+// mode 0: per-layer particle count and x-velocity sum of one step's state
+// (fixed point 2^40, so the sums do not depend on the order of the atomics);
+// mode 1: loss seed dL/dv_x = w[layer] / count[layer] of every particle
+__global__ __launch_bounds__(WG) void k_layers(Sys sys, const float* __restrict__ st, int mode,
+                                               uint32_t* __restrict__ cnt, unsigned long long* __restrict__ sum,
+                                               const double* __restrict__ w, float* __restrict__ seed) {
+    __shared__ uint32_t hc[LAYERS_MAX];
+    __shared__ unsigned long long hs[LAYERS_MAX];
+    KParams const P = sys.P[0];
+    uint64_t const N = sys.NT;
+    uint32_t const Z = P.dim[2];
+    if (mode == 0) {
+        for (uint32_t i = threadIdx.x; i < Z; i += WG) {
+            hc[i] = 0u;
+            hs[i] = 0ull;
+        }
+        __syncthreads();
+    }
+    for (uint64_t j = blockIdx.x * static_cast<uint64_t>(WG) + threadIdx.x; j < N;
+         j += static_cast<uint64_t>(gridDim.x) * WG) {
+        uint32_t const iz = mod_dim(static_cast<uint32_t>(floorf(st[2 * N + j] + P.half[2])), Z);
+        if (mode == 0) {
+            atomicAdd(&hc[iz], 1u);
+            atomicAdd(&hs[iz], static_cast<unsigned long long>(static_cast<long long>(
+                                   rint(static_cast<double>(st[3 * N + j]) * FIX_SCALE))));
+        } else {
+            uint32_t const c = cnt[iz];
+            seed[j] = c ? static_cast<float>(w[iz] / static_cast<double>(c)) : 0.0f;
+        }
+    }
+    if (mode == 0) {
+        __syncthreads();
+        for (uint32_t i = threadIdx.x; i < Z; i += WG) {
+            if (hc[i]) {
+                atomicAdd(cnt + i, hc[i]);
+                atomicAdd(sum + i, hs[i]);
+            }
+        }
+    }
+}
+
+// Reverse of one recorded SRD step (pathwise: cells, tables, bounces, axes and
+// ghosts of the step held fixed).  With v' the streamed velocity, u the cell
+// mean over the table and the ghosts, R the rotation, the step maps
+//   v'' = u + R (v' - u) + (g + f_c) dt e_x   (table particles of cells with more than one member)
+//   v'' = v'                                   (overflow particles, lone cells)
+// and v' = s v with s = -1 for a bounced particle.  The adjoint of a table
+// particle is mu_i = R^T lam_i + (I - R^T) sum_table(lam) / total, and
+//   dL/dg += dt sum lam_x,  dL/df_c += dt sum_c lam_x,  dL/dalpha += sum lam . dR/dalpha (v' - u).
+// lam_in: dL/dv'' of the step's output slots (x, y, z planes), the loss seed
+// of this step added to its x plane; lam_out: dL/dv of the input slots.
+// part[b], part[G + b]: this block's running sums of dL/dg and dL/dalpha (radians);
+// gf: dL/df per reference cell.
+__global__ __launch_bounds__(WG) void k_adjoint_step(Sys sys, const uint32_t* __restrict__ off,
+                                                     const float* __restrict__ u, const float* __restrict__ ax,
+                                                     const uint32_t* __restrict__ tot, const uint32_t* __restrict__ perm,
+                                                     const float* __restrict__ vp, const float* __restrict__ seed,
+                                                     const float* __restrict__ lam_in, float* __restrict__ lam_out,
+                                                     double* __restrict__ part, double* __restrict__ gf) {
+    __shared__ double red[2][WG / 64];
+    KParams const P = sys.P[0];
+    uint64_t const N = sys.NT, C = sys.CT;
+    double dg = 0.0, da = 0.0;
+    uint32_t const key = blockIdx.x * WG + threadIdx.x;
+    if (key < P.C) {
+        uint32_t const a = off[key], e = off[key + 1];
+        uint32_t const total = tot[key];
+        uint32_t const reals = min(e - a, P.cap);
+        auto lam = [&](uint32_t o, float* l) {
+            l[0] = lam_in[o] + seed[o];
+            l[1] = lam_in[N + o];
+            l[2] = lam_in[2 * N + o];
+        };
+        auto put = [&](uint32_t o, float const* m) {
+            uint32_t const pe = perm[o];
+            uint32_t const src = pe & ~BOUNCED;
+            float const sg = (pe & BOUNCED) ? -1.0f : 1.0f;
+            lam_out[src] = sg * m[0];
+            lam_out[N + src] = sg * m[1];
+            lam_out[2 * N + src] = sg * m[2];
+        };
+        uint32_t first_plain = a;
+        if (total > 1) {
+            float const axx = ax[key], axy = ax[C + key], axz = ax[2 * C + key];
+            float const ux = u[key], uy = u[C + key], uz = u[2 * C + key];
+            double sx = 0, sy = 0, sz = 0, sgx = 0;
+            for (uint32_t o = a; o < a + reals; ++o) {
+                float l[3];
+                lam(o, l);
+                sx += l[0];
+                sy += l[1];
+                sz += l[2];
+            }
+            float const inv = static_cast<float>(1.0 / static_cast<double>(total));
+            float const mx = static_cast<float>(sx) * inv, my = static_cast<float>(sy) * inv, mz = static_cast<float>(sz) * inv;
+            // (I - R^T) applied to the table mean of lam
+            auto rot_t = [&](float x, float y, float z, float* o3) {
+                float const d = x * axx + y * axy + z * axz;
+                float const px = x - axx * d, py = y - axy * d, pz = z - axz * d;
+                float const cx = py * axz - pz * axy, cy = pz * axx - px * axz, cz = px * axy - py * axx;
+                o3[0] = axx * d + P.cos_a * px - P.sin_a * cx;
+                o3[1] = axy * d + P.cos_a * py - P.sin_a * cy;
+                o3[2] = axz * d + P.cos_a * pz - P.sin_a * cz;
+            };
+            float rm[3];
+            rot_t(mx, my, mz, rm);
+            float const cmx = mx - rm[0], cmy = my - rm[1], cmz = mz - rm[2];
+            for (uint32_t o = a; o < a + reals; ++o) {
+                float l[3];
+                lam(o, l);
+                float rl[3];
+                rot_t(l[0], l[1], l[2], rl);
+                float const m[3] = {rl[0] + cmx, rl[1] + cmy, rl[2] + cmz};
+                // dR/dalpha (v' - u) = -sin q + cos (q x a), q the part of v' - u normal to the axis
+                float const wx = vp[o] - ux, wy = vp[N + o] - uy, wz = vp[2 * N + o] - uz;
+                float const d = wx * axx + wy * axy + wz * axz;
+                float const qx = wx - axx * d, qy = wy - axy * d, qz = wz - axz * d;
+                float const cx = qy * axz - qz * axy, cy = qz * axx - qx * axz, cz = qx * axy - qy * axx;
+                da += static_cast<double>(l[0]) * (-P.sin_a * qx + P.cos_a * cx) +
+                      static_cast<double>(l[1]) * (-P.sin_a * qy + P.cos_a * cy) +
+                      static_cast<double>(l[2]) * (-P.sin_a * qz + P.cos_a * cz);
+                sgx += l[0];
+                put(o, m);
+            }
+            double const gx = sgx * static_cast<double>(P.dt);
+            dg = gx;
+            uint32_t ix, iy, iz;
+            decode_key(P, key, ix, iy, iz);
+            gf[P.cb + ref_of(P, ix, iy, iz)] += gx;
+            first_plain = a + reals;
+        }
+        for (uint32_t o = first_plain; o < e; ++o) {
+            float l[3];
+            lam(o, l);
+            put(o, l);
+        }
+    }
+    // block sums, kept per block so the final sum has a fixed order
+    for (int d = 32; d >= 1; d >>= 1) {
+        dg += __shfl_down(dg, d, 64);
+        da += __shfl_down(da, d, 64);
+    }
+    uint32_t const lane = threadIdx.x & 63u, wv = threadIdx.x >> 6;
+    if (lane == 0) {
+        red[0][wv] = dg;
+        red[1][wv] = da;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double tg = 0, ta = 0;
+        for (int k = 0; k < WG / 64; ++k) {
+            tg += red[0][k];
+            ta += red[1][k];
+        }
+        part[blockIdx.x] += tg;
+        part[gridDim.x + blockIdx.x] += ta;
     }
 }
 
@@ -1157,6 +1368,25 @@ struct mpcd_hip_s {
     bool use_graph = true;
     hipGraphExec_t graph[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
     int graph_launches[2] = {0, 0};
+    // adjoint runs: tape of one checkpoint segment, checkpoints, adjoint state
+    struct Adj {
+        uint32_t K = 0, nck = 0, steps = 0;
+        uint32_t* perm = nullptr;    // [K][N] recorded permutation with the bounce flag
+        float* vp = nullptr;         // [K][3N] streamed velocities
+        float* seed = nullptr;       // [K][N] loss seed
+        uint32_t* off = nullptr;     // [K][C+1] cell offsets
+        float* u = nullptr;          // [K][3C] cell means
+        float* ax = nullptr;         // [K][3C] cell axes
+        uint32_t* tot = nullptr;     // [K][C] cell totals
+        float* ckpt = nullptr;       // [nck][6N] states at the checkpoints (first: the start state)
+        float* lam[2] = {nullptr, nullptr};
+        uint32_t* lcnt = nullptr;    // [steps][Z] layer counts
+        unsigned long long* lsum = nullptr;  // [steps][Z] layer x-velocity sums, fixed point
+        double* w = nullptr;         // [Z] dL/d(layer mean) / steps
+        double* part = nullptr;      // [2][blocks] per-block gradient sums
+        double* gf = nullptr;        // [C] dL/df
+        float* force = nullptr;      // [C] body force of the run
+    } adj;
     // timing
     bool timing = false;
     std::vector<hipEvent_t> ev;
@@ -1365,12 +1595,12 @@ int launch_sort(mpcd_hip_s* h) {
     return 4;
 }
 
-template <int ALGO, bool SAMPLE, int CAP>
-void launch_collide_t(mpcd_hip_s* h, uint32_t* rec) {
+template <int ALGO, bool SAMPLE, int CAP, bool ADJ = false>
+void launch_collide_t(mpcd_hip_s* h, uint32_t* rec, Tape const& T = Tape{}) {
     SampleBufs S{h->scnt, h->ssum};
-    hipLaunchKernelGGL((k_collide<ALGO, SAMPLE, CAP>), dim3(h->ntiles), dim3(WG), 0, h->stream, h->sys, h->st[h->cur],
-                       h->st[1 - h->cur], h->perm, h->offset, h->count, h->keys, rec, S, h->status, h->axis,
-                       h->wgcnt, h->wgvel);
+    hipLaunchKernelGGL((k_collide<ALGO, SAMPLE, CAP, ADJ>), dim3(h->ntiles), dim3(WG), 0, h->stream, h->sys,
+                       h->st[h->cur], h->st[1 - h->cur], h->perm, h->offset, h->count, h->keys, rec, S, h->status,
+                       h->axis, h->wgcnt, h->wgvel, T);
 }
 
 template <int ALGO, bool SAMPLE>
@@ -1547,10 +1777,13 @@ void get_component(mpcd_hip_s* h, float* dst, int first_plane, uint64_t first, u
         for (int a = 0; a < 3; ++a) dst[3 * i + a] = tmp[a * n + i];
 }
 
+void adj_free(mpcd_hip_s* h);
+
 void destroy(mpcd_hip_s* h) {
     if (!h) return;
     if (h->stream) (void)hipStreamSynchronize(h->stream);
     drop_graphs(h);
+    adj_free(h);
     for (auto* p : {h->st[0], h->st[1]}) if (p) (void)hipFree(p);
     for (void* p : {static_cast<void*>(h->perm), static_cast<void*>(h->keys), static_cast<void*>(h->cursor), static_cast<void*>(h->count),
                     static_cast<void*>(h->offset), static_cast<void*>(h->partial), static_cast<void*>(h->status),
@@ -1564,6 +1797,244 @@ void destroy(mpcd_hip_s* h) {
     if (h->ev_end) (void)hipEventDestroy(h->ev_end);
     if (h->stream) (void)hipStreamDestroy(h->stream);
     delete h;
+}
+
+// ------------------------------------------------------------ adjoint runs
+
+void adj_free(mpcd_hip_s* h) {
+    auto& a = h->adj;
+    for (void* p : {static_cast<void*>(a.perm), static_cast<void*>(a.vp), static_cast<void*>(a.seed),
+                    static_cast<void*>(a.off), static_cast<void*>(a.u), static_cast<void*>(a.ax),
+                    static_cast<void*>(a.tot), static_cast<void*>(a.ckpt), static_cast<void*>(a.lam[0]),
+                    static_cast<void*>(a.lam[1]), static_cast<void*>(a.lcnt), static_cast<void*>(a.lsum),
+                    static_cast<void*>(a.w), static_cast<void*>(a.part), static_cast<void*>(a.gf),
+                    static_cast<void*>(a.force)})
+        if (p) (void)hipFree(p);
+    a = mpcd_hip_s::Adj{};
+}
+
+template <typename T>
+void dalloc(T*& p, uint64_t count, const char* what) {
+    ck(hipMalloc(&p, sizeof(T) * std::max<uint64_t>(count, 1)), what);
+}
+
+// buffers for `steps` steps with checkpoints every K steps (K = 0: forward only)
+void adj_reserve(mpcd_hip_s* h, uint32_t steps, uint32_t K) {
+    auto& a = h->adj;
+    uint32_t const nck = K ? (steps + K - 1) / K : 1;
+    if (a.ckpt && a.K >= K && a.nck >= nck && a.steps >= steps) return;
+    ck(hipStreamSynchronize(h->stream), "sync");
+    adj_free(h);
+    uint64_t const N = h->N, C = h->C, Z = h->kp[0].dim[2];
+    dalloc(a.ckpt, 6 * N * nck, "alloc checkpoints");
+    dalloc(a.lcnt, uint64_t(steps) * Z, "alloc layer counts");
+    dalloc(a.lsum, uint64_t(steps) * Z, "alloc layer sums");
+    dalloc(a.w, Z, "alloc layer weights");
+    dalloc(a.force, C, "alloc force field");
+    if (K) {
+        dalloc(a.perm, uint64_t(K) * N, "alloc tape");
+        dalloc(a.vp, uint64_t(K) * 3 * N, "alloc tape");
+        dalloc(a.seed, uint64_t(K) * N, "alloc tape");
+        dalloc(a.off, uint64_t(K) * (C + 1), "alloc tape");
+        dalloc(a.u, uint64_t(K) * 3 * C, "alloc tape");
+        dalloc(a.ax, uint64_t(K) * 3 * C, "alloc tape");
+        dalloc(a.tot, uint64_t(K) * C, "alloc tape");
+        dalloc(a.lam[0], 3 * N, "alloc adjoint");
+        dalloc(a.lam[1], 3 * N, "alloc adjoint");
+        dalloc(a.part, 2 * ((C + WG - 1) / WG), "alloc gradient partials");
+        dalloc(a.gf, C, "alloc force gradient");
+    }
+    a.K = K;
+    a.nck = nck;
+    a.steps = steps;
+}
+
+// one SRD step with the body force; records the tape slot `j` when given
+void adj_step(mpcd_hip_s* h, float const* force, int j) {
+    auto& a = h->adj;
+    uint64_t const N = h->N, C = h->C;
+    if (!h->keys_valid) launch_keys(h);
+    if (!h->sorted_valid) launch_sort(h);
+    if (h->blocks_cell) hipLaunchKernelGGL(k_cell_axes, dim3(h->blocks_cell), dim3(WG), 0, h->stream, h->sys, h->axis);
+    if (h->blocks_wall) hipLaunchKernelGGL(k_wall_ghosts, dim3(h->blocks_wall), dim3(WG), 0, h->stream, h->sys, h->wgcnt, h->wgvel);
+    Tape T{};
+    T.force = force;
+    uint32_t* rec = nullptr;
+    if (j >= 0) {
+        T.vp = a.vp + uint64_t(j) * 3 * N;
+        T.u = a.u + uint64_t(j) * 3 * C;
+        T.tot = a.tot + uint64_t(j) * C;
+        rec = a.perm + uint64_t(j) * N;
+    }
+    switch (h->lds_cap) {
+        case 512: launch_collide_t<0, false, 512, true>(h, rec, T); break;
+        case 1152: launch_collide_t<0, false, 1152, true>(h, rec, T); break;
+        default: launch_collide_t<0, false, 2048, true>(h, rec, T); break;
+    }
+    ck(hipGetLastError(), "adjoint-run step");
+    if (j >= 0) {
+        ck(hipMemcpyAsync(a.off + uint64_t(j) * (C + 1), h->offset, 4 * (C + 1), hipMemcpyDeviceToDevice, h->stream), "tape");
+        ck(hipMemcpyAsync(a.ax + uint64_t(j) * 3 * C, h->axis, 12 * C, hipMemcpyDeviceToDevice, h->stream), "tape");
+    }
+    hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, h->stream, h->sys);
+    h->cur = 1 - h->cur;
+    h->step_index++;
+    h->keys_valid = true;
+    h->sorted_valid = false;
+}
+
+uint32_t layer_blocks(mpcd_hip_s* h) {
+    return static_cast<uint32_t>(std::max<uint64_t>(1, std::min<uint64_t>((h->N + WG - 1) / WG, 4096)));
+}
+
+struct ProfileResult {
+    double loss = 0, grad_drag = 0, grad_angle = 0;
+    std::vector<double> profile;
+};
+
+// Loss (and with K > 0 its pathwise gradient) of `steps` SRD steps from the
+// current state with drag g, rotation angle and body force f:
+//   L = mean over fluid z layers of (<vbar_x>(z) - target(z))^2,
+//   <vbar_x>(z) = mean over the steps of the layer's mean x velocity after each step.
+// The handle's state, step index and parameters are restored afterwards.
+ProfileResult profile_run(mpcd_hip_s* h, int steps, int K, float drag, float angle, float const* force_host,
+                          double const* target, size_t n_layers, float* grad_force) {
+    if (h->ns != 1) throw HipError{"adjoint runs take a single system"};
+    if (h->algorithm != 0) throw HipError{"adjoint runs implement the SRD rule"};
+    if (steps < 1) throw HipError{"steps must be positive"};
+    if (K < 0) throw HipError{"checkpoint interval must be positive"};
+    if (h->N >= BOUNCED) throw HipError{"adjoint runs take fewer than 2^31 particles"};
+    KParams& P = h->kp[0];
+    uint32_t const Z = P.dim[2];
+    if (Z > static_cast<uint32_t>(LAYERS_MAX)) throw HipError{"adjoint runs take at most 1024 z layers"};
+    uint32_t const zlo = P.perio_z ? 0u : 1u, zhi = P.perio_z ? Z : Z - 1;
+    if (n_layers != zhi - zlo)
+        throw HipError{"target holds " + std::to_string(n_layers) + " layers, the fluid has " + std::to_string(zhi - zlo)};
+    uint32_t const T = static_cast<uint32_t>(steps);
+    uint32_t const Kc = K ? std::min<uint32_t>(static_cast<uint32_t>(K), T) : 0u;
+    adj_reserve(h, T, Kc);
+    auto& a = h->adj;
+    uint64_t const N = h->N, C = h->C;
+    float const* force = nullptr;
+    if (force_host) {
+        ck(hipStreamSynchronize(h->stream), "sync");
+        ck(hipMemcpy(a.force, force_host, 4 * C, hipMemcpyHostToDevice), "upload force field");
+        force = a.force;
+    }
+
+    // save the start state and parameters, set the run's parameters
+    uint64_t const s0 = h->step_index;
+    KParams const saved = P;
+    ck(hipMemcpyAsync(a.ckpt, h->st[h->cur], 24 * N, hipMemcpyDeviceToDevice, h->stream), "checkpoint");
+    P.drag_dt = drag * P.dt;
+    P.sin_a = sinf((M_PI * static_cast<double>(angle)) / 180);
+    P.cos_a = cosf((M_PI * static_cast<double>(angle)) / 180);
+    auto restore = [&]() {
+        P = saved;
+        ck(hipMemcpyAsync(h->st[h->cur], a.ckpt, 24 * N, hipMemcpyDeviceToDevice, h->stream), "restore");
+        h->step_index = s0;
+        h->keys_valid = false;
+        h->sorted_valid = false;
+        upload_table(h);
+    };
+    ProfileResult res;
+    try {
+        h->keys_valid = false;
+        h->sorted_valid = false;
+        upload_table(h);
+        // forward pass: layer statistics of every step, checkpoints
+        ck(hipMemsetAsync(a.lcnt, 0, 4ull * T * Z, h->stream), "clear");
+        ck(hipMemsetAsync(a.lsum, 0, 8ull * T * Z, h->stream), "clear");
+        uint32_t const lb = layer_blocks(h);
+        for (uint32_t t = 1; t <= T; ++t) {
+            adj_step(h, force, -1);
+            hipLaunchKernelGGL(k_layers, dim3(lb), dim3(WG), 0, h->stream, h->sys, h->st[h->cur], 0,
+                               a.lcnt + uint64_t(t - 1) * Z, a.lsum + uint64_t(t - 1) * Z, nullptr, nullptr);
+            if (Kc && t % Kc == 0 && t < T)
+                ck(hipMemcpyAsync(a.ckpt + uint64_t(t / Kc) * 6 * N, h->st[h->cur], 24 * N, hipMemcpyDeviceToDevice,
+                                  h->stream), "checkpoint");
+        }
+        ck(hipGetLastError(), "forward pass");
+        std::vector<uint32_t> cnt(uint64_t(T) * Z);
+        std::vector<unsigned long long> sum(uint64_t(T) * Z);
+        ck(hipStreamSynchronize(h->stream), "forward pass");
+        uint32_t st = 0;
+        ck(hipMemcpy(&st, h->status, 4, hipMemcpyDeviceToHost), "status");
+        if (st & 1u) {
+            ck(hipMemset(h->status, 0, 4), "status");
+            throw HipError{"a cell holds more particles than one LDS tile"};
+        }
+        ck(hipMemcpy(cnt.data(), a.lcnt, 4ull * T * Z, hipMemcpyDeviceToHost), "download layers");
+        ck(hipMemcpy(sum.data(), a.lsum, 8ull * T * Z, hipMemcpyDeviceToHost), "download layers");
+        res.profile.assign(zhi - zlo, 0.0);
+        for (uint32_t t = 0; t < T; ++t)
+            for (uint32_t z = zlo; z < zhi; ++z) {
+                uint32_t const c = cnt[uint64_t(t) * Z + z];
+                double const m = c ? (static_cast<double>(static_cast<long long>(sum[uint64_t(t) * Z + z])) / FIX_SCALE) / c : 0.0;
+                res.profile[z - zlo] += m / T;
+            }
+        std::vector<double> w(Z, 0.0);
+        double const nl = static_cast<double>(zhi - zlo);
+        for (uint32_t z = zlo; z < zhi; ++z) {
+            double const d = res.profile[z - zlo] - target[z - zlo];
+            res.loss += d * d / nl;
+            w[z] = 2.0 * d / nl / T;
+        }
+        if (Kc) {
+            // reverse pass, one checkpoint segment at a time (recomputed with the tape)
+            ck(hipMemcpy(a.w, w.data(), 8ull * Z, hipMemcpyHostToDevice), "upload layer weights");
+            uint32_t const gb = static_cast<uint32_t>((C + WG - 1) / WG);
+            ck(hipMemsetAsync(a.lam[0], 0, 12 * N, h->stream), "clear");
+            ck(hipMemsetAsync(a.part, 0, 16ull * gb, h->stream), "clear");
+            ck(hipMemsetAsync(a.gf, 0, 8 * C, h->stream), "clear");
+            int li = 0;
+            uint32_t const nseg = (T + Kc - 1) / Kc;
+            for (int seg = static_cast<int>(nseg) - 1; seg >= 0; --seg) {
+                uint32_t const t0 = static_cast<uint32_t>(seg) * Kc, t1 = std::min(T, t0 + Kc);
+                ck(hipMemcpyAsync(h->st[h->cur], a.ckpt + uint64_t(seg) * 6 * N, 24 * N, hipMemcpyDeviceToDevice,
+                                  h->stream), "restore checkpoint");
+                h->step_index = s0 + t0;
+                h->keys_valid = false;
+                h->sorted_valid = false;
+                upload_table(h);
+                for (uint32_t t = t0 + 1; t <= t1; ++t) {
+                    int const j = static_cast<int>(t - t0 - 1);
+                    adj_step(h, force, j);
+                    hipLaunchKernelGGL(k_layers, dim3(lb), dim3(WG), 0, h->stream, h->sys, h->st[h->cur], 1,
+                                       a.lcnt + uint64_t(t - 1) * Z, nullptr, a.w, a.seed + uint64_t(j) * N);
+                }
+                for (uint32_t t = t1; t > t0; --t) {
+                    uint64_t const j = t - t0 - 1;
+                    hipLaunchKernelGGL(k_adjoint_step, dim3(gb), dim3(WG), 0, h->stream, h->sys, a.off + j * (C + 1),
+                                       a.u + j * 3 * C, a.ax + j * 3 * C, a.tot + j * C, a.perm + j * N,
+                                       a.vp + j * 3 * N, a.seed + j * N, a.lam[li], a.lam[1 - li], a.part, a.gf);
+                    li = 1 - li;
+                }
+                ck(hipGetLastError(), "reverse pass");
+            }
+            std::vector<double> part(2ull * gb), gf(C);
+            ck(hipStreamSynchronize(h->stream), "reverse pass");
+            ck(hipMemcpy(part.data(), a.part, 16ull * gb, hipMemcpyDeviceToHost), "download gradients");
+            for (uint32_t b = 0; b < gb; ++b) {
+                res.grad_drag += part[b];
+                res.grad_angle += part[gb + b];
+            }
+            res.grad_angle *= M_PI / 180.0;  // per degree
+            if (grad_force) {
+                ck(hipMemcpy(gf.data(), a.gf, 8 * C, hipMemcpyDeviceToHost), "download force gradient");
+                for (uint64_t c = 0; c < C; ++c) grad_force[c] = static_cast<float>(gf[c]);
+            }
+        }
+        restore();
+        ck(hipStreamSynchronize(h->stream), "restore");
+    } catch (...) {
+        try {
+            restore();
+        } catch (...) {
+        }
+        throw;
+    }
+    return res;
 }
 
 std::string validate(mpcd_params_t const& p) {
@@ -1857,6 +2328,35 @@ int mpcd_hip_get_tiling(const mpcd_hip_t* h, uint32_t* cells_per_tile, uint32_t*
     if (cells_per_tile) *cells_per_tile = h->kp[0].tile_cells;
     if (lds_capacity) *lds_capacity = h->lds_cap;
     return 0;
+}
+
+/* ---- adjoint runs ---- */
+
+int mpcd_hip_profile_loss(mpcd_hip_t* h, int steps, float drag, float rotation_angle, const float* force,
+                          const double* target, size_t n_layers, double* loss, double* profile) {
+    return guarded(h, [&] {
+        ck(hipSetDevice(h->device), "hipSetDevice");
+        if (!target) throw HipError{"target is required"};
+        ProfileResult const r = profile_run(h, steps, 0, drag, rotation_angle, force, target, n_layers, nullptr);
+        if (loss) *loss = r.loss;
+        if (profile) std::copy(r.profile.begin(), r.profile.end(), profile);
+    });
+}
+
+int mpcd_hip_profile_loss_grad(mpcd_hip_t* h, int steps, int checkpoint_every, float drag, float rotation_angle,
+                               const float* force, const double* target, size_t n_layers, double* loss,
+                               double* profile, double* grad_drag, double* grad_angle, float* grad_force) {
+    return guarded(h, [&] {
+        ck(hipSetDevice(h->device), "hipSetDevice");
+        if (!target) throw HipError{"target is required"};
+        if (checkpoint_every < 1) throw HipError{"checkpoint_every must be positive"};
+        ProfileResult const r =
+            profile_run(h, steps, checkpoint_every, drag, rotation_angle, force, target, n_layers, grad_force);
+        if (loss) *loss = r.loss;
+        if (profile) std::copy(r.profile.begin(), r.profile.end(), profile);
+        if (grad_drag) *grad_drag = r.grad_drag;
+        if (grad_angle) *grad_angle = r.grad_angle;
+    });
 }
 
 /* ---- ensembles ---- */

@@ -112,6 +112,31 @@ int mpcd_hip_last_timing(const mpcd_hip_t* h, double* collide_ms, double* total_
  * depend on it. */
 int mpcd_hip_set_tiling(mpcd_hip_t* h, uint32_t cells_per_tile, uint32_t lds_capacity);
 int mpcd_hip_get_tiling(const mpcd_hip_t* h, uint32_t* cells_per_tile, uint32_t* lds_capacity);
+/* ---- pathwise adjoint (no reference counterpart) ----
+ *
+ * Loss of `steps` SRD steps of a single-system handle from its current state,
+ * with drag g, rotation angle (degrees) and an x body force f per reference
+ * cell (v_x += g dt + f dt for the particles that collide; f = NULL: none):
+ *   L = mean over the fluid z layers of (<vbar_x>(z) - target(z))^2,
+ * <vbar_x>(z) = mean over the steps of the mean x velocity of the particles
+ * whose stored position lies in layer z after the step.  target holds one
+ * value per fluid layer (n_layers = Z - 2 with walls, Z without), host memory.
+ * The gradient is pathwise: cells, cell tables, wall bounces, rotation axes
+ * and ghosts of the trajectory held fixed (the step is then linear in the
+ * velocities); the trajectory is recomputed from checkpoints taken every
+ * checkpoint_every steps.  force and grad_force hold C floats in reference
+ * cell order, in host memory (a process may hold a second HIP runtime, e.g.
+ * torch's, whose device allocations this library cannot address safely);
+ * NULL: no force / no force gradient.  The handle's state,
+ * step index and parameters are unchanged afterwards (the same call repeats
+ * the same trajectory: common random numbers).  profile (n_layers doubles,
+ * host) may be NULL. */
+int mpcd_hip_profile_loss(mpcd_hip_t* h, int steps, float drag, float rotation_angle, const float* force,
+                          const double* target, size_t n_layers, double* loss, double* profile);
+int mpcd_hip_profile_loss_grad(mpcd_hip_t* h, int steps, int checkpoint_every, float drag, float rotation_angle,
+                               const float* force, const double* target, size_t n_layers, double* loss,
+                               double* profile, double* grad_drag, double* grad_angle, float* grad_force);
+
 /* ---- ensembles (no reference counterpart: the reference runs one system per
  * process, cpu_backend.cpp / cuda_backend_impl.cu) ---- */
 

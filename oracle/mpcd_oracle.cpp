@@ -381,6 +381,9 @@ struct Oracle {
     bool fixed_shift = false;  // known-answer tests: use `forced` instead of a drawn shift
     V3 forced = {0, 0, 0};
     float sin_a, cos_a;
+    // x body force per cell, v_x += g dt + f dt in the SRD rule (empty: none;
+    // the reference has no force field)
+    std::vector<float> force;
 
     // sampling, reference semantics (cpu_backend.cpp:72-122) in FP32 Kahan
     bool sampling_done = true;
@@ -540,10 +543,11 @@ struct Oracle {
                     V3 vperp = sub(v, vpar);
                     bv[i] = add(add(vpar, mul(vperp, cos_a)), mul(cross(vperp, axis), sin_a));
                 }
+                float const fdt = force.empty() ? 0.0f : force[c] * p.delta_t;
                 for (uint32_t i = 0; i < total; ++i) {
                     bv[i] = add(bv[i], mean);
                     bp[i] = wrap(sub(bp[i], shift));
-                    bv[i].x += drag;
+                    if (force.empty()) bv[i].x += drag; else bv[i].x += drag + fdt;
                 }
             }
             for (uint32_t i = 0; i < m; ++i) {
@@ -780,6 +784,112 @@ struct Oracle {
         }
     }
 
+    // Frozen-assignment model of an SRD run (counter mode), for checking a
+    // pathwise gradient: the real trajectory is stepped `steps` times with the
+    // oracle's own parameters, and beside it a double-precision "shadow"
+    // velocity of every particle follows the same cells, tables, bounces, axes
+    // and ghosts with the drag g, rotation angle and force field given here:
+    //   v' = -v if the particle bounced, else v;
+    //   table particles of cells with total > 1: v'' = u + R(angle)(v' - u) + (g + f_c) dt e_x,
+    //   u = (sum of table v' + ghost velocities) / total;   others: v'' = v'.
+    // Returns L = mean over the z layers [zlo, zhi) of (profile(z) - target(z))^2
+    // with profile(z) the mean over steps of the layer's mean x velocity
+    // (layer = z cell of the real position after the step), computed from the
+    // shadow velocities, or from the real ones when use_real is set.  The
+    // oracle's state is restored afterwards.
+    double frozen_profile_loss(int steps, double g, double angle_deg, const double* f, const double* target,
+                               uint32_t zlo, uint32_t zhi, int use_real, double* profile_out) {
+        if (p.rng_mode != 1 || p.algorithm != 0) return NAN;  // the shadow redraws: stateless draws, SRD only
+        std::vector<Particle> const saved = parts;
+        uint64_t const saved_step = step_index;
+        size_t const Np = parts.size();
+        std::vector<double> sv(3 * Np);
+        for (size_t i = 0; i < Np; ++i) {
+            sv[3 * i] = parts[i].vel.x;
+            sv[3 * i + 1] = parts[i].vel.y;
+            sv[3 * i + 2] = parts[i].vel.z;
+        }
+        double const ca = std::cos(M_PI * angle_deg / 180.0), sa = std::sin(M_PI * angle_deg / 180.0);
+        double const dt = p.delta_t;
+        std::vector<double> prof(zhi - zlo, 0.0);
+        std::vector<uint8_t> bounced(Np);
+        for (int t = 0; t < steps; ++t) {
+            if (!p.periodic[2]) {
+                float const z_wall = (0.5f * L.z - 1);
+                for (size_t i = 0; i < Np; ++i)
+                    bounced[i] = fabsf(parts[i].pos.z + parts[i].vel.z * p.delta_t) > z_wall;
+            }
+            translate();
+            for (size_t i = 0; i < Np; ++i)
+                if (bounced[i])
+                    for (int k = 0; k < 3; ++k) sv[3 * i + k] = -sv[3 * i + k];
+            // shadow collision with the draws of this step
+            float const sh = fabsf(shift.z);
+            bool const sign = shift.z > 0;
+            uint32_t const step = static_cast<uint32_t>(step_index);
+            for (uint32_t c = 0; c < C; ++c) {
+                uint32_t const m = std::min(cap, counter[c]);
+                bool layer, ghosts;
+                wall_flags(c, sign, &layer, &ghosts);
+                uint32_t added = 0;
+                if (ghosts)
+                    for (uint32_t i = 0; i < p.n; ++i)
+                        if ((rng.ghost_count_u(step, c, i) > sh) ^ layer ^ sign) ++added;
+                uint32_t const total = m + added;
+                if (total <= 1) continue;
+                double u[3] = {0, 0, 0};
+                for (uint32_t i = 0; i < m; ++i) {
+                    size_t const q = list[c + static_cast<size_t>(i) * C];
+                    for (int k = 0; k < 3; ++k) u[k] += sv[3 * q + k];
+                }
+                for (uint32_t i = 0; i < added; ++i) {
+                    V3 const gv = mul(rng.ghost_vel(step, c, i), p.thermal_velocity);
+                    u[0] += gv.x;
+                    u[1] += gv.y;
+                    u[2] += gv.z;
+                }
+                for (double& x : u) x /= total;
+                V3 const a3 = rng.srd_axis(step, c);
+                double const a[3] = {a3.x, a3.y, a3.z};
+                double const kick = (g + (f ? f[c] : 0.0)) * dt;
+                for (uint32_t i = 0; i < m; ++i) {
+                    size_t const q = list[c + static_cast<size_t>(i) * C];
+                    double w[3], par[3], perp[3];
+                    for (int k = 0; k < 3; ++k) w[k] = sv[3 * q + k] - u[k];
+                    double const d = w[0] * a[0] + w[1] * a[1] + w[2] * a[2];
+                    for (int k = 0; k < 3; ++k) {
+                        par[k] = a[k] * d;
+                        perp[k] = w[k] - par[k];
+                    }
+                    double const cr[3] = {perp[1] * a[2] - perp[2] * a[1], perp[2] * a[0] - perp[0] * a[2],
+                                          perp[0] * a[1] - perp[1] * a[0]};
+                    for (int k = 0; k < 3; ++k) sv[3 * q + k] = u[k] + par[k] + ca * perp[k] + sa * cr[k];
+                    sv[3 * q] += kick;
+                }
+            }
+            srd();
+            ++step_index;
+            std::vector<double> lsum(Z, 0.0);
+            std::vector<uint32_t> lcnt(Z, 0u);
+            for (size_t i = 0; i < Np; ++i) {
+                uint32_t const iz = static_cast<uint32_t>(floorf(parts[i].pos.z + half.z)) % Z;
+                lcnt[iz] += 1;
+                lsum[iz] += use_real ? static_cast<double>(parts[i].vel.x) : sv[3 * i];
+            }
+            for (uint32_t z = zlo; z < zhi; ++z)
+                if (lcnt[z]) prof[z - zlo] += lsum[z] / lcnt[z] / steps;
+        }
+        double loss = 0.0;
+        for (uint32_t z = zlo; z < zhi; ++z) {
+            double const d = prof[z - zlo] - target[z - zlo];
+            loss += d * d / (zhi - zlo);
+        }
+        if (profile_out) std::copy(prof.begin(), prof.end(), profile_out);
+        parts = saved;
+        step_index = saved_step;
+        return loss;
+    }
+
     // cpu_backend.cpp:111-122 (sum over S samples times 1/(S-1))
     void sample_mean(float* dens, float* vel) {
         double inverse = 1.0 / sample_counter;
@@ -862,6 +972,15 @@ unsigned long long orc_get_cells(void* h, unsigned* cell_of_particle, unsigned* 
     return k;
 }
 void orc_step(void* h, int n) { static_cast<Oracle*>(h)->step(n); }
+// x body force per cell (n_cells floats; NULL removes it)
+void orc_set_force(void* h, const float* f) {
+    Oracle* o = static_cast<Oracle*>(h);
+    if (f) o->force.assign(f, f + o->C); else o->force.clear();
+}
+double orc_frozen_profile_loss(void* h, int steps, double drag, double angle_deg, const double* force,
+                               const double* target, unsigned zlo, unsigned zhi, int use_real, double* profile) {
+    return static_cast<Oracle*>(h)->frozen_profile_loss(steps, drag, angle_deg, force, target, zlo, zhi, use_real, profile);
+}
 void orc_step_and_sample(void* h, int n) { static_cast<Oracle*>(h)->step_and_sample(n); }
 void orc_get_sample_mean(void* h, float* dens, float* vel) { static_cast<Oracle*>(h)->sample_mean(dens, vel); }
 void orc_get_sample_mean_f64(void* h, double* dens, double* vel) { static_cast<Oracle*>(h)->sample_mean_f64(dens, vel); }

@@ -102,6 +102,11 @@ def oracle_lib() -> ctypes.CDLL:
         lib.orc_counter_draws.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_uint, ctypes.c_uint, ctypes.c_uint, c_float_p]
         lib.orc_ext_direction.argtypes = [ctypes.c_int, c_float_p]
         lib.orc_sym_inverse.argtypes = [c_float_p, ctypes.c_float, c_float_p]
+        lib.orc_set_force.argtypes = [ctypes.c_void_p, c_float_p]
+        lib.orc_frozen_profile_loss.restype = ctypes.c_double
+        lib.orc_frozen_profile_loss.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                                c_double_p, c_double_p, ctypes.c_uint, ctypes.c_uint, ctypes.c_int,
+                                                c_double_p]
         _oracle = lib
     return _oracle
 
@@ -234,6 +239,27 @@ class Oracle:
         v = np.empty(3 * self.C, np.float64)
         self.lib.orc_get_sample_mean_f64(self.h, _ptr(d, c_double_p), _ptr(v, c_double_p))
         return d, v
+
+    def set_force(self, force):
+        """x body force per cell (reference cell order) applied in the SRD rule; None removes it."""
+        if force is None:
+            self.lib.orc_set_force(self.h, None)
+        else:
+            self._force = np.ascontiguousarray(force, dtype=np.float32)
+            self.lib.orc_set_force(self.h, _ptr(self._force, c_float_p))
+
+    def frozen_profile_loss(self, steps, drag, angle_deg, target, zlo, zhi, force=None, use_real=False):
+        """Profile loss of `steps` steps from the current state with the cells, tables,
+        bounces and draws of the oracle's own trajectory held fixed and the velocities
+        carried in float64 with the given drag, angle and force (see mpcd_oracle.cpp).
+        Returns (loss, profile); the oracle's state is unchanged."""
+        t = np.ascontiguousarray(target, dtype=np.float64)
+        prof = np.empty(zhi - zlo, np.float64)
+        f = None if force is None else np.ascontiguousarray(force, dtype=np.float64)
+        loss = self.lib.orc_frozen_profile_loss(self.h, steps, drag, angle_deg,
+                                                None if f is None else _ptr(f, c_double_p),
+                                                _ptr(t, c_double_p), zlo, zhi, int(use_real), _ptr(prof, c_double_p))
+        return loss, prof
 
 
 class Reference:

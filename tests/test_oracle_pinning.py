@@ -349,3 +349,73 @@ def test_oracle_poiseuille_viscosity_sanity():
     assert abs(eta_curv - theory) / theory < 0.08, (eta_curv, theory)
     assert np.abs(np.abs(walls) - L / 2).max() < 0.5      # no-slip walls at +-L/2
     assert abs(d.reshape(L + 2, 8, 8)[2:L].mean() - n) < 0.1
+
+
+# ---- frozen-assignment model (reference gradient of the pathwise adjoint) ----
+
+def _frozen_setup(seed=31):
+    volume, periodic, n = (6, 5, 10), (1, 1, 0), 8
+    o = Oracle(volume, periodic, n=n, drag=0.002, delta_t=0.02, rotation_deg=120.0, rng="counter", seed=(seed, 0))
+    o.step(3)
+    zlo, zhi = 1, volume[2] - 1
+    target = np.linspace(-0.02, 0.03, zhi - zlo)
+    return o, target, zlo, zhi
+
+
+def test_frozen_model_follows_the_real_trajectory():
+    """At the oracle's own parameters the float64 shadow velocities are the real
+    (bit-exact reference) velocities up to rounding: same loss and profile."""
+    o, target, zlo, zhi = _frozen_setup()
+    before = o.particles()
+    lr, pr = o.frozen_profile_loss(12, 0.002, 120.0, target, zlo, zhi, use_real=True)
+    ls, ps = o.frozen_profile_loss(12, 0.002, 120.0, target, zlo, zhi)
+    assert np.abs(pr - ps).max() < 1e-6
+    assert abs(lr - ls) <= 1e-5 * abs(lr)
+    after = o.particles()
+    assert np.array_equal(before[0], after[0]) and np.array_equal(before[1], after[1])
+    # and the real trajectory equals stepping the oracle and averaging the layers itself
+    prof = np.zeros(zhi - zlo)
+    q = Oracle((6, 5, 10), (1, 1, 0), n=8, drag=0.002, delta_t=0.02, rng="counter", seed=(31, 0))
+    q.step(3)
+    for _ in range(12):
+        q.step(1)
+        pos, vel = q.particles()
+        iz = np.floor(pos[:, 2] + np.float32(5.0)).astype(np.int64) % 10
+        for z in range(zlo, zhi):
+            prof[z - zlo] += vel[iz == z, 0].astype(np.float64).mean() / 12
+    assert np.abs(prof - pr).max() < 1e-12
+    assert abs(np.mean((prof - target) ** 2) - lr) < 1e-12
+
+
+def test_frozen_model_is_affine_in_the_force_terms():
+    """Shadow velocities are affine in g and f: the loss is an exact quadratic in g,
+    and a uniform force field equals the same drag."""
+    o, target, zlo, zhi = _frozen_setup()
+    h = 5e-4
+    L = [o.frozen_profile_loss(8, 0.002 + k * h, 120.0, target, zlo, zhi)[0] for k in range(-2, 3)]
+    d2a = L[0] - 2 * L[1] + L[2]
+    d2b = L[2] - 2 * L[3] + L[4]
+    assert abs(d2a - d2b) < 1e-9 * max(1.0, abs(d2a)) + 1e-14
+    assert d2a > 0
+    C = o.C
+    f = np.full(C, 7e-4)
+    lf = o.frozen_profile_loss(8, 0.002, 120.0, target, zlo, zhi, force=f)[0]
+    lg = o.frozen_profile_loss(8, 0.002 + 7e-4, 120.0, target, zlo, zhi)[0]
+    assert lf == lg
+
+
+def test_oracle_force_field_matches_drag():
+    """The oracle's float force path: a uniform f with g' = g - f gives v_x += (g' + f) dt,
+    identical to drag g when (g' dt + f dt) rounds to g dt; f = 0 is the plain rule bit for bit."""
+    o1 = Oracle((6, 5, 10), (1, 1, 0), n=8, drag=0.002, delta_t=0.02, rng="counter", seed=(5, 0))
+    o2 = Oracle((6, 5, 10), (1, 1, 0), n=8, drag=0.002, delta_t=0.02, rng="counter", seed=(5, 0))
+    o2.set_force(np.zeros(o2.C, np.float32))
+    o1.step(6)
+    o2.step(6)
+    assert np.array_equal(o1.particles()[1], o2.particles()[1])
+    o2.set_force(np.full(o2.C, 0.001, np.float32))
+    o1.step(4)
+    o2.step(4)
+    # force adds momentum n_collided * f dt per step in x
+    dv = o2.particles()[1][:, 0].astype(np.float64).sum() - o1.particles()[1][:, 0].astype(np.float64).sum()
+    assert dv > 0
