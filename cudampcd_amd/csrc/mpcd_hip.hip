@@ -18,9 +18,10 @@
 //             cells (fused histogram) and, when sampling, accumulates the mean
 //             fields in fixed point.
 //
-// Particle state is structure-of-arrays FP32 (x, y, z, vx, vy, vz planes,
-// 24 B per particle), double-buffered; the collide launch reads one buffer
-// and writes the other.  All floating-point code is compiled without
+// Particle state is one FP32 record per particle (x, y, z, vx, vy, vz: 24 B),
+// double-buffered; the collide launch reads one buffer and writes the other.
+// The gather of a particle then touches one or two cache lines instead of
+// six, and the cell-ordered write-out stays contiguous.  All floating-point code is compiled without
 // contraction so that per-particle arithmetic matches the CPU restatement
 // bit for bit.
 
@@ -84,6 +85,22 @@ struct Sys {
     uint32_t CT;                 // cells of all systems (stride of the axis planes)
     uint64_t NT;                 // particles of all systems (stride of the state planes)
 };
+
+// the 6 floats of particle i: x y z vx vy vz
+__host__ __device__ __forceinline__ uint64_t rec(uint64_t i) { return 6 * i; }
+
+__device__ __forceinline__ void load_rec(const float* __restrict__ st, uint64_t i, float* v) {
+    float2 const* const q = reinterpret_cast<float2 const*>(st + rec(i));
+    float2 const a = q[0], b = q[1], c = q[2];
+    v[0] = a.x; v[1] = a.y; v[2] = b.x; v[3] = b.y; v[4] = c.x; v[5] = c.y;
+}
+__device__ __forceinline__ void store_rec(float* __restrict__ st, uint64_t i, float px, float py, float pz, float vx,
+                                          float vy, float vz) {
+    float2* const q = reinterpret_cast<float2*>(st + rec(i));
+    q[0] = make_float2(px, py);
+    q[1] = make_float2(pz, vx);
+    q[2] = make_float2(vy, vz);
+}
 
 // system whose block range holds block b (base is non-decreasing, base[0] = 0)
 __device__ __forceinline__ uint32_t find_sys(uint32_t const* base, uint32_t ns, uint32_t b) {
@@ -297,7 +314,6 @@ __global__ __launch_bounds__(WG) void k_init(Sys sys, float* __restrict__ st) {
     uint32_t const s = find_sys(sys.blk_part, sys.ns, blockIdx.x);
     KParams const P = sys.P[s];
     uint32_t const b0 = sys.blk_part[s], nb = sys.blk_part[s + 1] - b0;
-    uint64_t const NT = sys.NT;
     float const sx = P.L[0], sy = P.L[1], sz = P.perio_z ? P.L[2] : P.L[2] - 2.0f;
     for (uint64_t i = static_cast<uint64_t>(blockIdx.x - b0) * WG + threadIdx.x; i < P.N;
          i += static_cast<uint64_t>(nb) * WG) {
@@ -308,12 +324,8 @@ __global__ __launch_bounds__(WG) void k_init(Sys sys, float* __restrict__ st) {
         box_muller(b.w[0], b.w[1], &g0, &g1);
         box_muller(b.w[2], b.w[3], &g2, &g3);
         uint64_t const j = P.pb + i;
-        st[j] = (uniform(a.w[0]) - 0.5f) * sx;
-        st[NT + j] = (uniform(a.w[1]) - 0.5f) * sy;
-        st[2 * NT + j] = (uniform(a.w[2]) - 0.5f) * sz;
-        st[3 * NT + j] = g0 * P.tv;
-        st[4 * NT + j] = g1 * P.tv;
-        st[5 * NT + j] = g2 * P.tv;
+        store_rec(st, j, (uniform(a.w[0]) - 0.5f) * sx, (uniform(a.w[1]) - 0.5f) * sy, (uniform(a.w[2]) - 0.5f) * sz,
+                  g0 * P.tv, g1 * P.tv, g2 * P.tv);
     }
 }
 
@@ -323,7 +335,6 @@ __global__ __launch_bounds__(WG) void k_keys(Sys sys, const float* __restrict__ 
     uint32_t const s = find_sys(sys.blk_part, sys.ns, blockIdx.x);
     KParams const P = sys.P[s];
     uint32_t const b0 = sys.blk_part[s], nb = sys.blk_part[s + 1] - b0;
-    uint64_t const NT = sys.NT;
     // the loop bound is uniform: wave_count needs whole waves
     for (uint64_t i0 = static_cast<uint64_t>(blockIdx.x - b0) * WG; i0 < P.N; i0 += static_cast<uint64_t>(nb) * WG) {
         uint64_t const i = i0 + threadIdx.x;
@@ -331,8 +342,9 @@ __global__ __launch_bounds__(WG) void k_keys(Sys sys, const float* __restrict__ 
         uint64_t const j = P.pb + i;
         uint32_t key = 0;
         if (valid) {
-            float px = st[j], py = st[NT + j], pz = st[2 * NT + j];
-            float vx = st[3 * NT + j], vy = st[4 * NT + j], vz = st[5 * NT + j];
+            float v[6];
+            load_rec(st, j, v);
+            float px = v[0], py = v[1], pz = v[2], vx = v[3], vy = v[4], vz = v[5];
             stream_shift(P, P.shift, px, py, pz, vx, vy, vz);
             uint32_t ix, iy, iz;
             cell_of(P, px, py, pz, ix, iy, iz);
@@ -593,6 +605,10 @@ __global__ void k_advance(Sys sys) {
 // TCM cells.  Sizes are compile-time so every LDS access is an immediate offset.
 constexpr int TCM = 64;
 
+// threads of the collide workgroup for an LDS capacity: small tiles run as
+// one-wave workgroups (many independent tiles per CU), large ones as four waves
+constexpr int tile_threads(int cap) { return cap <= 384 ? 64 : 256; }
+
 template <int CAP, int ALGO>
 struct Tile {
     float p[6][CAP];                 // x y z vx vy vz, cell order
@@ -604,7 +620,7 @@ struct Tile {
     float cforce[TCM];               // adjoint runs: force term f dt of each cell
     uint32_t gcnt[TCM];              // ghost count
     uint32_t goff[TCM];              // first ghost slot
-    uint32_t sw[WG / 64];            // scan scratch
+    uint32_t sw[4];                  // scan scratch
     uint16_t cell[CAP];              // tile-local cell of each slot
     float g[ALGO == 1 ? 6 : 1][ALGO == 1 ? CAP / 2 : CAP / 2];  // extended: ghosts, slot k*n + i;
                                                                // SRD: staged ghost velocities
@@ -620,7 +636,8 @@ struct Tile {
 template <int CAP, bool ADJ>
 __device__ void srd_cells(KParams const& P, Tile<CAP, 0>& s, uint32_t ncell, uint32_t const* off, uint32_t gb,
                           float const* __restrict__ gvel, uint32_t key0, Tape const& T, uint64_t CT) {
-    for (uint32_t k = threadIdx.x; k < ncell; k += WG) {
+    constexpr uint32_t NT = tile_threads(CAP);
+    for (uint32_t k = threadIdx.x; k < ncell; k += NT) {
         uint32_t const a = off[k] - gb, mc = off[k + 1] - gb - a;
         uint32_t const reals = mc < P.cap ? mc : P.cap;
         uint32_t const added = s.gcnt[k];
@@ -819,14 +836,22 @@ __device__ void extended_cell(KParams const& P, Tile<CAP, 1>& s, uint32_t k, uin
 
 constexpr int GQ = 4;               // particles gathered per thread per round
 
+// stride of the middle internal axis at local key `key` (cells of one block
+// of the slowest axis), the key distance of a neighbour along that axis
+__device__ __forceinline__ uint32_t mid_stride(KParams const& P, uint32_t key) {
+    uint32_t const b = P.mBLK[0] || P.mBLK[1] ? udiv(key, P.mBLK) : key / P.BLK;
+    return (b + 1 == P.nblk ? P.Blast : (1u << P.Bsh)) * P.D0;
+}
+
 template <int ALGO, bool SAMPLE, int CAP, bool ADJ = false>
-__global__ __launch_bounds__(WG, (ALGO == 0 ? 4 : 3)) void k_collide(Sys sys, const float* __restrict__ in, float* __restrict__ out,
+__global__ __launch_bounds__(tile_threads(CAP), (ALGO == 0 ? 1024 : 768) / tile_threads(CAP)) void k_collide(Sys sys, const float* __restrict__ in, float* __restrict__ out,
                                                 const uint32_t* __restrict__ perm, const uint32_t* __restrict__ offset,
                                                 uint32_t* __restrict__ count_next, uint32_t* __restrict__ keys,
                                                 uint32_t* __restrict__ perm_out, SampleBufs S,
                                                 uint32_t* __restrict__ status, float const* __restrict__ axis,
                                                 uint32_t const* __restrict__ gcnt, float const* __restrict__ gvel,
                                                 Tape T) {
+    constexpr uint32_t NT = tile_threads(CAP);
     __shared__ Tile<CAP, ALGO> s;
     uint32_t const tile = xcd_tile(blockIdx.x, gridDim.x);
     uint32_t const sy = sys.ns > 1 ? sys.tile_sys[tile] : 0u;
@@ -840,7 +865,7 @@ __global__ __launch_bounds__(WG, (ALGO == 0 ? 4 : 3)) void k_collide(Sys sys, co
     if (c0 >= P.C) return;
     uint32_t const nc = min(TC, P.C - c0);
     uint32_t const g0 = P.cb + c0;
-    for (uint32_t k = tid; k <= nc; k += WG) s.off[k] = offset[g0 + k];
+    for (uint32_t k = tid; k <= nc; k += NT) s.off[k] = offset[g0 + k];
     __syncthreads();
 
     uint32_t kb = 0;
@@ -858,21 +883,21 @@ __global__ __launch_bounds__(WG, (ALGO == 0 ? 4 : 3)) void k_collide(Sys sys, co
             // state and the next step's cell counts stay consistent, and the
             // step reports the condition as an error.
             if (tid == 0) atomicOr(status, 1u);
-            for (uint32_t i0 = 0; i0 < m; i0 += WG) {
+            for (uint32_t i0 = 0; i0 < m; i0 += NT) {
                 uint32_t const i = i0 + tid;
                 bool const valid = i < m;
                 uint32_t knext = 0;
                 if (valid) {
                     uint64_t const o = static_cast<uint64_t>(gb) + i;
                     uint32_t const src = perm[o];
-                    float px = in[src], py = in[N + src], pz = in[2 * N + src];
-                    float vx = in[3 * N + src], vy = in[4 * N + src], vz = in[5 * N + src];
+                    float v[6];
+                    load_rec(in, src, v);
+                    float px = v[0], py = v[1], pz = v[2], vx = v[3], vy = v[4], vz = v[5];
                     stream_shift(P, P.shift, px, py, pz, vx, vy, vz);
                     px = wrap1(px - P.shift[0], P.L[0]);
                     py = wrap1(py - P.shift[1], P.L[1]);
                     pz = wrap1(pz - P.shift[2], P.L[2]);
-                    out[o] = px; out[N + o] = py; out[2 * N + o] = pz;
-                    out[3 * N + o] = vx; out[4 * N + o] = vy; out[5 * N + o] = vz;
+                    store_rec(out, o, px, py, pz, vx, vy, vz);
                     if (perm_out) perm_out[o] = src;
                     stream_shift(P, P.shift_next, px, py, pz, vx, vy, vz);
                     uint32_t ix, iy, iz;
@@ -887,7 +912,7 @@ __global__ __launch_bounds__(WG, (ALGO == 0 ? 4 : 3)) void k_collide(Sys sys, co
             continue;
         }
         uint32_t const ncell = ke - kb;
-        for (uint32_t i = tid; i < m; i += WG) s.perm[i] = perm[gb + i];
+        for (uint32_t i = tid; i < m; i += NT) s.perm[i] = perm[gb + i];
         if constexpr (ALGO == 0) {
             // per-cell draws: axis, and for wall cells the ghost count and a
             // slot in the staging region (simulation_functions.cpp:137-146)
@@ -895,7 +920,7 @@ __global__ __launch_bounds__(WG, (ALGO == 0 ? 4 : 3)) void k_collide(Sys sys, co
             __syncthreads();
             bool const sign = P.shift[2] > 0;
             uint32_t const XY = P.dim[0] * P.dim[1];
-            for (uint32_t k = tid; k < ncell; k += WG) {
+            for (uint32_t k = tid; k < ncell; k += NT) {
                 uint32_t const key = c0 + kb + k;
                 uint64_t const gk = P.cb + key;
                 s.cm[3][k] = axis[gk];
@@ -918,7 +943,7 @@ __global__ __launch_bounds__(WG, (ALGO == 0 ? 4 : 3)) void k_collide(Sys sys, co
                 s.goff[k] = go;
             }
         }
-        for (uint32_t k = tid; k < ncell; k += WG) {
+        for (uint32_t k = tid; k < ncell; k += NT) {
             uint32_t const a = s.off[kb + k] - gb, e = s.off[kb + k + 1] - gb;
             for (uint32_t i = a; i < e; ++i) s.cell[i] = static_cast<uint16_t>(k);
         }
@@ -926,23 +951,22 @@ __global__ __launch_bounds__(WG, (ALGO == 0 ? 4 : 3)) void k_collide(Sys sys, co
         // gather this thread's particles, GQ at a time with all their loads in
         // flight together, then order each cell by source slot and re-apply
         // this step's streaming
-        for (uint32_t i0 = 0; i0 < m; i0 += GQ * WG) {
+        for (uint32_t i0 = 0; i0 < m; i0 += GQ * NT) {
             float g[GQ][6];
             uint32_t srcs[GQ];
 #pragma unroll
             for (int q = 0; q < GQ; ++q) {
-                uint32_t const i = i0 + tid + q * WG;
+                uint32_t const i = i0 + tid + q * NT;
                 srcs[q] = 0;
                 if (i < m) {
                     uint32_t const src = s.perm[i];
                     srcs[q] = src;
-#pragma unroll
-                    for (int c = 0; c < 6; ++c) g[q][c] = in[c * N + src];
+                    load_rec(in, src, g[q]);
                 }
             }
 #pragma unroll
             for (int q = 0; q < GQ; ++q) {
-                uint32_t const i = i0 + tid + q * WG;
+                uint32_t const i = i0 + tid + q * NT;
                 if (i < m) {
                     uint32_t const k = s.cell[i];
                     uint32_t const a = s.off[kb + k] - gb, e = s.off[kb + k + 1] - gb;
@@ -977,7 +1001,7 @@ __global__ __launch_bounds__(WG, (ALGO == 0 ? 4 : 3)) void k_collide(Sys sys, co
         if constexpr (ALGO == 0) {
             // stage the wall cells' ghost velocities, one wave per cell
             uint32_t const wave = tid >> 6, lane = tid & 63u;
-            for (uint32_t k = wave; k < ncell; k += WG / 64) {
+            for (uint32_t k = wave; k < ncell; k += NT / 64) {
                 uint32_t const go = s.goff[k];
                 if (go != ~0u) {
                     uint32_t const nf = 3 * s.gcnt[k];
@@ -991,7 +1015,7 @@ __global__ __launch_bounds__(WG, (ALGO == 0 ? 4 : 3)) void k_collide(Sys sys, co
         if constexpr (ALGO == 0) {
             srd_cells<CAP, ADJ>(P, s, ncell, s.off + kb, gb, gvel, c0 + kb, T, CT);
         } else {
-            for (uint32_t k = tid; k < ncell; k += WG) {
+            for (uint32_t k = tid; k < ncell; k += NT) {
                 uint32_t ix, iy, iz;
                 decode_key(P, c0 + kb + k, ix, iy, iz);
                 uint32_t const ref = ref_of(P, ix, iy, iz);
@@ -1000,16 +1024,21 @@ __global__ __launch_bounds__(WG, (ALGO == 0 ? 4 : 3)) void k_collide(Sys sys, co
             }
         }
         __syncthreads();
-        // next step's cell counts: keys near this batch's cells go to an LDS
-        // window (the free permutation scratch), the rest straight to memory
+        // next step's cell counts: keys near this batch's cells go to three LDS
+        // windows in the free permutation scratch (the batch's own rows and the
+        // rows one step away along the middle axis, smid keys away), the rest
+        // straight to memory.  Window bases may wrap below zero: distances are
+        // taken modulo 2^32, and only bins of real keys are ever non-zero.
         uint32_t* const hist = s.perm;
+        constexpr uint32_t W = static_cast<uint32_t>(CAP) / 3;
         uint32_t const span = ke - kb;
-        uint32_t const wlo = (CAP - span) / 2;
-        uint32_t const wb = g0 + kb > wlo ? g0 + kb - wlo : 0u;
-        for (uint32_t i = tid; i < static_cast<uint32_t>(CAP); i += WG) hist[i] = 0u;
+        uint32_t const wb0 = g0 + kb - (span < W ? (W - span) / 2 : 0u);
+        uint32_t const smid = mid_stride(P, c0 + kb);
+        uint32_t const wbm = wb0 - smid, wbp = wb0 + smid;
+        for (uint32_t i = tid; i < 3 * W; i += NT) hist[i] = 0u;
         __syncthreads();
         // per-particle finish, write-out, next-step cell count, sampling
-        for (uint32_t i0 = 0; i0 < m; i0 += WG) {
+        for (uint32_t i0 = 0; i0 < m; i0 += NT) {
             uint32_t const i = i0 + tid;
             bool const valid = i < m;
             uint32_t knext = 0;
@@ -1045,8 +1074,7 @@ __global__ __launch_bounds__(WG, (ALGO == 0 ? 4 : 3)) void k_collide(Sys sys, co
                     }
                 }
                 uint64_t const o = static_cast<uint64_t>(gb) + i;
-                out[o] = px; out[N + o] = py; out[2 * N + o] = pz;
-                out[3 * N + o] = vx; out[4 * N + o] = vy; out[5 * N + o] = vz;
+                store_rec(out, o, px, py, pz, vx, vy, vz);
                 if (SAMPLE) {
                     uint32_t ix, iy, iz;
                     cell_of(P, px, py, pz, ix, iy, iz);
@@ -1063,15 +1091,21 @@ __global__ __launch_bounds__(WG, (ALGO == 0 ? 4 : 3)) void k_collide(Sys sys, co
             }
             if (valid) {
                 keys[static_cast<uint64_t>(gb) + i] = knext;
-                uint32_t const w = knext - wb;
-                if (w < static_cast<uint32_t>(CAP)) atomicAdd(hist + w, 1u);
+                uint32_t w = knext - wb0;
+                if (w < W) atomicAdd(hist + w, 1u);
+                else if ((w = knext - wbm) < W) atomicAdd(hist + W + w, 1u);
+                else if ((w = knext - wbp) < W) atomicAdd(hist + 2 * W + w, 1u);
                 else atomicAdd(count_next + knext, 1u);
             }
         }
         __syncthreads();
-        for (uint32_t i = tid; i < static_cast<uint32_t>(CAP); i += WG) {
+        for (uint32_t i = tid; i < 3 * W; i += NT) {
             uint32_t const c = hist[i];
-            if (c) atomicAdd(count_next + wb + i, c);
+            if (c) {
+                uint32_t const j = i < W ? 0u : (i < 2 * W ? 1u : 2u);
+                uint32_t const base = j == 0 ? wb0 : (j == 1 ? wbm : wbp);
+                atomicAdd(count_next + (base + (i - j * W)), c);
+            }
         }
         __syncthreads();
         kb = ke;
@@ -1105,11 +1139,11 @@ __global__ __launch_bounds__(WG) void k_layers(Sys sys, const float* __restrict_
     }
     for (uint64_t j = blockIdx.x * static_cast<uint64_t>(WG) + threadIdx.x; j < N;
          j += static_cast<uint64_t>(gridDim.x) * WG) {
-        uint32_t const iz = mod_dim(static_cast<uint32_t>(floorf(st[2 * N + j] + P.half[2])), Z);
+        uint32_t const iz = mod_dim(static_cast<uint32_t>(floorf(st[rec(j) + 2] + P.half[2])), Z);
         if (mode == 0) {
             atomicAdd(&hc[iz], 1u);
             atomicAdd(&hs[iz], static_cast<unsigned long long>(static_cast<long long>(
-                                   rint(static_cast<double>(st[3 * N + j]) * FIX_SCALE))));
+                                   rint(static_cast<double>(st[rec(j) + 3]) * FIX_SCALE))));
         } else {
             uint32_t const c = cnt[iz];
             seed[j] = c ? static_cast<float>(w[iz] / static_cast<double>(c)) : 0.0f;
@@ -1459,7 +1493,7 @@ void upload_axes(int device) {
 }
 
 // LDS tile capacities compiled into k_collide (particles per tile)
-constexpr uint32_t TILE_CAPS[3] = {512, 1152, 2048};
+constexpr uint32_t TILE_CAPS[4] = {256, 512, 1152, 2048};
 
 // Cells per tile of every system for the LDS capacity `cap` (0: keep) and the
 // request `tc` (0: the largest that fits).  SRD: on average about 80% of the
@@ -1467,9 +1501,9 @@ constexpr uint32_t TILE_CAPS[3] = {512, 1152, 2048};
 // cell) must fit the cap/2 ghost region.
 void choose_tiling(mpcd_hip_s* h, uint32_t tc, uint32_t cap) {
     if (cap == 0) cap = h->lds_cap ? h->lds_cap : 1152u;
-    uint32_t chosen = TILE_CAPS[2];
+    uint32_t chosen = TILE_CAPS[3];
     for (uint32_t c : TILE_CAPS)
-        if (c >= cap) { chosen = c; break; }
+        if (c >= cap && (h->algorithm == 0 || c >= 512)) { chosen = c; break; }
     h->lds_cap = chosen;
     h->tile_req = tc;
     for (KParams& k : h->kp) {
@@ -1598,7 +1632,7 @@ int launch_sort(mpcd_hip_s* h) {
 template <int ALGO, bool SAMPLE, int CAP, bool ADJ = false>
 void launch_collide_t(mpcd_hip_s* h, uint32_t* rec, Tape const& T = Tape{}) {
     SampleBufs S{h->scnt, h->ssum};
-    hipLaunchKernelGGL((k_collide<ALGO, SAMPLE, CAP, ADJ>), dim3(h->ntiles), dim3(WG), 0, h->stream, h->sys,
+    hipLaunchKernelGGL((k_collide<ALGO, SAMPLE, CAP, ADJ>), dim3(h->ntiles), dim3(tile_threads(CAP)), 0, h->stream, h->sys,
                        h->st[h->cur], h->st[1 - h->cur], h->perm, h->offset, h->count, h->keys, rec, S, h->status,
                        h->axis, h->wgcnt, h->wgvel, T);
 }
@@ -1606,6 +1640,10 @@ void launch_collide_t(mpcd_hip_s* h, uint32_t* rec, Tape const& T = Tape{}) {
 template <int ALGO, bool SAMPLE>
 void launch_collide_cap(mpcd_hip_s* h, uint32_t* rec) {
     switch (h->lds_cap) {
+        case 256:
+            if constexpr (ALGO == 0) launch_collide_t<ALGO, SAMPLE, 256>(h, rec);
+            else throw HipError{"the extended rule needs an LDS capacity of at least 512"};
+            break;
         case 512: launch_collide_t<ALGO, SAMPLE, 512>(h, rec); break;
         case 1152: launch_collide_t<ALGO, SAMPLE, 1152>(h, rec); break;
         default: launch_collide_t<ALGO, SAMPLE, 2048>(h, rec); break;
@@ -1755,26 +1793,34 @@ void check_sys(mpcd_hip_s const* h, uint32_t s) {
     if (s >= h->ns) throw HipError{"system index " + std::to_string(s) + " out of range (" + std::to_string(h->ns) + " systems)"};
 }
 
-// xyz-interleaved host array <-> three SoA planes, slots [first, first + n)
-void put_component(mpcd_hip_s* h, const float* src, int first_plane) {
-    std::vector<float> tmp(3 * h->N);
-    for (uint64_t i = 0; i < h->N; ++i)
-        for (int a = 0; a < 3; ++a) tmp[a * h->N + i] = src[3 * i + a];
+// xyz-interleaved host array <-> three fields of the particle records, slots
+// [first, first + n); moved in chunks to bound the host buffer
+constexpr uint64_t XFER_CHUNK = uint64_t(1) << 22;
+
+void put_component(mpcd_hip_s* h, const float* src, int first_field) {
+    std::vector<float> tmp(6 * std::min<uint64_t>(h->N, XFER_CHUNK));
     ck(hipStreamSynchronize(h->stream), "sync");
-    ck(hipMemcpy(h->st[h->cur] + first_plane * h->N, tmp.data(), 4 * 3 * h->N, hipMemcpyHostToDevice), "upload");
+    for (uint64_t c0 = 0; c0 < h->N; c0 += XFER_CHUNK) {
+        uint64_t const n = std::min<uint64_t>(XFER_CHUNK, h->N - c0);
+        float* const d = h->st[h->cur] + rec(c0);
+        ck(hipMemcpy(tmp.data(), d, 24 * n, hipMemcpyDeviceToHost), "download");
+        for (uint64_t i = 0; i < n; ++i)
+            for (int a = 0; a < 3; ++a) tmp[rec(i) + first_field + a] = src[3 * (c0 + i) + a];
+        ck(hipMemcpy(d, tmp.data(), 24 * n, hipMemcpyHostToDevice), "upload");
+    }
     h->keys_valid = false;
     h->sorted_valid = false;
 }
 
-void get_component(mpcd_hip_s* h, float* dst, int first_plane, uint64_t first, uint64_t n) {
-    std::vector<float> tmp(3 * n);
+void get_component(mpcd_hip_s* h, float* dst, int first_field, uint64_t first, uint64_t n_all) {
+    std::vector<float> tmp(6 * std::min<uint64_t>(std::max<uint64_t>(n_all, 1), XFER_CHUNK));
     ck(hipStreamSynchronize(h->stream), "sync");
-    for (int a = 0; a < 3; ++a)
-        if (n)
-            ck(hipMemcpy(tmp.data() + a * n, h->st[h->cur] + (first_plane + a) * h->N + first, 4 * n,
-                         hipMemcpyDeviceToHost), "download");
-    for (uint64_t i = 0; i < n; ++i)
-        for (int a = 0; a < 3; ++a) dst[3 * i + a] = tmp[a * n + i];
+    for (uint64_t c0 = 0; c0 < n_all; c0 += XFER_CHUNK) {
+        uint64_t const n = std::min<uint64_t>(XFER_CHUNK, n_all - c0);
+        ck(hipMemcpy(tmp.data(), h->st[h->cur] + rec(first + c0), 24 * n, hipMemcpyDeviceToHost), "download");
+        for (uint64_t i = 0; i < n; ++i)
+            for (int a = 0; a < 3; ++a) dst[3 * (c0 + i) + a] = tmp[rec(i) + first_field + a];
+    }
 }
 
 void adj_free(mpcd_hip_s* h);
@@ -1867,6 +1913,7 @@ void adj_step(mpcd_hip_s* h, float const* force, int j) {
         rec = a.perm + uint64_t(j) * N;
     }
     switch (h->lds_cap) {
+        case 256: launch_collide_t<0, false, 256, true>(h, rec, T); break;
         case 512: launch_collide_t<0, false, 512, true>(h, rec, T); break;
         case 1152: launch_collide_t<0, false, 1152, true>(h, rec, T); break;
         default: launch_collide_t<0, false, 2048, true>(h, rec, T); break;
@@ -2316,8 +2363,8 @@ int mpcd_hip_last_timing(const mpcd_hip_t* h, double* collide_ms, double* total_
 
 int mpcd_hip_set_tiling(mpcd_hip_t* h, uint32_t cells_per_tile, uint32_t lds_capacity) {
     return guarded(h, [&] {
-        if (lds_capacity > TILE_CAPS[2])
-            throw HipError{"tile capacity above " + std::to_string(TILE_CAPS[2]) + " particles"};
+        if (lds_capacity > TILE_CAPS[3])
+            throw HipError{"tile capacity above " + std::to_string(TILE_CAPS[3]) + " particles"};
         choose_tiling(h, cells_per_tile, lds_capacity);
         layout(h);
     });
@@ -2428,7 +2475,7 @@ int mpcd_hip_autotune(mpcd_hip_t* h, int steps_per_trial, uint32_t* cells_per_ti
         };
         std::vector<Cand> cands;
         for (uint32_t cap : TILE_CAPS)
-            for (uint32_t tc : {0u, 48u, 32u, 16u}) cands.push_back({tc, cap});
+            for (uint32_t tc : {0u, 48u, 32u, 16u, 8u}) cands.push_back({tc, cap});
         bool const timing = h->timing;
         h->timing = false;
         hipEvent_t e0 = nullptr, e1 = nullptr;

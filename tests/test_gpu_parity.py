@@ -196,7 +196,7 @@ def test_sampling_matches_oracle(pympcd, algorithm):
 def test_determinism_and_tiling_independence(pympcd):
     cfg = ((24, 20, 14), (1, 1, 0), 10, "srd")
     states = []
-    for tiling in [(0, 0), (0, 0), (7, 512), (64, 1024), (1, 256)]:
+    for tiling in [(0, 0), (0, 0), (7, 512), (64, 1024), (1, 256), (10, 256), (64, 2048)]:
         p, sim, h = _sim(pympcd, cfg, seed=99)
         if tiling != (0, 0):
             h.set_tiling(*tiling)
