@@ -133,22 +133,21 @@ constexpr uint32_t BOUNCED = 0x80000000u;  // flag in a recorded permutation ent
 
 // ------------------------------------------------------------ helpers ---
 
+// fmodf for the arguments the fast path below does not cover (never on a
+// normal trajectory): one out-of-line copy instead of one per call site
+__device__ __noinline__ float wrap_slow(float x, float L) { return fmodf(x, L); }
+
 // periodic wrap fmodf(r + 1.5 L, L) - L/2 (simulation_functions.cpp:46-51).
 // fmod is exact; for x = r + 1.5 L in [0, 3L) it equals x, x - L or x - 2L,
-// and those subtractions are exact (Sterbenz), so the fast path returns the
-// same bits as fmodf; anything else takes fmodf itself.
+// and those subtractions are exact (Sterbenz), so the selects return the same
+// bits as fmodf; anything else takes fmodf itself.
 __device__ __forceinline__ float wrap1(float r, float L) {
     float const x = r + 1.5f * L;
-    float m;
-    if (x >= 0.0f && x < L) {
-        m = x;
-    } else if (x >= L && x < 2.0f * L) {
-        m = x - L;
-    } else if (x >= 2.0f * L && x - 2.0f * L < L) {
-        m = x - 2.0f * L;
-    } else {
-        m = fmodf(x, L);
-    }
+    float const L2 = 2.0f * L;
+    float m = x - L;
+    m = x < L ? x : m;
+    m = x >= L2 ? x - L2 : m;
+    if (__builtin_expect(!(x >= 0.0f && x - L2 < L), 0)) m = wrap_slow(x, L);
     return m - L * 0.5f;
 }
 
@@ -452,20 +451,56 @@ __global__ __launch_bounds__(WG) void k_scan_down(uint32_t* __restrict__ count, 
     if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) offset[C] = static_cast<uint32_t>(N);
 }
 
-// cell-ordered permutation: each particle takes the next free slot of its
-// cell (one returning atomic per distinct key of the wave, issued together)
+// cell-ordered permutation: each particle takes a free slot of its cell.
+// A block takes PERM_ITEMS consecutive particles; their keys lie close
+// together, so the block counts them in an LDS window around its first key
+// (returning LDS atomics give each particle its place inside the block's
+// share), reserves one range per distinct key with a single device atomic,
+// and writes the slots.  Keys outside the window take a device atomic of their
+// own.  The order inside a cell is not the final one: the collide kernel
+// orders every cell by source slot.
+constexpr int PERM_ITEMS = 4;
+constexpr uint32_t PERM_WIN = 3072;
+
 __global__ __launch_bounds__(WG) void k_perm(const uint32_t* __restrict__ keys, uint32_t* __restrict__ cursor,
                                              uint32_t* __restrict__ perm, uint64_t N) {
-    uint64_t const stride = static_cast<uint64_t>(gridDim.x) * WG;
-    uint64_t const span = (N + stride - 1) / stride * stride;
-    for (uint64_t j = blockIdx.x * static_cast<uint64_t>(WG) + threadIdx.x; j < span; j += stride) {
-        bool const valid = j < N;
-        uint32_t const key = valid ? keys[j] : 0u;
-        WaveGroup const g = wave_group(key, valid);
-        uint32_t base = 0;
-        if (valid && g.leader == static_cast<int>(__lane_id())) base = atomicAdd(cursor + key, g.size);
-        base = __shfl(base, g.leader, 64);
-        if (valid) perm[base + g.pos] = static_cast<uint32_t>(j);
+    __shared__ uint32_t cnt[PERM_WIN];
+    __shared__ uint32_t key0;
+    uint64_t const base = static_cast<uint64_t>(blockIdx.x) * (WG * PERM_ITEMS);
+    for (uint32_t i = threadIdx.x; i < PERM_WIN; i += WG) cnt[i] = 0u;
+    if (threadIdx.x == 0) {
+        uint32_t const k = keys[base];
+        key0 = k > PERM_WIN / 2 ? k - PERM_WIN / 2 : 0u;
+    }
+    __syncthreads();
+    uint32_t const lo = key0;
+    uint32_t key[PERM_ITEMS], local[PERM_ITEMS];
+#pragma unroll
+    for (int q = 0; q < PERM_ITEMS; ++q) {
+        uint64_t const j = base + q * WG + threadIdx.x;
+        key[q] = j < N ? keys[j] : ~0u;
+    }
+#pragma unroll
+    for (int q = 0; q < PERM_ITEMS; ++q) {
+        uint32_t const w = key[q] - lo;
+        local[q] = ~0u;
+        if (key[q] != ~0u && w < PERM_WIN) local[q] = atomicAdd(&cnt[w], 1u);
+    }
+    __syncthreads();
+    // one range per non-empty window bin
+    for (uint32_t i = threadIdx.x; i < PERM_WIN; i += WG) {
+        uint32_t const c = cnt[i];
+        if (c) cnt[i] = atomicAdd(cursor + lo + i, c);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < PERM_ITEMS; ++q) {
+        uint64_t const j = base + q * WG + threadIdx.x;
+        if (key[q] == ~0u) continue;
+        uint32_t slot;
+        if (local[q] != ~0u) slot = cnt[key[q] - lo] + local[q];
+        else slot = atomicAdd(cursor + key[q], 1u);
+        perm[slot] = static_cast<uint32_t>(j);
     }
 }
 
@@ -645,7 +680,24 @@ __device__ void srd_cells(KParams const& P, Tile<CAP, 0>& s, uint32_t ncell, uin
         s.go[k] = total > 1;
         if (total > 1) {
             float sx = 0, sy = 0, sz = 0;
-            for (uint32_t i = 0; i < reals; ++i) {
+            uint32_t i = 0;
+            // loads issued four particles at a time, sums kept in slot order
+            for (; i + 4 <= reals; i += 4) {
+                float x[4], y[4], z[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    x[u] = s.p[3][a + i + u];
+                    y[u] = s.p[4][a + i + u];
+                    z[u] = s.p[5][a + i + u];
+                }
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    sx = sx + x[u];
+                    sy = sy + y[u];
+                    sz = sz + z[u];
+                }
+            }
+            for (; i < reals; ++i) {
                 sx = sx + s.p[3][a + i];
                 sy = sy + s.p[4][a + i];
                 sz = sz + s.p[5][a + i];
@@ -1622,9 +1674,8 @@ int launch_sort(mpcd_hip_s* h) {
     hipLaunchKernelGGL(k_scan_reduce, dim3(h->nb), dim3(WG), 0, h->stream, h->count, C, h->partial);
     hipLaunchKernelGGL(k_scan_partials, dim3(1), dim3(1024), 0, h->stream, h->partial, h->nb);
     hipLaunchKernelGGL(k_scan_down, dim3(h->nb), dim3(WG), 0, h->stream, h->count, C, h->partial, h->offset, h->cursor, h->N);
-    uint64_t blocks = std::min<uint64_t>((h->N + WG - 1) / WG, 1u << 20);
-    if (blocks == 0) blocks = 1;
-    hipLaunchKernelGGL(k_perm, dim3(static_cast<uint32_t>(blocks)), dim3(WG), 0, h->stream, h->keys, h->cursor, h->perm, h->N);
+    uint64_t const blocks = (h->N + WG * PERM_ITEMS - 1) / (WG * PERM_ITEMS);
+    if (blocks) hipLaunchKernelGGL(k_perm, dim3(static_cast<uint32_t>(blocks)), dim3(WG), 0, h->stream, h->keys, h->cursor, h->perm, h->N);
     ck(hipGetLastError(), "sort");
     return 4;
 }
