@@ -465,8 +465,12 @@ constexpr uint32_t PERM_WIN = 3072;
 __global__ __launch_bounds__(WG) void k_perm(const uint32_t* __restrict__ keys, uint32_t* __restrict__ cursor,
                                              uint32_t* __restrict__ perm, uint64_t N) {
     __shared__ uint32_t cnt[PERM_WIN];
+    __shared__ uint32_t list[WG * PERM_ITEMS];   // non-empty bins (at most one per particle)
+    __shared__ uint32_t s_w[WG / 64];
     __shared__ uint32_t key0;
-    uint64_t const base = static_cast<uint64_t>(blockIdx.x) * (WG * PERM_ITEMS);
+    // neighbouring blocks fill the same cells: keep them on one XCD so their
+    // partial cache lines meet in one L2
+    uint64_t const base = static_cast<uint64_t>(xcd_tile(blockIdx.x, gridDim.x)) * (WG * PERM_ITEMS);
     for (uint32_t i = threadIdx.x; i < PERM_WIN; i += WG) cnt[i] = 0u;
     if (threadIdx.x == 0) {
         uint32_t const k = keys[base];
@@ -487,10 +491,24 @@ __global__ __launch_bounds__(WG) void k_perm(const uint32_t* __restrict__ keys, 
         if (key[q] != ~0u && w < PERM_WIN) local[q] = atomicAdd(&cnt[w], 1u);
     }
     __syncthreads();
-    // one range per non-empty window bin
-    for (uint32_t i = threadIdx.x; i < PERM_WIN; i += WG) {
-        uint32_t const c = cnt[i];
-        if (c) cnt[i] = atomicAdd(cursor + lo + i, c);
+    // one range per non-empty window bin: the non-empty bins are listed first
+    // (thread t owns bins [t B, t B + B)), so the device atomics run with
+    // full waves
+    constexpr int PB = PERM_WIN / WG;
+    uint32_t nz = 0;
+#pragma unroll
+    for (int k = 0; k < PB; ++k) nz += cnt[threadIdx.x * PB + k] != 0u;
+    uint32_t nz_tot;
+    uint32_t at = block_exclusive<WG>(nz, s_w, &nz_tot);
+#pragma unroll
+    for (int k = 0; k < PB; ++k) {
+        uint32_t const i = threadIdx.x * PB + k;
+        if (cnt[i]) list[at++] = i;
+    }
+    __syncthreads();
+    for (uint32_t t = threadIdx.x; t < nz_tot; t += WG) {
+        uint32_t const i = list[t];
+        cnt[i] = atomicAdd(cursor + lo + i, cnt[i]);
     }
     __syncthreads();
 #pragma unroll
