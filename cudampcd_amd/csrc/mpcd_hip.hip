@@ -259,6 +259,13 @@ __host__ __device__ __forceinline__ void decode_key(KParams const& P, uint32_t k
     iz = P.ord[0] == 2 ? q0 : (P.ord[1] == 2 ? q1 : q2);
 }
 
+// stride of the middle internal axis at local key `key` (cells of one block
+// of the slowest axis), the key distance of a neighbour along that axis
+__device__ __forceinline__ uint32_t mid_stride(KParams const& P, uint32_t key) {
+    uint32_t const b = P.mBLK[0] || P.mBLK[1] ? udiv(key, P.mBLK) : key / P.BLK;
+    return (b + 1 == P.nblk ? P.Blast : (1u << P.Bsh)) * P.D0;
+}
+
 // Groups the lanes of a wave by key without touching memory: every lane
 // learns the leader lane of its group and its position inside the group, the
 // leader learns the group size.  Must be called by whole waves in uniform
@@ -452,33 +459,46 @@ __global__ __launch_bounds__(WG) void k_scan_down(uint32_t* __restrict__ count, 
 }
 
 // cell-ordered permutation: each particle takes a free slot of its cell.
-// A block takes PERM_ITEMS consecutive particles; their keys lie close
-// together, so the block counts them in an LDS window around its first key
-// (returning LDS atomics give each particle its place inside the block's
-// share), reserves one range per distinct key with a single device atomic,
-// and writes the slots.  Keys outside the window take a device atomic of their
-// own.  The order inside a cell is not the final one: the collide kernel
-// orders every cell by source slot.
+// A block takes PERM_ITEMS * WG consecutive particles.  They sat in a short run
+// of cells and move at most about one cell, so their keys fall into three
+// windows: around the run's first key and one middle-axis stride either side.
+// The block counts them there in LDS (returning LDS atomics give each particle
+// its place inside the block's share of the key), reserves one range per
+// distinct key with a single device atomic (non-empty bins listed first, so the
+// atomics run with full waves) and writes the slots.  Keys outside the windows
+// take a device atomic of their own.  The order inside a cell is not the final
+// one: the collide kernel orders every cell by source slot.
 constexpr int PERM_ITEMS = 4;
-constexpr uint32_t PERM_WIN = 3072;
+constexpr uint32_t PERM_PW = 1024;            // bins per window
+constexpr uint32_t PERM_WIN = 3 * PERM_PW;
 
-__global__ __launch_bounds__(WG) void k_perm(const uint32_t* __restrict__ keys, uint32_t* __restrict__ cursor,
+__global__ __launch_bounds__(WG) void k_perm(Sys sys, const uint32_t* __restrict__ keys, uint32_t* __restrict__ cursor,
                                              uint32_t* __restrict__ perm, uint64_t N) {
     __shared__ uint32_t cnt[PERM_WIN];
     __shared__ uint32_t list[WG * PERM_ITEMS];   // non-empty bins (at most one per particle)
     __shared__ uint32_t s_w[WG / 64];
-    __shared__ uint32_t key0;
+    __shared__ uint32_t s_base[3];
     // neighbouring blocks fill the same cells: keep them on one XCD so their
     // partial cache lines meet in one L2
     uint64_t const base = static_cast<uint64_t>(xcd_tile(blockIdx.x, gridDim.x)) * (WG * PERM_ITEMS);
     for (uint32_t i = threadIdx.x; i < PERM_WIN; i += WG) cnt[i] = 0u;
     if (threadIdx.x == 0) {
         uint32_t const k = keys[base];
-        key0 = k > PERM_WIN / 2 ? k - PERM_WIN / 2 : 0u;
+        uint32_t lo = 0, hi = sys.ns;   // system holding key k
+        while (hi - lo > 1) {
+            uint32_t const mid = (lo + hi) >> 1;
+            if (sys.P[mid].cb <= k) lo = mid; else hi = mid;
+        }
+        KParams const& P = sys.P[lo];
+        uint32_t const smid = mid_stride(P, k - P.cb);
+        uint32_t const b0 = k - PERM_PW / 4;      // bases may wrap below zero (distances are modulo 2^32)
+        s_base[0] = b0;
+        s_base[1] = b0 - smid;
+        s_base[2] = b0 + smid;
     }
     __syncthreads();
-    uint32_t const lo = key0;
-    uint32_t key[PERM_ITEMS], local[PERM_ITEMS];
+    uint32_t const w0 = s_base[0], wm = s_base[1], wp = s_base[2];
+    uint32_t key[PERM_ITEMS], local[PERM_ITEMS], bin[PERM_ITEMS];
 #pragma unroll
     for (int q = 0; q < PERM_ITEMS; ++q) {
         uint64_t const j = base + q * WG + threadIdx.x;
@@ -486,14 +506,17 @@ __global__ __launch_bounds__(WG) void k_perm(const uint32_t* __restrict__ keys, 
     }
 #pragma unroll
     for (int q = 0; q < PERM_ITEMS; ++q) {
-        uint32_t const w = key[q] - lo;
-        local[q] = ~0u;
-        if (key[q] != ~0u && w < PERM_WIN) local[q] = atomicAdd(&cnt[w], 1u);
+        uint32_t w = key[q] - w0;
+        uint32_t b = ~0u;
+        if (w < PERM_PW) b = w;
+        else if ((w = key[q] - wm) < PERM_PW) b = PERM_PW + w;
+        else if ((w = key[q] - wp) < PERM_PW) b = 2 * PERM_PW + w;
+        if (key[q] == ~0u) b = ~0u;
+        bin[q] = b;
+        local[q] = b != ~0u ? atomicAdd(&cnt[b], 1u) : ~0u;
     }
     __syncthreads();
-    // one range per non-empty window bin: the non-empty bins are listed first
-    // (thread t owns bins [t B, t B + B)), so the device atomics run with
-    // full waves
+    // thread t lists its bins [t B, t B + B) that are non-empty
     constexpr int PB = PERM_WIN / WG;
     uint32_t nz = 0;
 #pragma unroll
@@ -508,16 +531,16 @@ __global__ __launch_bounds__(WG) void k_perm(const uint32_t* __restrict__ keys, 
     __syncthreads();
     for (uint32_t t = threadIdx.x; t < nz_tot; t += WG) {
         uint32_t const i = list[t];
-        cnt[i] = atomicAdd(cursor + lo + i, cnt[i]);
+        uint32_t const wdw = i / PERM_PW;
+        uint32_t const k = (wdw == 0 ? w0 : (wdw == 1 ? wm : wp)) + (i - wdw * PERM_PW);
+        cnt[i] = atomicAdd(cursor + k, cnt[i]);
     }
     __syncthreads();
 #pragma unroll
     for (int q = 0; q < PERM_ITEMS; ++q) {
         uint64_t const j = base + q * WG + threadIdx.x;
         if (key[q] == ~0u) continue;
-        uint32_t slot;
-        if (local[q] != ~0u) slot = cnt[key[q] - lo] + local[q];
-        else slot = atomicAdd(cursor + key[q], 1u);
+        uint32_t const slot = bin[q] != ~0u ? cnt[bin[q]] + local[q] : atomicAdd(cursor + key[q], 1u);
         perm[slot] = static_cast<uint32_t>(j);
     }
 }
@@ -905,13 +928,6 @@ __device__ void extended_cell(KParams const& P, Tile<CAP, 1>& s, uint32_t k, uin
 }
 
 constexpr int GQ = 4;               // particles gathered per thread per round
-
-// stride of the middle internal axis at local key `key` (cells of one block
-// of the slowest axis), the key distance of a neighbour along that axis
-__device__ __forceinline__ uint32_t mid_stride(KParams const& P, uint32_t key) {
-    uint32_t const b = P.mBLK[0] || P.mBLK[1] ? udiv(key, P.mBLK) : key / P.BLK;
-    return (b + 1 == P.nblk ? P.Blast : (1u << P.Bsh)) * P.D0;
-}
 
 template <int ALGO, bool SAMPLE, int CAP, bool ADJ = false>
 __global__ __launch_bounds__(tile_threads(CAP), (ALGO == 0 ? 1024 : 768) / tile_threads(CAP)) void k_collide(Sys sys, const float* __restrict__ in, float* __restrict__ out,
@@ -1693,7 +1709,7 @@ int launch_sort(mpcd_hip_s* h) {
     hipLaunchKernelGGL(k_scan_partials, dim3(1), dim3(1024), 0, h->stream, h->partial, h->nb);
     hipLaunchKernelGGL(k_scan_down, dim3(h->nb), dim3(WG), 0, h->stream, h->count, C, h->partial, h->offset, h->cursor, h->N);
     uint64_t const blocks = (h->N + WG * PERM_ITEMS - 1) / (WG * PERM_ITEMS);
-    if (blocks) hipLaunchKernelGGL(k_perm, dim3(static_cast<uint32_t>(blocks)), dim3(WG), 0, h->stream, h->keys, h->cursor, h->perm, h->N);
+    if (blocks) hipLaunchKernelGGL(k_perm, dim3(static_cast<uint32_t>(blocks)), dim3(WG), 0, h->stream, h->sys, h->keys, h->cursor, h->perm, h->N);
     ck(hipGetLastError(), "sort");
     return 4;
 }
