@@ -776,10 +776,22 @@ __device__ void srd_cells(KParams const& P, Tile<CAP, 0>& s, uint32_t ncell, uin
     }
 }
 
-// extended collision of one cell by one lane (simulation_functions.cpp:207-406)
+// Extended rule (simulation_functions.cpp:207-406) in three stages so that
+// the draws run on all lanes and only the order-sensitive sums stay serial:
+//   ext_cell_setup   one lane per cell: wall flags, ghost count (:137-146),
+//                    table size; stores ref, ghost count (bit 31: upper wall
+//                    layer) and the number of table particles kept
+//   ext_ghost        one lane per (cell, ghost): position and velocity (:147-167)
+//   ext_thermal      one lane per four table particles: their thermal Gaussians
+//                    (draw keyed by particle index / 4, as in the serial rule)
+//   extended_cell    one lane per cell: the collision itself
+constexpr uint32_t UPPER_LAYER = 0x80000000u;
+
 template <int CAP>
-__device__ void extended_cell(KParams const& P, Tile<CAP, 1>& s, uint32_t k, uint32_t a, uint32_t mc, uint32_t ref,
-                              uint32_t ix, uint32_t iy, uint32_t iz) {
+__device__ void ext_cell_setup(KParams const& P, Tile<CAP, 1>& s, uint32_t k, uint32_t mc, uint32_t key) {
+    uint32_t ix, iy, iz;
+    decode_key(P, key, ix, iy, iz);
+    uint32_t const ref = ref_of(P, ix, iy, iz);
     float const sh = fabsf(P.shift[2]);
     bool const sign = P.shift[2] > 0;
     float const zf = static_cast<float>(iz);
@@ -789,30 +801,62 @@ __device__ void extended_cell(KParams const& P, Tile<CAP, 1>& s, uint32_t k, uin
     uint32_t const added = ghosts ? ghost_count(P, ref, sh, layer, sign) : 0u;
     uint32_t total = reals_in + added;
     if (total > P.cap) total = P.cap;
-    uint32_t const rr = total - added;
+    s.cref[k] = ref;
+    s.gcnt[k] = added | (layer ? UPPER_LAYER : 0u);
+    s.goff[k] = total - added;   // table particles kept
     s.go[k] = total > 1;
+}
+
+template <int CAP>
+__device__ void ext_ghost(KParams const& P, Tile<CAP, 1>& s, uint32_t k, uint32_t g, uint32_t key) {
+    uint32_t ix, iy, iz;
+    decode_key(P, key, ix, iy, iz);
+    uint32_t const ref = s.cref[k];
+    float const sh = fabsf(P.shift[2]);
+    bool const sign = P.shift[2] > 0;
+    bool const layer = (s.gcnt[k] & UPPER_LAYER) != 0u;
     // cell centre (volume_container.hpp:86-91)
     float const ox = (0.5f + static_cast<float>(ix)) - P.half[0];
     float const oy = (0.5f + static_cast<float>(iy)) - P.half[1];
     float const oz = (0.5f + static_cast<float>(iz)) - P.half[2];
+    uint32_t const o = k * P.n + g;
+    Words const r = P.rs.at(P_GHOST_POS, g, ref, P.step);
+    float const uz = uniform(r.w[2]);
+    float const z = (layer ^ sign) ? sh * uz : sh + (1.0f - sh) * uz;
+    s.g[0][o] = (uniform(r.w[0]) - 0.5f) + ox;
+    s.g[1][o] = (uniform(r.w[1]) - 0.5f) + oy;
+    s.g[2][o] = (z - 0.5f) + oz;
+    Words const v = P.rs.at(P_GHOST_VEL, g, ref, P.step);
+    float g0, g1, g2, g3;
+    box_muller(v.w[0], v.w[1], &g0, &g1);
+    box_muller(v.w[2], v.w[3], &g2, &g3);
+    s.g[3][o] = g0 * P.tv;
+    s.g[4][o] = g1 * P.tv;
+    s.g[5][o] = g2 * P.tv;
+}
+
+// the four thermal Gaussians of draw `q` of cell `ref`
+__device__ __forceinline__ void thermal_quad(KParams const& P, uint32_t ref, uint32_t q, float* gq) {
+    Words const r = P.rs.at(P_THERM, q, ref, P.step);
+    box_muller(r.w[0], r.w[1], &gq[0], &gq[1]);
+    box_muller(r.w[2], r.w[3], &gq[2], &gq[3]);
+}
+
+// extended collision of one cell by one lane; ghosts and the table particles'
+// thermal Gaussians (in `gauss`, by LDS slot) were drawn by the stages above
+template <int CAP>
+__device__ void extended_cell(KParams const& P, Tile<CAP, 1>& s, uint32_t k, uint32_t a, uint32_t key,
+                              float const* gauss) {
+    uint32_t ix, iy, iz;
+    decode_key(P, key, ix, iy, iz);
+    uint32_t const ref = s.cref[k];
+    uint32_t const added = s.gcnt[k] & ~UPPER_LAYER;
+    uint32_t const rr = s.goff[k];
+    uint32_t const total = rr + added;
+    float const ox = (0.5f + static_cast<float>(ix)) - P.half[0];
+    float const oy = (0.5f + static_cast<float>(iy)) - P.half[1];
+    float const oz = (0.5f + static_cast<float>(iz)) - P.half[2];
     uint32_t const gb = k * P.n;
-    if (ghosts) {
-        for (uint32_t i = 0; i < added; ++i) {
-            Words const r = P.rs.at(P_GHOST_POS, i, ref, P.step);
-            float const uz = uniform(r.w[2]);
-            float const z = (layer ^ sign) ? sh * uz : sh + (1.0f - sh) * uz;
-            s.g[0][gb + i] = (uniform(r.w[0]) - 0.5f) + ox;
-            s.g[1][gb + i] = (uniform(r.w[1]) - 0.5f) + oy;
-            s.g[2][gb + i] = (z - 0.5f) + oz;
-            Words const v = P.rs.at(P_GHOST_VEL, i, ref, P.step);
-            float g0, g1, g2, g3;
-            box_muller(v.w[0], v.w[1], &g0, &g1);
-            box_muller(v.w[2], v.w[3], &g2, &g3);
-            s.g[3][gb + i] = g0 * P.tv;
-            s.g[4][gb + i] = g1 * P.tv;
-            s.g[5][gb + i] = g2 * P.tv;
-        }
-    }
     if (total <= 1) return;
     // particle i < rr lives at LDS slot a+i, ghost i >= rr at ghost slot gb+i-rr
     auto X = [&](int c, uint32_t i) -> float& { return i < rr ? s.p[c][a + i] : s.g[c][gb + (i - rr)]; };
@@ -867,15 +911,19 @@ __device__ void extended_cell(KParams const& P, Tile<CAP, 1>& s, uint32_t k, uin
         Sym I = {0, 0, 0, 0, 0, 0};
         float gq[4] = {0, 0, 0, 0};
         for (uint32_t i = 0; i < total; ++i) {
-            if ((i & 3u) == 0) {
-                Words const r = P.rs.at(P_THERM, i >> 2, ref, P.step);
-                box_muller(r.w[0], r.w[1], &gq[0], &gq[1]);
-                box_muller(r.w[2], r.w[3], &gq[2], &gq[3]);
+            float gi;
+            if (i < rr) {
+                gi = gauss[a + i];
+            } else {
+                // ghosts: the rest of the quad the last particles started, or a new one
+                if (i == rr || (i & 3u) == 0) thermal_quad(P, ref, i >> 2, gq);
+                uint32_t const qi = i & 3u;
+                gi = qi == 0 ? gq[0] : (qi == 1 ? gq[1] : (qi == 2 ? gq[2] : gq[3]));
             }
             float const rx = X(0, i), ry = X(1, i), rz = X(2, i);
             float ux = X(3, i), uy = X(4, i), uz = X(5, i);
             bool const side = rx * ax + ry * ay + rz * az < 0;
-            float const vr = gq[i & 3u] * P.tv;
+            float const vr = gi * P.tv;
             if (side) mean0 += vr; else mean1 += vr;
             Lx = Lx + (ry * uz - rz * uy);
             Ly = Ly + (rz * ux - rx * uz);
@@ -1101,13 +1149,32 @@ __global__ __launch_bounds__(tile_threads(CAP), (ALGO == 0 ? 1024 : 768) / tile_
         if constexpr (ALGO == 0) {
             srd_cells<CAP, ADJ>(P, s, ncell, s.off + kb, gb, gvel, c0 + kb, T, CT);
         } else {
-            for (uint32_t k = tid; k < ncell; k += NT) {
-                uint32_t ix, iy, iz;
-                decode_key(P, c0 + kb + k, ix, iy, iz);
-                uint32_t const ref = ref_of(P, ix, iy, iz);
-                uint32_t const a = s.off[kb + k] - gb, mc = s.off[kb + k + 1] - gb - a;
-                extended_cell<CAP>(P, s, k, a, mc, ref, ix, iy, iz);
+            for (uint32_t k = tid; k < ncell; k += NT)
+                ext_cell_setup<CAP>(P, s, k, s.off[kb + k + 1] - s.off[kb + k], c0 + kb + k);
+            __syncthreads();
+            // ghosts, one lane per (cell, ghost slot)
+            for (uint32_t t = tid; t < ncell * P.n; t += NT) {
+                uint32_t const k = t / P.n, g = t - k * P.n;
+                if (g < (s.gcnt[k] & ~UPPER_LAYER)) ext_ghost<CAP>(P, s, k, g, c0 + kb + k);
             }
+            // thermal Gaussians of the table particles, one lane per four
+            // (the permutation scratch is free until the histogram below)
+            float* const gauss = reinterpret_cast<float*>(s.perm);
+            for (uint32_t i = tid; i < m; i += NT) {
+                uint32_t const k = s.cell[i];
+                uint32_t const a = s.off[kb + k] - gb, li = i - a;
+                uint32_t const rr = s.goff[k];
+                if ((li & 3u) == 0 && li < rr && s.go[k]) {
+                    float gq[4];
+                    thermal_quad(P, s.cref[k], li >> 2, gq);
+#pragma unroll
+                    for (uint32_t u = 0; u < 4; ++u)
+                        if (li + u < rr) gauss[i + u] = gq[u];
+                }
+            }
+            __syncthreads();
+            for (uint32_t k = tid; k < ncell; k += NT)
+                extended_cell<CAP>(P, s, k, s.off[kb + k] - gb, c0 + kb + k, gauss);
         }
         __syncthreads();
         // next step's cell counts: keys near this batch's cells go to three LDS
