@@ -984,7 +984,12 @@ __global__ __launch_bounds__(tile_threads(CAP), (ALGO == 0 ? 1024 : 768) / tile_
                                                 uint32_t* __restrict__ perm_out, SampleBufs S,
                                                 uint32_t* __restrict__ status, float const* __restrict__ axis,
                                                 uint32_t const* __restrict__ gcnt, float const* __restrict__ gvel,
-                                                Tape T) {
+                                                Tape T, uint32_t io) {
+    // io bit 0: `in` holds this step's streamed state (the previous launch
+    // wrote it), so the gather skips the streaming; bit 1: write the next
+    // step's streamed state instead of the post-collision state.  Both forms
+    // are bit-identical to streaming the post-collision state here.
+    bool const in_streamed = (io & 1u) != 0, out_streamed = (io & 2u) != 0;
     constexpr uint32_t NT = tile_threads(CAP);
     __shared__ Tile<CAP, ALGO> s;
     uint32_t const tile = xcd_tile(blockIdx.x, gridDim.x);
@@ -1027,13 +1032,14 @@ __global__ __launch_bounds__(tile_threads(CAP), (ALGO == 0 ? 1024 : 768) / tile_
                     float v[6];
                     load_rec(in, src, v);
                     float px = v[0], py = v[1], pz = v[2], vx = v[3], vy = v[4], vz = v[5];
-                    stream_shift(P, P.shift, px, py, pz, vx, vy, vz);
+                    if (!in_streamed) stream_shift(P, P.shift, px, py, pz, vx, vy, vz);
                     px = wrap1(px - P.shift[0], P.L[0]);
                     py = wrap1(py - P.shift[1], P.L[1]);
                     pz = wrap1(pz - P.shift[2], P.L[2]);
-                    store_rec(out, o, px, py, pz, vx, vy, vz);
+                    if (!out_streamed) store_rec(out, o, px, py, pz, vx, vy, vz);
                     if (perm_out) perm_out[o] = src;
                     stream_shift(P, P.shift_next, px, py, pz, vx, vy, vz);
+                    if (out_streamed) store_rec(out, o, px, py, pz, vx, vy, vz);
                     uint32_t ix, iy, iz;
                     cell_of(P, px, py, pz, ix, iy, iz);
                     knext = P.cb + key_of(P, ix, iy, iz);
@@ -1114,7 +1120,7 @@ __global__ __launch_bounds__(tile_threads(CAP), (ALGO == 0 ? 1024 : 768) / tile_
                     uint32_t const d = a + r;
                     float px = g[q][0], py = g[q][1], pz = g[q][2];
                     float vx = g[q][3], vy = g[q][4], vz = g[q][5];
-                    bool const bounced = stream_shift(P, P.shift, px, py, pz, vx, vy, vz);
+                    bool const bounced = in_streamed ? false : stream_shift(P, P.shift, px, py, pz, vx, vy, vz);
                     s.p[0][d] = px; s.p[1][d] = py; s.p[2][d] = pz;
                     s.p[3][d] = vx; s.p[4][d] = vy; s.p[5][d] = vz;
                     uint64_t const o = static_cast<uint64_t>(gb) + d;
@@ -1227,7 +1233,7 @@ __global__ __launch_bounds__(tile_threads(CAP), (ALGO == 0 ? 1024 : 768) / tile_
                     }
                 }
                 uint64_t const o = static_cast<uint64_t>(gb) + i;
-                store_rec(out, o, px, py, pz, vx, vy, vz);
+                if (!out_streamed) store_rec(out, o, px, py, pz, vx, vy, vz);
                 if (SAMPLE) {
                     uint32_t ix, iy, iz;
                     cell_of(P, px, py, pz, ix, iy, iz);
@@ -1238,6 +1244,7 @@ __global__ __launch_bounds__(tile_threads(CAP), (ALGO == 0 ? 1024 : 768) / tile_
                     atomicAdd(S.sum + 3ull * rc + 2, static_cast<unsigned long long>(static_cast<long long>(rint(static_cast<double>(vz) * FIX_SCALE))));
                 }
                 stream_shift(P, P.shift_next, px, py, pz, vx, vy, vz);
+                if (out_streamed) store_rec(out, o, px, py, pz, vx, vy, vz);
                 uint32_t ix, iy, iz;
                 cell_of(P, px, py, pz, ix, iy, iz);
                 knext = P.cb + key_of(P, ix, iy, iz);
@@ -1553,7 +1560,7 @@ struct mpcd_hip_s {
     uint32_t tile_req = 0, lds_cap = 0;  // requested cells per tile (0: largest that fits), LDS capacity
     // captured steps, by input buffer and sampling
     bool use_graph = true;
-    hipGraphExec_t graph[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
+    hipGraphExec_t graph[2][2][4] = {};   // [input buffer][sampling][streamed in/out]
     int graph_launches[2] = {0, 0};
     // adjoint runs: tape of one checkpoint segment, checkpoints, adjoint state
     struct Adj {
@@ -1669,12 +1676,13 @@ void choose_tiling(mpcd_hip_s* h, uint32_t tc, uint32_t cap) {
 }
 
 void drop_graphs(mpcd_hip_s* h) {
-    for (auto& row : h->graph)
-        for (auto& g : row)
-            if (g) {
-                (void)hipGraphExecDestroy(g);
-                g = nullptr;
-            }
+    for (auto& plane : h->graph)
+        for (auto& row : plane)
+            for (auto& g : row)
+                if (g) {
+                    (void)hipGraphExecDestroy(g);
+                    g = nullptr;
+                }
 }
 
 // the system table with the step state of h->step_index
@@ -1782,23 +1790,23 @@ int launch_sort(mpcd_hip_s* h) {
 }
 
 template <int ALGO, bool SAMPLE, int CAP, bool ADJ = false>
-void launch_collide_t(mpcd_hip_s* h, uint32_t* rec, Tape const& T = Tape{}) {
+void launch_collide_t(mpcd_hip_s* h, uint32_t* rec, Tape const& T = Tape{}, uint32_t io = 0) {
     SampleBufs S{h->scnt, h->ssum};
     hipLaunchKernelGGL((k_collide<ALGO, SAMPLE, CAP, ADJ>), dim3(h->ntiles), dim3(tile_threads(CAP)), 0, h->stream, h->sys,
                        h->st[h->cur], h->st[1 - h->cur], h->perm, h->offset, h->count, h->keys, rec, S, h->status,
-                       h->axis, h->wgcnt, h->wgvel, T);
+                       h->axis, h->wgcnt, h->wgvel, T, io);
 }
 
 template <int ALGO, bool SAMPLE>
-void launch_collide_cap(mpcd_hip_s* h, uint32_t* rec) {
+void launch_collide_cap(mpcd_hip_s* h, uint32_t* rec, uint32_t io) {
     switch (h->lds_cap) {
         case 256:
-            if constexpr (ALGO == 0) launch_collide_t<ALGO, SAMPLE, 256>(h, rec);
+            if constexpr (ALGO == 0) launch_collide_t<ALGO, SAMPLE, 256>(h, rec, Tape{}, io);
             else throw HipError{"the extended rule needs an LDS capacity of at least 512"};
             break;
-        case 512: launch_collide_t<ALGO, SAMPLE, 512>(h, rec); break;
-        case 1152: launch_collide_t<ALGO, SAMPLE, 1152>(h, rec); break;
-        default: launch_collide_t<ALGO, SAMPLE, 2048>(h, rec); break;
+        case 512: launch_collide_t<ALGO, SAMPLE, 512>(h, rec, Tape{}, io); break;
+        case 1152: launch_collide_t<ALGO, SAMPLE, 1152>(h, rec, Tape{}, io); break;
+        default: launch_collide_t<ALGO, SAMPLE, 2048>(h, rec, Tape{}, io); break;
     }
 }
 
@@ -1815,7 +1823,10 @@ void ensure_sampling(mpcd_hip_s* h) {
 // The launches of one step on h->stream (also what a captured step graph
 // holds): sort, per-cell draws, collide, sampling fold, step-state advance.
 // Uses no memset, so the sequence can be captured and replayed.
-int enqueue_step(mpcd_hip_s* h, bool sample, bool first, bool sort, uint32_t* rec, hipEvent_t e0, hipEvent_t e1) {
+// io: bit 0 the input buffer holds this step's streamed state, bit 1 write the
+// next step's streamed state (see k_collide)
+int enqueue_step(mpcd_hip_s* h, bool sample, bool first, bool sort, uint32_t* rec, hipEvent_t e0, hipEvent_t e1,
+                 uint32_t io) {
     int launches = 0;
     if (sort) launches += launch_sort(h);
     if (h->algorithm == 0) {
@@ -1827,9 +1838,9 @@ int enqueue_step(mpcd_hip_s* h, bool sample, bool first, bool sort, uint32_t* re
     }
     if (e0) ck(hipEventRecord(e0, h->stream), "event");
     if (h->algorithm == 0) {
-        if (sample) launch_collide_cap<0, true>(h, rec); else launch_collide_cap<0, false>(h, rec);
+        if (sample) launch_collide_cap<0, true>(h, rec, io); else launch_collide_cap<0, false>(h, rec, io);
     } else {
-        if (sample) launch_collide_cap<1, true>(h, rec); else launch_collide_cap<1, false>(h, rec);
+        if (sample) launch_collide_cap<1, true>(h, rec, io); else launch_collide_cap<1, false>(h, rec, io);
     }
     ck(hipGetLastError(), "k_collide");
     if (e1) ck(hipEventRecord(e1, h->stream), "event");
@@ -1847,14 +1858,14 @@ int enqueue_step(mpcd_hip_s* h, bool sample, bool first, bool sort, uint32_t* re
 }
 
 // the captured step reading h->st[h->cur] (captured on first use)
-hipGraphExec_t step_graph(mpcd_hip_s* h, bool sample) {
-    hipGraphExec_t& ge = h->graph[h->cur][sample ? 1 : 0];
+hipGraphExec_t step_graph(mpcd_hip_s* h, bool sample, uint32_t io) {
+    hipGraphExec_t& ge = h->graph[h->cur][sample ? 1 : 0][io & 3u];
     if (ge) return ge;
     hipGraph_t g = nullptr;
     ck(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal), "begin capture");
     int n = 0;
     try {
-        n = enqueue_step(h, sample, false, true, nullptr, nullptr, nullptr);
+        n = enqueue_step(h, sample, false, true, nullptr, nullptr, nullptr, io);
     } catch (...) {
         (void)hipStreamEndCapture(h->stream, &g);
         if (g) (void)hipGraphDestroy(g);
@@ -1892,13 +1903,16 @@ void run_steps(mpcd_hip_s* h, int n_steps, bool sample) {
         }
         bool const first = sample && h->sampling_done;
         bool const graphed = h->use_graph && !h->timing && !h->record && !first && !h->sorted_valid;
+        // inside one call the intermediate states stay streamed; the last step
+        // writes the post-collision state the API reads
+        uint32_t const io = (s > 0 ? 1u : 0u) | (s + 1 < n_steps ? 2u : 0u);
         if (graphed) {
-            ck(hipGraphLaunch(step_graph(h, sample), h->stream), "launch step graph");
+            ck(hipGraphLaunch(step_graph(h, sample, io), h->stream), "launch step graph");
             h->launches += h->graph_launches[sample ? 1 : 0];
         } else {
             hipEvent_t const e0 = h->timing ? h->ev[2 * s] : nullptr;
             hipEvent_t const e1 = h->timing ? h->ev[2 * s + 1] : nullptr;
-            h->launches += enqueue_step(h, sample, first, !h->sorted_valid, h->record ? h->perm_rec : nullptr, e0, e1);
+            h->launches += enqueue_step(h, sample, first, !h->sorted_valid, h->record ? h->perm_rec : nullptr, e0, e1, io);
         }
         if (sample) {
             if (first) {

@@ -238,3 +238,23 @@ def test_oversized_cell_is_reported(pympcd):
     sim.set_particles(rng.uniform(-5, 5, size=(n, 3)).astype(np.float32), rng.normal(size=(n, 3)).astype(np.float32))
     sim.step(3)
     assert np.isfinite(sim.get_particle_velocities()).all()
+
+
+@pytest.mark.parametrize("algorithm", ["srd", "extended"])
+def test_one_call_equals_single_steps(pympcd, algorithm):
+    """Inside one step() call the intermediate states are kept streamed (the next
+    gather skips re-streaming); the result must equal single-step calls bit for bit."""
+    cfg = ((14, 12, 10), (1, 1, 0), 10, algorithm)
+    _, a, _ = _sim(pympcd, cfg, seed=8)
+    _, b, _ = _sim(pympcd, cfg, seed=8)
+    a.step(9)
+    a.step_and_sample(5)
+    for _ in range(9):
+        b.step(1)
+    for _ in range(5):
+        b.step_and_sample(1)
+    assert np.array_equal(a.get_particle_positions(), b.get_particle_positions())
+    assert np.array_equal(a.get_particle_velocities(), b.get_particle_velocities())
+    da, va = a.get_mean_fields_f64()
+    db, vb = b.get_mean_fields_f64()
+    assert np.array_equal(da, db) and np.array_equal(va, vb)
