@@ -469,18 +469,21 @@ __global__ __launch_bounds__(WG) void k_scan_down(uint32_t* __restrict__ count, 
 // take a device atomic of their own.  The order inside a cell is not the final
 // one: the collide kernel orders every cell by source slot.
 constexpr int PERM_ITEMS = 4;
-constexpr uint32_t PERM_PW = 1024;            // bins per window
+constexpr uint32_t PERM_PW = 512;             // bins per window
 constexpr uint32_t PERM_WIN = 3 * PERM_PW;
 
 __global__ __launch_bounds__(WG) void k_perm(Sys sys, const uint32_t* __restrict__ keys, uint32_t* __restrict__ cursor,
                                              uint32_t* __restrict__ perm, uint64_t N) {
-    __shared__ uint32_t cnt[PERM_WIN];
-    __shared__ uint32_t list[WG * PERM_ITEMS];   // non-empty bins (at most one per particle)
+    constexpr uint32_t ITEMS = WG * PERM_ITEMS;
+    __shared__ uint32_t cnt[PERM_WIN];   // count per bin, then the bin's reserved device base
+    __shared__ uint32_t pre[PERM_WIN];   // block-local exclusive prefix per bin
+    __shared__ uint32_t sj[ITEMS];       // first the non-empty bin list, then the entries in bin order
+    __shared__ uint16_t sb[ITEMS];       // bin of each ordered entry
     __shared__ uint32_t s_w[WG / 64];
     __shared__ uint32_t s_base[3];
     // neighbouring blocks fill the same cells: keep them on one XCD so their
     // partial cache lines meet in one L2
-    uint64_t const base = static_cast<uint64_t>(xcd_tile(blockIdx.x, gridDim.x)) * (WG * PERM_ITEMS);
+    uint64_t const base = static_cast<uint64_t>(xcd_tile(blockIdx.x, gridDim.x)) * ITEMS;
     for (uint32_t i = threadIdx.x; i < PERM_WIN; i += WG) cnt[i] = 0u;
     if (threadIdx.x == 0) {
         uint32_t const k = keys[base];
@@ -491,7 +494,9 @@ __global__ __launch_bounds__(WG) void k_perm(Sys sys, const uint32_t* __restrict
         }
         KParams const& P = sys.P[lo];
         uint32_t const smid = mid_stride(P, k - P.cb);
-        uint32_t const b0 = k - PERM_PW / 4;      // bases may wrap below zero (distances are modulo 2^32)
+        // the window starts one fastest-axis row (plus one) before the first key
+        uint32_t const back = min(P.D0 + 2u, PERM_PW / 2);
+        uint32_t const b0 = k - back;             // bases may wrap below zero (distances are modulo 2^32)
         s_base[0] = b0;
         s_base[1] = b0 - smid;
         s_base[2] = b0 + smid;
@@ -516,19 +521,31 @@ __global__ __launch_bounds__(WG) void k_perm(Sys sys, const uint32_t* __restrict
         local[q] = b != ~0u ? atomicAdd(&cnt[b], 1u) : ~0u;
     }
     __syncthreads();
-    // thread t lists its bins [t B, t B + B) that are non-empty
+    // thread t owns bins [t B, t B + B): one scan gives both the block-local
+    // prefix of the counts (high half) and the places of the non-empty bins in
+    // the bin list (low half); both totals stay below 2^16
     constexpr int PB = PERM_WIN / WG;
-    uint32_t nz = 0;
-#pragma unroll
-    for (int k = 0; k < PB; ++k) nz += cnt[threadIdx.x * PB + k] != 0u;
-    uint32_t nz_tot;
-    uint32_t at = block_exclusive<WG>(nz, s_w, &nz_tot);
+    uint32_t const b0 = threadIdx.x * PB;
+    uint32_t packed = 0;
 #pragma unroll
     for (int k = 0; k < PB; ++k) {
-        uint32_t const i = threadIdx.x * PB + k;
-        if (cnt[i]) list[at++] = i;
+        uint32_t const c = cnt[b0 + k];
+        packed += (c << 16) + (c != 0u ? 1u : 0u);
+    }
+    uint32_t tot_packed;
+    uint32_t const ex = block_exclusive<WG>(packed, s_w, &tot_packed);
+    uint32_t p = ex >> 16, at = ex & 0xffffu;
+    uint32_t const nz_tot = tot_packed & 0xffffu, in_win = tot_packed >> 16;
+    uint32_t* const list = sj;
+#pragma unroll
+    for (int k = 0; k < PB; ++k) {
+        uint32_t const c = cnt[b0 + k];
+        pre[b0 + k] = p;
+        p += c;
+        if (c) list[at++] = b0 + k;
     }
     __syncthreads();
+    // one device range per non-empty bin, with full waves
     for (uint32_t t = threadIdx.x; t < nz_tot; t += WG) {
         uint32_t const i = list[t];
         uint32_t const wdw = i / PERM_PW;
@@ -536,12 +553,24 @@ __global__ __launch_bounds__(WG) void k_perm(Sys sys, const uint32_t* __restrict
         cnt[i] = atomicAdd(cursor + k, cnt[i]);
     }
     __syncthreads();
+    // entries in bin order; keys outside the windows take their own slot
 #pragma unroll
     for (int q = 0; q < PERM_ITEMS; ++q) {
         uint64_t const j = base + q * WG + threadIdx.x;
         if (key[q] == ~0u) continue;
-        uint32_t const slot = bin[q] != ~0u ? cnt[bin[q]] + local[q] : atomicAdd(cursor + key[q], 1u);
-        perm[slot] = static_cast<uint32_t>(j);
+        if (bin[q] != ~0u) {
+            uint32_t const o = pre[bin[q]] + local[q];
+            sj[o] = static_cast<uint32_t>(j);
+            sb[o] = static_cast<uint16_t>(bin[q]);
+        } else {
+            perm[atomicAdd(cursor + key[q], 1u)] = static_cast<uint32_t>(j);
+        }
+    }
+    __syncthreads();
+    // neighbouring lanes write neighbouring slots of a cell
+    for (uint32_t o = threadIdx.x; o < in_win; o += WG) {
+        uint32_t const b = sb[o];
+        perm[cnt[b] + (o - pre[b])] = sj[o];
     }
 }
 
