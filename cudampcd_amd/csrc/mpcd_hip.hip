@@ -725,6 +725,7 @@ struct Tile {
     float cforce[TCM];               // adjoint runs: force term f dt of each cell
     uint32_t gcnt[TCM];              // ghost count
     uint32_t goff[TCM];              // first ghost slot
+    uint32_t wcell[TCM];             // SRD: cell of each staged wall slot
     uint32_t sw[4];                  // scan scratch
     uint16_t cell[CAP];              // tile-local cell of each slot
     float g[ALGO == 1 ? 6 : 1][ALGO == 1 ? CAP / 2 : CAP / 2];  // extended: ghosts, slot k*n + i;
@@ -1106,7 +1107,10 @@ __global__ __launch_bounds__(tile_threads(CAP), (ALGO == 0 ? 1024 : 768) / tile_
                     s.cref[k] = w;
                     gc = gcnt[P.wb + w];
                     uint32_t const slot = atomicAdd(&s.nwall, 1u);
-                    if ((slot + 1) * 3 * P.n <= static_cast<uint32_t>(CAP / 2)) go = slot * 3 * P.n;
+                    if ((slot + 1) * 3 * P.n <= static_cast<uint32_t>(CAP / 2)) {
+                        go = slot * 3 * P.n;
+                        s.wcell[slot] = k;
+                    }
                 }
                 s.gcnt[k] = gc;
                 s.goff[k] = go;
@@ -1117,6 +1121,26 @@ __global__ __launch_bounds__(tile_threads(CAP), (ALGO == 0 ? 1024 : 768) / tile_
             for (uint32_t i = a; i < e; ++i) s.cell[i] = static_cast<uint16_t>(k);
         }
         __syncthreads();
+        // SRD: the staged wall cells' ghost velocities are loaded into registers
+        // now, so their latency overlaps the gather's, and land in LDS below
+        constexpr int GPT = (CAP / 2 + NT - 1) / NT;
+        float gstage[ALGO == 0 ? GPT : 1];
+        uint32_t nstaged = 0;
+        if constexpr (ALGO == 0) {
+            uint32_t const n3 = 3 * P.n;
+            uint32_t const nslot = min(s.nwall, static_cast<uint32_t>(CAP / 2) / n3);
+            nstaged = nslot * n3;
+#pragma unroll
+            for (int q = 0; q < GPT; ++q) {
+                uint32_t const t = tid + q * NT;
+                gstage[q] = 0.0f;
+                if (t < nstaged) {
+                    uint32_t const slot = t / n3, f = t - slot * n3;
+                    uint32_t const k = s.wcell[slot];
+                    if (f < 3 * s.gcnt[k]) gstage[q] = gvel[P.gvb + static_cast<uint64_t>(n3) * s.cref[k] + f];
+                }
+            }
+        }
         // gather this thread's particles, GQ at a time with all their loads in
         // flight together, then order each cell by source slot and re-apply
         // this step's streaming
@@ -1168,16 +1192,14 @@ __global__ __launch_bounds__(tile_threads(CAP), (ALGO == 0 ? 1024 : 768) / tile_
             }
         }
         if constexpr (ALGO == 0) {
-            // stage the wall cells' ghost velocities, one wave per cell
-            uint32_t const wave = tid >> 6, lane = tid & 63u;
-            for (uint32_t k = wave; k < ncell; k += NT / 64) {
-                uint32_t const go = s.goff[k];
-                if (go != ~0u) {
-                    uint32_t const nf = 3 * s.gcnt[k];
-                    float const* const src = gvel + P.gvb + 3ull * P.n * s.cref[k];
-                    for (uint32_t f = lane; f < nf; f += 64) s.g[0][go + f] = src[f];
-                }
+#pragma unroll
+            for (int q = 0; q < GPT; ++q) {
+                uint32_t const t = tid + q * NT;
+                if (t < nstaged) s.g[0][t] = gstage[q];
             }
+        } else {
+            (void)nstaged;
+            (void)gstage;
         }
         __syncthreads();
         // per-cell stage
