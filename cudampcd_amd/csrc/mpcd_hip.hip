@@ -1608,6 +1608,7 @@ struct mpcd_hip_s {
     uint64_t sample_counter = 0;
     uint64_t step_index = 0;
     bool keys_valid = false, sorted_valid = false;
+    bool streamed_pending = false;   // the state buffer holds an intermediate (streamed) state
     uint32_t tile_req = 0, lds_cap = 0;  // requested cells per tile (0: largest that fits), LDS capacity
     // captured steps, by input buffer and sampling
     bool use_graph = true;
@@ -1809,6 +1810,12 @@ void set_err(mpcd_hip_s* h, std::string const& m) {
 template <typename F>
 int guarded(mpcd_hip_s* h, F&& f) {
     if (!h) return 1;
+    if (h->streamed_pending) {
+        // a step() call failed after writing an intermediate state: the
+        // post-collision state the API describes is gone
+        set_err(h, "the particle state was lost when an earlier step failed; destroy this handle");
+        return 3;
+    }
     try {
         f();
         return 0;
@@ -1957,6 +1964,7 @@ void run_steps(mpcd_hip_s* h, int n_steps, bool sample) {
         // inside one call the intermediate states stay streamed; the last step
         // writes the post-collision state the API reads
         uint32_t const io = (s > 0 ? 1u : 0u) | (s + 1 < n_steps ? 2u : 0u);
+        if (io & 2u) h->streamed_pending = true;
         if (graphed) {
             ck(hipGraphLaunch(step_graph(h, sample, io), h->stream), "launch step graph");
             h->launches += h->graph_launches[sample ? 1 : 0];
@@ -1965,6 +1973,7 @@ void run_steps(mpcd_hip_s* h, int n_steps, bool sample) {
             hipEvent_t const e1 = h->timing ? h->ev[2 * s + 1] : nullptr;
             h->launches += enqueue_step(h, sample, first, !h->sorted_valid, h->record ? h->perm_rec : nullptr, e0, e1, io);
         }
+        if (!(io & 2u)) h->streamed_pending = false;   // the post-collision state is written again
         if (sample) {
             if (first) {
                 h->sampling_done = false;
