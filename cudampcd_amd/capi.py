@@ -68,6 +68,8 @@ def lib() -> ctypes.CDLL:
             "mpcd_hip_system_sample_mean_f64": [_vp, ctypes.c_uint32, _f64p, _f64p, ctypes.c_size_t],
             "mpcd_hip_reset_sampling": [_vp],
             "mpcd_hip_use_graph": [_vp, ctypes.c_int],
+            "mpcd_hip_get_sampling_state": [_vp, _f64p, ctypes.c_size_t, _u64p, ctypes.POINTER(ctypes.c_int)],
+            "mpcd_hip_set_sampling_state": [_vp, _f64p, ctypes.c_size_t, ctypes.c_uint64, ctypes.c_int],
             "mpcd_hip_autotune": [_vp, ctypes.c_int, _u32p, _u32p, _f64p],
             "mpcd_hip_profile_loss": [_vp, ctypes.c_int, ctypes.c_float, ctypes.c_float, _f32p, _f64p,
                                       ctypes.c_size_t, _f64p, _f64p],
@@ -170,6 +172,18 @@ class Handle:
         v = [ctypes.c_uint64() for _ in range(4)]
         self._check(self.L.mpcd_hip_system_info(self.h, s, *[ctypes.byref(x) for x in v]))
         return tuple(int(x.value) for x in v)
+
+    def sampling_state(self):
+        """(accumulators (C, 4) float64, samples after the first, average running)."""
+        acc = np.empty((self.C, 4), np.float64)
+        cnt = ctypes.c_uint64()
+        act = ctypes.c_int()
+        self._check(self.L.mpcd_hip_get_sampling_state(self.h, _p(acc, _f64p), self.C, ctypes.byref(cnt), ctypes.byref(act)))
+        return acc, int(cnt.value), bool(act.value)
+
+    def set_sampling_state(self, acc, sample_counter: int, active: bool):
+        a = np.ascontiguousarray(acc, dtype=np.float64).reshape(self.C, 4)
+        self._check(self.L.mpcd_hip_set_sampling_state(self.h, _p(a, _f64p), self.C, sample_counter, int(active)))
 
     def autotune(self, steps_per_trial: int = 3):
         a = ctypes.c_uint32()

@@ -138,6 +138,8 @@ PYBIND11_MODULE(_pympcd, m) {
         .def("set_particles",
              [](SimulationHandle& self, py::array_t<float, py::array::c_style | py::array::forcecast> p,
                 py::array_t<float, py::array::c_style | py::array::forcecast> v) { self.setParticles(flat3(p), flat3(v)); })
+        .def("get_parameters", [](SimulationHandle const& self) { return self.getParameters(); },
+             "A copy of the parameters the simulation was created with.")
         .def_property_readonly("n_particles", [](SimulationHandle& self) { return hip_of(self).getNParticles(); })
         .def_property_readonly("seed", [](SimulationHandle& self) { return hip_of(self).seed(); })
         .def("_handle", [](SimulationHandle& self) { return reinterpret_cast<uintptr_t>(hip_of(self).handle()); },

@@ -2681,6 +2681,31 @@ int mpcd_hip_reset_sampling(mpcd_hip_t* h) {
     return guarded(h, [&] { h->sampling_done = true; });
 }
 
+int mpcd_hip_get_sampling_state(mpcd_hip_t* h, double* acc, size_t n_cells, uint64_t* sample_counter, int* active) {
+    return guarded(h, [&] {
+        if (n_cells != h->C) throw HipError{"cell array size does not match the grid"};
+        if (h->acc) {
+            ck(hipStreamSynchronize(h->stream), "sync");
+            ck(hipMemcpy(acc, h->acc, 32 * h->C, hipMemcpyDeviceToHost), "download accumulators");
+        } else {
+            std::fill(acc, acc + 4 * h->C, 0.0);
+        }
+        if (sample_counter) *sample_counter = h->sample_counter;
+        if (active) *active = h->sampling_done ? 0 : 1;
+    });
+}
+
+int mpcd_hip_set_sampling_state(mpcd_hip_t* h, const double* acc, size_t n_cells, uint64_t sample_counter, int active) {
+    return guarded(h, [&] {
+        if (n_cells != h->C) throw HipError{"cell array size does not match the grid"};
+        ensure_sampling(h);
+        ck(hipStreamSynchronize(h->stream), "sync");
+        ck(hipMemcpy(h->acc, acc, 32 * h->C, hipMemcpyHostToDevice), "upload accumulators");
+        h->sample_counter = sample_counter;
+        h->sampling_done = active == 0;
+    });
+}
+
 int mpcd_hip_use_graph(mpcd_hip_t* h, int enable) {
     return guarded(h, [&] {
         h->use_graph = enable != 0;

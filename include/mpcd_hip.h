@@ -162,6 +162,13 @@ int mpcd_hip_system_get_particles(mpcd_hip_t* h, uint32_t system, float* pos, fl
 int mpcd_hip_system_sample_mean(mpcd_hip_t* h, uint32_t system, float* density, float* velocity, size_t n_cells);
 int mpcd_hip_system_sample_mean_f64(mpcd_hip_t* h, uint32_t system, double* density, double* velocity, size_t n_cells);
 int mpcd_hip_reset_sampling(mpcd_hip_t* h);
+/* Sampling state, for checkpoints (the reference's backup writer is a stub,
+ * cuda_backend_impl.cu:77-79, cpu_backend.cpp:61-63): the FP64 accumulators
+ * (4 per cell: density and velocity sums of the per-step means, n_cells = all
+ * systems' cells), the samples after the first, and whether an average is
+ * running.  Setting it resumes the average exactly where it was saved. */
+int mpcd_hip_get_sampling_state(mpcd_hip_t* h, double* acc, size_t n_cells, uint64_t* sample_counter, int* active);
+int mpcd_hip_set_sampling_state(mpcd_hip_t* h, const double* acc, size_t n_cells, uint64_t sample_counter, int active);
 /* Replay each step from a HIP graph captured on first use (default on; steps
  * with timing, permutation recording or the first sample of an average run
  * eagerly).  The step state (index, grid shifts) lives in device memory and is
