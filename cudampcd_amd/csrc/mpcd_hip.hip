@@ -61,6 +61,8 @@ struct KParams {
     uint32_t mBLK[2], mD0[2], mBlast[2];  // division magics (multiplier, shift)
     uint32_t C;          // cells of this system
     int perio_z;
+    int channel;         // channel geometry: cylinder about the x axis (simulation_functions.cpp:11-24, 75-82)
+    float r2_init, r2_chan;  // its squared radius at initialisation (from X) and in streaming (from Z)
     float dt, drag_dt, tv, sin_a, cos_a;
     uint32_t n, cap;     // particles per cell, cell table capacity 4n
     Stream rs;
@@ -164,7 +166,21 @@ __device__ __forceinline__ bool stream_shift(KParams const& P, float const* sh, 
                                              float& vx, float& vy, float& vz) {
     float const dt = P.dt;
     bool bounced = false;
-    if (!P.perio_z) {
+    if (P.channel) {
+        // undo a move that leaves the cylinder and reverse the velocity (:75-82)
+        px = px + vx * dt;
+        py = py + vy * dt;
+        pz = pz + vz * dt;
+        if ((pz * pz + py * py) > P.r2_chan) {
+            px = px - vx * dt;
+            py = py - vy * dt;
+            pz = pz - vz * dt;
+            vx = -vx;
+            vy = -vy;
+            vz = -vz;
+            bounced = true;
+        }
+    } else if (!P.perio_z) {
         float const z_wall = (0.5f * P.L[2] - 1.0f);
         float const next_z = pz + vz * dt;
         if (fabsf(next_z) > z_wall) {
@@ -316,7 +332,9 @@ __device__ __forceinline__ uint32_t xcd_tile(uint32_t b, uint32_t G) {
 // seeded initial distribution (simulation_functions.cpp:7-33): positions
 // uniform in the fluid region (edge = volume - 2 (1 - periodic), :8-10),
 // Maxwell-Boltzmann velocities
-__global__ __launch_bounds__(WG) void k_init(Sys sys, float* __restrict__ st) {
+constexpr uint32_t INIT_ATTEMPTS = 1u << 16;   // channel rejection loop bound
+
+__global__ __launch_bounds__(WG) void k_init(Sys sys, float* __restrict__ st, uint32_t* __restrict__ status) {
     uint32_t const s = find_sys(sys.blk_part, sys.ns, blockIdx.x);
     KParams const P = sys.P[s];
     uint32_t const b0 = sys.blk_part[s], nb = sys.blk_part[s + 1] - b0;
@@ -324,14 +342,25 @@ __global__ __launch_bounds__(WG) void k_init(Sys sys, float* __restrict__ st) {
     for (uint64_t i = static_cast<uint64_t>(blockIdx.x - b0) * WG + threadIdx.x; i < P.N;
          i += static_cast<uint64_t>(nb) * WG) {
         uint32_t const lo = static_cast<uint32_t>(i);
-        Words const a = P.rs.at(P_INIT_POS, lo, 0u, 0);
+        // channel: redraw (attempt a keys the draw) until inside the cylinder (:11-24)
+        float px, py, pz;
+        for (uint32_t att = 0;; ++att) {
+            Words const a = P.rs.at(P_INIT_POS, lo, 0u, att);
+            px = (uniform(a.w[0]) - 0.5f) * sx;
+            py = (uniform(a.w[1]) - 0.5f) * sy;
+            pz = (uniform(a.w[2]) - 0.5f) * sz;
+            if (!(P.channel && (pz * pz + py * py) > P.r2_init)) break;
+            if (att + 1 == INIT_ATTEMPTS) {
+                atomicOr(status, 2u);
+                break;
+            }
+        }
         Words const b = P.rs.at(P_INIT_VEL, lo, 0u, 0);
         float g0, g1, g2, g3;
         box_muller(b.w[0], b.w[1], &g0, &g1);
         box_muller(b.w[2], b.w[3], &g2, &g3);
         uint64_t const j = P.pb + i;
-        store_rec(st, j, (uniform(a.w[0]) - 0.5f) * sx, (uniform(a.w[1]) - 0.5f) * sy, (uniform(a.w[2]) - 0.5f) * sz,
-                  g0 * P.tv, g1 * P.tv, g2 * P.tv);
+        store_rec(st, j, px, py, pz, g0 * P.tv, g1 * P.tv, g2 * P.tv);
     }
 }
 
@@ -1667,6 +1696,9 @@ void fill_geometry(mpcd_params_t const& p, KParams& k) {
     udiv_magic(k.D0, k.mD0);
     udiv_magic(k.Blast, k.mBlast);
     k.perio_z = p.periodicity[2];
+    k.channel = p.experiment == 1;
+    k.r2_init = (p.volume_size[0] - 2) * (p.volume_size[0] - 2) * 0.25f;
+    k.r2_chan = (p.volume_size[2] - 2) * (p.volume_size[2] - 2) * 0.25f;
     k.dt = p.delta_t;
     k.drag_dt = p.drag * p.delta_t;
     k.tv = p.thermal_velocity;
@@ -2174,6 +2206,7 @@ ProfileResult profile_run(mpcd_hip_s* h, int steps, int K, float drag, float ang
                           double const* target, size_t n_layers, float* grad_force) {
     if (h->ns != 1) throw HipError{"adjoint runs take a single system"};
     if (h->algorithm != 0) throw HipError{"adjoint runs implement the SRD rule"};
+    if (h->kp[0].channel) throw HipError{"adjoint runs implement the standard geometry (walls in z)"};
     if (steps < 1) throw HipError{"steps must be positive"};
     if (K < 0) throw HipError{"checkpoint interval must be positive"};
     if (h->N >= BOUNCED) throw HipError{"adjoint runs take fewer than 2^31 particles"};
@@ -2311,7 +2344,9 @@ ProfileResult profile_run(mpcd_hip_s* h, int steps, int K, float drag, float ang
 }
 
 std::string validate(mpcd_params_t const& p) {
-    if (p.experiment != 0) return "the hip backend implements the standard geometry only (experiment = \"standard\")";
+    if (p.experiment != 0 && p.experiment != 1) return "unknown experiment (standard or channel)";
+    if (p.experiment == 1 && !(p.volume_size[0] > 2.0f))
+        return "the channel needs more than 2 cells along x (its radius is (X - 2) / 2 at initialisation)";
     if (p.algorithm != 0 && p.algorithm != 1) return "Unsupported MPCD algorithm";
     for (int a = 0; a < 3; ++a)
         if (!(p.volume_size[a] >= 1.0f) || p.volume_size[a] != floorf(p.volume_size[a]))
@@ -2386,9 +2421,12 @@ int create_impl(const mpcd_params_t* params, uint32_t n_systems, mpcd_hip_t** ou
         ck(hipMemsetAsync(h->status, 0, 4, h->stream), "clear status");
         ck(hipMemsetAsync(h->count, 0, 4 * h->C, h->stream), "clear counts");
         if (h->blocks_part)
-            hipLaunchKernelGGL(k_init, dim3(h->blocks_part), dim3(WG), 0, h->stream, h->sys, h->st[0]);
+            hipLaunchKernelGGL(k_init, dim3(h->blocks_part), dim3(WG), 0, h->stream, h->sys, h->st[0], h->status);
         ck(hipGetLastError(), "k_init");
         ck(hipStreamSynchronize(h->stream), "init");
+        uint32_t st = 0;
+        ck(hipMemcpy(&st, h->status, 4, hipMemcpyDeviceToHost), "status");
+        if (st & 2u) throw HipError{"channel initialisation: the cylinder holds too little of the box"};
     } catch (HipError const& e) {
         destroy(h);
         return fail(e.msg);

@@ -26,7 +26,7 @@ extern "C" {
 /* Mirrors mpcd::SimulationParameters (include/mpcd/api/simulation_parameters.hpp:23-55),
  * numeric fields only, plus the options the reference fixes in code. */
 typedef struct mpcd_params_t {
-    int experiment;          /* 0 standard, 1 channel (channel: rejected by this backend) */
+    int experiment;          /* 0 standard, 1 channel: cylinder about x (simulation_functions.cpp:11-24, 75-82) */
     int algorithm;           /* 0 srd, 1 extended */
     int collision_kernel;    /* accepted for compatibility; this backend always runs its fused kernels */
     int periodicity[3];

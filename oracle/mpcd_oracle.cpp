@@ -309,14 +309,15 @@ struct Draws {
         box_muller(r.v[2], r.v[3], &g[2], &g[3]);
         return g[i & 3];
     }
-    V3 init_pos_u(uint64_t i) {
+    // attempt > 0: redraws of the channel geometry's rejection loop
+    V3 init_pos_u(uint64_t i, uint32_t attempt = 0) {
         if (mode == 0) {
             float a = xo.uniform();
             float b = xo.uniform();
             float c = xo.uniform();
             return {a, b, c};
         }
-        U4 r = ctr(P_INIT_POS, static_cast<uint32_t>(i), static_cast<uint32_t>(i >> 32), 0);
+        U4 r = ctr(P_INIT_POS, static_cast<uint32_t>(i), static_cast<uint32_t>(i >> 32), attempt);
         return {uf(r.v[0]), uf(r.v[1]), uf(r.v[2])};
     }
     V3 init_vel(uint64_t i) {
@@ -352,6 +353,7 @@ struct orc_params {
     unsigned long long seed_a, seed_b;  // mode 0: Xoshiro seed(a,b); mode 1: Philox key = seed_a
     unsigned system;        // mode 1: ensemble member id
     int ghost_xy_order;     // mode 0 only
+    int experiment;         // 0 standard, 1 channel (cylinder about the x axis)
 };
 }
 
@@ -450,11 +452,16 @@ struct Oracle {
         V3 scale = {L.x - static_cast<float>(2 * (1 - p.periodic[0])),
                     L.y - static_cast<float>(2 * (1 - p.periodic[1])),
                     L.z - static_cast<float>(2 * (1 - p.periodic[2]))};
+        // channel: rejection outside the radius taken from the x extent (:11, :23-24)
+        float const r2 = (p.volume[0] - 2) * (p.volume[0] - 2) * 0.25f;
         for (size_t i = 0; i < parts.size(); ++i) {
             Particle q = {};
-            V3 u = rng.init_pos_u(i);
-            V3 r = sub(u, V3{0.5f, 0.5f, 0.5f});
-            q.pos = {r.x * scale.x, r.y * scale.y, r.z * scale.z};
+            for (uint32_t attempt = 0;; ++attempt) {
+                V3 u = rng.init_pos_u(i, attempt);
+                V3 r = sub(u, V3{0.5f, 0.5f, 0.5f});
+                q.pos = {r.x * scale.x, r.y * scale.y, r.z * scale.z};
+                if (!(p.experiment == 1 && (q.pos.z * q.pos.z + q.pos.y * q.pos.y) > r2)) break;
+            }
             q.vel = mul(rng.init_vel(i), p.thermal_velocity);
             q.cell = index(q.pos);
             parts[i] = q;
@@ -466,9 +473,18 @@ struct Oracle {
         shift = fixed_shift ? forced : rng.shift(static_cast<uint32_t>(step_index));
         std::fill(counter.begin(), counter.end(), 0u);
         float const dt = p.delta_t;
+        // channel: radius from the z extent (:45), unlike distribute
+        float const r2 = (L.z - 2) * (L.z - 2) * 0.25f;
         for (size_t idx = 0; idx < parts.size(); ++idx) {
             Particle q = parts[idx];
-            if (!p.periodic[2]) {
+            if (p.experiment == 1) {
+                // simulation_functions.cpp:75-82: undo a move that leaves the cylinder, reverse the velocity
+                q.pos = add(q.pos, mul(q.vel, dt));
+                if ((q.pos.z * q.pos.z + q.pos.y * q.pos.y) > r2) {
+                    q.pos = sub(q.pos, mul(q.vel, dt));
+                    q.vel = neg(q.vel);
+                }
+            } else if (!p.periodic[2]) {
                 float z_wall = (0.5f * L.z - 1);
                 float next_z = q.pos.z + q.vel.z * dt;
                 if (fabsf(next_z) > z_wall) {

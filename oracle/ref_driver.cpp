@@ -59,11 +59,12 @@ struct ref_params {
     float thermal_velocity;
     int algorithm;         // 0 srd, 1 extended
     unsigned long long seed_a, seed_b;
+    int experiment;        // 0 standard, 1 channel
 };
 
 void* ref_create(const ref_params* q) {
     mpcd::SimulationParameters p;
-    p.experiment = mpcd::ExperimentType::standard;
+    p.experiment = q->experiment == 1 ? mpcd::ExperimentType::channel : mpcd::ExperimentType::standard;
     p.algorithm = q->algorithm == 0 ? mpcd::MPCDAlgorithm::srd : mpcd::MPCDAlgorithm::extended;
     for (int i = 0; i < 3; ++i) {
         p.volume_size[i] = q->volume[i];

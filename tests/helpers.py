@@ -6,7 +6,7 @@ import numpy as np
 
 
 def make_params(pympcd, fluid, periodicity=(1, 1, 0), n=10, algorithm="srd", seed=1234,
-                drag=0.001, delta_t=0.02, temperature=1.0, angle=120.0):
+                drag=0.001, delta_t=0.02, temperature=1.0, angle=120.0, experiment="standard"):
     p = pympcd.Params()
     p.n = n
     p.temperature = temperature
@@ -14,7 +14,7 @@ def make_params(pympcd, fluid, periodicity=(1, 1, 0), n=10, algorithm="srd", see
     p.volume_size = tuple(fluid)          # wall layers are added by the setter
     p.drag = drag
     p.delta_t = delta_t
-    p.experiment = "standard"
+    p.experiment = experiment
     p.algorithm = algorithm
     p.rotation_angle = angle
     p.seed = seed

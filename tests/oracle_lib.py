@@ -35,6 +35,7 @@ class OrcParams(ctypes.Structure):
         ("seed_b", ctypes.c_ulonglong),
         ("system", ctypes.c_uint),
         ("ghost_xy_order", ctypes.c_int),
+        ("experiment", ctypes.c_int),
     ]
 
 
@@ -50,6 +51,7 @@ class RefParams(ctypes.Structure):
         ("algorithm", ctypes.c_int),
         ("seed_a", ctypes.c_ulonglong),
         ("seed_b", ctypes.c_ulonglong),
+        ("experiment", ctypes.c_int),
     ]
 
 
@@ -150,7 +152,7 @@ class Oracle:
 
     def __init__(self, volume, periodic=(1, 1, 0), n=10, N=None, delta_t=0.02, drag=0.0,
                  thermal_velocity=1.0, rotation_deg=120.0, algorithm="srd", rng="counter",
-                 seed=(12345, 67890), system=0, ghost_xy_order=0, distribute=True):
+                 seed=(12345, 67890), system=0, ghost_xy_order=0, distribute=True, experiment="standard"):
         self.lib = oracle_lib()
         p = OrcParams()
         for i in range(3):
@@ -167,6 +169,7 @@ class Oracle:
         p.seed_a, p.seed_b = seed
         p.system = system
         p.ghost_xy_order = ghost_xy_order
+        p.experiment = 1 if experiment == "channel" else 0
         self.params = p
         self.h = self.lib.orc_create(ctypes.byref(p))
         self.N = int(p.N)
@@ -266,7 +269,7 @@ class Reference:
     """The reference CPU backend (cpu_backend.cpp) compiled in place, seeded."""
 
     def __init__(self, volume, periodic=(1, 1, 0), n=10, N=None, delta_t=0.02, drag=0.0,
-                 thermal_velocity=1.0, algorithm="srd", seed=(12345, 67890)):
+                 thermal_velocity=1.0, algorithm="srd", seed=(12345, 67890), experiment="standard"):
         self.lib = ref_lib()
         p = RefParams()
         for i in range(3):
@@ -279,6 +282,7 @@ class Reference:
         p.thermal_velocity = thermal_velocity
         p.algorithm = 0 if algorithm == "srd" else 1
         p.seed_a, p.seed_b = seed
+        p.experiment = 1 if experiment == "channel" else 0
         self.h = self.lib.ref_create(ctypes.byref(p))
         self.N = int(p.N)
         self.C = int(np.prod([int(x) for x in volume]))

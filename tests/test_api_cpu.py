@@ -109,7 +109,9 @@ def test_unknown_backend_rejected(pympcd):
 def test_invalid_configurations_fail_cleanly(pympcd):
     p = pympcd.Params()
     p.experiment = "channel"
-    with pytest.raises(RuntimeError, match="standard geometry"):
+    p.periodicity = (1, 1, 1)
+    p.volume_size = (2, 6, 6)
+    with pytest.raises(RuntimeError, match="more than 2 cells along x"):
         pympcd.Simulation(p, "hip")
     p = pympcd.Params()
     p.periodicity = (0, 1, 1)

@@ -49,7 +49,8 @@ def _sim(pympcd, cfg, seed=4242, drag=0.001):
 def _oracle(p, distribute=False):
     return Oracle(stored_volume(p), tuple(p.periodicity), n=p.n, drag=p.drag, delta_t=p.delta_t,
                   thermal_velocity=float(np.sqrt(p.temperature)), rotation_deg=p.rotation_angle,
-                  algorithm=p.algorithm, rng="counter", seed=(p.seed, 0), distribute=distribute)
+                  algorithm=p.algorithm, rng="counter", seed=(p.seed, 0), distribute=distribute,
+                  experiment=p.experiment)
 
 
 def test_device_philox_known_answers():
@@ -258,3 +259,38 @@ def test_one_call_equals_single_steps(pympcd, algorithm):
     da, va = a.get_mean_fields_f64()
     db, vb = b.get_mean_fields_f64()
     assert np.array_equal(da, db) and np.array_equal(va, vb)
+
+
+CHANNELS = [
+    # fluid extents, periodicity, n, algorithm: cylinder about x, radius (X-2)/2 at
+    # initialisation and (Z-2)/2 in streaming (simulation_functions.cpp:11-24, 45)
+    ((10, 8, 8), (1, 1, 1), 8, "srd"),
+    ((8, 10, 10), (1, 1, 1), 8, "srd"),
+    ((10, 8, 8), (1, 1, 1), 8, "extended"),
+    ((12, 9, 7), (1, 1, 0), 6, "srd"),
+]
+CIDS = [f"{a}-{'x'.join(map(str, f))}-p{''.join(map(str, p))}" for f, p, n, a in CHANNELS]
+
+
+@pytest.mark.parametrize("cfg", CHANNELS, ids=CIDS)
+def test_channel_matches_oracle(pympcd, cfg):
+    fluid, perio, n, algo = cfg
+    p = make_params(pympcd, fluid, perio, n=n, algorithm=algo, seed=606, drag=0.002, experiment="channel")
+    sim = pympcd.Simulation(p, "hip")
+    h = Handle(sim)
+    o = _oracle(p, distribute=True)
+    pos, vel = o.particles()
+    assert np.array_equal(sim.get_particle_positions(), pos)
+    assert np.array_equal(sim.get_particle_velocities(), vel)
+    sim.step(3)
+    h.record_permutation(True)
+    for _ in range(3):
+        o = _oracle(p)
+        o.set_particles(sim.get_particle_positions(), sim.get_particle_velocities())
+        o.set_step(h.step_index)
+        sim.step(1)
+        o.step(1)
+        perm = h.last_permutation()
+        opos, ovel = o.particles()
+        assert np.array_equal(sim.get_particle_positions(), opos[perm])
+        assert np.array_equal(sim.get_particle_velocities(), ovel[perm])

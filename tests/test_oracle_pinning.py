@@ -63,6 +63,46 @@ def test_oracle_reproduces_reference_bitwise(algorithm, volume, periodic, n):
 
 @needs_ref
 @pytest.mark.parametrize("algorithm", ["srd", "extended"])
+@pytest.mark.parametrize("volume,periodic", [
+    ((10, 8, 8), (1, 1, 1)),    # initial radius from x (4) > streaming radius from z (3)
+    ((8, 10, 10), (1, 1, 1)),   # initial radius 3 < streaming radius 4
+    ((10, 8, 8), (1, 1, 0)),    # with z-wall layers (and their ghosts) as well
+])
+def test_oracle_channel_reproduces_reference_bitwise(algorithm, volume, periodic):
+    """Channel geometry (simulation_functions.cpp:11-24, 45, 75-82): rejection
+    sampling inside a cylinder about x, bounce back off its wall."""
+    kw = dict(n=6, drag=0.01, delta_t=0.05, algorithm=algorithm, seed=(77, 88), experiment="channel")
+    o = Oracle(volume, periodic, rng="reference", **kw)
+    r = Reference(volume, periodic, **kw)
+    p0, v0 = o.particles()
+    q0, w0 = r.particles()
+    assert np.array_equal(p0, q0) and np.array_equal(v0, w0)
+    r2_init = np.float32((volume[0] - 2) ** 2 * 0.25)
+    assert (p0[:, 2] * p0[:, 2] + p0[:, 1] * p0[:, 1] <= r2_init).all()
+    for _ in range(10):
+        assert np.array_equal(o.translate(), r.translate())
+        assert np.array_equal(o.cells()[2], r.cells()[2])
+        o.collide()
+        r.collide()
+        p, v = o.particles()
+        q, w = r.particles()
+        assert np.array_equal(p, q) and np.array_equal(v, w)
+
+
+def test_counter_channel_initial_state_inside_cylinder():
+    o = Oracle((12, 9, 9), (1, 1, 1), n=5, rng="counter", seed=(3, 0), experiment="channel")
+    pos, _ = o.particles()
+    r2 = np.float32((12 - 2) ** 2 * 0.25)
+    assert (pos[:, 2] * pos[:, 2] + pos[:, 1] * pos[:, 1] <= r2).all()
+    s = Oracle((12, 9, 9), (1, 1, 1), n=5, rng="counter", seed=(3, 0))
+    spos, _ = s.particles()
+    # particles whose first draw already lies inside keep it (attempt 0 = the standard draw)
+    inside = spos[:, 2] * spos[:, 2] + spos[:, 1] * spos[:, 1] <= r2
+    assert np.array_equal(pos[inside], spos[inside]) and (~inside).any()
+
+
+@needs_ref
+@pytest.mark.parametrize("algorithm", ["srd", "extended"])
 def test_oracle_sampling_reproduces_reference(algorithm):
     kw = dict(n=10, drag=0.01, delta_t=0.02, algorithm=algorithm, seed=(5, 7))
     o = Oracle((6, 5, 10), (1, 1, 0), rng="reference", **kw)
