@@ -497,12 +497,14 @@ __global__ __launch_bounds__(WG) void k_scan_down(uint32_t* __restrict__ count, 
 // atomics run with full waves) and writes the slots.  Keys outside the windows
 // take a device atomic of their own.  The order inside a cell is not the final
 // one: the collide kernel orders every cell by source slot.
-constexpr int PERM_ITEMS = 4;
+constexpr int PERM_ITEMS = 8;
 constexpr uint32_t PERM_PW = 512;             // bins per window
 constexpr uint32_t PERM_WIN = 3 * PERM_PW;
 
-__global__ __launch_bounds__(WG) void k_perm(Sys sys, const uint32_t* __restrict__ keys, uint32_t* __restrict__ cursor,
-                                             uint32_t* __restrict__ perm, uint64_t N) {
+// one block of the permutation: particles [blk ITEMS, (blk + 1) ITEMS)
+__device__ __forceinline__ void perm_block(Sys const& sys, const uint32_t* __restrict__ keys,
+                                           uint32_t* __restrict__ cursor, uint32_t* __restrict__ perm, uint64_t N,
+                                           uint32_t blk) {
     constexpr uint32_t ITEMS = WG * PERM_ITEMS;
     __shared__ uint32_t cnt[PERM_WIN];   // count per bin, then the bin's reserved device base
     __shared__ uint32_t pre[PERM_WIN];   // block-local exclusive prefix per bin
@@ -510,12 +512,16 @@ __global__ __launch_bounds__(WG) void k_perm(Sys sys, const uint32_t* __restrict
     __shared__ uint16_t sb[ITEMS];       // bin of each ordered entry
     __shared__ uint32_t s_w[WG / 64];
     __shared__ uint32_t s_base[3];
-    // neighbouring blocks fill the same cells: keep them on one XCD so their
-    // partial cache lines meet in one L2
-    uint64_t const base = static_cast<uint64_t>(xcd_tile(blockIdx.x, gridDim.x)) * ITEMS;
+    uint64_t const base = static_cast<uint64_t>(blk) * ITEMS;
     for (uint32_t i = threadIdx.x; i < PERM_WIN; i += WG) cnt[i] = 0u;
+    uint32_t key[PERM_ITEMS], local[PERM_ITEMS], bin[PERM_ITEMS];
+#pragma unroll
+    for (int q = 0; q < PERM_ITEMS; ++q) {
+        uint64_t const j = base + q * WG + threadIdx.x;
+        key[q] = j < N ? keys[j] : ~0u;
+    }
     if (threadIdx.x == 0) {
-        uint32_t const k = keys[base];
+        uint32_t const k = key[0];   // keys[base]: base < N for every launched block
         uint32_t lo = 0, hi = sys.ns;   // system holding key k
         while (hi - lo > 1) {
             uint32_t const mid = (lo + hi) >> 1;
@@ -532,12 +538,6 @@ __global__ __launch_bounds__(WG) void k_perm(Sys sys, const uint32_t* __restrict
     }
     __syncthreads();
     uint32_t const w0 = s_base[0], wm = s_base[1], wp = s_base[2];
-    uint32_t key[PERM_ITEMS], local[PERM_ITEMS], bin[PERM_ITEMS];
-#pragma unroll
-    for (int q = 0; q < PERM_ITEMS; ++q) {
-        uint64_t const j = base + q * WG + threadIdx.x;
-        key[q] = j < N ? keys[j] : ~0u;
-    }
 #pragma unroll
     for (int q = 0; q < PERM_ITEMS; ++q) {
         uint32_t w = key[q] - w0;
@@ -601,6 +601,13 @@ __global__ __launch_bounds__(WG) void k_perm(Sys sys, const uint32_t* __restrict
         uint32_t const b = sb[o];
         perm[cnt[b] + (o - pre[b])] = sj[o];
     }
+}
+
+__global__ __launch_bounds__(WG) void k_perm(Sys sys, const uint32_t* __restrict__ keys, uint32_t* __restrict__ cursor,
+                                             uint32_t* __restrict__ perm, uint64_t N) {
+    // neighbouring blocks fill the same cells: keep them on one XCD so their
+    // partial cache lines meet in one L2
+    perm_block(sys, keys, cursor, perm, N, xcd_tile(blockIdx.x, gridDim.x));
 }
 
 // per-cell ghost count in a wall layer (simulation_functions.cpp:137-146)
@@ -674,12 +681,12 @@ __device__ __forceinline__ Sym sym_inverse(Sym const& m, float tol) {
 //   gvel[w][g][a]  ghost velocities, Maxwell-Boltzmann times sqrt(T) (:153-166)
 // Wall cells: the two z layers next to the walls for this step's shift,
 // w = (upper ? X*Y : 0) + x + X*y.
-__global__ __launch_bounds__(WG) void k_cell_axes(Sys sys, float* __restrict__ axis) {
-    uint32_t const s = find_sys(sys.blk_cell, sys.ns, blockIdx.x);
+__device__ __forceinline__ void axes_block(Sys const& sys, float* __restrict__ axis, uint32_t blk) {
+    uint32_t const s = find_sys(sys.blk_cell, sys.ns, blk);
     KParams const P = sys.P[s];
     uint32_t const b0 = sys.blk_cell[s], nb = sys.blk_cell[s + 1] - b0;
     uint64_t const CT = sys.CT;
-    for (uint32_t key = (blockIdx.x - b0) * WG + threadIdx.x; key < P.C; key += nb * WG) {
+    for (uint32_t key = (blk - b0) * WG + threadIdx.x; key < P.C; key += nb * WG) {
         uint32_t ix, iy, iz;
         decode_key(P, key, ix, iy, iz);
         float ax, ay, az;
@@ -691,15 +698,16 @@ __global__ __launch_bounds__(WG) void k_cell_axes(Sys sys, float* __restrict__ a
     }
 }
 
-__global__ __launch_bounds__(WG) void k_wall_ghosts(Sys sys, uint32_t* __restrict__ gcnt, float* __restrict__ gvel) {
-    uint32_t const s = find_sys(sys.blk_wall, sys.ns, blockIdx.x);
+__device__ __forceinline__ void ghosts_block(Sys const& sys, uint32_t* __restrict__ gcnt, float* __restrict__ gvel,
+                                             uint32_t blk) {
+    uint32_t const s = find_sys(sys.blk_wall, sys.ns, blk);
     KParams const P = sys.P[s];
     uint32_t const b0 = sys.blk_wall[s], nb = sys.blk_wall[s + 1] - b0;
     uint32_t const XY = P.dim[0] * P.dim[1];
     float const sh = fabsf(P.shift[2]);
     bool const sign = P.shift[2] > 0;
     uint32_t const iz_layer = static_cast<uint32_t>(P.L[2] - (sign ? 1.0f : 2.0f));
-    for (uint32_t w = (blockIdx.x - b0) * WG + threadIdx.x; w < 2 * XY; w += nb * WG) {
+    for (uint32_t w = (blk - b0) * WG + threadIdx.x; w < 2 * XY; w += nb * WG) {
         bool const layer = w >= XY;
         uint32_t const xy = layer ? w - XY : w;
         uint32_t const iz = layer ? iz_layer : static_cast<uint32_t>(sign);
@@ -717,6 +725,13 @@ __global__ __launch_bounds__(WG) void k_wall_ghosts(Sys sys, uint32_t* __restric
             out[3 * g + 2] = g2 * P.tv;
         }
     }
+}
+
+__global__ __launch_bounds__(WG) void k_cell_axes(Sys sys, float* __restrict__ axis) {
+    axes_block(sys, axis, blockIdx.x);
+}
+__global__ __launch_bounds__(WG) void k_wall_ghosts(Sys sys, uint32_t* __restrict__ gcnt, float* __restrict__ gvel) {
+    ghosts_block(sys, gcnt, gvel, blockIdx.x);
 }
 
 // one step forward of every system's step state: the shift of step t + 1 and
@@ -739,9 +754,9 @@ __global__ void k_advance(Sys sys) {
 // TCM cells.  Sizes are compile-time so every LDS access is an immediate offset.
 constexpr int TCM = 64;
 
-// threads of the collide workgroup for an LDS capacity: small tiles run as
-// one-wave workgroups (many independent tiles per CU), large ones as four waves
-constexpr int tile_threads(int cap) { return cap <= 384 ? 64 : 256; }
+// threads of the collide workgroup for an LDS capacity: more, smaller
+// workgroups keep more independent tiles in flight per CU
+constexpr int tile_threads(int cap) { return cap <= 256 ? 64 : (cap <= 640 ? 128 : 256); }
 
 template <int CAP, int ALGO>
 struct Tile {
@@ -1737,15 +1752,22 @@ void upload_axes(int device) {
 }
 
 // LDS tile capacities compiled into k_collide (particles per tile)
-constexpr uint32_t TILE_CAPS[4] = {256, 512, 1152, 2048};
+constexpr uint32_t TILE_CAPS[6] = {256, 384, 512, 640, 1152, 2048};
 
 // Cells per tile of every system for the LDS capacity `cap` (0: keep) and the
 // request `tc` (0: the largest that fits).  SRD: on average about 80% of the
 // tile's particle capacity; extended: the ghosts of a batch (at most n per
 // cell) must fit the cap/2 ghost region.
 void choose_tiling(mpcd_hip_s* h, uint32_t tc, uint32_t cap) {
-    if (cap == 0) cap = h->lds_cap ? h->lds_cap : 1152u;
-    uint32_t chosen = TILE_CAPS[3];
+    if (cap == 0 && h->lds_cap) cap = h->lds_cap;
+    if (cap == 0) {
+        // 512-particle tiles on two-wave workgroups, larger where the density
+        // makes single cells of several hundred particles likely
+        uint32_t nmax = 1;
+        for (KParams const& k : h->kp) nmax = std::max(nmax, k.n);
+        cap = std::max(512u, 6 * nmax);
+    }
+    uint32_t chosen = TILE_CAPS[5];
     for (uint32_t c : TILE_CAPS)
         if (c >= cap && (h->algorithm == 0 || c >= 512)) { chosen = c; break; }
     h->lds_cap = chosen;
@@ -1868,16 +1890,25 @@ void launch_keys(mpcd_hip_s* h) {
     h->sorted_valid = false;
 }
 
-int launch_sort(mpcd_hip_s* h) {
+void launch_scan(mpcd_hip_s* h) {
     uint32_t const C = static_cast<uint32_t>(h->C);
     hipLaunchKernelGGL(k_scan_reduce, dim3(h->nb), dim3(WG), 0, h->stream, h->count, C, h->partial);
     hipLaunchKernelGGL(k_scan_partials, dim3(1), dim3(1024), 0, h->stream, h->partial, h->nb);
     hipLaunchKernelGGL(k_scan_down, dim3(h->nb), dim3(WG), 0, h->stream, h->count, C, h->partial, h->offset, h->cursor, h->N);
-    uint64_t const blocks = (h->N + WG * PERM_ITEMS - 1) / (WG * PERM_ITEMS);
-    if (blocks) hipLaunchKernelGGL(k_perm, dim3(static_cast<uint32_t>(blocks)), dim3(WG), 0, h->stream, h->sys, h->keys, h->cursor, h->perm, h->N);
+}
+
+uint32_t perm_blocks(mpcd_hip_s const* h) {
+    return static_cast<uint32_t>((h->N + WG * PERM_ITEMS - 1) / (WG * PERM_ITEMS));
+}
+
+int launch_sort(mpcd_hip_s* h) {
+    launch_scan(h);
+    uint32_t const blocks = perm_blocks(h);
+    if (blocks) hipLaunchKernelGGL(k_perm, dim3(blocks), dim3(WG), 0, h->stream, h->sys, h->keys, h->cursor, h->perm, h->N);
     ck(hipGetLastError(), "sort");
     return 4;
 }
+
 
 template <int ALGO, bool SAMPLE, int CAP, bool ADJ = false>
 void launch_collide_t(mpcd_hip_s* h, uint32_t* rec, Tape const& T = Tape{}, uint32_t io = 0) {
@@ -1894,7 +1925,15 @@ void launch_collide_cap(mpcd_hip_s* h, uint32_t* rec, uint32_t io) {
             if constexpr (ALGO == 0) launch_collide_t<ALGO, SAMPLE, 256>(h, rec, Tape{}, io);
             else throw HipError{"the extended rule needs an LDS capacity of at least 512"};
             break;
+        case 384:
+            if constexpr (ALGO == 0) launch_collide_t<ALGO, SAMPLE, 384>(h, rec, Tape{}, io);
+            else throw HipError{"the extended rule needs an LDS capacity of at least 512"};
+            break;
         case 512: launch_collide_t<ALGO, SAMPLE, 512>(h, rec, Tape{}, io); break;
+        case 640:
+            if constexpr (ALGO == 0) launch_collide_t<ALGO, SAMPLE, 640>(h, rec, Tape{}, io);
+            else launch_collide_t<ALGO, SAMPLE, 1152>(h, rec, Tape{}, io);
+            break;
         case 1152: launch_collide_t<ALGO, SAMPLE, 1152>(h, rec, Tape{}, io); break;
         default: launch_collide_t<ALGO, SAMPLE, 2048>(h, rec, Tape{}, io); break;
     }
@@ -2172,7 +2211,9 @@ void adj_step(mpcd_hip_s* h, float const* force, int j) {
     }
     switch (h->lds_cap) {
         case 256: launch_collide_t<0, false, 256, true>(h, rec, T); break;
+        case 384: launch_collide_t<0, false, 384, true>(h, rec, T); break;
         case 512: launch_collide_t<0, false, 512, true>(h, rec, T); break;
+        case 640: launch_collide_t<0, false, 640, true>(h, rec, T); break;
         case 1152: launch_collide_t<0, false, 1152, true>(h, rec, T); break;
         default: launch_collide_t<0, false, 2048, true>(h, rec, T); break;
     }
@@ -2627,8 +2668,8 @@ int mpcd_hip_last_timing(const mpcd_hip_t* h, double* collide_ms, double* total_
 
 int mpcd_hip_set_tiling(mpcd_hip_t* h, uint32_t cells_per_tile, uint32_t lds_capacity) {
     return guarded(h, [&] {
-        if (lds_capacity > TILE_CAPS[3])
-            throw HipError{"tile capacity above " + std::to_string(TILE_CAPS[3]) + " particles"};
+        if (lds_capacity > TILE_CAPS[5])
+            throw HipError{"tile capacity above " + std::to_string(TILE_CAPS[5]) + " particles"};
         choose_tiling(h, cells_per_tile, lds_capacity);
         layout(h);
     });
