@@ -787,65 +787,63 @@ template <int CAP, bool ADJ>
 __device__ void srd_cells(KParams const& P, Tile<CAP, 0>& s, uint32_t ncell, uint32_t const* off, uint32_t gb,
                           float const* __restrict__ gvel, uint32_t key0, Tape const& T, uint64_t CT) {
     constexpr uint32_t NT = tile_threads(CAP);
-    for (uint32_t k = threadIdx.x; k < ncell; k += NT) {
+    constexpr uint32_t NW = NT / 64;
+    // three lanes per cell, one per velocity component (the three sums are
+    // independent chains, each kept in the reference's order), and the cells
+    // split between the waves of the workgroup
+    uint32_t const w = threadIdx.x / 64, lane = threadIdx.x % 64;
+    uint32_t const per_wave = (ncell + NW - 1) / NW;
+    for (uint32_t j = lane; j < 3 * per_wave; j += 64) {
+        uint32_t const kk = j / 3, c = j - 3 * kk;
+        uint32_t const k = w * per_wave + kk;
+        if (k >= ncell) break;
         uint32_t const a = off[k] - gb, mc = off[k + 1] - gb - a;
         uint32_t const reals = mc < P.cap ? mc : P.cap;
         uint32_t const added = s.gcnt[k];
         uint32_t const total = reals + added;
-        s.go[k] = total > 1;
+        float mean = 0.0f;
         if (total > 1) {
-            float sx = 0, sy = 0, sz = 0;
+            float const* const v = s.p[3 + c] + a;
+            float sum = 0.0f;
             uint32_t i = 0;
-            // loads issued four particles at a time, sums kept in slot order
-            for (; i + 4 <= reals; i += 4) {
-                float x[4], y[4], z[4];
+            // loads issued eight particles at a time, the sum kept in slot order
+            for (; i + 8 <= reals; i += 8) {
+                float x[8];
 #pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    x[u] = s.p[3][a + i + u];
-                    y[u] = s.p[4][a + i + u];
-                    z[u] = s.p[5][a + i + u];
-                }
+                for (int u = 0; u < 8; ++u) x[u] = v[i + u];
 #pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    sx = sx + x[u];
-                    sy = sy + y[u];
-                    sz = sz + z[u];
-                }
+                for (int u = 0; u < 8; ++u) sum = sum + x[u];
             }
-            for (; i < reals; ++i) {
-                sx = sx + s.p[3][a + i];
-                sy = sy + s.p[4][a + i];
-                sz = sz + s.p[5][a + i];
-            }
-            if (added) {
-                uint32_t const go = s.goff[k];
-                float const* const g = go != ~0u ? &s.g[0][go] : gvel + P.gvb + 3ull * P.n * s.cref[k];
-                for (uint32_t i = 0; i < added; ++i) {
-                    sx = sx + g[3 * i];
-                    sy = sy + g[3 * i + 1];
-                    sz = sz + g[3 * i + 2];
-                }
+            for (; i < reals; ++i) sum = sum + v[i];
+            // ghosts after the particles, in ghost order
+            uint32_t const go = s.goff[k];
+            if (go != ~0u) {
+                float const* const g = &s.g[0][go + c];
+                for (uint32_t q = 0; q < added; ++q) sum = sum + g[3 * q];
+            } else {
+                float const* const g = gvel + P.gvb + 3ull * P.n * s.cref[k] + c;
+                for (uint32_t q = 0; q < added; ++q) sum = sum + g[3 * q];
             }
             float const inv = 1.0f / static_cast<float>(total);
-            s.cm[0][k] = sx * inv;
-            s.cm[1][k] = sy * inv;
-            s.cm[2][k] = sz * inv;
+            mean = sum * inv;
+            s.cm[c][k] = mean;
         }
+        if (c == 0) s.go[k] = total > 1;
         if constexpr (ADJ) {
             uint64_t const gk = P.cb + key0 + k;
             if (T.tot) {
-                T.tot[gk] = total;
-                T.u[gk] = total > 1 ? s.cm[0][k] : 0.0f;
-                T.u[CT + gk] = total > 1 ? s.cm[1][k] : 0.0f;
-                T.u[2 * CT + gk] = total > 1 ? s.cm[2][k] : 0.0f;
+                if (c == 0) T.tot[gk] = total;
+                T.u[c * CT + gk] = mean;
             }
-            float f = 0.0f;
-            if (T.force) {
-                uint32_t ix, iy, iz;
-                decode_key(P, key0 + k, ix, iy, iz);
-                f = T.force[P.cb + ref_of(P, ix, iy, iz)] * P.dt;
+            if (c == 0) {
+                float f = 0.0f;
+                if (T.force) {
+                    uint32_t ix, iy, iz;
+                    decode_key(P, key0 + k, ix, iy, iz);
+                    f = T.force[P.cb + ref_of(P, ix, iy, iz)] * P.dt;
+                }
+                s.cforce[k] = f;
             }
-            s.cforce[k] = f;
         }
     }
 }
