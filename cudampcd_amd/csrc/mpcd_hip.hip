@@ -53,6 +53,8 @@ constexpr double FIX_SCALE = 1099511627776.0;   // 2^40, fixed-point velocity su
 struct KParams {
     float L[3];          // volume edges (wall layers included)
     float half[3];       // L / 2
+    float L15[3], L2[3], L25[3];  // 1.5 L, 2 L, 2.5 L (exact: L is a whole number up to 2^21)
+    uint32_t L3bits[3];  // bit pattern of 3 L
     uint32_t dim[3];     // X, Y, Z
     uint32_t ord[3];     // axis at internal position 0 (fastest), 1, 2
     uint32_t D0, D1, D2; // extents of the internal axes
@@ -139,18 +141,23 @@ constexpr uint32_t BOUNCED = 0x80000000u;  // flag in a recorded permutation ent
 // normal trajectory): one out-of-line copy instead of one per call site
 __device__ __noinline__ float wrap_slow(float x, float L) { return fmodf(x, L); }
 
-// periodic wrap fmodf(r + 1.5 L, L) - L/2 (simulation_functions.cpp:46-51).
-// fmod is exact; for x = r + 1.5 L in [0, 3L) it equals x, x - L or x - 2L,
-// and those subtractions are exact (Sterbenz), so the selects return the same
-// bits as fmodf; anything else takes fmodf itself.
-__device__ __forceinline__ float wrap1(float r, float L) {
-    float const x = r + 1.5f * L;
-    float const L2 = 2.0f * L;
-    float m = x - L;
-    m = x < L ? x : m;
-    m = x >= L2 ? x - L2 : m;
-    if (__builtin_expect(!(x >= 0.0f && x - L2 < L), 0)) m = wrap_slow(x, L);
-    return m - L * 0.5f;
+// periodic wrap fmodf(r + 1.5 L, L) - L/2 (simulation_functions.cpp:46-51) on
+// axis a.  For x = r + 1.5 L in [0, 3L), fmodf(x, L) = x - k L with k = 0, 1
+// or 2, and that subtraction is exact (Sterbenz); k L + L/2 is exact as well,
+// so one subtraction x - (k L + L/2) rounds exactly as fmodf(x, L) - L/2.
+// The three candidates are formed and the right one selected.  x outside
+// [0, 3L) (never on a normal trajectory) takes fmodf itself; as unsigned
+// integers the bits of a non-negative float order like its value, and those
+// of negative numbers and NaN lie above those of 3L.
+__device__ __forceinline__ float wrap1(float r, KParams const& P, int a) {
+    float const x = r + P.L15[a];
+    float y0 = x - P.half[a], y1 = x - P.L15[a], y2 = x - P.L25[a];
+    // keeps the three subtractions (the compiler would otherwise select the
+    // constant first and need register copies of the scalar operands)
+    asm("" : "+v"(y0), "+v"(y1), "+v"(y2));
+    float out = x >= P.L2[a] ? y2 : (x >= P.L[a] ? y1 : y0);
+    if (__builtin_expect(__float_as_uint(x) >= P.L3bits[a], 0)) out = wrap_slow(x, P.L[a]) - P.half[a];
+    return out;
 }
 
 // u % D with the common cases as a compare and a subtraction
@@ -206,9 +213,9 @@ __device__ __forceinline__ bool stream_shift(KParams const& P, float const* sh, 
         py = py + vy * dt;
         pz = pz + vz * dt;
     }
-    px = wrap1(px + sh[0], P.L[0]);
-    py = wrap1(py + sh[1], P.L[1]);
-    pz = wrap1(pz + sh[2], P.L[2]);
+    px = wrap1(px + sh[0], P, 0);
+    py = wrap1(py + sh[1], P, 1);
+    pz = wrap1(pz + sh[2], P, 2);
     return bounced;
 }
 
@@ -751,29 +758,39 @@ __global__ void k_advance(Sys sys) {
 }
 
 // LDS tile of the collide kernel: up to CAP particles of whole cells, up to
-// TCM cells.  Sizes are compile-time so every LDS access is an immediate offset.
-constexpr int TCM = 64;
+// tile_cell_max(CAP, ALGO) cells.  Sizes are compile-time so every LDS access
+// is an immediate offset.
+constexpr int TCM = 64;   // largest cells per tile of any tile
+// SRD tiles of up to 448 particles keep 32 cells, so that ten two-wave
+// workgroups (five waves per SIMD) fit the 160 KB of LDS of a CU
+constexpr int tile_cell_max(int cap, int algo) { return algo == 0 && cap <= 448 ? 32 : TCM; }
 
 // threads of the collide workgroup for an LDS capacity: more, smaller
 // workgroups keep more independent tiles in flight per CU
 constexpr int tile_threads(int cap) { return cap <= 256 ? 64 : (cap <= 640 ? 128 : 256); }
 
+// workgroups per CU the collide kernel is compiled for (its register budget)
+constexpr int tile_blocks(int cap, int algo) {
+    return algo == 0 ? (cap <= 448 ? 1280 : 1024) / tile_threads(cap) : 768 / tile_threads(cap);
+}
+
 template <int CAP, int ALGO>
 struct Tile {
+    static constexpr int TC = tile_cell_max(CAP, ALGO);
     float p[6][CAP];                 // x y z vx vy vz, cell order
     uint32_t perm[CAP];              // source slot, arrival order; later scratch
-    float cm[6][TCM];                // per cell: mean velocity xyz, axis xyz
-    uint32_t off[TCM + 1];           // cell offsets of the tile
-    uint32_t go[TCM];                // collision happened
-    uint32_t cref[TCM];              // reference cell index
-    float cforce[TCM];               // adjoint runs: force term f dt of each cell
-    uint32_t gcnt[TCM];              // ghost count
-    uint32_t goff[TCM];              // first ghost slot
-    uint32_t wcell[TCM];             // SRD: cell of each staged wall slot
+    float cm[6][TC];                 // per cell: mean velocity xyz, axis xyz
+    uint32_t off[TC + 1];            // cell offsets of the tile
+    uint32_t go[TC];                 // collision happened
+    uint32_t cref[TC];               // reference cell index
+    float cforce[TC];                // adjoint runs: force term f dt of each cell
+    uint32_t gcnt[TC];               // ghost count
+    uint32_t goff[TC];               // first ghost slot
+    uint32_t wcell[TC];              // SRD: cell of each staged wall slot
     uint32_t sw[4];                  // scan scratch
-    uint16_t cell[CAP];              // tile-local cell of each slot
-    float g[ALGO == 1 ? 6 : 1][ALGO == 1 ? CAP / 2 : CAP / 2];  // extended: ghosts, slot k*n + i;
-                                                               // SRD: staged ghost velocities
+    uint8_t cell[CAP];               // tile-local cell of each slot
+    float g[ALGO == 1 ? 6 : 1][CAP / 2];  // extended: ghosts, slot k*n + i;
+                                          // SRD: staged ghost velocities
     uint32_t nwall;                  // SRD: wall cells of the batch staged so far
 };
 
@@ -1041,16 +1058,16 @@ __device__ void extended_cell(KParams const& P, Tile<CAP, 1>& s, uint32_t k, uin
         uy = uy - (rz * Lx - rx * Lz);
         uz = uz - (rx * Ly - ry * Lx);
         s.p[3][a + i] = ux; s.p[4][a + i] = uy; s.p[5][a + i] = uz;
-        s.p[0][a + i] = wrap1((cx + rx) - P.shift[0], P.L[0]);
-        s.p[1][a + i] = wrap1((cy + ry) - P.shift[1], P.L[1]);
-        s.p[2][a + i] = wrap1((cz + rz) - P.shift[2], P.L[2]);
+        s.p[0][a + i] = wrap1((cx + rx) - P.shift[0], P, 0);
+        s.p[1][a + i] = wrap1((cy + ry) - P.shift[1], P, 1);
+        s.p[2][a + i] = wrap1((cz + rz) - P.shift[2], P, 2);
     }
 }
 
-constexpr int GQ = 4;               // particles gathered per thread per round
+constexpr int GQ = 2;               // particles gathered per thread per round
 
 template <int ALGO, bool SAMPLE, int CAP, bool ADJ = false>
-__global__ __launch_bounds__(tile_threads(CAP), (ALGO == 0 ? 1024 : 768) / tile_threads(CAP)) void k_collide(Sys sys, const float* __restrict__ in, float* __restrict__ out,
+__global__ __launch_bounds__(tile_threads(CAP), tile_blocks(CAP, ALGO)) void k_collide(Sys sys, const float* __restrict__ in, float* __restrict__ out,
                                                 const uint32_t* __restrict__ perm, const uint32_t* __restrict__ offset,
                                                 uint32_t* __restrict__ count_next, uint32_t* __restrict__ keys,
                                                 uint32_t* __restrict__ perm_out, SampleBufs S,
@@ -1105,9 +1122,9 @@ __global__ __launch_bounds__(tile_threads(CAP), (ALGO == 0 ? 1024 : 768) / tile_
                     load_rec(in, src, v);
                     float px = v[0], py = v[1], pz = v[2], vx = v[3], vy = v[4], vz = v[5];
                     if (!in_streamed) stream_shift(P, P.shift, px, py, pz, vx, vy, vz);
-                    px = wrap1(px - P.shift[0], P.L[0]);
-                    py = wrap1(py - P.shift[1], P.L[1]);
-                    pz = wrap1(pz - P.shift[2], P.L[2]);
+                    px = wrap1(px - P.shift[0], P, 0);
+                    py = wrap1(py - P.shift[1], P, 1);
+                    pz = wrap1(pz - P.shift[2], P, 2);
                     if (!out_streamed) store_rec(out, o, px, py, pz, vx, vy, vz);
                     if (perm_out) perm_out[o] = src;
                     stream_shift(P, P.shift_next, px, py, pz, vx, vy, vz);
@@ -1160,7 +1177,7 @@ __global__ __launch_bounds__(tile_threads(CAP), (ALGO == 0 ? 1024 : 768) / tile_
         }
         for (uint32_t k = tid; k < ncell; k += NT) {
             uint32_t const a = s.off[kb + k] - gb, e = s.off[kb + k + 1] - gb;
-            for (uint32_t i = a; i < e; ++i) s.cell[i] = static_cast<uint16_t>(k);
+            for (uint32_t i = a; i < e; ++i) s.cell[i] = static_cast<uint8_t>(k);
         }
         __syncthreads();
         // SRD: the staged wall cells' ghost velocities are loaded into registers
@@ -1301,9 +1318,9 @@ __global__ __launch_bounds__(tile_threads(CAP), (ALGO == 0 ? 1024 : 768) / tile_
                 float vx = s.p[3][i], vy = s.p[4][i], vz = s.p[5][i];
                 if (slot >= P.cap) {
                     // table overflow: un-shifted, no collision (simulation_functions.cpp:90-93)
-                    px = wrap1(px - P.shift[0], P.L[0]);
-                    py = wrap1(py - P.shift[1], P.L[1]);
-                    pz = wrap1(pz - P.shift[2], P.L[2]);
+                    px = wrap1(px - P.shift[0], P, 0);
+                    py = wrap1(py - P.shift[1], P, 1);
+                    pz = wrap1(pz - P.shift[2], P, 2);
                 } else if (ALGO == 0 && s.go[k]) {
                     // SRD rotation about the cell axis (simulation_functions.cpp:176-188)
                     float const mx = s.cm[0][k], my = s.cm[1][k], mz = s.cm[2][k];
@@ -1316,9 +1333,9 @@ __global__ __launch_bounds__(tile_threads(CAP), (ALGO == 0 ? 1024 : 768) / tile_
                     vx = ((wx + qx * P.cos_a) + cxp * P.sin_a) + mx;
                     vy = ((wy + qy * P.cos_a) + cyp * P.sin_a) + my;
                     vz = ((wz + qz * P.cos_a) + czp * P.sin_a) + mz;
-                    px = wrap1(px - P.shift[0], P.L[0]);
-                    py = wrap1(py - P.shift[1], P.L[1]);
-                    pz = wrap1(pz - P.shift[2], P.L[2]);
+                    px = wrap1(px - P.shift[0], P, 0);
+                    py = wrap1(py - P.shift[1], P, 1);
+                    pz = wrap1(pz - P.shift[2], P, 2);
                     if constexpr (ADJ) {
                         vx += P.drag_dt + s.cforce[k];   // f = 0 adds exactly drag_dt
                     } else {
@@ -1690,6 +1707,11 @@ void fill_geometry(mpcd_params_t const& p, KParams& k) {
     for (int a = 0; a < 3; ++a) {
         k.L[a] = p.volume_size[a];
         k.half[a] = p.volume_size[a] * 0.5f;
+        k.L15[a] = p.volume_size[a] * 1.5f;
+        k.L2[a] = p.volume_size[a] * 2.0f;
+        k.L25[a] = p.volume_size[a] * 2.5f;
+        float const l3 = p.volume_size[a] * 3.0f;
+        std::memcpy(&k.L3bits[a], &l3, 4);
         k.dim[a] = static_cast<uint32_t>(p.volume_size[a]);
     }
     k.C = k.dim[0] * k.dim[1] * k.dim[2];
@@ -1750,7 +1772,8 @@ void upload_axes(int device) {
 }
 
 // LDS tile capacities compiled into k_collide (particles per tile)
-constexpr uint32_t TILE_CAPS[6] = {256, 384, 512, 640, 1152, 2048};
+constexpr uint32_t TILE_CAPS[7] = {256, 384, 448, 512, 640, 1152, 2048};
+constexpr uint32_t TILE_CAP_MAX = TILE_CAPS[6];
 
 // Cells per tile of every system for the LDS capacity `cap` (0: keep) and the
 // request `tc` (0: the largest that fits).  SRD: on average about 80% of the
@@ -1759,23 +1782,29 @@ constexpr uint32_t TILE_CAPS[6] = {256, 384, 512, 640, 1152, 2048};
 void choose_tiling(mpcd_hip_s* h, uint32_t tc, uint32_t cap) {
     if (cap == 0 && h->lds_cap) cap = h->lds_cap;
     if (cap == 0) {
-        // 512-particle tiles on two-wave workgroups, larger where the density
-        // makes single cells of several hundred particles likely
+        // SRD: 384-particle batches on two-wave workgroups (ten per CU), the
+        // extended rule 512; larger where the density makes single cells of
+        // several hundred particles likely
         uint32_t nmax = 1;
         for (KParams const& k : h->kp) nmax = std::max(nmax, k.n);
-        cap = std::max(512u, 6 * nmax);
+        cap = std::max(h->algorithm == 0 ? 384u : 512u, 6 * nmax);
     }
-    uint32_t chosen = TILE_CAPS[5];
+    uint32_t chosen = TILE_CAP_MAX;
     for (uint32_t c : TILE_CAPS)
         if (c >= cap && (h->algorithm == 0 || c >= 512)) { chosen = c; break; }
     h->lds_cap = chosen;
     h->tile_req = tc;
     for (KParams& k : h->kp) {
         uint32_t const n = std::max<uint32_t>(1, k.n);
+        // cells per tile: about one batch of particles (SRD; a tile whose
+        // cells hold more is processed in two batches), half of one for the
+        // extended rule, whose ghosts share the LDS tile
         uint32_t const tc_max = h->algorithm == 1 ? std::max<uint32_t>(1, (chosen / 2) / n)
-                                                  : std::max<uint32_t>(1, (chosen * 4 / 5) / n);
-        uint32_t t = tc ? std::min(tc, tc_max) : tc_max;
-        k.tile_cells = std::min<uint32_t>(t, static_cast<uint32_t>(TCM));
+                                                  : std::max<uint32_t>(1, (chosen + n / 2) / n);
+        // an explicit request may exceed the heuristic (a tile whose cells
+        // hold more than the capacity is processed in several batches)
+        uint32_t const t = tc ? tc : tc_max;
+        k.tile_cells = std::min<uint32_t>(t, static_cast<uint32_t>(tile_cell_max(static_cast<int>(chosen), h->algorithm)));
     }
 }
 
@@ -1925,6 +1954,10 @@ void launch_collide_cap(mpcd_hip_s* h, uint32_t* rec, uint32_t io) {
             break;
         case 384:
             if constexpr (ALGO == 0) launch_collide_t<ALGO, SAMPLE, 384>(h, rec, Tape{}, io);
+            else throw HipError{"the extended rule needs an LDS capacity of at least 512"};
+            break;
+        case 448:
+            if constexpr (ALGO == 0) launch_collide_t<ALGO, SAMPLE, 448>(h, rec, Tape{}, io);
             else throw HipError{"the extended rule needs an LDS capacity of at least 512"};
             break;
         case 512: launch_collide_t<ALGO, SAMPLE, 512>(h, rec, Tape{}, io); break;
@@ -2210,6 +2243,7 @@ void adj_step(mpcd_hip_s* h, float const* force, int j) {
     switch (h->lds_cap) {
         case 256: launch_collide_t<0, false, 256, true>(h, rec, T); break;
         case 384: launch_collide_t<0, false, 384, true>(h, rec, T); break;
+        case 448: launch_collide_t<0, false, 448, true>(h, rec, T); break;
         case 512: launch_collide_t<0, false, 512, true>(h, rec, T); break;
         case 640: launch_collide_t<0, false, 640, true>(h, rec, T); break;
         case 1152: launch_collide_t<0, false, 1152, true>(h, rec, T); break;
@@ -2387,9 +2421,11 @@ std::string validate(mpcd_params_t const& p) {
     if (p.experiment == 1 && !(p.volume_size[0] > 2.0f))
         return "the channel needs more than 2 cells along x (its radius is (X - 2) / 2 at initialisation)";
     if (p.algorithm != 0 && p.algorithm != 1) return "Unsupported MPCD algorithm";
-    for (int a = 0; a < 3; ++a)
+    for (int a = 0; a < 3; ++a) {
         if (!(p.volume_size[a] >= 1.0f) || p.volume_size[a] != floorf(p.volume_size[a]))
             return "volume_size must hold positive whole numbers of cells";
+        if (p.volume_size[a] > 2097152.0f) return "volume_size must stay at or below 2^21 cells per axis";
+    }
     if (p.periodicity[0] == 0 || p.periodicity[1] == 0)
         return "the hip backend supports walls in z only (periodicity (1, 1, 0) or (1, 1, 1))";
     if (p.periodicity[2] == 0 && p.volume_size[2] < 3.0f) return "walls in z need at least 3 cell layers";
@@ -2666,8 +2702,8 @@ int mpcd_hip_last_timing(const mpcd_hip_t* h, double* collide_ms, double* total_
 
 int mpcd_hip_set_tiling(mpcd_hip_t* h, uint32_t cells_per_tile, uint32_t lds_capacity) {
     return guarded(h, [&] {
-        if (lds_capacity > TILE_CAPS[5])
-            throw HipError{"tile capacity above " + std::to_string(TILE_CAPS[5]) + " particles"};
+        if (lds_capacity > TILE_CAP_MAX)
+            throw HipError{"tile capacity above " + std::to_string(TILE_CAP_MAX) + " particles"};
         choose_tiling(h, cells_per_tile, lds_capacity);
         layout(h);
     });
