@@ -804,16 +804,11 @@ template <int CAP, bool ADJ>
 __device__ void srd_cells(KParams const& P, Tile<CAP, 0>& s, uint32_t ncell, uint32_t const* off, uint32_t gb,
                           float const* __restrict__ gvel, uint32_t key0, Tape const& T, uint64_t CT) {
     constexpr uint32_t NT = tile_threads(CAP);
-    constexpr uint32_t NW = NT / 64;
     // three lanes per cell, one per velocity component (the three sums are
-    // independent chains, each kept in the reference's order), and the cells
-    // split between the waves of the workgroup
-    uint32_t const w = threadIdx.x / 64, lane = threadIdx.x % 64;
-    uint32_t const per_wave = (ncell + NW - 1) / NW;
-    for (uint32_t j = lane; j < 3 * per_wave; j += 64) {
-        uint32_t const kk = j / 3, c = j - 3 * kk;
-        uint32_t const k = w * per_wave + kk;
-        if (k >= ncell) break;
+    // independent chains, each kept in the reference's order), packed into as
+    // few waves as possible
+    for (uint32_t j = threadIdx.x; j < 3 * ncell; j += NT) {
+        uint32_t const k = j / 3, c = j - 3 * k;
         uint32_t const a = off[k] - gb, mc = off[k + 1] - gb - a;
         uint32_t const reals = mc < P.cap ? mc : P.cap;
         uint32_t const added = s.gcnt[k];
@@ -1175,9 +1170,11 @@ __global__ __launch_bounds__(tile_threads(CAP), tile_blocks(CAP, ALGO)) void k_c
                 s.goff[k] = go;
             }
         }
-        for (uint32_t k = tid; k < ncell; k += NT) {
+        // tile-local cell of every slot, four lanes per cell
+        for (uint32_t t = tid; t < 4 * ncell; t += NT) {
+            uint32_t const k = t / 4;
             uint32_t const a = s.off[kb + k] - gb, e = s.off[kb + k + 1] - gb;
-            for (uint32_t i = a; i < e; ++i) s.cell[i] = static_cast<uint8_t>(k);
+            for (uint32_t i = a + (t & 3u); i < e; i += 4) s.cell[i] = static_cast<uint8_t>(k);
         }
         __syncthreads();
         // SRD: the staged wall cells' ghost velocities are loaded into registers

@@ -197,7 +197,10 @@ def test_sampling_matches_oracle(pympcd, algorithm):
 def test_determinism_and_tiling_independence(pympcd):
     cfg = ((24, 20, 14), (1, 1, 0), 10, "srd")
     states = []
-    for tiling in [(0, 0), (0, 0), (7, 512), (64, 1024), (1, 256), (10, 256), (64, 2048)]:
+    # (32, 256): every tile spans two or more batches; 384/448: the 32-cell
+    # tiles of five-wave workgroups
+    for tiling in [(0, 0), (0, 0), (7, 512), (64, 1024), (1, 256), (10, 256), (64, 2048), (32, 256),
+                   (20, 448), (32, 448), (30, 384)]:
         p, sim, h = _sim(pympcd, cfg, seed=99)
         if tiling != (0, 0):
             h.set_tiling(*tiling)
