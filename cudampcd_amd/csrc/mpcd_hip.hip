@@ -762,8 +762,9 @@ __global__ void k_advance(Sys sys) {
 // is an immediate offset.
 constexpr int TCM = 64;   // largest cells per tile of any tile
 // SRD tiles of up to 448 particles keep 32 cells, so that ten two-wave
-// workgroups (five waves per SIMD) fit the 160 KB of LDS of a CU
-constexpr int tile_cell_max(int cap, int algo) { return algo == 0 && cap <= 448 ? 32 : TCM; }
+// workgroups (five waves per SIMD) fit the 160 KB of LDS of a CU; extended
+// tiles of 512 particles too (seven workgroups per CU instead of six)
+constexpr int tile_cell_max(int cap, int algo) { return cap <= (algo == 0 ? 448 : 512) ? 32 : TCM; }
 
 // threads of the collide workgroup for an LDS capacity: more, smaller
 // workgroups keep more independent tiles in flight per CU
@@ -771,7 +772,7 @@ constexpr int tile_threads(int cap) { return cap <= 256 ? 64 : (cap <= 640 ? 128
 
 // workgroups per CU the collide kernel is compiled for (its register budget)
 constexpr int tile_blocks(int cap, int algo) {
-    return algo == 0 ? (cap <= 448 ? 1280 : 1024) / tile_threads(cap) : 768 / tile_threads(cap);
+    return algo == 0 ? (cap <= 448 ? 1280 : 1024) / tile_threads(cap) : (cap <= 512 ? 1024 : 768) / tile_threads(cap);
 }
 
 template <int CAP, int ALGO>
