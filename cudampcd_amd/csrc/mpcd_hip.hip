@@ -784,13 +784,14 @@ struct Tile {
     uint32_t off[TC + 1];            // cell offsets of the tile
     uint32_t go[TC];                 // collision happened
     uint32_t cref[TC];               // reference cell index
-    float cforce[TC];                // adjoint runs: force term f dt of each cell
+    float cforce[ALGO == 0 ? TC : 1];  // SRD adjoint runs: force term f dt of each cell
     uint32_t gcnt[TC];               // ghost count
     uint32_t goff[TC];               // first ghost slot
-    uint32_t wcell[TC];              // SRD: cell of each staged wall slot
+    uint32_t wcell[TC];              // SRD: cell of each staged wall slot; extended: first ghost slot
+    uint32_t wpre[ALGO == 1 ? TC + 1 : 1];  // extended: wall cells of the tile before each cell
     uint32_t sw[4];                  // scan scratch
     uint8_t cell[CAP];               // tile-local cell of each slot
-    float g[ALGO == 1 ? 6 : 1][CAP / 2];  // extended: ghosts, slot k*n + i;
+    float g[ALGO == 1 ? 6 : 1][CAP / 2];  // extended: ghosts, n slots per wall cell of the batch;
                                           // SRD: staged ghost velocities
     uint32_t nwall;                  // SRD: wall cells of the batch staged so far
 };
@@ -872,6 +873,16 @@ __device__ void srd_cells(KParams const& P, Tile<CAP, 0>& s, uint32_t ncell, uin
 //   extended_cell    one lane per cell: the collision itself
 constexpr uint32_t UPPER_LAYER = 0x80000000u;
 
+// whether cell layer iz receives ghosts this step: the z layers next to the
+// walls for the current shift (simulation_functions.cpp:137-146); `layer`
+// tells the upper one
+__device__ __forceinline__ bool ghost_layer(KParams const& P, uint32_t iz, bool& layer) {
+    bool const sign = P.shift[2] > 0;
+    float const zf = static_cast<float>(iz);
+    layer = (zf == (P.L[2] - (sign ? 1.0f : 2.0f)));
+    return (!P.perio_z) && ((zf == static_cast<float>(sign)) || layer);
+}
+
 template <int CAP>
 __device__ void ext_cell_setup(KParams const& P, Tile<CAP, 1>& s, uint32_t k, uint32_t mc, uint32_t key) {
     uint32_t ix, iy, iz;
@@ -879,9 +890,8 @@ __device__ void ext_cell_setup(KParams const& P, Tile<CAP, 1>& s, uint32_t k, ui
     uint32_t const ref = ref_of(P, ix, iy, iz);
     float const sh = fabsf(P.shift[2]);
     bool const sign = P.shift[2] > 0;
-    float const zf = static_cast<float>(iz);
-    bool const layer = (zf == (P.L[2] - (sign ? 1.0f : 2.0f)));
-    bool const ghosts = (!P.perio_z) && ((zf == static_cast<float>(sign)) || layer);
+    bool layer;
+    bool const ghosts = ghost_layer(P, iz, layer);
     uint32_t const reals_in = mc < P.cap ? mc : P.cap;
     uint32_t const added = ghosts ? ghost_count(P, ref, sh, layer, sign) : 0u;
     uint32_t total = reals_in + added;
@@ -904,7 +914,7 @@ __device__ void ext_ghost(KParams const& P, Tile<CAP, 1>& s, uint32_t k, uint32_
     float const ox = (0.5f + static_cast<float>(ix)) - P.half[0];
     float const oy = (0.5f + static_cast<float>(iy)) - P.half[1];
     float const oz = (0.5f + static_cast<float>(iz)) - P.half[2];
-    uint32_t const o = k * P.n + g;
+    uint32_t const o = s.wcell[k] + g;   // the cell's first ghost slot
     Words const r = P.rs.at(P_GHOST_POS, g, ref, P.step);
     float const uz = uniform(r.w[2]);
     float const z = (layer ^ sign) ? sh * uz : sh + (1.0f - sh) * uz;
@@ -941,7 +951,7 @@ __device__ void extended_cell(KParams const& P, Tile<CAP, 1>& s, uint32_t k, uin
     float const ox = (0.5f + static_cast<float>(ix)) - P.half[0];
     float const oy = (0.5f + static_cast<float>(iy)) - P.half[1];
     float const oz = (0.5f + static_cast<float>(iz)) - P.half[2];
-    uint32_t const gb = k * P.n;
+    uint32_t const gb = s.wcell[k];
     if (total <= 1) return;
     // particle i < rr lives at LDS slot a+i, ghost i >= rr at ghost slot gb+i-rr
     auto X = [&](int c, uint32_t i) -> float& { return i < rr ? s.p[c][a + i] : s.g[c][gb + (i - rr)]; };
@@ -1090,13 +1100,30 @@ __global__ __launch_bounds__(tile_threads(CAP), tile_blocks(CAP, ALGO)) void k_c
     uint32_t const nc = min(TC, P.C - c0);
     uint32_t const g0 = P.cb + c0;
     for (uint32_t k = tid; k <= nc; k += NT) s.off[k] = offset[g0 + k];
+    if constexpr (ALGO == 1) {
+        // wall cells of the tile (those that receive ghosts) counted before
+        // each cell: a batch holds at most CAP/2 ghosts, n per wall cell
+        if (tid < 64) {
+            bool wall = false;
+            if (tid < nc) {
+                uint32_t ix, iy, iz;
+                bool layer;
+                decode_key(P, c0 + tid, ix, iy, iz);
+                wall = ghost_layer(P, iz, layer);
+            }
+            uint64_t const mask = __ballot(wall);
+            if (tid <= nc) s.wpre[tid] = static_cast<uint32_t>(__popcll(mask & ((1ull << tid) - 1ull)));
+            if (tid == 0 && nc == 64) s.wpre[64] = static_cast<uint32_t>(__popcll(mask));
+        }
+    }
     __syncthreads();
 
     uint32_t kb = 0;
     while (kb < nc) {
         // batch = longest run of whole cells from kb that fits the LDS tile
         // (offsets are non-decreasing, so the fitting cells form a prefix)
-        bool const fits = (kb + 1 + tid <= nc) && (s.off[kb + 1 + tid] - s.off[kb] <= cap);
+        bool fits = (kb + 1 + tid <= nc) && (s.off[kb + 1 + tid] - s.off[kb] <= cap);
+        if constexpr (ALGO == 1) fits = fits && (s.wpre[kb + 1 + tid] - s.wpre[kb]) * P.n <= static_cast<uint32_t>(CAP / 2);
         uint32_t const nfit = static_cast<uint32_t>(__syncthreads_count(fits));
         uint32_t const ke = kb + (nfit > 0 ? nfit : 1u);
         uint32_t const gb = s.off[kb];
@@ -1263,8 +1290,10 @@ __global__ __launch_bounds__(tile_threads(CAP), tile_blocks(CAP, ALGO)) void k_c
         if constexpr (ALGO == 0) {
             srd_cells<CAP, ADJ>(P, s, ncell, s.off + kb, gb, gvel, c0 + kb, T, CT);
         } else {
-            for (uint32_t k = tid; k < ncell; k += NT)
+            for (uint32_t k = tid; k < ncell; k += NT) {
                 ext_cell_setup<CAP>(P, s, k, s.off[kb + k + 1] - s.off[kb + k], c0 + kb + k);
+                s.wcell[k] = (s.wpre[kb + k] - s.wpre[kb]) * P.n;
+            }
             __syncthreads();
             // ghosts, one lane per (cell, ghost slot)
             for (uint32_t t = tid; t < ncell * P.n; t += NT) {
@@ -1778,30 +1807,27 @@ constexpr uint32_t TILE_CAP_MAX = TILE_CAPS[6];
 // tile's particle capacity; extended: the ghosts of a batch (at most n per
 // cell) must fit the cap/2 ghost region.
 void choose_tiling(mpcd_hip_s* h, uint32_t tc, uint32_t cap) {
+    uint32_t nmax = 1;
+    for (KParams const& k : h->kp) nmax = std::max(nmax, k.n);
+    // the extended rule needs 512 or more and room for the ghosts of one wall
+    // cell (n) in its half-capacity ghost region
+    if (h->algorithm == 1 && TILE_CAP_MAX / 2 < nmax) throw HipError{"the extended rule supports n up to 1024"};
     if (cap == 0 && h->lds_cap) cap = h->lds_cap;
-    if (cap == 0) {
-        // SRD: 384-particle batches on two-wave workgroups (ten per CU), the
-        // extended rule 512; larger where the density makes single cells of
-        // several hundred particles likely
-        uint32_t nmax = 1;
-        for (KParams const& k : h->kp) nmax = std::max(nmax, k.n);
-        cap = std::max(h->algorithm == 0 ? 384u : 512u, 6 * nmax);
-    }
+    // default: SRD 384-particle batches on two-wave workgroups (ten per CU),
+    // the extended rule 512; larger where the density makes single cells of
+    // several hundred particles likely
+    if (cap == 0) cap = std::max(h->algorithm == 0 ? 384u : 512u, 6 * nmax);
     uint32_t chosen = TILE_CAP_MAX;
     for (uint32_t c : TILE_CAPS)
-        if (c >= cap && (h->algorithm == 0 || c >= 512)) { chosen = c; break; }
+        if (c >= cap && (h->algorithm == 0 || (c >= 512 && c / 2 >= nmax))) { chosen = c; break; }
     h->lds_cap = chosen;
     h->tile_req = tc;
     for (KParams& k : h->kp) {
         uint32_t const n = std::max<uint32_t>(1, k.n);
-        // cells per tile: about one batch of particles (SRD; a tile whose
-        // cells hold more is processed in two batches), half of one for the
-        // extended rule, whose ghosts share the LDS tile
-        uint32_t const tc_max = h->algorithm == 1 ? std::max<uint32_t>(1, (chosen / 2) / n)
-                                                  : std::max<uint32_t>(1, (chosen + n / 2) / n);
-        // an explicit request may exceed the heuristic (a tile whose cells
-        // hold more than the capacity is processed in several batches)
-        uint32_t const t = tc ? tc : tc_max;
+        // cells per tile: about one batch of particles by default; a tile
+        // whose cells hold more (also on an explicit request) is processed in
+        // several batches
+        uint32_t const t = tc ? tc : std::max<uint32_t>(1, (chosen + n / 2) / n);
         k.tile_cells = std::min<uint32_t>(t, static_cast<uint32_t>(tile_cell_max(static_cast<int>(chosen), h->algorithm)));
     }
 }

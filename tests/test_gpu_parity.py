@@ -210,6 +210,38 @@ def test_determinism_and_tiling_independence(pympcd):
         assert np.array_equal(s[0], states[0][0]) and np.array_equal(s[1], states[0][1])
 
 
+def test_extended_tiling_independence_and_ghost_bound(pympcd):
+    """Extended rule, 5 cell layers (2 of them wall layers), n = 40: a 32-cell
+    tile holds about 13 wall cells (520 ghosts), more than the 256-slot ghost
+    region of a 512 tile, so batches are cut by the ghost bound as well as by
+    the particle capacity.  Every tiling gives the same bits, and one step
+    with the smallest tiles matches the oracle bitwise."""
+    cfg = ((12, 10, 3), (1, 1, 0), 40, "extended")
+    states = []
+    for tiling in [(0, 0), (32, 512), (8, 512), (64, 1152), (32, 2048)]:
+        p, sim, h = _sim(pympcd, cfg, seed=77)
+        if tiling != (0, 0):
+            h.set_tiling(*tiling)
+        sim.step(6)
+        states.append((sim.get_particle_positions(), sim.get_particle_velocities()))
+    for st in states[1:]:
+        assert np.array_equal(st[0], states[0][0]) and np.array_equal(st[1], states[0][1])
+    p, sim, h = _sim(pympcd, cfg, seed=77)
+    h.set_tiling(32, 512)
+    sim.step(2)
+    h.record_permutation(True)
+    pos, vel = sim.get_particle_positions(), sim.get_particle_velocities()
+    o = _oracle(p)
+    o.set_particles(pos, vel)
+    o.set_step(h.step_index)
+    sim.step(1)
+    o.step(1)
+    perm = h.last_permutation()
+    opos, ovel = o.particles()
+    assert np.array_equal(sim.get_particle_positions(), opos[perm].astype(np.float32))
+    assert np.array_equal(sim.get_particle_velocities(), ovel[perm].astype(np.float32))
+
+
 def test_set_get_roundtrip_and_errors(pympcd):
     p, sim, h = _sim(pympcd, ((5, 5, 6), (1, 1, 0), 4, "srd"))
     rng = np.random.default_rng(3)
