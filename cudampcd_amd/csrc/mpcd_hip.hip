@@ -761,10 +761,17 @@ __global__ void k_advance(Sys sys) {
 // tile_cell_max(CAP, ALGO) cells.  Sizes are compile-time so every LDS access
 // is an immediate offset.
 constexpr int TCM = 64;   // largest cells per tile of any tile
-// SRD tiles of up to 448 particles keep 32 cells, so that ten two-wave
-// workgroups (five waves per SIMD) fit the 160 KB of LDS of a CU; extended
-// tiles of 512 particles too (seven workgroups per CU instead of six)
-constexpr int tile_cell_max(int cap, int algo) { return cap <= (algo == 0 ? 448 : 512) ? 32 : TCM; }
+// Small tiles keep fewer cells so that more workgroups fit the 160 KB of LDS
+// of a CU (allocated in 512-byte granules): SRD 384-particle tiles 24 cells
+// (twelve two-wave workgroups, six waves per SIMD), SRD 448 and extended 512
+// tiles 32 cells (ten and seven workgroups)
+constexpr int tile_cell_max(int cap, int algo) {
+    return algo == 0 ? (cap <= 384 ? 24 : (cap <= 448 ? 32 : TCM)) : (cap <= 512 ? 32 : TCM);
+}
+// floats of the ghost region: extended ghosts (per coordinate), SRD staged
+// wall-cell ghost velocities (slightly less than half the capacity for the
+// 384-particle tile, to stay within 26 LDS granules)
+constexpr int ghost_floats(int cap, int algo) { return algo == 0 && cap == 384 ? cap / 2 - 32 : cap / 2; }
 
 // threads of the collide workgroup for an LDS capacity: more, smaller
 // workgroups keep more independent tiles in flight per CU
@@ -772,7 +779,8 @@ constexpr int tile_threads(int cap) { return cap <= 256 ? 64 : (cap <= 640 ? 128
 
 // workgroups per CU the collide kernel is compiled for (its register budget)
 constexpr int tile_blocks(int cap, int algo) {
-    return algo == 0 ? (cap <= 448 ? 1280 : 1024) / tile_threads(cap) : (cap <= 512 ? 1024 : 768) / tile_threads(cap);
+    return algo == 0 ? (cap <= 384 ? 1536 : (cap <= 448 ? 1280 : 1024)) / tile_threads(cap)
+                     : (cap <= 512 ? 1024 : 768) / tile_threads(cap);
 }
 
 template <int CAP, int ALGO>
@@ -791,7 +799,8 @@ struct Tile {
     uint32_t wpre[ALGO == 1 ? TC + 1 : 1];  // extended: wall cells of the tile before each cell
     uint32_t sw[4];                  // scan scratch
     uint8_t cell[CAP];               // tile-local cell of each slot
-    float g[ALGO == 1 ? 6 : 1][CAP / 2];  // extended: ghosts, n slots per wall cell of the batch;
+    static constexpr int GS = ghost_floats(CAP, ALGO);
+    float g[ALGO == 1 ? 6 : 1][GS];  // extended: ghosts, n slots per wall cell of the batch;
                                           // SRD: staged ghost velocities
     uint32_t nwall;                  // SRD: wall cells of the batch staged so far
 };
@@ -1070,7 +1079,10 @@ __device__ void extended_cell(KParams const& P, Tile<CAP, 1>& s, uint32_t k, uin
     }
 }
 
-constexpr int GQ = 2;               // particles gathered per thread per round
+// particles gathered per thread per round (one for the 384-particle tile,
+// which keeps it within 80 VGPRs)
+template <int CAP>
+constexpr int gather_quota() { return CAP <= 384 ? 1 : 2; }
 
 template <int ALGO, bool SAMPLE, int CAP, bool ADJ = false>
 __global__ __launch_bounds__(tile_threads(CAP), tile_blocks(CAP, ALGO)) void k_collide(Sys sys, const float* __restrict__ in, float* __restrict__ out,
@@ -1102,7 +1114,7 @@ __global__ __launch_bounds__(tile_threads(CAP), tile_blocks(CAP, ALGO)) void k_c
     for (uint32_t k = tid; k <= nc; k += NT) s.off[k] = offset[g0 + k];
     if constexpr (ALGO == 1) {
         // wall cells of the tile (those that receive ghosts) counted before
-        // each cell: a batch holds at most CAP/2 ghosts, n per wall cell
+        // each cell: a batch holds at most GS ghosts, n per wall cell
         if (tid < 64) {
             bool wall = false;
             if (tid < nc) {
@@ -1123,7 +1135,7 @@ __global__ __launch_bounds__(tile_threads(CAP), tile_blocks(CAP, ALGO)) void k_c
         // batch = longest run of whole cells from kb that fits the LDS tile
         // (offsets are non-decreasing, so the fitting cells form a prefix)
         bool fits = (kb + 1 + tid <= nc) && (s.off[kb + 1 + tid] - s.off[kb] <= cap);
-        if constexpr (ALGO == 1) fits = fits && (s.wpre[kb + 1 + tid] - s.wpre[kb]) * P.n <= static_cast<uint32_t>(CAP / 2);
+        if constexpr (ALGO == 1) fits = fits && (s.wpre[kb + 1 + tid] - s.wpre[kb]) * P.n <= static_cast<uint32_t>(s.GS);
         uint32_t const nfit = static_cast<uint32_t>(__syncthreads_count(fits));
         uint32_t const ke = kb + (nfit > 0 ? nfit : 1u);
         uint32_t const gb = s.off[kb];
@@ -1189,7 +1201,7 @@ __global__ __launch_bounds__(tile_threads(CAP), tile_blocks(CAP, ALGO)) void k_c
                     s.cref[k] = w;
                     gc = gcnt[P.wb + w];
                     uint32_t const slot = atomicAdd(&s.nwall, 1u);
-                    if ((slot + 1) * 3 * P.n <= static_cast<uint32_t>(CAP / 2)) {
+                    if ((slot + 1) * 3 * P.n <= static_cast<uint32_t>(s.GS)) {
                         go = slot * 3 * P.n;
                         s.wcell[slot] = k;
                     }
@@ -1207,12 +1219,12 @@ __global__ __launch_bounds__(tile_threads(CAP), tile_blocks(CAP, ALGO)) void k_c
         __syncthreads();
         // SRD: the staged wall cells' ghost velocities are loaded into registers
         // now, so their latency overlaps the gather's, and land in LDS below
-        constexpr int GPT = (CAP / 2 + NT - 1) / NT;
+        constexpr int GPT = (Tile<CAP, ALGO>::GS + NT - 1) / NT;
         float gstage[ALGO == 0 ? GPT : 1];
         uint32_t nstaged = 0;
         if constexpr (ALGO == 0) {
             uint32_t const n3 = 3 * P.n;
-            uint32_t const nslot = min(s.nwall, static_cast<uint32_t>(CAP / 2) / n3);
+            uint32_t const nslot = min(s.nwall, static_cast<uint32_t>(s.GS) / n3);
             nstaged = nslot * n3;
 #pragma unroll
             for (int q = 0; q < GPT; ++q) {
@@ -1228,6 +1240,7 @@ __global__ __launch_bounds__(tile_threads(CAP), tile_blocks(CAP, ALGO)) void k_c
         // gather this thread's particles, GQ at a time with all their loads in
         // flight together, then order each cell by source slot and re-apply
         // this step's streaming
+        constexpr int GQ = gather_quota<CAP>();
         for (uint32_t i0 = 0; i0 < m; i0 += GQ * NT) {
             float g[GQ][6];
             uint32_t srcs[GQ];
