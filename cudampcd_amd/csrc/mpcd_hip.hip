@@ -513,10 +513,12 @@ __device__ __forceinline__ void perm_block(Sys const& sys, const uint32_t* __res
                                            uint32_t* __restrict__ cursor, uint32_t* __restrict__ perm, uint64_t N,
                                            uint32_t blk) {
     constexpr uint32_t ITEMS = WG * PERM_ITEMS;
+    // 17.4 KB in all, so eight blocks (the 32 waves a CU holds) fit the LDS
+    static_assert(ITEMS <= 65536 && PERM_WIN < 65536, "prefixes, bins and entries in 16 bits");
     __shared__ uint32_t cnt[PERM_WIN];   // count per bin, then the bin's reserved device base
-    __shared__ uint32_t pre[PERM_WIN];   // block-local exclusive prefix per bin
-    __shared__ uint32_t sj[ITEMS];       // first the non-empty bin list, then the entries in bin order
-    __shared__ uint16_t sb[ITEMS];       // bin of each ordered entry
+    __shared__ uint16_t pre[PERM_WIN];   // block-local exclusive prefix per bin
+    __shared__ uint32_t sj[ITEMS];       // first the non-empty bin list, then the entries in bin
+                                         // order: bin << 16 | particle - base
     __shared__ uint32_t s_w[WG / 64];
     __shared__ uint32_t s_base[3];
     uint64_t const base = static_cast<uint64_t>(blk) * ITEMS;
@@ -576,7 +578,7 @@ __device__ __forceinline__ void perm_block(Sys const& sys, const uint32_t* __res
 #pragma unroll
     for (int k = 0; k < PB; ++k) {
         uint32_t const c = cnt[b0 + k];
-        pre[b0 + k] = p;
+        pre[b0 + k] = static_cast<uint16_t>(p);
         p += c;
         if (c) list[at++] = b0 + k;
     }
@@ -596,8 +598,7 @@ __device__ __forceinline__ void perm_block(Sys const& sys, const uint32_t* __res
         if (key[q] == ~0u) continue;
         if (bin[q] != ~0u) {
             uint32_t const o = pre[bin[q]] + local[q];
-            sj[o] = static_cast<uint32_t>(j);
-            sb[o] = static_cast<uint16_t>(bin[q]);
+            sj[o] = (bin[q] << 16) | static_cast<uint32_t>(q * WG + threadIdx.x);
         } else {
             perm[atomicAdd(cursor + key[q], 1u)] = static_cast<uint32_t>(j);
         }
@@ -605,8 +606,8 @@ __device__ __forceinline__ void perm_block(Sys const& sys, const uint32_t* __res
     __syncthreads();
     // neighbouring lanes write neighbouring slots of a cell
     for (uint32_t o = threadIdx.x; o < in_win; o += WG) {
-        uint32_t const b = sb[o];
-        perm[cnt[b] + (o - pre[b])] = sj[o];
+        uint32_t const e = sj[o], b = e >> 16;
+        perm[cnt[b] + (o - pre[b])] = static_cast<uint32_t>(base) + (e & 0xffffu);
     }
 }
 
@@ -777,7 +778,12 @@ constexpr int ghost_floats(int cap, int algo) { return algo == 0 && cap == 384 ?
 // workgroups keep more independent tiles in flight per CU
 constexpr int tile_threads(int cap) { return cap <= 256 ? 64 : (cap <= 640 ? 128 : 256); }
 
-// workgroups per CU the collide kernel is compiled for (its register budget)
+// Second launch bound of the collide kernel.  HIP reads it as waves per SIMD,
+// so these targets (12 for the 384-particle tile) are more than the LDS lets
+// run; the compiler then schedules for the smallest register count it can
+// reach.  The bound that matches the real occupancy (6, 5 or 4 waves) was
+// measured slower for the extended rule (500x500x40 cells, n = 10: 5.87 ->
+// 6.25 ms per step) and equal for SRD, so the larger target stays.
 constexpr int tile_blocks(int cap, int algo) {
     return algo == 0 ? (cap <= 384 ? 1536 : (cap <= 448 ? 1280 : 1024)) / tile_threads(cap)
                      : (cap <= 512 ? 1024 : 768) / tile_threads(cap);
@@ -1125,7 +1131,9 @@ __global__ __launch_bounds__(tile_threads(CAP), tile_blocks(CAP, ALGO)) void k_c
             }
             uint64_t const mask = __ballot(wall);
             if (tid <= nc) s.wpre[tid] = static_cast<uint32_t>(__popcll(mask & ((1ull << tid) - 1ull)));
-            if (tid == 0 && nc == 64) s.wpre[64] = static_cast<uint32_t>(__popcll(mask));
+            if constexpr (Tile<CAP, ALGO>::TC >= 64) {
+                if (tid == 0 && nc == 64) s.wpre[64] = static_cast<uint32_t>(__popcll(mask));
+            }
         }
     }
     __syncthreads();
