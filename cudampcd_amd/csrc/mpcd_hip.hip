@@ -706,8 +706,13 @@ __device__ __forceinline__ void axes_block(Sys const& sys, float* __restrict__ a
     }
 }
 
+// A block takes WG wall cells at a time: one lane per cell draws the ghost
+// count, then the cells' n ghost slots are spread over all lanes (slot g of
+// cell k on lane k n + g), so the velocity draws run on full waves and
+// neighbouring lanes store neighbouring ghosts.
 __device__ __forceinline__ void ghosts_block(Sys const& sys, uint32_t* __restrict__ gcnt, float* __restrict__ gvel,
                                              uint32_t blk) {
+    __shared__ uint32_t s_cnt[WG];
     uint32_t const s = find_sys(sys.blk_wall, sys.ns, blk);
     KParams const P = sys.P[s];
     uint32_t const b0 = sys.blk_wall[s], nb = sys.blk_wall[s + 1] - b0;
@@ -715,23 +720,36 @@ __device__ __forceinline__ void ghosts_block(Sys const& sys, uint32_t* __restric
     float const sh = fabsf(P.shift[2]);
     bool const sign = P.shift[2] > 0;
     uint32_t const iz_layer = static_cast<uint32_t>(P.L[2] - (sign ? 1.0f : 2.0f));
-    for (uint32_t w = (blk - b0) * WG + threadIdx.x; w < 2 * XY; w += nb * WG) {
+    // reference index of wall cell w: w < XY the layer at z = sign, else the upper one
+    auto ref_of_wall = [&](uint32_t w) {
         bool const layer = w >= XY;
-        uint32_t const xy = layer ? w - XY : w;
-        uint32_t const iz = layer ? iz_layer : static_cast<uint32_t>(sign);
-        uint32_t const ref = xy + XY * iz;
-        uint32_t const added = ghost_count(P, ref, sh, layer, sign);
-        gcnt[P.wb + w] = added;
-        float* const out = gvel + P.gvb + 3ull * P.n * w;
-        for (uint32_t g = 0; g < added; ++g) {
-            Words const v = P.rs.at(P_GHOST_VEL, g, ref, P.step);
-            float g0, g1, g2, g3;
-            box_muller(v.w[0], v.w[1], &g0, &g1);
-            box_muller(v.w[2], v.w[3], &g2, &g3);
-            out[3 * g] = g0 * P.tv;
-            out[3 * g + 1] = g1 * P.tv;
-            out[3 * g + 2] = g2 * P.tv;
+        return (layer ? w - XY : w) + XY * (layer ? iz_layer : static_cast<uint32_t>(sign));
+    };
+    // the chunk start is the same on every lane, so the barriers are uniform
+    for (uint32_t c0 = (blk - b0) * WG; c0 < 2 * XY; c0 += nb * WG) {
+        uint32_t const w = c0 + threadIdx.x;
+        uint32_t added = 0;
+        if (w < 2 * XY) {
+            added = ghost_count(P, ref_of_wall(w), sh, w >= XY, sign);
+            gcnt[P.wb + w] = added;
         }
+        s_cnt[threadIdx.x] = added;
+        __syncthreads();
+        uint32_t const ncell = min(static_cast<uint32_t>(WG), 2 * XY - c0);
+        for (uint32_t t = threadIdx.x; t < ncell * P.n; t += WG) {
+            uint32_t const k = t / P.n, g = t - k * P.n;
+            if (g < s_cnt[k]) {
+                Words const v = P.rs.at(P_GHOST_VEL, g, ref_of_wall(c0 + k), P.step);
+                float g0, g1, g2, g3;
+                box_muller(v.w[0], v.w[1], &g0, &g1);
+                box_muller(v.w[2], v.w[3], &g2, &g3);
+                float* const out = gvel + P.gvb + 3ull * P.n * c0 + 3ull * t;
+                out[0] = g0 * P.tv;
+                out[1] = g1 * P.tv;
+                out[2] = g2 * P.tv;
+            }
+        }
+        __syncthreads();
     }
 }
 
