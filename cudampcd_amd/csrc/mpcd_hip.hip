@@ -1782,14 +1782,14 @@ void fill_geometry(mpcd_params_t const& p, KParams& k) {
     }
     k.C = k.dim[0] * k.dim[1] * k.dim[2];
     // internal cell order: smallest extent fastest, the slowest axis in
-    // blocks of up to 1024 rows (one block when it is shorter).  Neighbours
+    // blocks of up to 128 rows (one block when it is shorter).  Neighbours
     // along the middle axis then lie one block height of keys apart (the
     // permutation and the next-step histogram give them windows of their
     // own), and only moves across a block edge land far away, where each
-    // takes a device atomic of its own: 1024-row blocks measured 9.4 -> 8.6 ms
-    // per step against 16-row ones at 1000x1000x22 cells, n = 20.  The cap
-    // bounds the neighbour distance of very large grids and keeps the block
-    // size within 32 bits.
+    // takes a device atomic of its own.  At 1000x1000x22 cells, n = 20:
+    // 16-row blocks 9.4 ms per step, 128 rows 8.55 ms; 256 and 1024 rows were
+    // as fast, but put the middle-axis neighbours beyond what the L2 keeps
+    // (k_collide fetched 18.6 GB per launch instead of 13.9).
     uint32_t ord[3] = {0, 1, 2};
     std::stable_sort(ord, ord + 3, [&](uint32_t x, uint32_t y) { return k.dim[x] < k.dim[y]; });
     for (int a = 0; a < 3; ++a) k.ord[a] = ord[a];
@@ -1797,11 +1797,11 @@ void fill_geometry(mpcd_params_t const& p, KParams& k) {
     k.D1 = k.dim[ord[1]];
     k.D2 = k.dim[ord[2]];
     k.Bsh = 0;
-    while ((1u << k.Bsh) < k.D2 && k.Bsh < 10) ++k.Bsh;
+    while ((1u << k.Bsh) < k.D2 && k.Bsh < 7) ++k.Bsh;
     k.nblk = (k.D2 + (1u << k.Bsh) - 1) >> k.Bsh;
     k.Blast = k.D2 - ((k.nblk - 1) << k.Bsh);
     // one block: every key is below C, so key / C = 0 selects it (2^Bsh D1 D0
-    // could pass 2^32 there); several blocks: D2 > 1024, so 1024 D1 D0 < C
+    // could pass 2^32 there); several blocks: D2 > 128, so 128 D1 D0 < C
     k.BLK = k.nblk == 1 ? k.C : (1u << k.Bsh) * k.D1 * k.D0;
     udiv_magic(k.BLK, k.mBLK);
     udiv_magic(k.D0, k.mD0);
