@@ -35,6 +35,12 @@ CONFIGS = [
     ((5, 4, 7), (1, 1, 0), 3, "extended"),
     ((4, 4, 6), (1, 1, 0), 1, "extended"),
     ((20, 16, 12), (1, 1, 0), 10, "extended"),
+    # longest axis above 128 cells: the internal key cuts it into 128-row
+    # blocks (the last one shorter), so particles cross block edges, whose
+    # keys fall outside the permutation's and histogram's windows
+    ((6, 4, 300), (1, 1, 0), 5, "srd"),
+    ((300, 5, 6), (1, 1, 1), 4, "srd"),
+    ((5, 4, 260), (1, 1, 0), 5, "extended"),
 ]
 IDS = [f"{a}-{'x'.join(map(str, f))}-p{''.join(map(str, p))}-n{n}" for f, p, n, a in CONFIGS]
 
