@@ -6,14 +6,17 @@
 //   scan      offsets[c] = exclusive prefix of the per-cell counts of this
 //             step (counts were accumulated by the previous collide launch);
 //             counts are cleared for the next step in the same pass.
-//   perm      perm[offsets[key_j] + rank_j] = j: cell-ordered permutation of
-//             the particle array (rank from wave-aggregated atomics).
+//   perm      cell-ordered permutation of the particle array: a block of
+//             particles counts its keys in LDS windows and reserves one range
+//             per distinct key with a single device atomic; perm[slot] = j.
+//   draws     SRD only: the rotation axis of every cell and the ghost counts
+//             and velocities of the wall cells, from the counter-based stream.
 //   collide   one workgroup per tile of consecutive cells: gathers its cells'
 //             particles through perm into LDS, orders each cell by source
 //             index (stable counting-sort order, identical to the reference's
 //             cell list, simulation_functions.cpp:84-94), re-applies the
 //             streaming of this step, runs the SRD or extended collision, writes
-//             the particles out in cell order (SoA, coalesced), and already
+//             the particles out in cell order (contiguous records), and already
 //             streams them with the next step's shift to count the next step's
 //             cells (fused histogram) and, when sampling, accumulates the mean
 //             fields in fixed point.
