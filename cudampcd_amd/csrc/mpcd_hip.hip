@@ -1092,18 +1092,34 @@ __device__ void extended_cell(KParams const& P, Tile<CAP, 1>& s, uint32_t k, uin
         float const nz = (J.xz * Lx) + (J.yz * Ly) + (J.zz * Lz);
         Lx = nx; Ly = ny; Lz = nz;
     }
-    for (uint32_t i = 0; i < rr; ++i) {
-        float const rx = s.p[0][a + i], ry = s.p[1][a + i], rz = s.p[2][a + i];
-        float ux = s.p[3][a + i] + mx, uy = s.p[4][a + i] + my, uz = s.p[5][a + i] + mz;
-        ux += P.drag_dt;
-        ux = ux - (ry * Lz - rz * Ly);
-        uy = uy - (rz * Lx - rx * Lz);
-        uz = uz - (rx * Ly - ry * Lx);
-        s.p[3][a + i] = ux; s.p[4][a + i] = uy; s.p[5][a + i] = uz;
-        s.p[0][a + i] = wrap1((cx + rx) - P.shift[0], P, 0);
-        s.p[1][a + i] = wrap1((cy + ry) - P.shift[1], P, 1);
-        s.p[2][a + i] = wrap1((cz + rz) - P.shift[2], P, 2);
-    }
+    // the way back to the lab frame is per particle: ext_finish runs it on all
+    // lanes from these (this cell's ghost count, reference index and ghost
+    // slot are not read again, so they carry the correction)
+    s.cm[0][k] = cx; s.cm[1][k] = cy; s.cm[2][k] = cz;
+    s.cm[3][k] = mx; s.cm[4][k] = my; s.cm[5][k] = mz;
+    s.gcnt[k] = __float_as_uint(Lx);
+    s.cref[k] = __float_as_uint(Ly);
+    s.wcell[k] = __float_as_uint(Lz);
+}
+
+// table particle at LDS slot i of cell k after extended_cell: velocity back
+// to the lab frame with drag and the angular-momentum correction, position
+// back from the centre of mass and un-shifted (simulation_functions.cpp:384-391)
+template <int CAP>
+__device__ __forceinline__ void ext_finish(KParams const& P, Tile<CAP, 1>& s, uint32_t k, uint32_t i) {
+    float const cx = s.cm[0][k], cy = s.cm[1][k], cz = s.cm[2][k];
+    float const mx = s.cm[3][k], my = s.cm[4][k], mz = s.cm[5][k];
+    float const Lx = __uint_as_float(s.gcnt[k]), Ly = __uint_as_float(s.cref[k]), Lz = __uint_as_float(s.wcell[k]);
+    float const rx = s.p[0][i], ry = s.p[1][i], rz = s.p[2][i];
+    float ux = s.p[3][i] + mx, uy = s.p[4][i] + my, uz = s.p[5][i] + mz;
+    ux += P.drag_dt;
+    ux = ux - (ry * Lz - rz * Ly);
+    uy = uy - (rz * Lx - rx * Lz);
+    uz = uz - (rx * Ly - ry * Lx);
+    s.p[3][i] = ux; s.p[4][i] = uy; s.p[5][i] = uz;
+    s.p[0][i] = wrap1((cx + rx) - P.shift[0], P, 0);
+    s.p[1][i] = wrap1((cy + ry) - P.shift[1], P, 1);
+    s.p[2][i] = wrap1((cz + rz) - P.shift[2], P, 2);
 }
 
 // particles gathered per thread per round (one for the 384-particle tile,
@@ -1360,6 +1376,12 @@ __global__ __launch_bounds__(tile_threads(CAP), tile_blocks(CAP, ALGO)) void k_c
             __syncthreads();
             for (uint32_t k = tid; k < ncell; k += NT)
                 extended_cell<CAP>(P, s, k, s.off[kb + k] - gb, c0 + kb + k, gauss);
+            __syncthreads();
+            // table particles of cells with more than one member, one lane each
+            for (uint32_t i = tid; i < m; i += NT) {
+                uint32_t const k = s.cell[i];
+                if (s.go[k] && i - (s.off[kb + k] - gb) < s.goff[k]) ext_finish<CAP>(P, s, k, i);
+            }
         }
         __syncthreads();
         // next step's cell counts: keys near this batch's cells go to three LDS
