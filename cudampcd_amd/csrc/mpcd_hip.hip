@@ -36,6 +36,7 @@
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #pragma clang fp contract(off)
@@ -998,15 +999,45 @@ __device__ void extended_cell(KParams const& P, Tile<CAP, 1>& s, uint32_t k, uin
     float ax = c_axes[select][0], ay = c_axes[select][1], az = c_axes[select][2];
     if (flip) { ax = -ax; ay = -ay; az = -az; }
 
+    // members in order: the table particles (LDS slots a ...), then the
+    // ghosts (ghost slots gb ...); the loops read MB members at a time and
+    // keep every sum in member order
+    // (coordinate c of a member at base[c * S + i]: S = CAP for particles,
+    // GS for ghosts, compile-time so the six rows are immediate LDS offsets)
+    constexpr uint32_t MB = 2;   // members per batch of reads (more leave too few registers)
+    auto members = [&](auto&& f) {
+        auto part = [&](float* const base, uint32_t const cnt, auto stride) {
+            constexpr uint32_t S = decltype(stride)::value;
+            uint32_t i = 0;
+            for (; i + MB <= cnt; i += MB) {
+                float x[MB][6];
+#pragma unroll
+                for (int u = 0; u < MB; ++u)
+#pragma unroll
+                    for (int c = 0; c < 6; ++c) x[u][c] = base[c * S + i + u];
+#pragma unroll
+                for (int u = 0; u < MB; ++u) f(base, S, i + u, x[u]);
+            }
+            for (; i < cnt; ++i) {
+                float x[6];
+#pragma unroll
+                for (int c = 0; c < 6; ++c) x[c] = base[c * S + i];
+                f(base, S, i, x);
+            }
+        };
+        part(&s.p[0][0] + a, rr, std::integral_constant<uint32_t, static_cast<uint32_t>(CAP)>{});
+        part(&s.g[0][0] + gb, added, std::integral_constant<uint32_t, static_cast<uint32_t>(Tile<CAP, 1>::GS)>{});
+    };
+
     float cx = 0, cy = 0, cz = 0, mx = 0, my = 0, mz = 0;
-    for (uint32_t i = 0; i < total; ++i) {
-        cx = cx + (X(0, i) - ox);
-        cy = cy + (X(1, i) - oy);
-        cz = cz + (X(2, i) - oz);
-        mx = mx + X(3, i);
-        my = my + X(4, i);
-        mz = mz + X(5, i);
-    }
+    members([&](float*, uint32_t, uint32_t, float const* x) {
+        cx = cx + (x[0] - ox);
+        cy = cy + (x[1] - oy);
+        cz = cz + (x[2] - oz);
+        mx = mx + x[3];
+        my = my + x[4];
+        mz = mz + x[5];
+    });
     float const inv = 1.0f / static_cast<float>(total);
     cx = cx * inv + ox;
     cy = cy * inv + oy;
@@ -1017,16 +1048,16 @@ __device__ void extended_cell(KParams const& P, Tile<CAP, 1>& s, uint32_t k, uin
 
     float proj0 = 0.0f;
     int size0 = 0;
-    for (uint32_t i = 0; i < total; ++i) {
-        float const rx = X(0, i) - cx, ry = X(1, i) - cy, rz = X(2, i) - cz;
-        float const ux = X(3, i) - mx, uy = X(4, i) - my, uz = X(5, i) - mz;
-        X(0, i) = rx; X(1, i) = ry; X(2, i) = rz;
-        X(3, i) = ux; X(4, i) = uy; X(5, i) = uz;
+    members([&](float* base, uint32_t S, uint32_t i, float const* x) {
+        float const rx = x[0] - cx, ry = x[1] - cy, rz = x[2] - cz;
+        float const ux = x[3] - mx, uy = x[4] - my, uz = x[5] - mz;
+        base[i] = rx; base[S + i] = ry; base[2 * S + i] = rz;
+        base[3 * S + i] = ux; base[4 * S + i] = uy; base[5 * S + i] = uz;
         if (rx * ax + ry * ay + rz * az < 0) {
             proj0 += ux * ax + uy * ay + uz * az;
             size0 += 1;
         }
-    }
+    });
     int const size1 = static_cast<int>(total) - size0;
     float const proj1 = -proj0 / static_cast<float>(size1);
     proj0 = proj0 / static_cast<float>(size0);
