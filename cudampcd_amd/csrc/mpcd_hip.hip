@@ -990,8 +990,6 @@ __device__ void extended_cell(KParams const& P, Tile<CAP, 1>& s, uint32_t k, uin
     float const oz = (0.5f + static_cast<float>(iz)) - P.half[2];
     uint32_t const gb = s.wcell[k];
     if (total <= 1) return;
-    // particle i < rr lives at LDS slot a+i, ghost i >= rr at ghost slot gb+i-rr
-    auto X = [&](int c, uint32_t i) -> float& { return i < rr ? s.p[c][a + i] : s.g[c][gb + (i - rr)]; };
 
     Words const e = P.rs.at(P_EXT, 0, ref, P.step);
     int const select = static_cast<int>(e.w[0] / 2) % 13;
@@ -1004,9 +1002,11 @@ __device__ void extended_cell(KParams const& P, Tile<CAP, 1>& s, uint32_t k, uin
     // keep every sum in member order
     // (coordinate c of a member at base[c * S + i]: S = CAP for particles,
     // GS for ghosts, compile-time so the six rows are immediate LDS offsets)
-    constexpr uint32_t MB = 2;   // members per batch of reads (more leave too few registers)
-    auto members = [&](auto&& f) {
-        auto part = [&](float* const base, uint32_t const cnt, auto stride) {
+    // (reads of MB members issued together: 2 where the pass keeps little
+    // state, 1 in the collision pass, whose sums leave no registers for more)
+    auto members = [&](auto mb, auto&& f) {
+        constexpr uint32_t MB = decltype(mb)::value;
+        auto part = [&](float* const base, uint32_t const cnt, uint32_t const first, auto stride) {
             constexpr uint32_t S = decltype(stride)::value;
             uint32_t i = 0;
             for (; i + MB <= cnt; i += MB) {
@@ -1016,21 +1016,24 @@ __device__ void extended_cell(KParams const& P, Tile<CAP, 1>& s, uint32_t k, uin
 #pragma unroll
                     for (int c = 0; c < 6; ++c) x[u][c] = base[c * S + i + u];
 #pragma unroll
-                for (int u = 0; u < MB; ++u) f(base, S, i + u, x[u]);
+                for (int u = 0; u < MB; ++u) f(base, S, i + u, first + i + u, x[u]);
             }
             for (; i < cnt; ++i) {
                 float x[6];
 #pragma unroll
                 for (int c = 0; c < 6; ++c) x[c] = base[c * S + i];
-                f(base, S, i, x);
+                f(base, S, i, first + i, x);
             }
         };
-        part(&s.p[0][0] + a, rr, std::integral_constant<uint32_t, static_cast<uint32_t>(CAP)>{});
-        part(&s.g[0][0] + gb, added, std::integral_constant<uint32_t, static_cast<uint32_t>(Tile<CAP, 1>::GS)>{});
+        // (first: member index of the part's first member)
+        part(&s.p[0][0] + a, rr, 0u, std::integral_constant<uint32_t, static_cast<uint32_t>(CAP)>{});
+        part(&s.g[0][0] + gb, added, rr, std::integral_constant<uint32_t, static_cast<uint32_t>(Tile<CAP, 1>::GS)>{});
     };
 
     float cx = 0, cy = 0, cz = 0, mx = 0, my = 0, mz = 0;
-    members([&](float*, uint32_t, uint32_t, float const* x) {
+    using two = std::integral_constant<uint32_t, 2>;
+    using one = std::integral_constant<uint32_t, 1>;
+    members(two{}, [&](float*, uint32_t, uint32_t, uint32_t, float const* x) {
         cx = cx + (x[0] - ox);
         cy = cy + (x[1] - oy);
         cz = cz + (x[2] - oz);
@@ -1048,7 +1051,7 @@ __device__ void extended_cell(KParams const& P, Tile<CAP, 1>& s, uint32_t k, uin
 
     float proj0 = 0.0f;
     int size0 = 0;
-    members([&](float* base, uint32_t S, uint32_t i, float const* x) {
+    members(two{}, [&](float* base, uint32_t S, uint32_t i, uint32_t, float const* x) {
         float const rx = x[0] - cx, ry = x[1] - cy, rz = x[2] - cz;
         float const ux = x[3] - mx, uy = x[4] - my, uz = x[5] - mz;
         base[i] = rx; base[S + i] = ry; base[2 * S + i] = rz;
@@ -1072,7 +1075,7 @@ __device__ void extended_cell(KParams const& P, Tile<CAP, 1>& s, uint32_t k, uin
         float mean0 = 0.0f, mean1 = 0.0f;
         Sym I = {0, 0, 0, 0, 0, 0};
         float gq[4] = {0, 0, 0, 0};
-        for (uint32_t i = 0; i < total; ++i) {
+        members(one{}, [&](float* base, uint32_t S, uint32_t li, uint32_t i, float const* x) {
             float gi;
             if (i < rr) {
                 gi = gauss[a + i];
@@ -1082,8 +1085,8 @@ __device__ void extended_cell(KParams const& P, Tile<CAP, 1>& s, uint32_t k, uin
                 uint32_t const qi = i & 3u;
                 gi = qi == 0 ? gq[0] : (qi == 1 ? gq[1] : (qi == 2 ? gq[2] : gq[3]));
             }
-            float const rx = X(0, i), ry = X(1, i), rz = X(2, i);
-            float ux = X(3, i), uy = X(4, i), uz = X(5, i);
+            float const rx = x[0], ry = x[1], rz = x[2];
+            float ux = x[3], uy = x[4], uz = x[5];
             bool const side = rx * ax + ry * ay + rz * az < 0;
             float const vr = gi * P.tv;
             if (side) mean0 += vr; else mean1 += vr;
@@ -1094,7 +1097,7 @@ __device__ void extended_cell(KParams const& P, Tile<CAP, 1>& s, uint32_t k, uin
             ux = ux + ax * sc;
             uy = uy + ay * sc;
             uz = uz + az * sc;
-            X(3, i) = ux; X(4, i) = uy; X(5, i) = uz;
+            base[3 * S + li] = ux; base[4 * S + li] = uy; base[5 * S + li] = uz;
             float const qx = rx * rx, qy = ry * ry, qz = rz * rz;
             I.xx += qy + qz;
             I.yy += qx + qz;
@@ -1102,21 +1105,21 @@ __device__ void extended_cell(KParams const& P, Tile<CAP, 1>& s, uint32_t k, uin
             I.xy += -rx * ry;
             I.xz += -rx * rz;
             I.yz += -ry * rz;
-        }
+        });
         float const a0 = mean0 / static_cast<float>(size0), a1 = mean1 / static_cast<float>(size1);
         float const v0x = ax * a0, v0y = ay * a0, v0z = az * a0;
         float const v1x = ax * a1, v1y = ay * a1, v1z = az * a1;
-        for (uint32_t i = 0; i < total; ++i) {
-            float const rx = X(0, i), ry = X(1, i), rz = X(2, i);
+        members(two{}, [&](float* base, uint32_t S, uint32_t li, uint32_t, float const* x) {
+            float const rx = x[0], ry = x[1], rz = x[2];
             bool const side = rx * ax + ry * ay + rz * az < 0;
-            float const ux = X(3, i) - (side ? v0x : v1x);
-            float const uy = X(4, i) - (side ? v0y : v1y);
-            float const uz = X(5, i) - (side ? v0z : v1z);
-            X(3, i) = ux; X(4, i) = uy; X(5, i) = uz;
+            float const ux = x[3] - (side ? v0x : v1x);
+            float const uy = x[4] - (side ? v0y : v1y);
+            float const uz = x[5] - (side ? v0z : v1z);
+            base[3 * S + li] = ux; base[4 * S + li] = uy; base[5 * S + li] = uz;
             Lx = Lx - (ry * uz - rz * uy);
             Ly = Ly - (rz * ux - rx * uz);
             Lz = Lz - (rx * uy - ry * ux);
-        }
+        });
         Sym const J = sym_inverse(I, 1e-5f);
         float const nx = (J.xx * Lx) + (J.xy * Ly) + (J.xz * Lz);
         float const ny = (J.xy * Lx) + (J.yy * Ly) + (J.yz * Lz);
