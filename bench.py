@@ -145,7 +145,7 @@ def main():
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
-              "(the step would shard as x-slabs with particle halo exchange, see DESIGN.md)", file=sys.stderr)
+              "(the step would shard as x-slabs with a particle halo exchange between neighbours)", file=sys.stderr)
         sys.exit(2)
 
     import cudampcd_amd as pympcd
