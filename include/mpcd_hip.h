@@ -108,7 +108,7 @@ int mpcd_hip_enable_timing(mpcd_hip_t* h, int enable);
  * kernels and of the whole step sequence, from HIP events on the handle's stream. */
 int mpcd_hip_last_timing(const mpcd_hip_t* h, double* collide_ms, double* total_ms, int* launches);
 /* Kernel launch shape: cells per tile (at most 24 for the SRD capacity 384,
- * 32 for SRD 448 and extended 512, else 64) and LDS particle capacity of one
+ * 32 for SRD 448, 48 for extended 512, else 64) and LDS particle capacity of one
  * batch (rounded up to 256,
  * 384, 448, 512, 640, 1152 or 2048; 256 runs one-wave workgroups, 384 to 640
  * two-wave workgroups, larger ones four waves; 256 to 448 are SRD only, and

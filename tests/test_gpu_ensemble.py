@@ -87,7 +87,8 @@ def test_autotune_keeps_results(pympcd):
     tuned = pympcd.Ensemble(params)
     plain = pympcd.Ensemble(params)
     tc, cap, ms = tuned.autotune(2)
-    assert cap in (256, 384, 512, 640, 1152, 2048) and 1 <= tc <= 64 and ms > 0
+    # any compiled LDS capacity can win the timing (include/mpcd_hip.h)
+    assert cap in (256, 384, 448, 512, 640, 1152, 2048) and 1 <= tc <= 64 and ms > 0
     steps = Handle(tuned).step_index
     plain.step(steps)
     tuned.step(3)
