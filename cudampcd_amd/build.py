@@ -36,6 +36,10 @@ HIPCC_FLAGS = [
     "-ffp-contract=off",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall", "-Wno-unused-function",
+    # the collide kernel asks for more waves per SIMD than its LDS allows (a
+    # scheduling target, measured faster; see tile_blocks in mpcd_hip.hip),
+    # which the compiler reports as a missed occupancy target
+    "-Wno-pass-failed",
 ]
 
 
