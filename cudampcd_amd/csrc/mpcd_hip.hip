@@ -524,7 +524,6 @@ __device__ __forceinline__ void perm_block(Sys const& sys, const uint32_t* __res
     __shared__ uint32_t sj[ITEMS];       // first the non-empty bin list, then the entries in bin
                                          // order: bin << 16 | particle - base
     __shared__ uint32_t s_w[WG / 64];
-    __shared__ uint32_t s_base[3];
     uint64_t const base = static_cast<uint64_t>(blk) * ITEMS;
     for (uint32_t i = threadIdx.x; i < PERM_WIN; i += WG) cnt[i] = 0u;
     uint32_t key[PERM_ITEMS], local[PERM_ITEMS], bin[PERM_ITEMS];
@@ -533,24 +532,21 @@ __device__ __forceinline__ void perm_block(Sys const& sys, const uint32_t* __res
         uint64_t const j = base + q * WG + threadIdx.x;
         key[q] = j < N ? keys[j] : ~0u;
     }
-    if (threadIdx.x == 0) {
-        uint32_t const k = key[0];   // keys[base]: base < N for every launched block
-        uint32_t lo = 0, hi = sys.ns;   // system holding key k
-        while (hi - lo > 1) {
-            uint32_t const mid = (lo + hi) >> 1;
-            if (sys.P[mid].cb <= k) lo = mid; else hi = mid;
-        }
-        KParams const& P = sys.P[lo];
-        uint32_t const smid = mid_stride(P, k - P.cb);
-        // the window starts one fastest-axis row (plus one) before the first key
-        uint32_t const back = min(P.D0 + 2u, PERM_PW / 2);
-        uint32_t const b0 = k - back;             // bases may wrap below zero (distances are modulo 2^32)
-        s_base[0] = b0;
-        s_base[1] = b0 - smid;
-        s_base[2] = b0 + smid;
+    // window bases from the block's first key, computed by every lane (the
+    // values are uniform, so the key and the system's parameters are read
+    // alongside the keys above, not after them)
+    uint32_t const k0 = keys[base];   // base < N for every launched block
+    uint32_t lo = 0, hi = sys.ns;      // system holding key k0
+    while (hi - lo > 1) {
+        uint32_t const mid = (lo + hi) >> 1;
+        if (sys.P[mid].cb <= k0) lo = mid; else hi = mid;
     }
-    __syncthreads();
-    uint32_t const w0 = s_base[0], wm = s_base[1], wp = s_base[2];
+    KParams const& P = sys.P[lo];
+    uint32_t const smid = mid_stride(P, k0 - P.cb);
+    // the window starts one fastest-axis row (plus one) before the first key;
+    // bases may wrap below zero (distances are taken modulo 2^32)
+    uint32_t const w0 = k0 - min(P.D0 + 2u, PERM_PW / 2), wm = w0 - smid, wp = w0 + smid;
+    __syncthreads();   // the bins are cleared
 #pragma unroll
     for (int q = 0; q < PERM_ITEMS; ++q) {
         uint32_t w = key[q] - w0;
