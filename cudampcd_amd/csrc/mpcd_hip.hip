@@ -1566,102 +1566,204 @@ __global__ __launch_bounds__(WG) void k_layers(Sys sys, const float* __restrict_
 // of this step added to its x plane; lam_out: dL/dv of the input slots.
 // part[b], part[G + b]: this block's running sums of dL/dg and dL/dalpha (radians);
 // gf: dL/df per reference cell.
-__global__ __launch_bounds__(WG) void k_adjoint_step(Sys sys, const uint32_t* __restrict__ off,
-                                                     const float* __restrict__ u, const float* __restrict__ ax,
-                                                     const uint32_t* __restrict__ tot, const uint32_t* __restrict__ perm,
-                                                     const float* __restrict__ vp, const float* __restrict__ seed,
-                                                     const float* __restrict__ lam_in, float* __restrict__ lam_out,
-                                                     double* __restrict__ part, double* __restrict__ gf) {
+// Reverse of one SRD step with the cells, tables, bounces and axes of the
+// recorded forward step: a table particle of a colliding cell gets
+// mu = R^T lam + (I - R^T) (sum of the table's lam) / total, every other
+// particle keeps lam, and each is scattered back to its source slot (negated
+// after a bounce); dL/dg, dL/dalpha and dL/df accumulate in FP64.
+// 256 cells per block.  The block's particles' adjoint values are staged in
+// LDS with coalesced loads; one lane per cell forms the cell's table sums (in
+// slot order, FP64) and its rotation of the mean; then every lane updates one
+// particle.  A block whose cells hold more particles than the staging buffer
+// runs the same arithmetic one lane per cell, reading memory directly.
+constexpr uint32_t ADJ_STAGE = 3072;
+
+struct AdjIn {
+    const uint32_t* off;
+    const float* u;
+    const float* ax;
+    const uint32_t* tot;
+    const uint32_t* perm;
+    const float* vp;
+    const float* seed;
+    const float* lam_in;
+    float* lam_out;
+};
+
+// R^T x: x rotated by -alpha about the axis (axx, axy, axz)
+__device__ __forceinline__ void adj_rot_t(KParams const& P, float axx, float axy, float axz, float x, float y, float z,
+                                          float* o3) {
+    float const d = x * axx + y * axy + z * axz;
+    float const px = x - axx * d, py = y - axy * d, pz = z - axz * d;
+    float const cx = py * axz - pz * axy, cy = pz * axx - px * axz, cz = px * axy - py * axx;
+    o3[0] = axx * d + P.cos_a * px - P.sin_a * cx;
+    o3[1] = axy * d + P.cos_a * py - P.sin_a * cy;
+    o3[2] = axz * d + P.cos_a * pz - P.sin_a * cz;
+}
+
+// scatter of one particle's adjoint back to its source slot (sign of a bounce)
+__device__ __forceinline__ void adj_put(AdjIn const& A, uint64_t N, uint32_t o, float const* m) {
+    uint32_t const pe = A.perm[o];
+    uint32_t const src = pe & ~BOUNCED;
+    float const sg = (pe & BOUNCED) ? -1.0f : 1.0f;
+    A.lam_out[src] = sg * m[0];
+    A.lam_out[N + src] = sg * m[1];
+    A.lam_out[2 * N + src] = sg * m[2];
+}
+
+// dL/dalpha contribution of table particle o: l . dR/dalpha (v' - u), with
+// dR/dalpha (v' - u) = -sin q + cos (q x a), q the part of v' - u normal to a
+__device__ __forceinline__ double adj_dalpha(KParams const& P, AdjIn const& A, uint64_t N, uint32_t o, float const* l,
+                                             float axx, float axy, float axz, float ux, float uy, float uz) {
+    float const wx = A.vp[o] - ux, wy = A.vp[N + o] - uy, wz = A.vp[2 * N + o] - uz;
+    float const d = wx * axx + wy * axy + wz * axz;
+    float const qx = wx - axx * d, qy = wy - axy * d, qz = wz - axz * d;
+    float const cx = qy * axz - qz * axy, cy = qz * axx - qx * axz, cz = qx * axy - qy * axx;
+    return static_cast<double>(l[0]) * (-P.sin_a * qx + P.cos_a * cx) +
+           static_cast<double>(l[1]) * (-P.sin_a * qy + P.cos_a * cy) +
+           static_cast<double>(l[2]) * (-P.sin_a * qz + P.cos_a * cz);
+}
+
+// the whole reverse step of cell `key` on one lane (blocks too large to stage)
+__device__ void adjoint_cell_serial(KParams const& P, AdjIn const& A, uint64_t N, uint64_t C, uint32_t key,
+                                    double* __restrict__ gf, double& dg, double& da) {
+    uint32_t const a = A.off[key], e = A.off[key + 1];
+    uint32_t const total = A.tot[key];
+    uint32_t const reals = min(e - a, P.cap);
+    auto lam = [&](uint32_t o, float* l) {
+        l[0] = A.lam_in[o] + A.seed[o];
+        l[1] = A.lam_in[N + o];
+        l[2] = A.lam_in[2 * N + o];
+    };
+    uint32_t first_plain = a;
+    if (total > 1) {
+        float const axx = A.ax[key], axy = A.ax[C + key], axz = A.ax[2 * C + key];
+        float const ux = A.u[key], uy = A.u[C + key], uz = A.u[2 * C + key];
+        double sx = 0, sy = 0, sz = 0;
+        for (uint32_t o = a; o < a + reals; ++o) {
+            float l[3];
+            lam(o, l);
+            sx += l[0];
+            sy += l[1];
+            sz += l[2];
+        }
+        float const inv = static_cast<float>(1.0 / static_cast<double>(total));
+        float const mx = static_cast<float>(sx) * inv, my = static_cast<float>(sy) * inv, mz = static_cast<float>(sz) * inv;
+        float rm[3];
+        adj_rot_t(P, axx, axy, axz, mx, my, mz, rm);
+        float const cmx = mx - rm[0], cmy = my - rm[1], cmz = mz - rm[2];
+        for (uint32_t o = a; o < a + reals; ++o) {
+            float l[3];
+            lam(o, l);
+            float rl[3];
+            adj_rot_t(P, axx, axy, axz, l[0], l[1], l[2], rl);
+            float const m[3] = {rl[0] + cmx, rl[1] + cmy, rl[2] + cmz};
+            da += adj_dalpha(P, A, N, o, l, axx, axy, axz, ux, uy, uz);
+            adj_put(A, N, o, m);
+        }
+        // dL/dg: dt times the x adjoint summed over the table (the same sum as sx)
+        double const gx = sx * static_cast<double>(P.dt);
+        dg += gx;
+        uint32_t ix, iy, iz;
+        decode_key(P, key, ix, iy, iz);
+        gf[P.cb + ref_of(P, ix, iy, iz)] += gx;
+        first_plain = a + reals;
+    }
+    for (uint32_t o = first_plain; o < e; ++o) {
+        float l[3];
+        lam(o, l);
+        adj_put(A, N, o, l);
+    }
+}
+
+__global__ __launch_bounds__(WG) void k_adjoint_step(Sys sys, AdjIn A, double* __restrict__ part,
+                                                     double* __restrict__ gf) {
+    __shared__ float sl[3][ADJ_STAGE];   // adjoint (plus loss seed) of the block's particles
+    __shared__ uint8_t scell[ADJ_STAGE]; // block-local cell of each staged particle
+    __shared__ uint32_t soff[WG + 1];
+    __shared__ uint32_t sreal[WG];       // table particles of a colliding cell, else 0
+    __shared__ float scell_v[9][WG];     // axis, cell mean u, (I - R^T) applied to the table mean
     __shared__ double red[2][WG / 64];
     KParams const P = sys.P[0];
     uint64_t const N = sys.NT, C = sys.CT;
+    uint32_t const tid = threadIdx.x;
+    uint32_t const k0 = blockIdx.x * WG;
+    uint32_t const ncell = min(static_cast<uint32_t>(WG), P.C - k0);   // k0 < P.C for every launched block
+    for (uint32_t k = tid; k <= ncell; k += WG) soff[k] = A.off[k0 + k];
+    __syncthreads();
+    uint32_t const pa = soff[0], m = soff[ncell] - pa;
     double dg = 0.0, da = 0.0;
-    uint32_t const key = blockIdx.x * WG + threadIdx.x;
-    if (key < P.C) {
-        uint32_t const a = off[key], e = off[key + 1];
-        uint32_t const total = tot[key];
-        uint32_t const reals = min(e - a, P.cap);
-        auto lam = [&](uint32_t o, float* l) {
-            l[0] = lam_in[o] + seed[o];
-            l[1] = lam_in[N + o];
-            l[2] = lam_in[2 * N + o];
-        };
-        auto put = [&](uint32_t o, float const* m) {
-            uint32_t const pe = perm[o];
-            uint32_t const src = pe & ~BOUNCED;
-            float const sg = (pe & BOUNCED) ? -1.0f : 1.0f;
-            lam_out[src] = sg * m[0];
-            lam_out[N + src] = sg * m[1];
-            lam_out[2 * N + src] = sg * m[2];
-        };
-        uint32_t first_plain = a;
-        if (total > 1) {
-            float const axx = ax[key], axy = ax[C + key], axz = ax[2 * C + key];
-            float const ux = u[key], uy = u[C + key], uz = u[2 * C + key];
-            double sx = 0, sy = 0, sz = 0, sgx = 0;
-            for (uint32_t o = a; o < a + reals; ++o) {
-                float l[3];
-                lam(o, l);
-                sx += l[0];
-                sy += l[1];
-                sz += l[2];
+    if (m <= ADJ_STAGE) {
+        for (uint32_t i = tid; i < m; i += WG) {
+            uint64_t const o = pa + i;
+            sl[0][i] = A.lam_in[o] + A.seed[o];
+            sl[1][i] = A.lam_in[N + o];
+            sl[2][i] = A.lam_in[2 * N + o];
+        }
+        if (tid < ncell)
+            for (uint32_t i = soff[tid] - pa; i < soff[tid + 1] - pa; ++i) scell[i] = static_cast<uint8_t>(tid);
+        __syncthreads();
+        if (tid < ncell) {
+            uint32_t const key = k0 + tid;
+            uint32_t const a = soff[tid] - pa, e = soff[tid + 1] - pa;
+            uint32_t const total = A.tot[key];
+            uint32_t const reals = min(e - a, P.cap);
+            sreal[tid] = 0u;
+            if (total > 1) {
+                float const axx = A.ax[key], axy = A.ax[C + key], axz = A.ax[2 * C + key];
+                double sx = 0, sy = 0, sz = 0;
+                for (uint32_t i = a; i < a + reals; ++i) {
+                    sx += sl[0][i];
+                    sy += sl[1][i];
+                    sz += sl[2][i];
+                }
+                float const inv = static_cast<float>(1.0 / static_cast<double>(total));
+                float const mx = static_cast<float>(sx) * inv, my = static_cast<float>(sy) * inv,
+                            mz = static_cast<float>(sz) * inv;
+                float rm[3];
+                adj_rot_t(P, axx, axy, axz, mx, my, mz, rm);
+                scell_v[0][tid] = axx; scell_v[1][tid] = axy; scell_v[2][tid] = axz;
+                scell_v[3][tid] = A.u[key]; scell_v[4][tid] = A.u[C + key]; scell_v[5][tid] = A.u[2 * C + key];
+                scell_v[6][tid] = mx - rm[0]; scell_v[7][tid] = my - rm[1]; scell_v[8][tid] = mz - rm[2];
+                sreal[tid] = reals;
+                double const gx = sx * static_cast<double>(P.dt);
+                dg = gx;
+                uint32_t ix, iy, iz;
+                decode_key(P, key, ix, iy, iz);
+                gf[P.cb + ref_of(P, ix, iy, iz)] += gx;
             }
-            float const inv = static_cast<float>(1.0 / static_cast<double>(total));
-            float const mx = static_cast<float>(sx) * inv, my = static_cast<float>(sy) * inv, mz = static_cast<float>(sz) * inv;
-            // (I - R^T) applied to the table mean of lam
-            auto rot_t = [&](float x, float y, float z, float* o3) {
-                float const d = x * axx + y * axy + z * axz;
-                float const px = x - axx * d, py = y - axy * d, pz = z - axz * d;
-                float const cx = py * axz - pz * axy, cy = pz * axx - px * axz, cz = px * axy - py * axx;
-                o3[0] = axx * d + P.cos_a * px - P.sin_a * cx;
-                o3[1] = axy * d + P.cos_a * py - P.sin_a * cy;
-                o3[2] = axz * d + P.cos_a * pz - P.sin_a * cz;
-            };
-            float rm[3];
-            rot_t(mx, my, mz, rm);
-            float const cmx = mx - rm[0], cmy = my - rm[1], cmz = mz - rm[2];
-            for (uint32_t o = a; o < a + reals; ++o) {
-                float l[3];
-                lam(o, l);
+        }
+        __syncthreads();
+        for (uint32_t i = tid; i < m; i += WG) {
+            uint32_t const k = scell[i];
+            uint32_t const o = pa + i;
+            float const l[3] = {sl[0][i], sl[1][i], sl[2][i]};
+            if (i - (soff[k] - pa) < sreal[k]) {
+                float const axx = scell_v[0][k], axy = scell_v[1][k], axz = scell_v[2][k];
                 float rl[3];
-                rot_t(l[0], l[1], l[2], rl);
-                float const m[3] = {rl[0] + cmx, rl[1] + cmy, rl[2] + cmz};
-                // dR/dalpha (v' - u) = -sin q + cos (q x a), q the part of v' - u normal to the axis
-                float const wx = vp[o] - ux, wy = vp[N + o] - uy, wz = vp[2 * N + o] - uz;
-                float const d = wx * axx + wy * axy + wz * axz;
-                float const qx = wx - axx * d, qy = wy - axy * d, qz = wz - axz * d;
-                float const cx = qy * axz - qz * axy, cy = qz * axx - qx * axz, cz = qx * axy - qy * axx;
-                da += static_cast<double>(l[0]) * (-P.sin_a * qx + P.cos_a * cx) +
-                      static_cast<double>(l[1]) * (-P.sin_a * qy + P.cos_a * cy) +
-                      static_cast<double>(l[2]) * (-P.sin_a * qz + P.cos_a * cz);
-                sgx += l[0];
-                put(o, m);
+                adj_rot_t(P, axx, axy, axz, l[0], l[1], l[2], rl);
+                float const mm[3] = {rl[0] + scell_v[6][k], rl[1] + scell_v[7][k], rl[2] + scell_v[8][k]};
+                da += adj_dalpha(P, A, N, o, l, axx, axy, axz, scell_v[3][k], scell_v[4][k], scell_v[5][k]);
+                adj_put(A, N, o, mm);
+            } else {
+                adj_put(A, N, o, l);
             }
-            double const gx = sgx * static_cast<double>(P.dt);
-            dg = gx;
-            uint32_t ix, iy, iz;
-            decode_key(P, key, ix, iy, iz);
-            gf[P.cb + ref_of(P, ix, iy, iz)] += gx;
-            first_plain = a + reals;
         }
-        for (uint32_t o = first_plain; o < e; ++o) {
-            float l[3];
-            lam(o, l);
-            put(o, l);
-        }
+    } else if (tid < ncell) {
+        adjoint_cell_serial(P, A, N, C, k0 + tid, gf, dg, da);
     }
     // block sums, kept per block so the final sum has a fixed order
     for (int d = 32; d >= 1; d >>= 1) {
         dg += __shfl_down(dg, d, 64);
         da += __shfl_down(da, d, 64);
     }
-    uint32_t const lane = threadIdx.x & 63u, wv = threadIdx.x >> 6;
+    uint32_t const lane = tid & 63u, wv = tid >> 6;
     if (lane == 0) {
         red[0][wv] = dg;
         red[1][wv] = da;
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
+    if (tid == 0) {
         double tg = 0, ta = 0;
         for (int k = 0; k < WG / 64; ++k) {
             tg += red[0][k];
@@ -2392,8 +2494,10 @@ void adj_step(mpcd_hip_s* h, float const* force, int j) {
     h->sorted_valid = false;
 }
 
+// blocks of k_layers: each flushes its LDS histogram with two atomics per
+// layer onto the same few addresses, so fewer, longer blocks (four per CU)
 uint32_t layer_blocks(mpcd_hip_s* h) {
-    return static_cast<uint32_t>(std::max<uint64_t>(1, std::min<uint64_t>((h->N + WG - 1) / WG, 4096)));
+    return static_cast<uint32_t>(std::max<uint64_t>(1, std::min<uint64_t>((h->N + WG - 1) / WG, 1024)));
 }
 
 struct ProfileResult {
@@ -2515,9 +2619,9 @@ ProfileResult profile_run(mpcd_hip_s* h, int steps, int K, float drag, float ang
                 }
                 for (uint32_t t = t1; t > t0; --t) {
                     uint64_t const j = t - t0 - 1;
-                    hipLaunchKernelGGL(k_adjoint_step, dim3(gb), dim3(WG), 0, h->stream, h->sys, a.off + j * (C + 1),
-                                       a.u + j * 3 * C, a.ax + j * 3 * C, a.tot + j * C, a.perm + j * N,
-                                       a.vp + j * 3 * N, a.seed + j * N, a.lam[li], a.lam[1 - li], a.part, a.gf);
+                    AdjIn const in{a.off + j * (C + 1), a.u + j * 3 * C, a.ax + j * 3 * C, a.tot + j * C, a.perm + j * N,
+                                   a.vp + j * 3 * N, a.seed + j * N, a.lam[li], a.lam[1 - li]};
+                    hipLaunchKernelGGL(k_adjoint_step, dim3(gb), dim3(WG), 0, h->stream, h->sys, in, a.part, a.gf);
                     li = 1 - li;
                 }
                 ck(hipGetLastError(), "reverse pass");
