@@ -2495,9 +2495,11 @@ void adj_step(mpcd_hip_s* h, float const* force, int j) {
 }
 
 // blocks of k_layers: each flushes its LDS histogram with two atomics per
-// layer onto the same few addresses, so fewer, longer blocks (four per CU)
+// layer onto the same few addresses, so fewer, longer blocks (two per CU;
+// 64x64x52 cells, n = 10: 4096 blocks 64 us per launch, 1024 25 us, 512 a
+// little less)
 uint32_t layer_blocks(mpcd_hip_s* h) {
-    return static_cast<uint32_t>(std::max<uint64_t>(1, std::min<uint64_t>((h->N + WG - 1) / WG, 1024)));
+    return static_cast<uint32_t>(std::max<uint64_t>(1, std::min<uint64_t>((h->N + WG - 1) / WG, 512)));
 }
 
 struct ProfileResult {
