@@ -753,11 +753,13 @@ __device__ __forceinline__ void ghosts_block(Sys const& sys, uint32_t* __restric
     }
 }
 
-__global__ __launch_bounds__(WG) void k_cell_axes(Sys sys, float* __restrict__ axis) {
-    axes_block(sys, axis, blockIdx.x);
-}
-__global__ __launch_bounds__(WG) void k_wall_ghosts(Sys sys, uint32_t* __restrict__ gcnt, float* __restrict__ gvel) {
-    ghosts_block(sys, gcnt, gvel, blockIdx.x);
+// The SRD per-cell draws of a step in one launch: blocks below nbw draw the
+// wall cells' ghosts (fewer, compute-heavy blocks, dispatched first), the
+// rest the cells' axes (memory-heavy), so the two kinds overlap.
+__global__ __launch_bounds__(WG) void k_cell_draws(Sys sys, float* __restrict__ axis, uint32_t* __restrict__ gcnt,
+                                                   float* __restrict__ gvel, uint32_t nbw) {
+    if (blockIdx.x < nbw) ghosts_block(sys, gcnt, gvel, blockIdx.x);
+    else axes_block(sys, axis, blockIdx.x - nbw);
 }
 
 // one step forward of every system's step state: the shift of step t + 1 and
@@ -835,7 +837,7 @@ struct Tile {
 // SRD per-cell stage of one batch (simulation_functions.cpp:133-182): one lane
 // per cell sums its particles in slot order and its ghosts in ghost order (the
 // reference's order).  Ghost counts and velocities of wall cells (drawn by
-// k_wall_ghosts) were staged into LDS while the particles were gathered:
+// k_cell_draws) were staged into LDS while the particles were gathered:
 // s.gcnt[k] ghosts at s.g[0][s.goff[k] ...], or, when the staging region was
 // full (s.goff[k] == ~0u), read from memory at gvel + 3 n s.cref[k].
 template <int CAP, bool ADJ>
@@ -2141,6 +2143,16 @@ int guarded(mpcd_hip_s* h, F&& f) {
     return 2;
 }
 
+// the SRD per-cell draws of the step in the system table (k_cell_draws)
+int launch_draws(mpcd_hip_s* h) {
+    uint32_t const nb = h->blocks_wall + h->blocks_cell;
+    if (!nb) return 0;
+    hipLaunchKernelGGL(k_cell_draws, dim3(nb), dim3(WG), 0, h->stream, h->sys, h->axis, h->wgcnt, h->wgvel,
+                       h->blocks_wall);
+    ck(hipGetLastError(), "cell draws");
+    return 1;
+}
+
 void launch_keys(mpcd_hip_s* h) {
     ck(hipMemsetAsync(h->count, 0, 4 * h->C, h->stream), "clear counts");
     if (h->blocks_part)
@@ -2223,11 +2235,7 @@ int enqueue_step(mpcd_hip_s* h, bool sample, bool first, bool sort, uint32_t* re
     int launches = 0;
     if (sort) launches += launch_sort(h);
     if (h->algorithm == 0) {
-        if (h->blocks_cell) hipLaunchKernelGGL(k_cell_axes, dim3(h->blocks_cell), dim3(WG), 0, h->stream, h->sys, h->axis);
-        if (h->blocks_wall)
-            hipLaunchKernelGGL(k_wall_ghosts, dim3(h->blocks_wall), dim3(WG), 0, h->stream, h->sys, h->wgcnt, h->wgvel);
-        ck(hipGetLastError(), "cell draws");
-        launches += (h->blocks_cell ? 1 : 0) + (h->blocks_wall ? 1 : 0);
+        launches += launch_draws(h);
     }
     if (e0) ck(hipEventRecord(e0, h->stream), "event");
     if (h->algorithm == 0) {
@@ -2462,8 +2470,7 @@ void adj_step(mpcd_hip_s* h, float const* force, int j) {
     uint64_t const N = h->N, C = h->C;
     if (!h->keys_valid) launch_keys(h);
     if (!h->sorted_valid) launch_sort(h);
-    if (h->blocks_cell) hipLaunchKernelGGL(k_cell_axes, dim3(h->blocks_cell), dim3(WG), 0, h->stream, h->sys, h->axis);
-    if (h->blocks_wall) hipLaunchKernelGGL(k_wall_ghosts, dim3(h->blocks_wall), dim3(WG), 0, h->stream, h->sys, h->wgcnt, h->wgvel);
+    launch_draws(h);
     Tape T{};
     T.force = force;
     uint32_t* rec = nullptr;
