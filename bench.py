@@ -253,7 +253,7 @@ def bench_ensemble(args, cfg):
     ens = pympcd.Ensemble(params)
     h = Handle(ens)
     N, C = h.N, h.C
-    tc, cap, tune_ms = ens.autotune(2)
+    tc, cap, tune_ms = ens.autotune(4)   # steps per trial: two ranked the tile shapes unreliably
     print(f"[bench] autotuned tiles: {tc} cells (system 0), LDS capacity {cap}, {tune_ms:.2f} ms/step", file=sys.stderr, flush=True)
     ens.step(args.warmup)
     print(f"[bench] timing {args.steps} batched steps", file=sys.stderr, flush=True)
