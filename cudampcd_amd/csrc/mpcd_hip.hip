@@ -789,17 +789,24 @@ constexpr int TCM = 64;   // largest cells per tile of any tile
 // Small tiles keep fewer cells so that more workgroups fit the 160 KB of LDS
 // of a CU (allocated in 512-byte granules): SRD 384-particle tiles 24 cells
 // (twelve two-wave workgroups, six waves per SIMD), SRD 448 tiles 32 cells
-// (ten workgroups).  Extended 512-particle tiles take 48 cells (six
-// workgroups): its per-cell stage runs one lane per cell, and more cells per
-// batch beat the seventh workgroup that 32-cell tiles leave room for
-// (500x500x40 cells, n = 10: 4.63 against 4.90 ms per step)
+// (ten workgroups).  Extended 512-particle tiles take 48 cells (seven
+// workgroups with the 192-slot ghost region): its per-cell stage runs one
+// lane per cell, so more cells per batch keep more lanes busy (500x500x40
+// cells, n = 10, six workgroups: 4.63 ms per step against 4.90 with 32 cells)
 constexpr int tile_cell_max(int cap, int algo) {
     return algo == 0 ? (cap <= 384 ? 24 : (cap <= 448 ? 32 : TCM)) : (cap <= 512 ? 48 : TCM);
 }
 // floats of the ghost region: extended ghosts (per coordinate), SRD staged
 // wall-cell ghost velocities (slightly less than half the capacity for the
-// 384-particle tile, to stay within 26 LDS granules)
-constexpr int ghost_floats(int cap, int algo) { return algo == 0 && cap == 384 ? cap / 2 - 32 : cap / 2; }
+// 384-particle tile, to stay within 26 LDS granules).
+// Extended 512-particle tiles keep 192 ghost slots: a 48-cell batch rarely
+// holds more than a few wall cells, a batch whose ghosts would overflow ends
+// early, and the smaller region lets seven workgroups share a CU instead of
+// six (500x500x40 cells, n = 10: 4.63 -> 4.16 ms per step).  A region must
+// hold one wall cell's n ghosts (choose_tiling).
+constexpr int ghost_floats(int cap, int algo) {
+    return algo == 0 && cap == 384 ? cap / 2 - 32 : (algo == 1 && cap == 512 ? 192 : cap / 2);
+}
 
 // threads of the collide workgroup for an LDS capacity: more, smaller
 // workgroups keep more independent tiles in flight per CU
@@ -2025,7 +2032,7 @@ void choose_tiling(mpcd_hip_s* h, uint32_t tc, uint32_t cap) {
     uint32_t nmax = 1;
     for (KParams const& k : h->kp) nmax = std::max(nmax, k.n);
     // the extended rule needs 512 or more and room for the ghosts of one wall
-    // cell (n) in its half-capacity ghost region
+    // cell (n) in its ghost region
     if (h->algorithm == 1 && TILE_CAP_MAX / 2 < nmax) throw HipError{"the extended rule supports n up to 1024"};
     if (cap == 0 && h->lds_cap) cap = h->lds_cap;
     // default: SRD 384-particle batches on two-wave workgroups (twelve per
@@ -2034,7 +2041,10 @@ void choose_tiling(mpcd_hip_s* h, uint32_t tc, uint32_t cap) {
     if (cap == 0) cap = std::max(h->algorithm == 0 ? 384u : 512u, 6 * nmax);
     uint32_t chosen = TILE_CAP_MAX;
     for (uint32_t c : TILE_CAPS)
-        if (c >= cap && (h->algorithm == 0 || (c >= 512 && c / 2 >= nmax))) { chosen = c; break; }
+        if (c >= cap && (h->algorithm == 0 || (c >= 512 && static_cast<uint32_t>(ghost_floats(static_cast<int>(c), 1)) >= nmax))) {
+            chosen = c;
+            break;
+        }
     h->lds_cap = chosen;
     h->tile_req = tc;
     for (KParams& k : h->kp) {
