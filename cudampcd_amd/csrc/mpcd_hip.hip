@@ -789,8 +789,8 @@ constexpr int TCM = 64;   // largest cells per tile of any tile
 // Small tiles keep fewer cells so that more workgroups fit the 160 KB of LDS
 // of a CU (allocated in 512-byte granules): SRD 384-particle tiles 24 cells
 // (twelve two-wave workgroups, six waves per SIMD), SRD 448 tiles 32 cells
-// (ten workgroups).  Extended 512-particle tiles take 48 cells (seven
-// workgroups with the 192-slot ghost region): its per-cell stage runs one
+// (ten workgroups).  Extended 512-particle tiles take 48 cells (eight
+// workgroups with the 112-slot ghost region): its per-cell stage runs one
 // lane per cell, so more cells per batch keep more lanes busy (500x500x40
 // cells, n = 10, six workgroups: 4.63 ms per step against 4.90 with 32 cells)
 constexpr int tile_cell_max(int cap, int algo) {
@@ -799,13 +799,14 @@ constexpr int tile_cell_max(int cap, int algo) {
 // floats of the ghost region: extended ghosts (per coordinate), SRD staged
 // wall-cell ghost velocities (slightly less than half the capacity for the
 // 384-particle tile, to stay within 26 LDS granules).
-// Extended 512-particle tiles keep 192 ghost slots: a 48-cell batch rarely
+// Extended 512-particle tiles keep 112 ghost slots: a 48-cell batch rarely
 // holds more than a few wall cells, a batch whose ghosts would overflow ends
-// early, and the smaller region lets seven workgroups share a CU instead of
-// six (500x500x40 cells, n = 10: 4.63 -> 4.16 ms per step).  A region must
-// hold one wall cell's n ghosts (choose_tiling).
+// early, and the smaller region lets eight workgroups share a CU instead of
+// six (500x500x40 cells, n = 10: 256 slots 4.63 ms per step, 192 slots and
+// seven workgroups 4.16, 112 slots and eight 3.79).  A region must hold one
+// wall cell's n ghosts (choose_tiling).
 constexpr int ghost_floats(int cap, int algo) {
-    return algo == 0 && cap == 384 ? cap / 2 - 32 : (algo == 1 && cap == 512 ? 192 : cap / 2);
+    return algo == 0 && cap == 384 ? cap / 2 - 32 : (algo == 1 && cap == 512 ? 112 : cap / 2);
 }
 
 // threads of the collide workgroup for an LDS capacity: more, smaller
