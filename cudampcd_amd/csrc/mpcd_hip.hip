@@ -1585,12 +1585,14 @@ __global__ __launch_bounds__(WG) void k_layers(Sys sys, const float* __restrict_
 // mu = R^T lam + (I - R^T) (sum of the table's lam) / total, every other
 // particle keeps lam, and each is scattered back to its source slot (negated
 // after a bounce); dL/dg, dL/dalpha and dL/df accumulate in FP64.
-// 256 cells per block.  The block's particles' adjoint values are staged in
+// ADJ_CELLS cells per block (half the lanes take a cell in the cell stage,
+// all of them a particle; the 25 KB of LDS let six blocks share a CU).  The block's particles' adjoint values are staged in
 // LDS with coalesced loads; one lane per cell forms the cell's table sums (in
 // slot order, FP64) and its rotation of the mean; then every lane updates one
 // particle.  A block whose cells hold more particles than the staging buffer
 // runs the same arithmetic one lane per cell, reading memory directly.
-constexpr uint32_t ADJ_STAGE = 3072;
+constexpr uint32_t ADJ_CELLS = 128;    // cells per block
+constexpr uint32_t ADJ_STAGE = 1536;    // particles staged per block (12 n cells' worth at n = 10)
 
 struct AdjIn {
     const uint32_t* off;
@@ -1694,15 +1696,16 @@ __global__ __launch_bounds__(WG) void k_adjoint_step(Sys sys, AdjIn A, double* _
                                                      double* __restrict__ gf) {
     __shared__ float sl[3][ADJ_STAGE];   // adjoint (plus loss seed) of the block's particles
     __shared__ uint8_t scell[ADJ_STAGE]; // block-local cell of each staged particle
-    __shared__ uint32_t soff[WG + 1];
-    __shared__ uint32_t sreal[WG];       // table particles of a colliding cell, else 0
-    __shared__ float scell_v[9][WG];     // axis, cell mean u, (I - R^T) applied to the table mean
+    __shared__ uint32_t soff[ADJ_CELLS + 1];
+    __shared__ uint32_t sreal[ADJ_CELLS];      // table particles of a colliding cell, else 0
+    __shared__ float scell_v[9][ADJ_CELLS];    // axis, cell mean u, (I - R^T) applied to the table mean
     __shared__ double red[2][WG / 64];
     KParams const P = sys.P[0];
     uint64_t const N = sys.NT, C = sys.CT;
     uint32_t const tid = threadIdx.x;
-    uint32_t const k0 = blockIdx.x * WG;
-    uint32_t const ncell = min(static_cast<uint32_t>(WG), P.C - k0);   // k0 < P.C for every launched block
+    static_assert(ADJ_CELLS <= WG && ADJ_CELLS <= 256, "one lane per cell, cell index in a byte");
+    uint32_t const k0 = blockIdx.x * ADJ_CELLS;
+    uint32_t const ncell = min(ADJ_CELLS, P.C - k0);   // k0 < P.C for every launched block
     for (uint32_t k = tid; k <= ncell; k += WG) soff[k] = A.off[k0 + k];
     __syncthreads();
     uint32_t const pa = soff[0], m = soff[ncell] - pa;
@@ -2471,7 +2474,7 @@ void adj_reserve(mpcd_hip_s* h, uint32_t steps, uint32_t K) {
         dalloc(a.tot, uint64_t(K) * C, "alloc tape");
         dalloc(a.lam[0], 3 * N, "alloc adjoint");
         dalloc(a.lam[1], 3 * N, "alloc adjoint");
-        dalloc(a.part, 2 * ((C + WG - 1) / WG), "alloc gradient partials");
+        dalloc(a.part, 2 * ((C + ADJ_CELLS - 1) / ADJ_CELLS), "alloc gradient partials");
         dalloc(a.gf, C, "alloc force gradient");
     }
     a.K = K;
@@ -2621,7 +2624,7 @@ ProfileResult profile_run(mpcd_hip_s* h, int steps, int K, float drag, float ang
         if (Kc) {
             // reverse pass, one checkpoint segment at a time (recomputed with the tape)
             ck(hipMemcpy(a.w, w.data(), 8ull * Z, hipMemcpyHostToDevice), "upload layer weights");
-            uint32_t const gb = static_cast<uint32_t>((C + WG - 1) / WG);
+            uint32_t const gb = static_cast<uint32_t>((C + ADJ_CELLS - 1) / ADJ_CELLS);
             ck(hipMemsetAsync(a.lam[0], 0, 12 * N, h->stream), "clear");
             ck(hipMemsetAsync(a.part, 0, 16ull * gb, h->stream), "clear");
             ck(hipMemsetAsync(a.gf, 0, 8 * C, h->stream), "clear");
