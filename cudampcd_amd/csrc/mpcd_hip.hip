@@ -1585,14 +1585,16 @@ __global__ __launch_bounds__(WG) void k_layers(Sys sys, const float* __restrict_
 // mu = R^T lam + (I - R^T) (sum of the table's lam) / total, every other
 // particle keeps lam, and each is scattered back to its source slot (negated
 // after a bounce); dL/dg, dL/dalpha and dL/df accumulate in FP64.
-// ADJ_CELLS cells per block (half the lanes take a cell in the cell stage,
-// all of them a particle; the 25 KB of LDS let six blocks share a CU).  The block's particles' adjoint values are staged in
+// ADJ_CELLS cells per block (a quarter of the lanes take a cell in the cell
+// stage, all of them a particle; about 13 KB of LDS, eight blocks per CU:
+// 43 us per step at 64x64x52 cells, n = 10, against 48 with 128 cells per
+// block and 60 with 256).  The block's particles' adjoint values are staged in
 // LDS with coalesced loads; one lane per cell forms the cell's table sums (in
 // slot order, FP64) and its rotation of the mean; then every lane updates one
 // particle.  A block whose cells hold more particles than the staging buffer
 // runs the same arithmetic one lane per cell, reading memory directly.
-constexpr uint32_t ADJ_CELLS = 128;    // cells per block
-constexpr uint32_t ADJ_STAGE = 1536;    // particles staged per block (12 n cells' worth at n = 10)
+constexpr uint32_t ADJ_CELLS = 64;     // cells per block
+constexpr uint32_t ADJ_STAGE = 768;     // particles staged per block (12 n cells' worth at n = 10)
 
 struct AdjIn {
     const uint32_t* off;
