@@ -1793,6 +1793,17 @@ __global__ __launch_bounds__(WG) void k_adjoint_step(Sys sys, AdjIn A, double* _
     }
 }
 
+// largest per-cell count (the capacity a batch needs for the fullest cell)
+__global__ __launch_bounds__(WG) void k_count_max(const uint32_t* __restrict__ count, uint32_t C,
+                                                  uint32_t* __restrict__ out) {
+    uint32_t m = 0;
+    for (uint64_t i = blockIdx.x * static_cast<uint64_t>(WG) + threadIdx.x; i < C;
+         i += static_cast<uint64_t>(gridDim.x) * WG)
+        m = max(m, count[i]);
+    for (int d = 32; d >= 1; d >>= 1) m = max(m, static_cast<uint32_t>(__shfl_down(m, d, 64)));
+    if ((threadIdx.x & 63u) == 0) atomicMax(out, m);
+}
+
 // fold one sampled step into the FP64 accumulators (per reference cell)
 __global__ __launch_bounds__(WG) void k_sample_fold(uint32_t C, uint32_t* __restrict__ cnt,
                                                     unsigned long long* __restrict__ sum, double* __restrict__ acc,
@@ -1892,6 +1903,7 @@ struct mpcd_hip_s {
     uint32_t* partial = nullptr;
     uint32_t nb = 0;
     uint32_t* status = nullptr;
+    uint32_t* cmax = nullptr;        // largest per-cell count (k_count_max)
     uint32_t* perm_rec = nullptr;
     bool record = false, recorded = false;
     float* axis = nullptr;           // SRD per-cell axes of the current step
@@ -2182,6 +2194,24 @@ void launch_keys(mpcd_hip_s* h) {
     h->sorted_valid = false;
 }
 
+// After the cells were counted from scratch (initial state, set_particles,
+// set_step_index): a cell fuller than one LDS batch would be streamed without
+// collision and fail the step, so the batch capacity is raised to fit it when
+// a compiled one does (up to 2048 particles; never lowered again here).
+void fit_capacity(mpcd_hip_s* h) {
+    ck(hipMemsetAsync(h->cmax, 0, 4, h->stream), "clear");
+    uint32_t const blocks = static_cast<uint32_t>(std::max<uint64_t>(1, std::min<uint64_t>((h->C + WG - 1) / WG, 1024)));
+    hipLaunchKernelGGL(k_count_max, dim3(blocks), dim3(WG), 0, h->stream, h->count, static_cast<uint32_t>(h->C), h->cmax);
+    ck(hipGetLastError(), "k_count_max");
+    uint32_t m = 0;
+    ck(hipMemcpyAsync(&m, h->cmax, 4, hipMemcpyDeviceToHost, h->stream), "count max");
+    ck(hipStreamSynchronize(h->stream), "count max");
+    if (m > h->lds_cap && m <= TILE_CAP_MAX) {
+        choose_tiling(h, h->tile_req, m);
+        layout(h);
+    }
+}
+
 void launch_scan(mpcd_hip_s* h) {
     uint32_t const C = static_cast<uint32_t>(h->C);
     hipLaunchKernelGGL(k_scan_reduce, dim3(h->nb), dim3(WG), 0, h->stream, h->count, C, h->partial);
@@ -2321,6 +2351,7 @@ void run_steps(mpcd_hip_s* h, int n_steps, bool sample) {
         if (!h->keys_valid) {
             launch_keys(h);
             h->launches += 2;
+            fit_capacity(h);
         }
         bool const first = sample && h->sampling_done;
         bool const graphed = h->use_graph && !h->timing && !h->record && !first && !h->sorted_valid;
@@ -2422,6 +2453,7 @@ void destroy(mpcd_hip_s* h) {
     for (auto* p : {h->st[0], h->st[1]}) if (p) (void)hipFree(p);
     for (void* p : {static_cast<void*>(h->perm), static_cast<void*>(h->keys), static_cast<void*>(h->cursor), static_cast<void*>(h->count),
                     static_cast<void*>(h->offset), static_cast<void*>(h->partial), static_cast<void*>(h->status),
+                    static_cast<void*>(h->cmax),
                     static_cast<void*>(h->perm_rec), static_cast<void*>(h->scnt), static_cast<void*>(h->ssum),
                     static_cast<void*>(h->axis), static_cast<void*>(h->wgcnt), static_cast<void*>(h->wgvel),
                     static_cast<void*>(h->acc), static_cast<void*>(h->d_kp), static_cast<void*>(h->d_blk),
@@ -2757,6 +2789,7 @@ int create_impl(const mpcd_params_t* params, uint32_t n_systems, mpcd_hip_t** ou
         h->nb = static_cast<uint32_t>((h->C + SCAN_CHUNK - 1) / SCAN_CHUNK);
         ck(hipMalloc(&h->partial, 4 * std::max<uint32_t>(h->nb, 1)), "alloc scan partials");
         ck(hipMalloc(&h->status, 4), "alloc status");
+        ck(hipMalloc(&h->cmax, 4), "alloc count max");
         ck(hipMemsetAsync(h->status, 0, 4, h->stream), "clear status");
         ck(hipMemsetAsync(h->count, 0, 4 * h->C, h->stream), "clear counts");
         if (h->blocks_part)

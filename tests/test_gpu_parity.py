@@ -283,6 +283,31 @@ def test_oversized_cell_is_reported(pympcd):
 
 
 @pytest.mark.parametrize("algorithm", ["srd", "extended"])
+def test_full_cell_raises_capacity_and_matches_oracle(pympcd, algorithm):
+    """A cell fuller than the default LDS batch (600 particles at one point,
+    one velocity, so they share a cell whatever the grid shift): the batch
+    capacity is raised to fit it and the step matches the oracle, whose
+    table takes the first 4n particles of the cell and un-shifts the rest."""
+    p, sim, h = _sim(pympcd, ((6, 5, 8), (1, 1, 0), 10, algorithm))
+    cap0 = h.tiling()[1]
+    pos, vel = sim.get_particle_positions().copy(), sim.get_particle_velocities().copy()
+    pos[:600] = (0.3, -0.6, -3.6)      # the fluid layer next to the lower wall: ghosts join for some shifts
+    vel[:600] = (0.4, -0.1, 0.2)
+    sim.set_particles(pos, vel)
+    h.record_permutation(True)
+    o = _oracle(p)
+    o.set_particles(pos, vel)
+    o.set_step(h.step_index)
+    sim.step(1)
+    o.step(1)
+    assert h.tiling()[1] > cap0 and h.tiling()[1] >= 600
+    perm = h.last_permutation()
+    gpos, gvel = sim.get_particle_positions(), sim.get_particle_velocities()
+    opos, ovel = o.particles()
+    assert np.array_equal(gpos, opos[perm]) and np.array_equal(gvel, ovel[perm])
+
+
+@pytest.mark.parametrize("algorithm", ["srd", "extended"])
 def test_one_call_equals_single_steps(pympcd, algorithm):
     """Inside one step() call the intermediate states are kept streamed (the next
     gather skips re-streaming); the result must equal single-step calls bit for bit."""
