@@ -79,9 +79,13 @@ def test_forward_loss_matches_oracle_and_leaves_state(pympcd):
     assert r2.loss == r.loss
 
 
-def test_gradient_matches_frozen_model_finite_differences(pympcd):
+# n = 8: the reverse step stages each block's 64 cells in LDS; n = 20: the
+# blocks hold more particles than the staging buffer (768) and take the
+# one-lane-per-cell path
+@pytest.mark.parametrize("n", [8, 20], ids=["staged", "per-cell"])
+def test_gradient_matches_frozen_model_finite_differences(pympcd, n):
     """Adjoint gradient vs central differences of the oracle's float64 frozen model."""
-    p, sim, target = _setup(pympcd)
+    p, sim, target = _setup(pympcd, n=n)
     Z = stored_volume(p)[2]
     C = _cells(p)
     r = adjoint.profile_loss_grad(sim, STEPS, 5, p.drag, p.rotation_angle, target)
