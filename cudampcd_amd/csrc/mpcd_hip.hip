@@ -846,6 +846,14 @@ struct Tile {
     uint32_t nwall;                  // SRD: wall cells of the batch staged so far
 };
 
+// sampled steps: per-batch LDS bins of the mean-field accumulators (count and
+// the three fixed-point velocity sums of each cell, indexed by cell key)
+template <uint32_t NB>
+struct SampleBins {
+    unsigned long long sum[3][NB];
+    uint32_t cnt[NB];
+};
+
 // SRD per-cell stage of one batch (simulation_functions.cpp:133-182): one lane
 // per cell sums its particles in slot order and its ghosts in ghost order (the
 // reference's order).  Ghost counts and velocities of wall cells (drawn by
@@ -1189,6 +1197,7 @@ __global__ __launch_bounds__(tile_threads(CAP), tile_blocks(CAP, ALGO)) void k_c
     bool const in_streamed = (io & 1u) != 0, out_streamed = (io & 2u) != 0;
     constexpr uint32_t NT = tile_threads(CAP);
     __shared__ Tile<CAP, ALGO> s;
+    __shared__ SampleBins<SAMPLE ? 2 * (static_cast<uint32_t>(CAP) / 3) : 1u> sb;
     uint32_t const tile = xcd_tile(blockIdx.x, gridDim.x);
     uint32_t const sy = sys.ns > 1 ? sys.tile_sys[tile] : 0u;
     KParams const P = sys.P[sy];
@@ -1443,6 +1452,20 @@ __global__ __launch_bounds__(tile_threads(CAP), tile_blocks(CAP, ALGO)) void k_c
         uint32_t const smid = mid_stride(P, c0 + kb);
         uint32_t const wbm = wb0 - smid, wbp = wb0 + smid;
         for (uint32_t i = tid; i < 3 * W; i += NT) hist[i] = 0u;
+        // sampling: un-shifted cells lie in the batch's rows or one step back
+        // along each axis the shift is positive on, one step ahead where it is
+        // negative, so two windows (own rows, and the middle-axis neighbour on
+        // the shift's side) take their counts and fixed-point sums in LDS
+        float const shm = P.ord[1] == 0 ? P.shift[0] : (P.ord[1] == 1 ? P.shift[1] : P.shift[2]);
+        uint32_t const wbs = shm > 0.0f ? wbm : wbp;
+        if constexpr (SAMPLE) {
+            for (uint32_t i = tid; i < 2 * W; i += NT) {
+                sb.cnt[i] = 0u;
+                sb.sum[0][i] = 0ull;
+                sb.sum[1][i] = 0ull;
+                sb.sum[2][i] = 0ull;
+            }
+        }
         __syncthreads();
         // per-particle finish, write-out, next-step cell count, sampling
         for (uint32_t i0 = 0; i0 < m; i0 += NT) {
@@ -1482,14 +1505,26 @@ __global__ __launch_bounds__(tile_threads(CAP), tile_blocks(CAP, ALGO)) void k_c
                 }
                 uint64_t const o = static_cast<uint64_t>(gb) + i;
                 if (!out_streamed) store_rec(out, o, px, py, pz, vx, vy, vz);
-                if (SAMPLE) {
+                if constexpr (SAMPLE) {
                     uint32_t ix, iy, iz;
                     cell_of(P, px, py, pz, ix, iy, iz);
-                    uint64_t const rc = P.cb + ref_of(P, ix, iy, iz);
-                    atomicAdd(S.cnt + rc, 1u);
-                    atomicAdd(S.sum + 3ull * rc + 0, static_cast<unsigned long long>(static_cast<long long>(rint(static_cast<double>(vx) * FIX_SCALE))));
-                    atomicAdd(S.sum + 3ull * rc + 1, static_cast<unsigned long long>(static_cast<long long>(rint(static_cast<double>(vy) * FIX_SCALE))));
-                    atomicAdd(S.sum + 3ull * rc + 2, static_cast<unsigned long long>(static_cast<long long>(rint(static_cast<double>(vz) * FIX_SCALE))));
+                    uint32_t const ks = P.cb + key_of(P, ix, iy, iz);
+                    unsigned long long const fx = static_cast<unsigned long long>(static_cast<long long>(rint(static_cast<double>(vx) * FIX_SCALE)));
+                    unsigned long long const fy = static_cast<unsigned long long>(static_cast<long long>(rint(static_cast<double>(vy) * FIX_SCALE)));
+                    unsigned long long const fz = static_cast<unsigned long long>(static_cast<long long>(rint(static_cast<double>(vz) * FIX_SCALE)));
+                    uint32_t b = ks - wb0;
+                    if (b >= W) b = (ks - wbs < W) ? W + (ks - wbs) : ~0u;
+                    if (b != ~0u) {
+                        atomicAdd(&sb.cnt[b], 1u);
+                        atomicAdd(&sb.sum[0][b], fx);
+                        atomicAdd(&sb.sum[1][b], fy);
+                        atomicAdd(&sb.sum[2][b], fz);
+                    } else {
+                        atomicAdd(S.cnt + ks, 1u);
+                        atomicAdd(S.sum + 3ull * ks + 0, fx);
+                        atomicAdd(S.sum + 3ull * ks + 1, fy);
+                        atomicAdd(S.sum + 3ull * ks + 2, fz);
+                    }
                 }
                 stream_shift(P, P.shift_next, px, py, pz, vx, vy, vz);
                 if (out_streamed) store_rec(out, o, px, py, pz, vx, vy, vz);
@@ -1513,6 +1548,19 @@ __global__ __launch_bounds__(tile_threads(CAP), tile_blocks(CAP, ALGO)) void k_c
                 uint32_t const j = i < W ? 0u : (i < 2 * W ? 1u : 2u);
                 uint32_t const base = j == 0 ? wb0 : (j == 1 ? wbm : wbp);
                 atomicAdd(count_next + (base + (i - j * W)), c);
+            }
+        }
+        if constexpr (SAMPLE) {
+            for (uint32_t i = tid; i < 2 * W; i += NT) {
+                uint32_t const c = sb.cnt[i];
+                if (c) {
+                    // consecutive bins are consecutive keys: coalesced atomics
+                    uint64_t const rc = i < W ? wb0 + i : wbs + (i - W);
+                    atomicAdd(S.cnt + rc, c);
+                    atomicAdd(S.sum + 3ull * rc + 0, sb.sum[0][i]);
+                    atomicAdd(S.sum + 3ull * rc + 1, sb.sum[1][i]);
+                    atomicAdd(S.sum + 3ull * rc + 2, sb.sum[2][i]);
+                }
             }
         }
         __syncthreads();
@@ -1804,7 +1852,9 @@ __global__ __launch_bounds__(WG) void k_count_max(const uint32_t* __restrict__ c
     if ((threadIdx.x & 63u) == 0) atomicMax(out, m);
 }
 
-// fold one sampled step into the FP64 accumulators (per reference cell)
+// fold one sampled step into the FP64 accumulators; the per-step sums and the
+// accumulators are both indexed by global cell key (reference order is
+// restored where they leave the device)
 __global__ __launch_bounds__(WG) void k_sample_fold(uint32_t C, uint32_t* __restrict__ cnt,
                                                     unsigned long long* __restrict__ sum, double* __restrict__ acc,
                                                     int first) {
@@ -2807,13 +2857,33 @@ int create_impl(const mpcd_params_t* params, uint32_t n_systems, mpcd_hip_t** ou
     return 0;
 }
 
-// FP64 accumulators of cells [first, first + n) scaled by 1/sample_counter
-// (the reference divides the sum of S samples by S-1, cpu_backend.cpp:85-114)
+// global cell key -> global reference cell index (system's first cell +
+// x + X (y + Y z)); both orders cover the same range [cb, cb + C) per system
+std::vector<uint32_t> ref_of_keys(mpcd_hip_t* h) {
+    std::vector<uint32_t> r(h->C);
+    for (KParams const& k : h->kp)
+        for (uint32_t key = 0; key < k.C; ++key) {
+            uint32_t ix, iy, iz;
+            decode_key(k, key, ix, iy, iz);
+            r[k.cb + key] = k.cb + ref_of(k, ix, iy, iz);
+        }
+    return r;
+}
+
+// FP64 accumulators of cells [first, first + n) in reference order (one whole
+// system or the whole grid) scaled by 1/sample_counter (the reference divides
+// the sum of S samples by S-1, cpu_backend.cpp:85-114)
 void sample_mean_impl(mpcd_hip_t* h, uint64_t first, uint64_t n, float* d32, float* v32, double* d64, double* v64) {
     std::vector<double> acc(4 * n, 0.0);
     if (h->acc) {
+        std::vector<double> by_key(4 * n);
         ck(hipStreamSynchronize(h->stream), "sync");
-        ck(hipMemcpy(acc.data(), h->acc + 4 * first, 8 * 4 * n, hipMemcpyDeviceToHost), "download accumulators");
+        ck(hipMemcpy(by_key.data(), h->acc + 4 * first, 8 * 4 * n, hipMemcpyDeviceToHost), "download accumulators");
+        std::vector<uint32_t> const ref = ref_of_keys(h);
+        for (uint64_t g = 0; g < n; ++g) {
+            uint64_t const c = ref[first + g] - first;
+            for (int k = 0; k < 4; ++k) acc[4 * c + k] = by_key[4 * g + k];
+        }
     }
     double const inverse = 1.0 / static_cast<double>(h->sample_counter);
     for (uint64_t c = 0; c < n; ++c) {
@@ -2947,14 +3017,8 @@ int mpcd_hip_peek_sort(mpcd_hip_t* h, uint32_t* cell_of_particle, uint32_t* coun
             ck(hipMemcpy(pm.data(), h->perm, 4 * h->N, hipMemcpyDeviceToHost), "download perm");
         }
         ck(hipMemcpy(off.data(), h->offset, 4 * (h->C + 1), hipMemcpyDeviceToHost), "download offsets");
-        // global key -> global reference cell index (system's first cell + x + X (y + Y z))
-        std::vector<uint32_t> ref_of_key(h->C), key_of_ref(h->C);
-        for (KParams const& k : h->kp)
-            for (uint32_t key = 0; key < k.C; ++key) {
-                uint32_t ix, iy, iz;
-                decode_key(k, key, ix, iy, iz);
-                ref_of_key[k.cb + key] = k.cb + ref_of(k, ix, iy, iz);
-            }
+        std::vector<uint32_t> const ref_of_key = ref_of_keys(h);
+        std::vector<uint32_t> key_of_ref(h->C);
         for (uint32_t key = 0; key < h->C; ++key) key_of_ref[ref_of_key[key]] = key;
         if (cell_of_particle)
             for (uint64_t j = 0; j < h->N; ++j) cell_of_particle[j] = ref_of_key[kr[j]];
@@ -3095,8 +3159,12 @@ int mpcd_hip_get_sampling_state(mpcd_hip_t* h, double* acc, size_t n_cells, uint
     return guarded(h, [&] {
         if (n_cells != h->C) throw HipError{"cell array size does not match the grid"};
         if (h->acc) {
+            std::vector<double> by_key(4 * h->C);
             ck(hipStreamSynchronize(h->stream), "sync");
-            ck(hipMemcpy(acc, h->acc, 32 * h->C, hipMemcpyDeviceToHost), "download accumulators");
+            ck(hipMemcpy(by_key.data(), h->acc, 32 * h->C, hipMemcpyDeviceToHost), "download accumulators");
+            std::vector<uint32_t> const ref = ref_of_keys(h);
+            for (uint64_t g = 0; g < h->C; ++g)
+                for (int k = 0; k < 4; ++k) acc[4 * uint64_t(ref[g]) + k] = by_key[4 * g + k];
         } else {
             std::fill(acc, acc + 4 * h->C, 0.0);
         }
@@ -3109,8 +3177,12 @@ int mpcd_hip_set_sampling_state(mpcd_hip_t* h, const double* acc, size_t n_cells
     return guarded(h, [&] {
         if (n_cells != h->C) throw HipError{"cell array size does not match the grid"};
         ensure_sampling(h);
+        std::vector<double> by_key(4 * h->C);
+        std::vector<uint32_t> const ref = ref_of_keys(h);
+        for (uint64_t g = 0; g < h->C; ++g)
+            for (int k = 0; k < 4; ++k) by_key[4 * g + k] = acc[4 * uint64_t(ref[g]) + k];
         ck(hipStreamSynchronize(h->stream), "sync");
-        ck(hipMemcpy(h->acc, acc, 32 * h->C, hipMemcpyHostToDevice), "upload accumulators");
+        ck(hipMemcpy(h->acc, by_key.data(), 32 * h->C, hipMemcpyHostToDevice), "upload accumulators");
         h->sample_counter = sample_counter;
         h->sampling_done = active == 0;
     });

@@ -170,9 +170,16 @@ def test_step_matches_oracle(pympcd, cfg):
     assert bitwise == total
 
 
-@pytest.mark.parametrize("algorithm", ["srd", "extended"])
-def test_sampling_matches_oracle(pympcd, algorithm):
-    cfg = ((6, 5, 8), (1, 1, 0), 10, algorithm)
+# the small grid fits one tile; the long ones span many tiles and several
+# 128-row blocks of the cell key, so batches accumulate their samples in LDS
+# windows and send cells outside them (block edges, periodic wrap) to memory
+SAMPLING_CFGS = [((6, 5, 8), (1, 1, 0), 10, "srd"), ((6, 5, 8), (1, 1, 0), 10, "extended"),
+                 ((6, 4, 300), (1, 1, 0), 5, "srd"), ((300, 5, 6), (1, 1, 1), 4, "srd"),
+                 ((5, 4, 260), (1, 1, 0), 5, "extended")]
+
+
+@pytest.mark.parametrize("cfg", SAMPLING_CFGS, ids=lambda c: f"{c[3]}-{'x'.join(map(str, c[0]))}")
+def test_sampling_matches_oracle(pympcd, cfg):
     p, sim, h = _sim(pympcd, cfg, drag=0.01)
     sim.step(2)
     o = _oracle(p)
