@@ -848,6 +848,9 @@ struct Tile {
 
 // sampled steps: per-batch LDS bins of the mean-field accumulators (count and
 // the three fixed-point velocity sums of each cell, indexed by cell key)
+// keys per sampling window: the batch's cells and a margin for the step back
+// or ahead along the two faster internal axes (cells outside go to memory)
+__host__ __device__ constexpr uint32_t sample_window(int cap) { return static_cast<uint32_t>(cap) / 5; }
 template <uint32_t NB>
 struct SampleBins {
     unsigned long long sum[3][NB];
@@ -1197,7 +1200,8 @@ __global__ __launch_bounds__(tile_threads(CAP), tile_blocks(CAP, ALGO)) void k_c
     bool const in_streamed = (io & 1u) != 0, out_streamed = (io & 2u) != 0;
     constexpr uint32_t NT = tile_threads(CAP);
     __shared__ Tile<CAP, ALGO> s;
-    __shared__ SampleBins<SAMPLE ? 2 * (static_cast<uint32_t>(CAP) / 3) : 1u> sb;
+    constexpr uint32_t WS = sample_window(CAP);
+    __shared__ SampleBins<SAMPLE ? 2 * WS : 1u> sb;
     uint32_t const tile = xcd_tile(blockIdx.x, gridDim.x);
     uint32_t const sy = sys.ns > 1 ? sys.tile_sys[tile] : 0u;
     KParams const P = sys.P[sy];
@@ -1452,14 +1456,21 @@ __global__ __launch_bounds__(tile_threads(CAP), tile_blocks(CAP, ALGO)) void k_c
         uint32_t const smid = mid_stride(P, c0 + kb);
         uint32_t const wbm = wb0 - smid, wbp = wb0 + smid;
         for (uint32_t i = tid; i < 3 * W; i += NT) hist[i] = 0u;
-        // sampling: un-shifted cells lie in the batch's rows or one step back
+        // sampling: un-shifted cells lie in the batch's cells or one step back
         // along each axis the shift is positive on, one step ahead where it is
-        // negative, so two windows (own rows, and the middle-axis neighbour on
-        // the shift's side) take their counts and fixed-point sums in LDS
+        // negative, so two windows (the batch's keys widened on the shift's side
+        // by one cell along the fastest axis and one row (D0 keys) along the
+        // slowest, and the same keys one middle-axis step away on the shift's
+        // side) take their counts and fixed-point sums in LDS
+        float const sh0 = P.ord[0] == 0 ? P.shift[0] : (P.ord[0] == 1 ? P.shift[1] : P.shift[2]);
         float const shm = P.ord[1] == 0 ? P.shift[0] : (P.ord[1] == 1 ? P.shift[1] : P.shift[2]);
-        uint32_t const wbs = shm > 0.0f ? wbm : wbp;
+        float const sh2 = P.ord[2] == 0 ? P.shift[0] : (P.ord[2] == 1 ? P.shift[1] : P.shift[2]);
+        uint32_t const lo_off = (sh0 > 0.0f ? 1u : 0u) + (sh2 > 0.0f ? P.D0 : 0u);   // keys below the batch
+        uint32_t const need = span + 1u + P.D0;
+        uint32_t const ws0 = g0 + kb - lo_off - (need < WS ? (WS - need) / 2 : 0u);
+        uint32_t const wss = shm > 0.0f ? ws0 - smid : ws0 + smid;
         if constexpr (SAMPLE) {
-            for (uint32_t i = tid; i < 2 * W; i += NT) {
+            for (uint32_t i = tid; i < 2 * WS; i += NT) {
                 sb.cnt[i] = 0u;
                 sb.sum[0][i] = 0ull;
                 sb.sum[1][i] = 0ull;
@@ -1512,8 +1523,8 @@ __global__ __launch_bounds__(tile_threads(CAP), tile_blocks(CAP, ALGO)) void k_c
                     unsigned long long const fx = static_cast<unsigned long long>(static_cast<long long>(rint(static_cast<double>(vx) * FIX_SCALE)));
                     unsigned long long const fy = static_cast<unsigned long long>(static_cast<long long>(rint(static_cast<double>(vy) * FIX_SCALE)));
                     unsigned long long const fz = static_cast<unsigned long long>(static_cast<long long>(rint(static_cast<double>(vz) * FIX_SCALE)));
-                    uint32_t b = ks - wb0;
-                    if (b >= W) b = (ks - wbs < W) ? W + (ks - wbs) : ~0u;
+                    uint32_t b = ks - ws0;
+                    if (b >= WS) b = (ks - wss < WS) ? WS + (ks - wss) : ~0u;
                     if (b != ~0u) {
                         atomicAdd(&sb.cnt[b], 1u);
                         atomicAdd(&sb.sum[0][b], fx);
@@ -1551,11 +1562,11 @@ __global__ __launch_bounds__(tile_threads(CAP), tile_blocks(CAP, ALGO)) void k_c
             }
         }
         if constexpr (SAMPLE) {
-            for (uint32_t i = tid; i < 2 * W; i += NT) {
+            for (uint32_t i = tid; i < 2 * WS; i += NT) {
                 uint32_t const c = sb.cnt[i];
                 if (c) {
                     // consecutive bins are consecutive keys: coalesced atomics
-                    uint64_t const rc = i < W ? wb0 + i : wbs + (i - W);
+                    uint64_t const rc = i < WS ? ws0 + i : wss + (i - WS);
                     atomicAdd(S.cnt + rc, c);
                     atomicAdd(S.sum + 3ull * rc + 0, sb.sum[0][i]);
                     atomicAdd(S.sum + 3ull * rc + 1, sb.sum[1][i]);
