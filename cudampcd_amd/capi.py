@@ -173,6 +173,10 @@ class Handle:
         self._check(self.L.mpcd_hip_system_info(self.h, s, *[ctypes.byref(x) for x in v]))
         return tuple(int(x.value) for x in v)
 
+    def reset_sampling(self):
+        """End the running average: the next sampled step starts a new one."""
+        self._check(self.L.mpcd_hip_reset_sampling(self.h))
+
     def sampling_state(self):
         """(accumulators (C, 4) float64, samples after the first, average running)."""
         acc = np.empty((self.C, 4), np.float64)
