@@ -138,6 +138,8 @@ def main():
     ap.add_argument("--seed", type=int, default=None, help="default 0x5EED0000 + config number")
     ap.add_argument("--tiling", default="",
                     help="cells_per_tile,lds_capacity for k_collide (results do not depend on it)")
+    ap.add_argument("--sample", action="store_true",
+                    help="time step_and_sample (a mean-field sample every step) instead of step")
     ap.add_argument("--copy-gb", type=float, default=0.0,
                     help="only run the plain copy kernel over this many GB (profiling reference) and exit")
     args = ap.parse_args()
@@ -176,10 +178,14 @@ def main():
 
     print(f"[bench] warmup {args.warmup} steps", file=sys.stderr, flush=True)
     sim.step(args.warmup)
+    run = sim.step
+    if args.sample:
+        sim.step_and_sample(2)      # the first sample of an average runs eagerly
+        run = sim.step_and_sample
     h.enable_timing(True)
-    print(f"[bench] timing {args.steps} steps", file=sys.stderr, flush=True)
+    print(f"[bench] timing {args.steps} {'sampled ' if args.sample else ''}steps", file=sys.stderr, flush=True)
     t0 = time.perf_counter()
-    sim.step(args.steps)            # returns after the device finished
+    run(args.steps)                 # returns after the device finished
     wall = time.perf_counter() - t0
     collide_ms, total_ms, launches = h.last_timing()
 
@@ -195,7 +201,8 @@ def main():
     traffic = load_traffic(args.config)
     roofline = {
         "bound": "hbm",
-        "kernel": "k_collide (gather + stream + sort-in-cell + collide + next-step histogram)",
+        "kernel": "k_collide (gather + stream + sort-in-cell + collide + next-step histogram"
+                  + (" + mean-field sums)" if args.sample else ")"),
         "achieved": round(achieved, 1),
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
@@ -224,12 +231,13 @@ def main():
         "ms_per_step": round(ms_per_step, 4),
         "higher_is_better": True,
         "scaling": "weak",
-        "vs_baseline": round(value / REFERENCE_PSTEPS, 3) if args.config == "c2" else None,
+        "vs_baseline": round(value / REFERENCE_PSTEPS, 3) if args.config == "c2" and not args.sample else None,
         "dtype": "f32",
         "data": "synthetic (seeded uniform positions, Maxwell-Boltzmann velocities)",
         "config": {"workload": cfg["workload"], "cells": [int(v) for v in p.volume_size], "particles": N,
                    "algorithm": cfg["algorithm"], "n": cfg["n"], "delta_t": 0.02, "drag": 0.001,
-                   "tiling": list(h.tiling()), "parallelism": "single GPU"},
+                   "tiling": list(h.tiling()), "parallelism": "single GPU",
+                   "sampling": "a mean-field sample every step (step_and_sample)" if args.sample else "none (step)"},
         "roofline": roofline,
         "step_roofline": step_roofline,
         "device_ms": {"collide_total": round(collide_ms, 3), "step_total": round(total_ms, 3), "launches": launches},
