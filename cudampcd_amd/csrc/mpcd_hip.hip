@@ -19,7 +19,9 @@
 //             the particles out in cell order (contiguous records), and already
 //             streams them with the next step's shift to count the next step's
 //             cells (fused histogram) and, when sampling, accumulates the mean
-//             fields in fixed point.
+//             fields in fixed point (LDS bins per batch, then per-step sums
+//             indexed by cell key; the host reorders them to reference order
+//             where they leave the device).
 //
 // Particle state is one FP32 record per particle (x, y, z, vx, vy, vz: 24 B),
 // double-buffered; the collide launch reads one buffer and writes the other.

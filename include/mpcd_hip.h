@@ -167,7 +167,8 @@ int mpcd_hip_system_sample_mean_f64(mpcd_hip_t* h, uint32_t system, double* dens
 int mpcd_hip_reset_sampling(mpcd_hip_t* h);
 /* Sampling state, for checkpoints (the reference's backup writer is a stub,
  * cuda_backend_impl.cu:77-79, cpu_backend.cpp:61-63): the FP64 accumulators
- * (4 per cell: density and velocity sums of the per-step means, n_cells = all
+ * (4 per cell in the order of the mean-field arrays, cell first_cell + x +
+ * X (y + Y z): density and velocity sums of the per-step means, n_cells = all
  * systems' cells), the samples after the first, and whether an average is
  * running.  Setting it resumes the average exactly where it was saved. */
 int mpcd_hip_get_sampling_state(mpcd_hip_t* h, double* acc, size_t n_cells, uint64_t* sample_counter, int* active);
