@@ -96,6 +96,13 @@ struct Sys {
     uint64_t NT;                 // particles of all systems (stride of the state planes)
 };
 
+// Cells larger than the LDS tile are listed after the status word (status[1]
+// count, then (system, local key) pairs) for the big-cell launch, which runs
+// BIG_BLOCKS workgroups over the list.  The list holds one entry per
+// N / (smallest tile capacity + 1) + 1, so it cannot overflow.
+constexpr uint32_t BIG_BLOCKS = 256;
+inline uint64_t big_list_words(uint64_t n) { return 2 + 2 * (n / 257 + 2); }
+
 // the 6 floats of particle i: x y z vx vy vz
 __host__ __device__ __forceinline__ uint64_t rec(uint64_t i) { return 6 * i; }
 
@@ -770,8 +777,9 @@ __global__ __launch_bounds__(WG) void k_cell_draws(Sys sys, float* __restrict__ 
 
 // one step forward of every system's step state: the shift of step t + 1 and
 // the next shift (the grid shift is a pure function of (seed, system, step))
-__global__ void k_advance(Sys sys) {
+__global__ void k_advance(Sys sys, uint32_t* __restrict__ big) {
     uint32_t const s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s == 0) big[0] = 0u;   // the step's big cells are done
     if (s >= sys.ns) return;
     KParams* const P = sys.P + s;
     uint32_t const step = P->step + 1;
@@ -850,6 +858,71 @@ struct Tile {
 
 // sampled steps: per-batch LDS bins of the mean-field accumulators (count and
 // the three fixed-point velocity sums of each cell, indexed by cell key)
+// Stable LSD radix sort of one cell's m source slots by one workgroup, for a
+// cell larger than the LDS tile (rare): 8-bit digits, `passes` of them (2
+// when every slot is below 2^16, else 4), from src through tmp into dst
+// (src -> tmp -> dst [-> tmp -> dst]).  Each chunk of NT slots is placed with
+// the digit's running base, the counts of that digit in lower waves, and the
+// lane's rank among the wave's lanes holding the same digit (eight ballots).
+// sm: 256 * (1 + NT / 64) words of LDS.
+template <uint32_t NT>
+__device__ void sort_slots(const uint32_t* src, uint32_t* tmp, uint32_t* dst, uint32_t m, int passes,
+                           uint32_t* sm) {
+    constexpr uint32_t NW = NT / 64;
+    static_assert(NT % 64 == 0, "whole waves");
+    uint32_t* const base = sm;
+    uint32_t* const wc = sm + 256;
+    uint32_t const tid = threadIdx.x, w = tid / 64;
+    for (int pass = 0; pass < passes; ++pass) {
+        const uint32_t* const a = pass == 0 ? src : ((pass & 1) ? tmp : dst);
+        uint32_t* const b = (pass & 1) ? dst : tmp;
+        uint32_t const sh = 8u * static_cast<uint32_t>(pass);
+        for (uint32_t i = tid; i < 256 * (NW + 1); i += NT) sm[i] = 0u;
+        __syncthreads();
+        for (uint32_t i = tid; i < m; i += NT) atomicAdd(&base[(a[i] >> sh) & 255u], 1u);
+        __syncthreads();
+        if (tid == 0) {
+            uint32_t acc = 0;
+            for (uint32_t d = 0; d < 256; ++d) {
+                uint32_t const c = base[d];
+                base[d] = acc;
+                acc += c;
+            }
+        }
+        __syncthreads();
+        for (uint32_t c0 = 0; c0 < m; c0 += NT) {   // uniform bound: whole waves take the ballots
+            uint32_t const i = c0 + tid;
+            bool const valid = i < m;
+            uint32_t const v = valid ? a[i] : 0u;
+            uint32_t const d = (v >> sh) & 255u;
+            uint64_t mask = __ballot(valid);
+            for (uint32_t bit = 0; bit < 8; ++bit) {
+                bool const on = ((d >> bit) & 1u) != 0;
+                uint64_t const bb = __ballot(valid && on);
+                mask &= on ? bb : ~bb;
+            }
+            uint32_t const rank = static_cast<uint32_t>(__popcll(mask & __lanemask_lt()));
+            if (valid && rank == 0) wc[w * 256 + d] = static_cast<uint32_t>(__popcll(mask));
+            __syncthreads();
+            if (valid) {
+                uint32_t pos = base[d] + rank;
+                for (uint32_t q = 0; q < w; ++q) pos += wc[q * 256 + d];
+                b[pos] = v;
+            }
+            __syncthreads();
+            for (uint32_t dd = tid; dd < 256; dd += NT) {
+                uint32_t t = 0;
+                for (uint32_t q = 0; q < NW; ++q) {
+                    t += wc[q * 256 + dd];
+                    wc[q * 256 + dd] = 0u;
+                }
+                base[dd] += t;
+            }
+            __syncthreads();
+        }
+    }
+}
+
 // keys per sampling window: the batch's cells and a margin for the step back
 // or ahead along the two faster internal axes (cells outside go to memory)
 __host__ __device__ constexpr uint32_t sample_window(int cap) { return static_cast<uint32_t>(cap) / 5; }
@@ -1187,14 +1260,14 @@ __device__ __forceinline__ void ext_finish(KParams const& P, Tile<CAP, 1>& s, ui
 template <int CAP>
 constexpr int gather_quota() { return CAP <= 384 ? 1 : 2; }
 
-template <int ALGO, bool SAMPLE, int CAP, bool ADJ = false>
-__global__ __launch_bounds__(tile_threads(CAP), tile_blocks(CAP, ALGO)) void k_collide(Sys sys, const float* __restrict__ in, float* __restrict__ out,
-                                                const uint32_t* __restrict__ perm, const uint32_t* __restrict__ offset,
-                                                uint32_t* __restrict__ count_next, uint32_t* __restrict__ keys,
-                                                uint32_t* __restrict__ perm_out, SampleBufs S,
-                                                uint32_t* __restrict__ status, float const* __restrict__ axis,
-                                                uint32_t const* __restrict__ gcnt, float const* __restrict__ gvel,
-                                                Tape T, uint32_t io) {
+template <int ALGO, bool SAMPLE, int CAP, bool ADJ, bool BIG>
+__device__ __forceinline__ void collide_tile(Sys const& sys, const float* __restrict__ in, float* __restrict__ out,
+                                             const uint32_t* __restrict__ perm, const uint32_t* __restrict__ offset,
+                                             uint32_t* __restrict__ count_next, uint32_t* __restrict__ keys,
+                                             uint32_t* __restrict__ perm_out, SampleBufs const& S,
+                                             uint32_t* __restrict__ status, float const* __restrict__ axis,
+                                             uint32_t const* __restrict__ gcnt, float const* __restrict__ gvel,
+                                             Tape const& T, uint32_t io, uint32_t tile) {
     // io bit 0: `in` holds this step's streamed state (the previous launch
     // wrote it), so the gather skips the streaming; bit 1: write the next
     // step's streamed state instead of the post-collision state.  Both forms
@@ -1204,15 +1277,16 @@ __global__ __launch_bounds__(tile_threads(CAP), tile_blocks(CAP, ALGO)) void k_c
     __shared__ Tile<CAP, ALGO> s;
     constexpr uint32_t WS = sample_window(CAP);
     __shared__ SampleBins<SAMPLE ? 2 * WS : 1u> sb;
-    uint32_t const tile = xcd_tile(blockIdx.x, gridDim.x);
-    uint32_t const sy = sys.ns > 1 ? sys.tile_sys[tile] : 0u;
+    // BIG: `tile` is an entry of the big-cell list, the cell as a one-cell tile
+    uint32_t* const big = status + 1;
+    uint32_t const sy = BIG ? big[1 + 2 * tile] : (sys.ns > 1 ? sys.tile_sys[tile] : 0u);
     KParams const P = sys.P[sy];
-    uint32_t const cap = CAP, TC = P.tile_cells;
+    uint32_t const cap = CAP, TC = BIG ? 1u : P.tile_cells;
     uint32_t const tid = threadIdx.x;
     uint64_t const N = sys.NT;     // stride of the state planes
     uint64_t const CT = sys.CT;    // stride of the axis planes
     // tile -> cells [c0, c0 + nc) of system sy (local keys); g0 = global key of c0
-    uint32_t const c0 = (tile - (sys.ns > 1 ? sys.tile0[sy] : 0u)) * TC;
+    uint32_t const c0 = BIG ? big[2 + 2 * tile] : (tile - (sys.ns > 1 ? sys.tile0[sy] : 0u)) * TC;
     if (c0 >= P.C) return;
     uint32_t const nc = min(TC, P.C - c0);
     uint32_t const g0 = P.cb + c0;
@@ -1246,20 +1320,19 @@ __global__ __launch_bounds__(tile_threads(CAP), tile_blocks(CAP, ALGO)) void k_c
         uint32_t const nfit = static_cast<uint32_t>(__syncthreads_count(fits));
         uint32_t const ke = kb + (nfit > 0 ? nfit : 1u);
         uint32_t const gb = s.off[kb];
-        uint32_t const m = s.off[ke] - gb;
-        if (m > cap) {
-            // A single cell larger than the LDS tile: its particles are moved
-            // through unchanged (streamed and un-shifted, no collision) so the
-            // state and the next step's cell counts stay consistent, and the
-            // step reports the condition as an error.
-            if (tid == 0) atomicOr(status, 1u);
-            for (uint32_t i0 = 0; i0 < m; i0 += NT) {
+        uint32_t m = s.off[ke] - gb;
+        // particles [i_begin, i_end) of the batch's range, source slots at
+        // srcs[gb + i], moved through without collision: streamed, un-shifted
+        // (simulation_functions.cpp:90-93), written, sampled, streamed with the
+        // next shift and counted for the next step
+        auto pass_through = [&](const uint32_t* srcs, uint32_t i_begin, uint32_t i_end) {
+            for (uint32_t i0 = i_begin; i0 < i_end; i0 += NT) {
                 uint32_t const i = i0 + tid;
-                bool const valid = i < m;
+                bool const valid = i < i_end;
                 uint32_t knext = 0;
                 if (valid) {
                     uint64_t const o = static_cast<uint64_t>(gb) + i;
-                    uint32_t const src = perm[o];
+                    uint32_t const src = srcs[o];
                     float v[6];
                     load_rec(in, src, v);
                     float px = v[0], py = v[1], pz = v[2], vx = v[3], vy = v[4], vz = v[5];
@@ -1269,21 +1342,65 @@ __global__ __launch_bounds__(tile_threads(CAP), tile_blocks(CAP, ALGO)) void k_c
                     pz = wrap1(pz - P.shift[2], P, 2);
                     if (!out_streamed) store_rec(out, o, px, py, pz, vx, vy, vz);
                     if (perm_out) perm_out[o] = src;
+                    uint32_t ix, iy, iz;
+                    if constexpr (SAMPLE) {
+                        cell_of(P, px, py, pz, ix, iy, iz);
+                        uint64_t const ks = P.cb + key_of(P, ix, iy, iz);
+                        atomicAdd(S.cnt + ks, 1u);
+                        atomicAdd(S.sum + 3ull * ks + 0, static_cast<unsigned long long>(static_cast<long long>(rint(static_cast<double>(vx) * FIX_SCALE))));
+                        atomicAdd(S.sum + 3ull * ks + 1, static_cast<unsigned long long>(static_cast<long long>(rint(static_cast<double>(vy) * FIX_SCALE))));
+                        atomicAdd(S.sum + 3ull * ks + 2, static_cast<unsigned long long>(static_cast<long long>(rint(static_cast<double>(vz) * FIX_SCALE))));
+                    }
                     stream_shift(P, P.shift_next, px, py, pz, vx, vy, vz);
                     if (out_streamed) store_rec(out, o, px, py, pz, vx, vy, vz);
-                    uint32_t ix, iy, iz;
                     cell_of(P, px, py, pz, ix, iy, iz);
                     knext = P.cb + key_of(P, ix, iy, iz);
                 }
                 wave_count(count_next, knext, valid);
                 if (valid) keys[static_cast<uint64_t>(gb) + i] = knext;
             }
+        };
+        if (m > cap && (ADJ || (BIG && P.cap > cap))) {
+            // A single cell larger than the LDS tile in a taped run, or whose
+            // collision table (4n) would not fit the tile either: its particles
+            // are moved through without collision so the state and the next
+            // step's cell counts stay consistent, and the step reports the
+            // condition as an error.
+            if (tid == 0) atomicOr(status, 1u);
+            pass_through(perm, 0u, m);
             __syncthreads();
             kb = ke;
             continue;
         }
+        if (!BIG && m > cap) {
+            // A single cell larger than the LDS tile: listed for the big-cell
+            // launch that follows
+            if (tid == 0) {
+                uint32_t const e = atomicAdd(big, 1u);
+                big[1 + 2 * e] = sy;
+                big[2 + 2 * e] = c0 + kb;
+            }
+            kb = ke;
+            continue;
+        }
+        // BIG: its source slots are sorted (into its share of the key array,
+        // through its share of the output state, both free until written
+        // below), its first P.cap particles collide as a one-cell batch, and
+        // the rest (the reference's table overflow) are moved through after it.
+        const uint32_t* psrc = perm;
+        uint32_t tail = 0;
+        if (BIG && m > cap) {
+            static_assert(6 * CAP >= 256 * (1 + NT / 64), "sort scratch fits the particle arrays");
+            sort_slots<NT>(perm + gb, reinterpret_cast<uint32_t*>(out) + rec(gb), keys + gb, m, N > 65536u ? 4 : 2,
+                           reinterpret_cast<uint32_t*>(&s.p[0][0]));
+            psrc = keys;
+            tail = m - P.cap;
+            m = P.cap;
+            if (tid == 0) s.off[ke] = gb + m;
+            __syncthreads();
+        }
         uint32_t const ncell = ke - kb;
-        for (uint32_t i = tid; i < m; i += NT) s.perm[i] = perm[gb + i];
+        for (uint32_t i = tid; i < m; i += NT) s.perm[i] = psrc[gb + i];
         if constexpr (ALGO == 0) {
             // per-cell draws: axis, and for wall cells the ghost count and a
             // slot in the staging region (simulation_functions.cpp:137-146)
@@ -1577,6 +1694,12 @@ __global__ __launch_bounds__(tile_threads(CAP), tile_blocks(CAP, ALGO)) void k_c
             }
         }
         __syncthreads();
+        if (BIG && tail) {
+            pass_through(psrc, m, m + tail);
+            __syncthreads();
+            if (tid == 0) s.off[ke] = gb + m + tail;
+            __syncthreads();
+        }
         kb = ke;
     }
 }
@@ -2295,12 +2418,41 @@ int launch_sort(mpcd_hip_s* h) {
 }
 
 
+// The collide launch: one workgroup per tile; BIG: the step's second launch,
+// BIG_BLOCKS workgroups over the cells the first listed as larger than their
+// tile (usually none: every workgroup returns at once)
+template <int ALGO, bool SAMPLE, int CAP, bool ADJ = false, bool BIG = false>
+__global__ __launch_bounds__(tile_threads(CAP), tile_blocks(CAP, ALGO)) void k_collide(
+    Sys sys, const float* __restrict__ in, float* __restrict__ out, const uint32_t* __restrict__ perm,
+    const uint32_t* __restrict__ offset, uint32_t* __restrict__ count_next, uint32_t* __restrict__ keys,
+    uint32_t* __restrict__ perm_out, SampleBufs S, uint32_t* __restrict__ status, float const* __restrict__ axis,
+    uint32_t const* __restrict__ gcnt, float const* __restrict__ gvel, Tape T, uint32_t io) {
+    if constexpr (BIG) {
+        uint32_t const n = status[1];
+        for (uint32_t e = blockIdx.x; e < n; e += gridDim.x) {
+            collide_tile<ALGO, SAMPLE, CAP, false, true>(sys, in, out, perm, offset, count_next, keys, perm_out, S,
+                                                         status, axis, gcnt, gvel, T, io, e);
+            __syncthreads();
+        }
+    } else {
+        collide_tile<ALGO, SAMPLE, CAP, ADJ, false>(sys, in, out, perm, offset, count_next, keys, perm_out, S, status,
+                                                    axis, gcnt, gvel, T, io, xcd_tile(blockIdx.x, gridDim.x));
+    }
+}
+
 template <int ALGO, bool SAMPLE, int CAP, bool ADJ = false>
 void launch_collide_t(mpcd_hip_s* h, uint32_t* rec, Tape const& T = Tape{}, uint32_t io = 0) {
     SampleBufs S{h->scnt, h->ssum};
     hipLaunchKernelGGL((k_collide<ALGO, SAMPLE, CAP, ADJ>), dim3(h->ntiles), dim3(tile_threads(CAP)), 0, h->stream, h->sys,
                        h->st[h->cur], h->st[1 - h->cur], h->perm, h->offset, h->count, h->keys, rec, S, h->status,
                        h->axis, h->wgcnt, h->wgvel, T, io);
+    if constexpr (!ADJ) {
+        // cells the first launch listed as larger than the tile (usually
+        // none: every workgroup returns at once)
+        hipLaunchKernelGGL((k_collide<ALGO, SAMPLE, CAP, false, true>), dim3(BIG_BLOCKS), dim3(tile_threads(CAP)), 0,
+                           h->stream, h->sys, h->st[h->cur], h->st[1 - h->cur], h->perm, h->offset, h->count, h->keys,
+                           rec, S, h->status, h->axis, h->wgcnt, h->wgvel, T, io);
+    }
 }
 
 template <int ALGO, bool SAMPLE>
@@ -2366,7 +2518,7 @@ int enqueue_step(mpcd_hip_s* h, bool sample, bool first, bool sort, uint32_t* re
         ck(hipGetLastError(), "k_sample_fold");
         launches += 1;
     }
-    hipLaunchKernelGGL(k_advance, dim3((h->ns + 63) / 64), dim3(64), 0, h->stream, h->sys);
+    hipLaunchKernelGGL(k_advance, dim3((h->ns + 63) / 64), dim3(64), 0, h->stream, h->sys, h->status + 1);
     ck(hipGetLastError(), "k_advance");
     return launches + 1;
 }
@@ -2463,7 +2615,8 @@ void run_steps(mpcd_hip_s* h, int n_steps, bool sample) {
         ck(hipMemsetAsync(h->status, 0, 4, h->stream), "status");
         ck(hipStreamSynchronize(h->stream), "status");
         throw HipError{"a cell holds more particles than one LDS tile (" + std::to_string(h->lds_cap) +
-                       "); raise the tile capacity with mpcd_hip_set_tiling"};
+                       ") and its collision table (4n) does not fit the tile either, or a taped run met such a "
+                       "cell; raise the tile capacity with mpcd_hip_set_tiling"};
     }
 }
 
@@ -2609,7 +2762,7 @@ void adj_step(mpcd_hip_s* h, float const* force, int j) {
         ck(hipMemcpyAsync(a.off + uint64_t(j) * (C + 1), h->offset, 4 * (C + 1), hipMemcpyDeviceToDevice, h->stream), "tape");
         ck(hipMemcpyAsync(a.ax + uint64_t(j) * 3 * C, h->axis, 12 * C, hipMemcpyDeviceToDevice, h->stream), "tape");
     }
-    hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, h->stream, h->sys);
+    hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, h->stream, h->sys, h->status + 1);
     h->cur = 1 - h->cur;
     h->step_index++;
     h->keys_valid = true;
@@ -2851,9 +3004,9 @@ int create_impl(const mpcd_params_t* params, uint32_t n_systems, mpcd_hip_t** ou
         }
         h->nb = static_cast<uint32_t>((h->C + SCAN_CHUNK - 1) / SCAN_CHUNK);
         ck(hipMalloc(&h->partial, 4 * std::max<uint32_t>(h->nb, 1)), "alloc scan partials");
-        ck(hipMalloc(&h->status, 4), "alloc status");
+        ck(hipMalloc(&h->status, 4 * big_list_words(h->N)), "alloc status and big-cell list");
         ck(hipMalloc(&h->cmax, 4), "alloc count max");
-        ck(hipMemsetAsync(h->status, 0, 4, h->stream), "clear status");
+        ck(hipMemsetAsync(h->status, 0, 4 * big_list_words(h->N), h->stream), "clear status");
         ck(hipMemsetAsync(h->count, 0, 4 * h->C, h->stream), "clear counts");
         if (h->blocks_part)
             hipLaunchKernelGGL(k_init, dim3(h->blocks_part), dim3(WG), 0, h->stream, h->sys, h->st[0], h->status);

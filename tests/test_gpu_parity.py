@@ -271,22 +271,43 @@ def test_set_get_roundtrip_and_errors(pympcd):
     assert np.isfinite(sim.get_particle_positions()).all()
 
 
-def test_oversized_cell_is_reported(pympcd):
-    """A cell beyond one LDS tile is streamed without collision and reported."""
-    p, sim, h = _sim(pympcd, ((10, 10, 10), (1, 1, 1), 4, "srd"))
-    cap = h.tiling()[1]
-    n = sim.n_particles
-    assert n > cap
-    pos = np.zeros((n, 3), np.float32)
-    vel = np.zeros((n, 3), np.float32)
+@pytest.mark.parametrize("algorithm", ["srd", "extended"])
+def test_cell_beyond_largest_tile_matches_oracle(pympcd, algorithm):
+    """2500 particles at one point (one cell, more than the largest LDS tile,
+    2048): the cell's source slots are sorted inside the collide kernel, its
+    first 4n collide and the rest are moved through un-shifted, as the
+    oracle's table does.  A plain step from the initial state, then two
+    sampled steps from the resulting states (the big cell now in cell order,
+    mostly not collided): states and the mean fields equal to the oracle's."""
+    p, sim, h = _sim(pympcd, ((10, 10, 10), (1, 1, 0), 5, algorithm))
+    assert sim.n_particles > 2500
+    pos, vel = sim.get_particle_positions().copy(), sim.get_particle_velocities().copy()
+    pos[:2500] = (0.3, -0.6, -3.6)      # next to the lower wall: ghosts join for some shifts
+    vel[:2500] = (0.4, -0.1, 0.2)
     sim.set_particles(pos, vel)
-    with pytest.raises(RuntimeError, match="LDS"):
-        sim.step(1)
-    # the state stays consistent: a normal configuration steps again
-    rng = np.random.default_rng(5)
-    sim.set_particles(rng.uniform(-5, 5, size=(n, 3)).astype(np.float32), rng.normal(size=(n, 3)).astype(np.float32))
-    sim.step(3)
-    assert np.isfinite(sim.get_particle_velocities()).all()
+    h.record_permutation(True)
+    o = _oracle(p)
+    for sampled in (False, True, True):
+        o.set_particles(sim.get_particle_positions(), sim.get_particle_velocities())
+        o.set_step(h.step_index)
+        if sampled:
+            sim.step_and_sample(1)
+            o.step_and_sample(1)
+        else:
+            sim.step(1)
+            o.step(1)
+        assert h.tiling()[1] <= 2048
+        perm = h.last_permutation()
+        assert np.array_equal(np.sort(perm), np.arange(sim.n_particles))
+        gpos, gvel = sim.get_particle_positions(), sim.get_particle_velocities()
+        opos, ovel = o.particles()
+        assert np.array_equal(gpos, opos[perm]) and np.array_equal(gvel, ovel[perm])
+    X, Y, Z = stored_volume(p)
+    d64, v64 = sim.get_mean_fields_f64()
+    od, ov = o.sample_mean_f64()
+    assert d64.max() >= 2000
+    assert np.array_equal(d64, od.reshape(Z, Y, X).transpose(2, 1, 0))
+    assert np.abs(v64 - ov.reshape(Z, Y, X, 3).transpose(2, 1, 0, 3)).max() < 1e-9
 
 
 @pytest.mark.parametrize("algorithm", ["srd", "extended"])
