@@ -78,7 +78,7 @@ def make_params(pympcd, fluid, n, algorithm, drag, seed, system_id=0):
 
 
 def algorithmic_bytes(n_particles: int, n_cells: int) -> dict:
-    """Compulsory HBM bytes of each kernel of one step."""
+    """Compulsory HBM bytes of each kernel of one step: every input read once and every output written once."""
     collide = 56 * n_particles + 12 * n_cells   # perm 4 + gathered state 24 + state out 24 + next key 4; offsets 4 + count RMW 8
     scan = 20 * n_cells                         # counts read twice, offsets + cursors written, counts cleared
     perm = 8 * n_particles + 8 * n_cells        # key 4 + perm slot 4; cursor RMW 8
@@ -146,7 +146,7 @@ def main():
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path "
+        print("bench.py measures one GPU: this library has no multi-GPU path "
               "(the step would shard as x-slabs with a particle halo exchange between neighbours)", file=sys.stderr)
         sys.exit(2)
 

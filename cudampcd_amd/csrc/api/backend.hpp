@@ -2,7 +2,7 @@
 //
 // Same interface as the reference's plugin boundary mpcd::Backend
 // (src/libmpcd/common/backend.hpp:11-31) and create_backend
-// (src/libmpcd/common/backend_factory.hpp:15-27); this build registers the
+// (src/libmpcd/common/backend_factory.hpp:15-27); this library registers the
 // "hip" backend.  Unknown names throw std::runtime_error("Unsupported backend type").
 #pragma once
 

@@ -26,7 +26,8 @@
 //     +,-,*,/,sqrt and integer bit operations only, so that a GPU using the
 //     same key layout produces the same bits.
 //
-// Parity status of each function (how it is pinned) is listed in DESIGN.md.
+// Each function is pinned in tests/test_oracle_pinning.py against the compiled
+// reference CPU backend, closed forms, invariants or brute force on tiny inputs.
 //
 // Every floating-point expression keeps the operation order of the reference
 // (vector_3d.hpp operator definitions), and the file is compiled with
