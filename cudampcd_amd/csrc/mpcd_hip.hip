@@ -2942,6 +2942,7 @@ std::string validate(mpcd_params_t const& p) {
         return "the hip backend supports walls in z only (periodicity (1, 1, 0) or (1, 1, 1))";
     if (p.periodicity[2] == 0 && p.volume_size[2] < 3.0f) return "walls in z need at least 3 cell layers";
     if (p.n == 0) return "n must be positive";
+    if (p.N == 0) return "particle count N must be positive";
     if (p.N >= 0xffffffffull) return "particle count must be below 2^32";
     double const cells = static_cast<double>(p.volume_size[0]) * p.volume_size[1] * p.volume_size[2];
     if (cells >= 4294967295.0) return "cell count must be below 2^32";

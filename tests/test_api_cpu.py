@@ -125,6 +125,10 @@ def test_invalid_configurations_fail_cleanly(pympcd):
     p.volume_size = (2 ** 21 + 1, 1, 10)   # the wrap's exact fast path needs 2.5 L below 2^23
     with pytest.raises(RuntimeError, match="2\\^21"):
         pympcd.Simulation(p, "hip")
+    p = pympcd.Params()
+    p.N = 0                                  # an empty system: rejected, not launched with empty grids
+    with pytest.raises(RuntimeError, match="N must be positive"):
+        pympcd.Simulation(p, "hip")
 
 
 @pytest.mark.skipif(_has_gpu(), reason="a GPU is present")
