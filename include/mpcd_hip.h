@@ -53,7 +53,11 @@ const char* mpcd_hip_last_error(const mpcd_hip_t* h);
  * (cuda_backend.cu:7-8, backend_context.cu:17-127) and CPUBackend::CPUBackend
  * (cpu_backend.cpp:8-21): allocates device state and distributes particles
  * uniformly in the fluid region with Maxwell-Boltzmann velocities
- * (simulation_functions.cpp:7-33), seeded from params->seed. */
+ * (simulation_functions.cpp:7-33), seeded from params->seed.  Returns nonzero,
+ * with the reason in mpcd_hip_last_error(NULL), for parameters the step does
+ * not support: n or N zero, extents not whole positive numbers or above 2^21,
+ * walls on x or y, fewer than 3 layers between z walls, 2^32 or more
+ * particles or cells, an unknown algorithm or experiment, no device. */
 int mpcd_hip_create(const mpcd_params_t* params, mpcd_hip_t** out);
 /* Replaces the backend destructor (cuda_backend_impl.cu:19-21). */
 int mpcd_hip_destroy(mpcd_hip_t* h);
