@@ -41,6 +41,12 @@ CONFIGS = [
     ((6, 4, 300), (1, 1, 0), 5, "srd"),
     ((300, 5, 6), (1, 1, 1), 4, "srd"),
     ((5, 4, 260), (1, 1, 0), 5, "extended"),
+    # the smallest grids: one periodic cell (every particle in it, the shift
+    # wraps within one cell length) and walled channels one cell wide
+    ((1, 1, 1), (1, 1, 1), 6, "srd"),
+    ((2, 1, 3), (1, 1, 0), 4, "srd"),
+    ((1, 1, 1), (1, 1, 1), 6, "extended"),
+    ((1, 2, 3), (1, 1, 0), 4, "extended"),
 ]
 IDS = [f"{a}-{'x'.join(map(str, f))}-p{''.join(map(str, p))}-n{n}" for f, p, n, a in CONFIGS]
 
