@@ -73,11 +73,13 @@ def build_hip(verbose: bool = True) -> Path:
 def build_python(verbose: bool = True) -> Path:
     import pybind11
 
-    srcs = [CSRC / "bindings.cpp", CSRC / "api" / "hip_backend.cpp", CSRC / "api" / "simulation_handle.cpp"]
+    # the API sources use the reference's include layout (<mpcd/api/...>,
+    # "common/backend.hpp") so hip_backend/ also compiles inside its tree
+    srcs = [CSRC / "bindings.cpp", CSRC / "hip_backend" / "hip_backend.cpp", CSRC / "api" / "simulation_handle.cpp"]
     tmp = EXT.with_name(EXT.name + ".tmp")
     cmd = [
         os.environ.get("CXX", "g++"), "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall",
-        f"-I{INCLUDE}", f"-I{CSRC}", f"-I{CSRC / 'api'}",
+        f"-I{INCLUDE}", f"-I{CSRC}",
         f"-I{pybind11.get_include()}", f"-I{sysconfig.get_paths()['include']}",
         *srcs,
         f"-L{PKG}", "-lmpcd_hip", "-Wl,-rpath,$ORIGIN",

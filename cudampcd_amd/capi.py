@@ -75,6 +75,7 @@ def lib() -> ctypes.CDLL:
                                       ctypes.c_size_t, _f64p, _f64p],
             "mpcd_hip_profile_loss_grad": [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_float, _f32p,
                                            _f64p, ctypes.c_size_t, _f64p, _f64p, _f64p, _f64p, _f32p],
+            "mpcd_hip_tune_capacities": [ctypes.c_int, ctypes.c_uint32, _u32p, ctypes.c_uint32, _u32p],
             "mpcd_hip_bench_copy": [ctypes.c_size_t, ctypes.c_int, _f64p],
             "mpcd_hip_selftest_philox": [ctypes.c_uint32, ctypes.c_uint32, _u32p, _u32p, ctypes.c_size_t],
             "mpcd_hip_selftest_math": [_f32p, _f32p, ctypes.c_size_t, ctypes.c_int],
@@ -195,6 +196,16 @@ class Handle:
         ms = ctypes.c_double()
         self._check(self.L.mpcd_hip_autotune(self.h, steps_per_trial, ctypes.byref(a), ctypes.byref(b), ctypes.byref(ms)))
         return a.value, b.value, ms.value
+
+
+def tune_capacities(algorithm: str, nmax: int) -> list[int]:
+    """LDS capacities the autotuner tries (no device needed)."""
+    out = np.zeros(16, np.uint32)
+    n = ctypes.c_uint32(0)
+    if lib().mpcd_hip_tune_capacities(0 if algorithm == "srd" else 1, nmax, _p(out, _u32p), len(out),
+                                       ctypes.byref(n)) != 0:
+        raise RuntimeError("mpcd_hip_tune_capacities failed")
+    return [int(v) for v in out[: n.value]]
 
 
 def bench_copy(nbytes: int, reps: int = 10) -> float:

@@ -15,7 +15,7 @@ import numpy as np
 
 from . import capi
 
-_SCALARS = ("n", "temperature", "drag", "delta_t", "rotation_angle", "system_id", "N", "device_id")
+_SCALARS = ("n", "temperature", "thermal_velocity", "drag", "delta_t", "rotation_angle", "system_id", "N", "device_id")
 
 
 def save(sim, path) -> Path:
@@ -27,30 +27,41 @@ def save(sim, path) -> Path:
         "positions": sim.get_particle_positions(),
         "velocities": sim.get_particle_velocities(),
         "step_index": np.uint64(h.step_index),
-        "seed": np.uint64(sim.seed),
-        "volume_size": np.asarray(p.volume_size, np.float32),
-        "periodicity": np.asarray(p.periodicity, np.int32),
-        "algorithm": np.asarray(p.algorithm),
-        "experiment": np.asarray(p.experiment),
         "sampling_acc": acc,
         "sampling_count": np.uint64(count),
         "sampling_active": np.bool_(active),
     }
-    for k in _SCALARS:
-        data[k] = np.asarray(getattr(p, k))
+    data.update(param_record(p, sim.seed))
     path = Path(path)
     np.savez(path, **data)
     return path if path.suffix == ".npz" else path.with_name(path.name + ".npz")
 
 
-def restore(path, backend: str = "hip"):
-    """A new Simulation in the saved state."""
+def param_record(p, seed) -> dict:
+    """The arrays a checkpoint keeps of Params `p` (and the stream's seed)."""
+    data = {
+        "seed": np.uint64(seed),
+        "volume_size": np.asarray(p.volume_size, np.float32),
+        "periodicity": np.asarray(p.periodicity, np.int32),
+        "algorithm": np.asarray(p.algorithm),
+        "experiment": np.asarray(p.experiment),
+    }
+    for k in _SCALARS:
+        data[k] = np.asarray(getattr(p, k))
+    return data
+
+
+def params_of(z):
+    """The Params a checkpoint archive was saved with."""
     import cudampcd_amd as pympcd
 
-    z = np.load(path, allow_pickle=False)
     p = pympcd.Params()
     p.n = int(z["n"])
     p.temperature = float(z["temperature"])
+    # the thermal velocity as it was: recomputing it from the stored float32
+    # temperature can change its last bit
+    if "thermal_velocity" in z:
+        p.thermal_velocity = float(z["thermal_velocity"])
     perio = tuple(int(v) for v in z["periodicity"])
     stored = [float(v) for v in z["volume_size"]]
     p.periodicity = perio
@@ -65,6 +76,15 @@ def restore(path, backend: str = "hip"):
     p.seed = int(z["seed"])
     if tuple(float(v) for v in p.volume_size) != tuple(stored) or int(p.N) != int(z["N"]):
         raise ValueError("checkpoint parameters do not reproduce the saved grid")
+    return p
+
+
+def restore(path, backend: str = "hip"):
+    """A new Simulation in the saved state."""
+    import cudampcd_amd as pympcd
+
+    z = np.load(path, allow_pickle=False)
+    p = params_of(z)
     sim = pympcd.Simulation(p, backend)
     sim.set_particles(z["positions"], z["velocities"])
     h = capi.Handle(sim)

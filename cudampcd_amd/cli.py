@@ -139,6 +139,9 @@ def make_params(pympcd, r: RunSettings, seed: int | None = None):
     p = pympcd.Params()
     p.n = r.n
     p.temperature = r.temperature
+    # the parameter file reader takes the square root of the float temperature
+    # (simulation_parameters.cpp:182), not of the double the Params setter sees
+    p.thermal_velocity = float(np.sqrt(np.float32(r.temperature)))
     p.periodicity = r.periodicity
     p.volume_size = r.volume_size              # fluid cells; the setter adds the wall layers
     p.drag = r.drag

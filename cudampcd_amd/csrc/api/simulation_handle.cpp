@@ -1,9 +1,9 @@
-#include "simulation_handle.hpp"
+#include <mpcd/api/simulation_handle.hpp>
 
 #include <stdexcept>
 
-#include "backend.hpp"
-#include "hip_backend.hpp"
+#include "common/backend.hpp"
+#include "hip_backend/hip_backend.hpp"
 
 namespace mpcd {
 

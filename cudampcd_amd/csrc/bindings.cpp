@@ -22,10 +22,11 @@
 #include <cstdint>
 #include <vector>
 
-#include "api/backend.hpp"
-#include "api/hip_backend.hpp"
-#include "api/simulation_handle.hpp"
-#include "api/simulation_parameters.hpp"
+#include <mpcd/api/simulation_handle.hpp>
+#include <mpcd/api/simulation_parameters.hpp>
+
+#include "common/backend.hpp"
+#include "hip_backend/hip_backend.hpp"
 
 namespace py = pybind11;
 
@@ -268,6 +269,11 @@ PYBIND11_MODULE(_pympcd, m) {
                 else if (k == "sorting") p.collision_kernel = CollisionKernel::sorting;
                 else p.collision_kernel = CollisionKernel::optimized;
             })
+        // Not a property of the reference's Params: the temperature setter derives
+        // it as sqrt(double T) while the parameter file reader takes sqrtf of the
+        // float T (simulation_parameters.cpp:182), which can differ in the last
+        // bit; checkpoints and the file driver set it directly.
+        .def_readwrite("thermal_velocity", &SimulationParameters::thermal_velocity)
         .def_readwrite("rotation_angle", &SimulationParameters::rotation_angle)
         .def_property(
             "seed", [](const SimulationParameters& p) -> py::object { if (p.seed_set) return py::int_(p.seed); return py::none(); },

@@ -10,7 +10,7 @@
 #include <string>
 #include <vector>
 
-#include "simulation_parameters.hpp"
+#include <mpcd/api/simulation_parameters.hpp>
 
 namespace mpcd {
 

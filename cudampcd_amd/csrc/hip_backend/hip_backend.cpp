@@ -3,6 +3,8 @@
 #include <random>
 #include <stdexcept>
 #include <string>
+#include <type_traits>
+#include <utility>
 
 namespace mpcd::hip {
 
@@ -10,6 +12,44 @@ uint64_t fresh_seed() {
     std::random_device rd;
     return (static_cast<uint64_t>(rd()) << 32) ^ rd();
 }
+
+// The reference's SimulationParameters (include/mpcd/api/simulation_parameters.hpp:23-55)
+// has no rotation_angle, seed / seed_set or system_id: it fixes the angle to 120
+// degrees (simulation_functions.cpp:118-119), seeds from /dev/urandom
+// (random.hpp:39-46) and runs one system.  These accessors read the fields
+// where the parameter struct has them and take the reference's choices where
+// it does not, so this file compiles unchanged against either struct.
+namespace {
+template <typename P, typename = void>
+struct has_rotation_angle : std::false_type {};
+template <typename P>
+struct has_rotation_angle<P, std::void_t<decltype(std::declval<P const&>().rotation_angle)>> : std::true_type {};
+template <typename P, typename = void>
+struct has_seed : std::false_type {};
+template <typename P>
+struct has_seed<P, std::void_t<decltype(std::declval<P const&>().seed), decltype(std::declval<P const&>().seed_set)>>
+    : std::true_type {};
+template <typename P, typename = void>
+struct has_system_id : std::false_type {};
+template <typename P>
+struct has_system_id<P, std::void_t<decltype(std::declval<P const&>().system_id)>> : std::true_type {};
+
+template <typename P>
+float rotation_angle_of(P const& p) {
+    if constexpr (has_rotation_angle<P>::value) return p.rotation_angle;
+    else return 120.0f;
+}
+template <typename P>
+uint64_t seed_of(P const& p) {
+    if constexpr (has_seed<P>::value) return p.seed_set ? static_cast<uint64_t>(p.seed) : fresh_seed();
+    else return fresh_seed();
+}
+template <typename P>
+uint32_t system_id_of(P const& p) {
+    if constexpr (has_system_id<P>::value) return static_cast<uint32_t>(p.system_id);
+    else return 0u;
+}
+}  // namespace
 
 mpcd_params_t to_c_params(SimulationParameters const& params, uint64_t seed) {
     mpcd_params_t p{};
@@ -27,14 +67,14 @@ mpcd_params_t to_c_params(SimulationParameters const& params, uint64_t seed) {
     p.temperature = params.temperature;
     p.thermal_velocity = params.thermal_velocity;
     p.drag = params.drag;
-    p.rotation_angle = params.rotation_angle;
+    p.rotation_angle = rotation_angle_of(params);
     p.seed = seed;
-    p.system_id = params.system_id;
+    p.system_id = system_id_of(params);
     return p;
 }
 
 HipBackend::HipBackend(SimulationParameters const& params) : Backend(params) {
-    seed_ = params.seed_set ? params.seed : fresh_seed();
+    seed_ = seed_of(params);
     mpcd_params_t const p = to_c_params(params, seed_);
     if (mpcd_hip_create(&p, &h_) != 0) throw std::runtime_error(mpcd_hip_last_error(nullptr));
     uint64_t c = 0;
@@ -104,7 +144,7 @@ HipEnsemble::HipEnsemble(std::vector<SimulationParameters> const& params) : para
     if (params.empty()) throw std::runtime_error("an ensemble needs at least one system");
     std::vector<mpcd_params_t> c;
     for (auto const& p : params) {
-        seeds_.push_back(p.seed_set ? p.seed : fresh_seed());
+        seeds_.push_back(seed_of(p));
         c.push_back(to_c_params(p, seeds_.back()));
     }
     if (mpcd_hip_create_ensemble(c.data(), static_cast<uint32_t>(c.size()), &h_) != 0)

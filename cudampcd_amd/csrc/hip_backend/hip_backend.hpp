@@ -9,7 +9,9 @@
 #include <cstdint>
 #include <vector>
 
-#include "backend.hpp"
+#include <mpcd/api/simulation_parameters.hpp>
+
+#include "common/backend.hpp"
 #include "mpcd_hip.h"
 
 namespace mpcd::hip {

@@ -20,7 +20,7 @@ namespace mpcd {
 
 enum class ExperimentType {
     standard,  // 3D periodic domain, optional walls in z
-    channel,   // cylindrical channel (not implemented by the hip backend)
+    channel,   // cylindrical channel about the x axis (simulation_functions.cpp:11-24, 75-82)
 };
 
 enum class MPCDAlgorithm {

@@ -1711,21 +1711,26 @@ __device__ __forceinline__ void collide_tile(Sys const& sys, const float* __rest
 
 constexpr int LAYERS_MAX = 1024;
 
-// mode 0: per-layer particle count and x-velocity sum of one step's state
-// (fixed point 2^40, so the sums do not depend on the order of the atomics);
+// mode 0: per-layer particle count and x-velocity sum of one step's state in
+// fixed point, so the sums do not depend on the order of the atomics: each
+// velocity is rounded to a multiple of 2^-40, q = rint(v 2^40), and split into
+// q >> 20 (sum[z], units of 2^-20) and q & (2^20 - 1) (sum[Z + z], units of
+// 2^-40), so neither 64-bit word overflows for |v| < 2^10 and fewer than 2^32
+// particles (one word at 2^-40 would overflow at |sum| = 2^23);
 // mode 1: loss seed dL/dv_x = w[layer] / count[layer] of every particle
 __global__ __launch_bounds__(WG) void k_layers(Sys sys, const float* __restrict__ st, int mode,
                                                uint32_t* __restrict__ cnt, unsigned long long* __restrict__ sum,
                                                const double* __restrict__ w, float* __restrict__ seed) {
     __shared__ uint32_t hc[LAYERS_MAX];
-    __shared__ unsigned long long hs[LAYERS_MAX];
+    __shared__ unsigned long long hs[2][LAYERS_MAX];
     KParams const P = sys.P[0];
     uint64_t const N = sys.NT;
     uint32_t const Z = P.dim[2];
     if (mode == 0) {
         for (uint32_t i = threadIdx.x; i < Z; i += WG) {
             hc[i] = 0u;
-            hs[i] = 0ull;
+            hs[0][i] = 0ull;
+            hs[1][i] = 0ull;
         }
         __syncthreads();
     }
@@ -1733,9 +1738,10 @@ __global__ __launch_bounds__(WG) void k_layers(Sys sys, const float* __restrict_
          j += static_cast<uint64_t>(gridDim.x) * WG) {
         uint32_t const iz = mod_dim(static_cast<uint32_t>(floorf(st[rec(j) + 2] + P.half[2])), Z);
         if (mode == 0) {
+            long long const q = static_cast<long long>(rint(static_cast<double>(st[rec(j) + 3]) * FIX_SCALE));
             atomicAdd(&hc[iz], 1u);
-            atomicAdd(&hs[iz], static_cast<unsigned long long>(static_cast<long long>(
-                                   rint(static_cast<double>(st[rec(j) + 3]) * FIX_SCALE))));
+            atomicAdd(&hs[0][iz], static_cast<unsigned long long>(q >> 20));
+            atomicAdd(&hs[1][iz], static_cast<unsigned long long>(q & 0xfffffll));
         } else {
             uint32_t const c = cnt[iz];
             seed[j] = c ? static_cast<float>(w[iz] / static_cast<double>(c)) : 0.0f;
@@ -1746,7 +1752,8 @@ __global__ __launch_bounds__(WG) void k_layers(Sys sys, const float* __restrict_
         for (uint32_t i = threadIdx.x; i < Z; i += WG) {
             if (hc[i]) {
                 atomicAdd(cnt + i, hc[i]);
-                atomicAdd(sum + i, hs[i]);
+                atomicAdd(sum + i, hs[0][i]);
+                atomicAdd(sum + Z + i, hs[1][i]);
             }
         }
     }
@@ -2129,7 +2136,7 @@ struct mpcd_hip_s {
         float* ckpt = nullptr;       // [nck][6N] states at the checkpoints (first: the start state)
         float* lam[2] = {nullptr, nullptr};
         uint32_t* lcnt = nullptr;    // [steps][Z] layer counts
-        unsigned long long* lsum = nullptr;  // [steps][Z] layer x-velocity sums, fixed point
+        unsigned long long* lsum = nullptr;  // [steps][2][Z] layer x-velocity sums, fixed point (k_layers)
         double* w = nullptr;         // [Z] dL/d(layer mean) / steps
         double* part = nullptr;      // [2][blocks] per-block gradient sums
         double* gf = nullptr;        // [C] dL/df
@@ -2232,23 +2239,44 @@ constexpr uint32_t TILE_CAP_MAX = TILE_CAPS[6];
 // request `tc` (0: the largest that fits).  SRD: on average about 80% of the
 // tile's particle capacity; extended: the ghosts of a batch (at most n per
 // cell) must fit the cap/2 ghost region.
-void choose_tiling(mpcd_hip_s* h, uint32_t tc, uint32_t cap) {
-    uint32_t nmax = 1;
-    for (KParams const& k : h->kp) nmax = std::max(nmax, k.n);
-    // the extended rule needs 512 or more and room for the ghosts of one wall
-    // cell (n) in its ghost region
-    if (h->algorithm == 1 && TILE_CAP_MAX / 2 < nmax) throw HipError{"the extended rule supports n up to 1024"};
-    if (cap == 0 && h->lds_cap) cap = h->lds_cap;
+// The compiled capacity a request of `cap` particles runs with (0: the
+// default): the smallest at least as large that the rule can use (the
+// extended rule needs 512 or more and room for one wall cell's n ghosts in its
+// ghost region).
+uint32_t pick_capacity(int algorithm, uint32_t nmax, uint32_t cap) {
     // default: SRD 384-particle batches on two-wave workgroups (twelve per
     // CU), the extended rule 512; larger where the density makes single cells
     // of several hundred particles likely
-    if (cap == 0) cap = std::max(h->algorithm == 0 ? 384u : 512u, 6 * nmax);
-    uint32_t chosen = TILE_CAP_MAX;
+    if (cap == 0) cap = std::max(algorithm == 0 ? 384u : 512u, 6 * nmax);
     for (uint32_t c : TILE_CAPS)
-        if (c >= cap && (h->algorithm == 0 || (c >= 512 && static_cast<uint32_t>(ghost_floats(static_cast<int>(c), 1)) >= nmax))) {
-            chosen = c;
-            break;
-        }
+        if (c >= cap && (algorithm == 0 || (c >= 512 && static_cast<uint32_t>(ghost_floats(static_cast<int>(c), 1)) >= nmax)))
+            return c;
+    return TILE_CAP_MAX;
+}
+
+// Capacities the autotuner tries: every compiled one the rule can run whose
+// batch holds a whole collision table (4n of the densest system).  With a
+// smaller one, a cell fuller than the batch could not collide in the big-cell
+// launch either, and the step would fail.  When no capacity holds the table
+// (n above 512), only `current`.
+std::vector<uint32_t> tune_capacities(int algorithm, uint32_t nmax, uint32_t current) {
+    std::vector<uint32_t> out;
+    for (uint32_t c : TILE_CAPS) {
+        uint32_t const chosen = pick_capacity(algorithm, nmax, c);
+        if (chosen >= 4 * nmax && std::find(out.begin(), out.end(), chosen) == out.end()) out.push_back(chosen);
+    }
+    if (out.empty()) out.push_back(current);
+    return out;
+}
+
+void choose_tiling(mpcd_hip_s* h, uint32_t tc, uint32_t cap) {
+    uint32_t nmax = 1;
+    for (KParams const& k : h->kp) nmax = std::max(nmax, k.n);
+    // the extended rule needs room for the ghosts of one wall cell (n) in its
+    // ghost region
+    if (h->algorithm == 1 && TILE_CAP_MAX / 2 < nmax) throw HipError{"the extended rule supports n up to 1024"};
+    if (cap == 0 && h->lds_cap) cap = h->lds_cap;
+    uint32_t const chosen = pick_capacity(h->algorithm, nmax, cap);
     h->lds_cap = chosen;
     h->tile_req = tc;
     for (KParams& k : h->kp) {
@@ -2711,7 +2739,7 @@ void adj_reserve(mpcd_hip_s* h, uint32_t steps, uint32_t K) {
     uint64_t const N = h->N, C = h->C, Z = h->kp[0].dim[2];
     dalloc(a.ckpt, 6 * N * nck, "alloc checkpoints");
     dalloc(a.lcnt, uint64_t(steps) * Z, "alloc layer counts");
-    dalloc(a.lsum, uint64_t(steps) * Z, "alloc layer sums");
+    dalloc(a.lsum, uint64_t(steps) * 2 * Z, "alloc layer sums");
     dalloc(a.w, Z, "alloc layer weights");
     dalloc(a.force, C, "alloc force field");
     if (K) {
@@ -2835,19 +2863,19 @@ ProfileResult profile_run(mpcd_hip_s* h, int steps, int K, float drag, float ang
         upload_table(h);
         // forward pass: layer statistics of every step, checkpoints
         ck(hipMemsetAsync(a.lcnt, 0, 4ull * T * Z, h->stream), "clear");
-        ck(hipMemsetAsync(a.lsum, 0, 8ull * T * Z, h->stream), "clear");
+        ck(hipMemsetAsync(a.lsum, 0, 16ull * T * Z, h->stream), "clear");
         uint32_t const lb = layer_blocks(h);
         for (uint32_t t = 1; t <= T; ++t) {
             adj_step(h, force, -1);
             hipLaunchKernelGGL(k_layers, dim3(lb), dim3(WG), 0, h->stream, h->sys, h->st[h->cur], 0,
-                               a.lcnt + uint64_t(t - 1) * Z, a.lsum + uint64_t(t - 1) * Z, nullptr, nullptr);
+                               a.lcnt + uint64_t(t - 1) * Z, a.lsum + uint64_t(t - 1) * 2 * Z, nullptr, nullptr);
             if (Kc && t % Kc == 0 && t < T)
                 ck(hipMemcpyAsync(a.ckpt + uint64_t(t / Kc) * 6 * N, h->st[h->cur], 24 * N, hipMemcpyDeviceToDevice,
                                   h->stream), "checkpoint");
         }
         ck(hipGetLastError(), "forward pass");
         std::vector<uint32_t> cnt(uint64_t(T) * Z);
-        std::vector<unsigned long long> sum(uint64_t(T) * Z);
+        std::vector<unsigned long long> sum(uint64_t(T) * 2 * Z);
         ck(hipStreamSynchronize(h->stream), "forward pass");
         uint32_t st = 0;
         ck(hipMemcpy(&st, h->status, 4, hipMemcpyDeviceToHost), "status");
@@ -2856,12 +2884,15 @@ ProfileResult profile_run(mpcd_hip_s* h, int steps, int K, float drag, float ang
             throw HipError{"a cell holds more particles than one LDS tile"};
         }
         ck(hipMemcpy(cnt.data(), a.lcnt, 4ull * T * Z, hipMemcpyDeviceToHost), "download layers");
-        ck(hipMemcpy(sum.data(), a.lsum, 8ull * T * Z, hipMemcpyDeviceToHost), "download layers");
+        ck(hipMemcpy(sum.data(), a.lsum, 16ull * T * Z, hipMemcpyDeviceToHost), "download layers");
         res.profile.assign(zhi - zlo, 0.0);
         for (uint32_t t = 0; t < T; ++t)
             for (uint32_t z = zlo; z < zhi; ++z) {
                 uint32_t const c = cnt[uint64_t(t) * Z + z];
-                double const m = c ? (static_cast<double>(static_cast<long long>(sum[uint64_t(t) * Z + z])) / FIX_SCALE) / c : 0.0;
+                // sum of rint(v 2^40) = hi 2^20 + lo, both exact integers
+                double const hi = static_cast<double>(static_cast<long long>(sum[uint64_t(t) * 2 * Z + z]));
+                double const lo = static_cast<double>(sum[uint64_t(t) * 2 * Z + Z + z]);
+                double const m = c ? ((hi * 1048576.0 + lo) / FIX_SCALE) / c : 0.0;
                 res.profile[z - zlo] += m / T;
             }
         std::vector<double> w(Z, 0.0);
@@ -3373,8 +3404,10 @@ int mpcd_hip_autotune(mpcd_hip_t* h, int steps_per_trial, uint32_t* cells_per_ti
         struct Cand {
             uint32_t tc, cap;
         };
+        uint32_t nmax = 1;
+        for (KParams const& k : h->kp) nmax = std::max(nmax, k.n);
         std::vector<Cand> cands;
-        for (uint32_t cap : TILE_CAPS)
+        for (uint32_t cap : tune_capacities(h->algorithm, nmax, h->lds_cap))
             for (uint32_t tc : {0u, 48u, 32u, 16u, 8u}) cands.push_back({tc, cap});
         bool const timing = h->timing;
         h->timing = false;
@@ -3382,7 +3415,8 @@ int mpcd_hip_autotune(mpcd_hip_t* h, int steps_per_trial, uint32_t* cells_per_ti
         ck(hipEventCreate(&e0), "event");
         ck(hipEventCreate(&e1), "event");
         double best = 1e300;
-        Cand pick = {h->tile_req, h->lds_cap};
+        Cand const before = {h->tile_req, h->lds_cap};
+        Cand pick = before;
         std::vector<std::pair<uint32_t, uint32_t>> seen;
         try {
             for (Cand const& c : cands) {
@@ -3407,6 +3441,12 @@ int mpcd_hip_autotune(mpcd_hip_t* h, int steps_per_trial, uint32_t* cells_per_ti
             (void)hipEventDestroy(e0);
             (void)hipEventDestroy(e1);
             h->timing = timing;
+            // the handle keeps the tile shape it had before tuning
+            try {
+                choose_tiling(h, before.tc, before.cap);
+                layout(h);
+            } catch (...) {
+            }
             throw;
         }
         (void)hipEventDestroy(e0);
@@ -3418,6 +3458,14 @@ int mpcd_hip_autotune(mpcd_hip_t* h, int steps_per_trial, uint32_t* cells_per_ti
         if (lds_capacity) *lds_capacity = h->lds_cap;
         if (ms_per_step) *ms_per_step = best / steps_per_trial;
     });
+}
+
+int mpcd_hip_tune_capacities(int algorithm, uint32_t nmax, uint32_t* caps, uint32_t max_caps, uint32_t* n_caps) {
+    if (!caps || !n_caps || (algorithm != 0 && algorithm != 1)) return 1;
+    std::vector<uint32_t> const c = tune_capacities(algorithm, std::max<uint32_t>(nmax, 1), 0);
+    *n_caps = static_cast<uint32_t>(c.size());
+    for (uint32_t i = 0; i < c.size() && i < max_caps; ++i) caps[i] = c[i];
+    return 0;
 }
 
 int mpcd_hip_bench_copy(size_t bytes, int reps, double* ms_per_copy) {

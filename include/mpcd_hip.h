@@ -188,6 +188,11 @@ int mpcd_hip_use_graph(mpcd_hip_t* h, int enable);
  * systems advance by these steps; results do not depend on the tile shape. */
 int mpcd_hip_autotune(mpcd_hip_t* h, int steps_per_trial, uint32_t* cells_per_tile, uint32_t* lds_capacity,
                       double* ms_per_step);
+/* LDS capacities the autotuner tries for `algorithm` (0 srd, 1 extended) when
+ * the densest system has nmax particles per cell: the compiled capacities the
+ * rule can run whose batch holds a whole collision table (4 nmax).  Writes up
+ * to max_caps of them and their number to *n_caps; needs no device. */
+int mpcd_hip_tune_capacities(int algorithm, uint32_t nmax, uint32_t* caps, uint32_t max_caps, uint32_t* n_caps);
 
 /* Copies `bytes` between two device buffers `reps` times with a plain float4
  * kernel and returns the mean time per copy: the practical HBM reference. */

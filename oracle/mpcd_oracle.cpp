@@ -1034,6 +1034,28 @@ void orc_counter_draws(void* h, int purpose, unsigned step, unsigned cell, unsig
         }
     }
 }
+// `count` wall-ghost positions of one cell (x, y, z in [0, 1] each, before the
+// cell centre is added): mode 0 from the stream with the reference's rejection
+// loop, mode 1 ghost k = i of the counter stream
+void orc_ghost_pos_draws(void* h, unsigned step, unsigned cell, unsigned count, float shift, int layer, int sign,
+                         float* out3) {
+    auto* o = static_cast<Oracle*>(h);
+    for (unsigned i = 0; i < count; ++i) {
+        oracle::V3 const g = o->rng.ghost_pos(step, cell, i, shift, layer != 0, sign != 0);
+        out3[3 * i] = g.x;
+        out3[3 * i + 1] = g.y;
+        out3[3 * i + 2] = g.z;
+    }
+}
+// extended-rule direction draws of cells [cell0, cell0 + count): the index of
+// the discrete direction (0..12) and the sign bit, then the gate uniform
+void orc_ext_axis_draws(void* h, unsigned step, unsigned cell0, unsigned count, int* select, int* flip, float* gate) {
+    auto* o = static_cast<Oracle*>(h);
+    for (unsigned i = 0; i < count; ++i) {
+        o->rng.ext_axis(step, cell0 + i, select + i, flip + i);
+        gate[i] = o->rng.ext_gate(step, cell0 + i);
+    }
+}
 void orc_ext_direction(int select, float* out) {
     oracle::V3 a = Oracle::ext_direction(select);
     out[0] = a.x; out[1] = a.y; out[2] = a.z;

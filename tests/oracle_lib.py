@@ -103,6 +103,11 @@ def oracle_lib() -> ctypes.CDLL:
         lib.orc_xoshiro_gaussian.argtypes = [ctypes.c_ulonglong, ctypes.c_ulonglong, ctypes.c_ulonglong, c_float_p]
         lib.orc_counter_draws.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_uint, ctypes.c_uint, ctypes.c_uint, c_float_p]
         lib.orc_ext_direction.argtypes = [ctypes.c_int, c_float_p]
+        lib.orc_ghost_pos_draws.argtypes = [ctypes.c_void_p, ctypes.c_uint, ctypes.c_uint, ctypes.c_uint,
+                                            ctypes.c_float, ctypes.c_int, ctypes.c_int, c_float_p]
+        c_int_p = ctypes.POINTER(ctypes.c_int)
+        lib.orc_ext_axis_draws.argtypes = [ctypes.c_void_p, ctypes.c_uint, ctypes.c_uint, ctypes.c_uint, c_int_p,
+                                           c_int_p, c_float_p]
         lib.orc_sym_inverse.argtypes = [c_float_p, ctypes.c_float, c_float_p]
         lib.orc_set_force.argtypes = [ctypes.c_void_p, c_float_p]
         lib.orc_frozen_profile_loss.restype = ctypes.c_double
@@ -217,6 +222,23 @@ class Oracle:
         out = np.empty(count * (3 if purpose == 5 else 1), np.float32)
         self.lib.orc_counter_draws(self.h, purpose, step, cell, count, _ptr(out, c_float_p))
         return out.reshape(count, 3) if purpose == 5 else out
+
+    def ghost_positions(self, step, cell, count, shift, layer, sign):
+        """`count` wall-ghost positions of one cell, each coordinate in [0, 1]
+        (reference mode: the stream's rejection loop; counter mode: ghost k = i)."""
+        out = np.empty((count, 3), np.float32)
+        self.lib.orc_ghost_pos_draws(self.h, step, cell, count, float(shift), int(layer), int(sign),
+                                     _ptr(out, c_float_p))
+        return out
+
+    def ext_axis_draws(self, step, cell0, count):
+        """(direction index 0..12, sign bit, gate uniform) of cells [cell0, cell0 + count)."""
+        sel = np.empty(count, np.int32)
+        flip = np.empty(count, np.int32)
+        gate = np.empty(count, np.float32)
+        ip = ctypes.POINTER(ctypes.c_int)
+        self.lib.orc_ext_axis_draws(self.h, step, cell0, count, _ptr(sel, ip), _ptr(flip, ip), _ptr(gate, c_float_p))
+        return sel, flip, gate
 
     def cells(self):
         cell = np.empty(self.N, np.uint32)

@@ -188,3 +188,24 @@ def test_ensemble_without_device_is_an_error(pympcd):
     a.volume_size = (4, 4, 4)
     with pytest.raises(RuntimeError, match="device"):
         pympcd.Ensemble([a, a])
+
+
+def test_autotuner_tries_only_capacities_that_hold_a_collision_table():
+    """A trial capacity below 4n would meet a cell fuller than the batch whose
+    table the big-cell launch cannot collide either, and fail the step; the
+    tuner skips such capacities (and the extended rule's capacities below 512
+    or without room for one wall cell's n ghosts)."""
+    from cudampcd_amd import capi
+
+    compiled = [256, 384, 448, 512, 640, 1152, 2048]
+    assert capi.tune_capacities("srd", 10) == compiled
+    assert capi.tune_capacities("srd", 100) == [448, 512, 640, 1152, 2048]
+    assert capi.tune_capacities("srd", 150) == [640, 1152, 2048]
+    assert capi.tune_capacities("srd", 512) == [2048]
+    for n in (1, 10, 64, 100, 150, 300, 512):
+        for algo in ("srd", "extended"):
+            caps = capi.tune_capacities(algo, n)
+            assert caps and all(c >= 4 * n and c in compiled for c in caps), (algo, n, caps)
+            if algo == "extended":
+                assert min(caps) >= 512
+    assert capi.tune_capacities("extended", 10) == [512, 640, 1152, 2048]

@@ -144,3 +144,46 @@ def test_schedule_snapshots(tmp_path, monkeypatch):
     # average_samples_over 1: every sample is written alone
     got, _ = _run(tmp_path, monkeypatch, 2, 3, 1, 1)
     assert got == {3: 3.0, 4: 4.0, 5: 5.0}
+
+
+def test_thermal_velocity_follows_the_file_reader():
+    """The reference's file reader sets thermal_velocity = sqrtf(float T)
+    (simulation_parameters.cpp:182); the Params temperature setter takes the
+    square root of the double, which differs in the last bit for T = 0.9."""
+    import cudampcd_amd as pympcd
+
+    text = (GOLDEN / "cli_input.txt").read_text().replace("temperature                  0.5",
+                                                         "temperature                  0.9")
+    r = cli.parse_input(text)
+    p = cli.make_params(pympcd, r)
+    want = np.sqrt(np.float32(0.9))
+    assert np.float32(p.thermal_velocity) == want
+    assert np.float32(np.sqrt(0.9)) != want            # the case the reader's order matters for
+    q = pympcd.Params()
+    q.temperature = 0.9
+    assert np.float32(q.thermal_velocity) == np.float32(np.sqrt(0.9))
+
+
+def test_checkpoint_params_round_trip_bitwise():
+    """A checkpoint restores the thermal velocity it was saved with (at T = 0.9
+    the float32 temperature's square root differs from the saved value)."""
+    import cudampcd_amd as pympcd
+    from cudampcd_amd import checkpoint
+
+    p = pympcd.Params()
+    p.n = 7
+    p.temperature = 0.9
+    p.periodicity = (1, 1, 0)
+    p.volume_size = (6, 5, 8)
+    p.drag = 0.003
+    p.delta_t = 0.05
+    p.rotation_angle = 97.0
+    p.algorithm = "extended"
+    p.seed = 4321
+    rec = checkpoint.param_record(p, 4321)
+    q = checkpoint.params_of(rec)
+    for k in ("n", "temperature", "thermal_velocity", "drag", "delta_t", "rotation_angle", "N",
+              "volume_size", "periodicity", "algorithm", "experiment", "seed"):
+        assert getattr(q, k) == getattr(p, k), k
+    legacy = {k: v for k, v in rec.items() if k != "thermal_velocity"}
+    assert checkpoint.params_of(legacy).thermal_velocity != p.thermal_velocity

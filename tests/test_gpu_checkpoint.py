@@ -12,9 +12,12 @@ from helpers import make_params
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("algorithm", ["srd", "extended"])
-def test_restart_continues_bitwise(pympcd, tmp_path, algorithm):
-    p = make_params(pympcd, (10, 8, 12), (1, 1, 0), n=8, algorithm=algorithm, seed=321, drag=0.002)
+# T = 0.9: the thermal velocity (wall ghosts, extended thermal draws) must be
+# restored as saved, not recomputed from the float32 temperature
+@pytest.mark.parametrize("algorithm,temperature", [("srd", 1.0), ("extended", 1.0), ("srd", 0.9), ("extended", 0.9)])
+def test_restart_continues_bitwise(pympcd, tmp_path, algorithm, temperature):
+    p = make_params(pympcd, (10, 8, 12), (1, 1, 0), n=8, algorithm=algorithm, seed=321, drag=0.002,
+                    temperature=temperature)
     a = pympcd.Simulation(p, "hip")
     a.step(12)
     a.step_and_sample(5)

@@ -16,6 +16,7 @@ import pytest
 from cudampcd_amd.capi import Handle
 from cudampcd_amd.physics import poiseuille_viscosity, srd_viscosity
 from helpers import make_params
+from oracle_lib import Oracle
 
 pytestmark = pytest.mark.gpu
 
@@ -52,6 +53,54 @@ def test_momentum_energy_conservation(pympcd, algorithm):
     if algorithm == "srd":
         E0, E1 = (v0 ** 2).sum(), (v1 ** 2).sum()
         assert abs(E1 - E0) / E0 < 1e-5
+
+
+@pytest.mark.parametrize("algorithm", ["srd", "extended"])
+def test_walled_cells_conserve_momentum(pympcd, algorithm):
+    """Walls in z: every collision cell outside the two ghost layers conserves
+    the momentum of its particles plus g dt each (simulation_functions.cpp:121,
+    187, 384-391; the extended rule's thermal and angular terms sum to zero over
+    the cell), a lone particle keeps its velocity (no drag, Q1), and the ghost
+    layers exchange momentum with the walls.  Device velocities after a step
+    against the streamed velocities and shifted cells of the oracle's
+    streaming half-step from the same state (bit-exact with the device's)."""
+    g, dt, n = 0.002, 0.02, 10
+    p = make_params(pympcd, (20, 16, 12), (1, 1, 0), n=n, algorithm=algorithm, seed=515, drag=g, delta_t=dt)
+    sim = pympcd.Simulation(p, "hip")
+    h = Handle(sim)
+    sim.step(3)
+    h.record_permutation(True)
+    X, Y, Z = (int(v) for v in p.volume_size)
+    cells = np.arange(X * Y * Z)
+    checked = 0
+    for _ in range(3):
+        pos, vel = sim.get_particle_positions(), sim.get_particle_velocities()
+        o = Oracle((X, Y, Z), (1, 1, 0), n=n, drag=g, delta_t=dt, algorithm=algorithm, rng="counter",
+                   seed=(p.seed, 0), distribute=False)
+        o.set_particles(pos, vel)
+        o.set_step(h.step_index)
+        shift = o.translate()
+        cell, cnt, _ = o.cells()
+        _, streamed = o.particles()
+        sim.step(1)
+        perm = h.last_permutation()
+        after = np.empty_like(vel)
+        after[perm] = sim.get_particle_velocities()       # back to the input order
+        dP = np.zeros((len(cells), 3))
+        np.add.at(dP, cell, after.astype(np.float64) - streamed)
+        sign = int(shift[2] > 0)
+        cz = cells // (X * Y)
+        ghost_layer = (cz == sign) | (cz == Z - (1 if sign else 2))
+        full = ~ghost_layer & (cnt >= 2) & (cnt <= 4 * n)
+        lone = ~ghost_layer & (cnt == 1)
+        want = np.zeros_like(dP)
+        want[:, 0] = cnt * np.float64(np.float32(g * dt))
+        assert np.abs(dP - want)[full].max() < 5e-5
+        assert not dP[lone].any()
+        assert np.abs(dP[ghost_layer & (cnt > 0)]).max() > 0.1
+        checked += int(full.sum())
+    # the fluid's shifted cells span Z - 1 layers, two of them ghost layers
+    assert checked > 0.9 * 3 * X * Y * (Z - 3)
 
 
 def test_drag_momentum_gain(pympcd):

@@ -359,6 +359,68 @@ def test_extended_directions():
     assert np.allclose(np.linalg.norm(dirs, axis=1), 1, atol=1e-6)
 
 
+# ---- counter-mode draws that replace a rejection loop or a stateful draw ----
+
+def _accepted_by_reference(z, shift, layer, sign):
+    """Acceptance of the reference's ghost-z rejection loop
+    (simulation_functions.cpp:155-161): it redraws while
+    ((z < shift) xor layer) xor sign."""
+    return ~(((z < np.float32(shift)) ^ bool(layer)) ^ bool(sign))
+
+
+@pytest.mark.parametrize("shift", [0.03, 0.31, 0.5, 0.77, 0.98])
+@pytest.mark.parametrize("layer,sign", [(0, 0), (0, 1), (1, 0), (1, 1)])
+def test_counter_ghost_z_lies_in_the_accepted_half_cell_and_is_uniform(layer, sign, shift):
+    """Counter mode maps one uniform onto the region the reference's rejection
+    loop accepts ([shift, 1] when layer xor sign is 0, [0, shift) otherwise).
+    Every draw must pass the reference's own acceptance predicate (a swapped
+    layer/sign branch would put every draw in the rejected region), be uniform
+    there, and follow the same distribution as the reference-mode stream, which
+    runs the rejection loop itself (and is pinned bitwise to the compiled
+    reference above).  x and y stay uniform on [0, 1]."""
+    o = Oracle((4, 4, 4), (1, 1, 0), n=1, rng="counter", seed=(2024, 7), distribute=False)
+    g = np.concatenate([o.ghost_positions(step, cell, 25, shift, layer, sign)
+                        for step in range(4) for cell in range(100)]).astype(np.float64)
+    z = g[:, 2].astype(np.float32)
+    assert _accepted_by_reference(z, shift, layer, sign).all()
+    lo, hi = (0.0, shift) if (layer ^ sign) else (shift, 1.0)
+    assert stats.kstest(g[:, 2], "uniform", args=(lo, hi - lo)).pvalue > 1e-3
+    for a in (0, 1):
+        assert stats.kstest(g[:, a], "uniform").pvalue > 1e-3
+    r = Oracle((4, 4, 4), (1, 1, 0), n=1, rng="reference", seed=(31337, 4242), distribute=False)
+    gr = r.ghost_positions(0, 0, len(g), shift, layer, sign).astype(np.float64)
+    assert _accepted_by_reference(gr[:, 2].astype(np.float32), shift, layer, sign).all()
+    assert stats.ks_2samp(g[:, 2], gr[:, 2]).pvalue > 1e-3
+
+
+def test_counter_extended_directions_uniform_over_26_and_independent():
+    """Counter mode draws the extended rule's direction index as
+    genUniformInt(0, 12) = (u32 / 2) % 13 and its sign as (u32 / 2) % 2 from two
+    words of one Philox block (random.hpp:134-136, simulation_functions.cpp:276,
+    293): the 26 signed directions are equally likely, index and sign are
+    independent, the gate uniform (third word) is uniform and independent of
+    the direction, and the joint table matches the reference-mode stream's."""
+    o = Oracle((4, 4, 4), (1, 1, 1), n=1, rng="counter", seed=(99, 3), distribute=False)
+    draws = [o.ext_axis_draws(step, 0, 20000) for step in range(13)]
+    sel = np.concatenate([d[0] for d in draws])
+    flip = np.concatenate([d[1] for d in draws])
+    gate = np.concatenate([d[2] for d in draws]).astype(np.float64)
+    assert sel.min() == 0 and sel.max() == 12 and set(np.unique(flip)) == {0, 1}
+    table = np.zeros((13, 2))
+    np.add.at(table, (sel, flip), 1)
+    assert stats.chisquare(table.ravel()).pvalue > 1e-3
+    assert stats.chi2_contingency(table).pvalue > 1e-3
+    assert stats.kstest(gate, "uniform").pvalue > 1e-3
+    # gate independent of the direction: its mean per direction index
+    means = np.array([gate[sel == k].mean() for k in range(13)])
+    assert np.abs(means - 0.5).max() < 5 * math.sqrt(1 / 12 / (len(gate) / 13))
+    r = Oracle((4, 4, 4), (1, 1, 1), n=1, rng="reference", seed=(5, 6), distribute=False)
+    rs, rf, _ = r.ext_axis_draws(0, 0, len(sel))
+    rtable = np.zeros((13, 2))
+    np.add.at(rtable, (rs, rf), 1)
+    assert stats.chi2_contingency(np.stack([table.ravel(), rtable.ravel()])).pvalue > 1e-3
+
+
 def test_oracle_poiseuille_viscosity_sanity():
     """Small developed channel on the oracle: a parabolic profile whose viscosity
     is within 8% of SRD theory.
