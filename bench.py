@@ -85,19 +85,24 @@ def algorithmic_bytes(n_particles: int, n_cells: int) -> dict:
     return {"collide": collide, "scan": scan, "perm": perm, "step": collide + scan + perm}
 
 
-def cpu_baseline(n: int, seconds_target: float = 15.0) -> dict:
-    """Reference CPU backend (oracle/_ref, compiled from the reference sources) on a
-    bounded slice of the same workload, one host core."""
-    sys.path.insert(0, str(ROOT / "tests"))
-    from oracle_lib import Oracle, Reference, ref_available  # checker libraries, timed here only
+def host_cpu() -> dict:
+    """CPU model and core counts of the host the baseline runs on."""
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        usable = os.cpu_count()
+    return {"cpu_model": model, "nproc": os.cpu_count(), "usable_cpus": usable}
 
-    fluid = (100, 100, 20)
-    volume = (fluid[0], fluid[1], fluid[2] + 2)
-    kind = "reference" if ref_available() else "port"
-    if kind == "reference":
-        sim = Reference(volume, (1, 1, 0), n=n, drag=0.001, delta_t=0.02, algorithm="srd", seed=(11, 13))
-    else:
-        sim = Oracle(volume, (1, 1, 0), n=n, drag=0.001, delta_t=0.02, algorithm="srd", rng="reference", seed=(11, 13))
+
+def _time_cpu(sim, seconds_target: float):
     sim.step(1)
     t0 = time.perf_counter()
     steps = 0
@@ -106,12 +111,42 @@ def cpu_baseline(n: int, seconds_target: float = 15.0) -> dict:
         steps += 1
         if time.perf_counter() - t0 >= seconds_target:
             break
-    dt = time.perf_counter() - t0
-    value = sim.N * steps / dt
+    return sim.N * steps / (time.perf_counter() - t0), steps
+
+
+def cpu_baseline(n: int, seconds_target: float = 15.0) -> dict:
+    """Reference CPU backend (oracle/_ref, compiled from the reference sources;
+    single-threaded, as the reference is) on one host core: a bounded sample of
+    the workload, a cross-section slice of the same channel (100x100 of the
+    1000x1000 cells, the same gap, walls, n and thermodynamics: the CPU cost per
+    particle-step does not depend on the cross-section), plus the plain SRD
+    Poiseuille case of the reference README (10x10x100 cells, n = 10)."""
+    sys.path.insert(0, str(ROOT / "tests"))
+    from oracle_lib import Oracle, Reference, ref_available  # checker libraries, timed here only
+
+    kind = "reference" if ref_available() else "port"
+
+    def make(volume, nn):
+        if kind == "reference":
+            return Reference(volume, (1, 1, 0), n=nn, drag=0.001, delta_t=0.02, algorithm="srd", seed=(11, 13))
+        return Oracle(volume, (1, 1, 0), n=nn, drag=0.001, delta_t=0.02, algorithm="srd", rng="reference", seed=(11, 13))
+
+    fluid = (100, 100, 20)
+    sim = make((fluid[0], fluid[1], fluid[2] + 2), n)
+    value, steps = _time_cpu(sim, seconds_target)
+    n_slice = sim.N
+    del sim
+    c1 = make((10, 10, 102), 10)
+    c1_value, c1_steps = _time_cpu(c1, seconds_target / 2)
+    what = "reference CPU backend (cpu_backend.cpp + simulation_functions.cpp, g++ -O3)" if kind == "reference" \
+        else "oracle restatement"
     return {"value": value, "unit": "particle-steps/s", "cores": 1, "kind": kind,
-            "sample": f"{'reference CPU backend (cpu_backend.cpp + simulation_functions.cpp, g++ -O3)' if kind == 'reference' else 'oracle restatement'}: "
-                      f"SRD {fluid[0]}x{fluid[1]}x{fluid[2]} cells, n={n}, walls in z ({sim.N} particles), "
-                      f"{steps} steps after 1 warm-up, single thread"}
+            "sample": f"{what}: SRD {fluid[0]}x{fluid[1]}x{fluid[2]} cells (a cross-section slice of the "
+                      f"1000x1000x20 channel), n={n}, walls in z ({n_slice} particles), {steps} steps after 1 "
+                      f"warm-up, single thread",
+            "host": host_cpu(),
+            "c1": {"value": c1_value, "unit": "particle-steps/s", "steps": c1_steps,
+                   "workload": "SRD Poiseuille 10x10x100 cells, n=10, walls in z (1e5 particles), single thread"}}
 
 
 def load_traffic(config: str):
@@ -140,6 +175,8 @@ def main():
                     help="cells_per_tile,lds_capacity for k_collide (results do not depend on it)")
     ap.add_argument("--sample", action="store_true",
                     help="time step_and_sample (a mean-field sample every step) instead of step")
+    ap.add_argument("--single", action="store_true",
+                    help="c2: time only the plain step (skip the sampled-step and extended-rule lines)")
     ap.add_argument("--copy-gb", type=float, default=0.0,
                     help="only run the plain copy kernel over this many GB (profiling reference) and exit")
     args = ap.parse_args()
@@ -167,47 +204,81 @@ def main():
         return bench_ensemble(args, cfg)
     if args.config == "c5":
         return bench_adjoint(args, cfg)
-    p = make_params(pympcd, cfg["fluid"], cfg["n"], cfg["algorithm"], 0.001, args.seed)
+    tiling = tuple(int(x) for x in args.tiling.split(",")) if args.tiling else None
+    out = measure(args.config, args.seed, args.steps, args.warmup, args.sample, tiling)
+    if args.config == "c2" and not args.sample and not args.single:
+        # the sampled step (the reference's viscosity workflow) and the extended
+        # rule, timed in the same run so their numbers carry the driver's clock
+        out["extra"] = {
+            "c2_sampled": summary(measure("c2", args.seed, args.steps, args.warmup, True, None)),
+            "c3": summary(measure("c3", 0x5EED0003, args.steps, args.warmup, False, None)),
+        }
+    if not args.no_cpu:
+        print("[bench] cpu baseline", file=sys.stderr, flush=True)
+        out["cpu_baseline"] = cpu_baseline(CONFIGS[args.config].get("n", 10))
+    print(json.dumps(out), flush=True)
+
+
+def summary(r: dict) -> dict:
+    """The figures of a secondary measurement kept in the main line."""
+    return {k: r[k] for k in ("value", "unit", "ms_per_step", "steps", "warmup", "config", "roofline", "device_ms")}
+
+
+def measure(config, seed, steps, warmup, sample, tiling) -> dict:
+    """One single-system workload: build it, warm up, time `steps` steps (or
+    sampled steps) with the state resident in HBM, and the collide launches
+    with HIP events on the handle's stream."""
+    import gc
+
+    import cudampcd_amd as pympcd
+    from cudampcd_amd import capi
+    from cudampcd_amd.capi import Handle
+
+    cfg = CONFIGS[config]
+    p = make_params(pympcd, cfg["fluid"], cfg["n"], cfg["algorithm"], 0.001, seed)
     print(f"[bench] creating {cfg['workload']} ({p.N} particles)", file=sys.stderr, flush=True)
     sim = pympcd.Simulation(p, "hip")
     h = Handle(sim)
     N, C = h.N, h.C
-    if args.tiling:
-        tc, cap = (int(x) for x in args.tiling.split(","))
-        h.set_tiling(tc, cap)
+    if tiling:
+        h.set_tiling(*tiling)
 
-    print(f"[bench] warmup {args.warmup} steps", file=sys.stderr, flush=True)
-    sim.step(args.warmup)
+    print(f"[bench] warmup {warmup} steps", file=sys.stderr, flush=True)
+    sim.step(warmup)
     run = sim.step
-    if args.sample:
+    if sample:
         sim.step_and_sample(2)      # the first sample of an average runs eagerly
         run = sim.step_and_sample
     h.enable_timing(True)
-    print(f"[bench] timing {args.steps} {'sampled ' if args.sample else ''}steps", file=sys.stderr, flush=True)
+    print(f"[bench] timing {steps} {'sampled ' if sample else ''}steps", file=sys.stderr, flush=True)
     t0 = time.perf_counter()
-    run(args.steps)                 # returns after the device finished
+    run(steps)                      # returns after the device finished
     wall = time.perf_counter() - t0
     collide_ms, total_ms, launches = h.last_timing()
+    tiles = list(h.tiling())
+    volume = [int(v) for v in p.volume_size]
+    del sim, h
+    gc.collect()
 
     copy_bytes = 4 * 10 ** 9
     copy_ms = capi.bench_copy(copy_bytes, 5)
     copy_gbps = 2 * copy_bytes / copy_ms / 1e6
 
-    ms_per_step = wall * 1e3 / args.steps
-    value = N * args.steps / wall
+    ms_per_step = wall * 1e3 / steps
+    value = N * steps / wall
     ab = algorithmic_bytes(N, C)
-    collide_avg_ms = collide_ms / args.steps
+    collide_avg_ms = collide_ms / steps
     achieved = ab["collide"] / (collide_avg_ms * 1e-3) / 1e9
-    traffic = load_traffic(args.config)
+    traffic = load_traffic(config)
     roofline = {
         "bound": "hbm",
         "kernel": "k_collide (gather + stream + sort-in-cell + collide + next-step histogram"
-                  + (" + mean-field sums)" if args.sample else ")"),
+                  + (" + mean-field sums)" if sample else ")"),
         "achieved": round(achieved, 1),
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 4),
-        "traffic": traffic.get("bytes_per_launch") if traffic and traffic.get("particles") == N else None,
+        "traffic": traffic.get("bytes_per_launch") if traffic and traffic.get("particles") == N and not sample else None,
         "algorithmic_bytes_per_launch": ab["collide"],
         "avg_launch_ms": round(collide_avg_ms, 4),
         "copy_kernel_GBps": round(copy_gbps, 1),
@@ -219,33 +290,29 @@ def main():
         "achieved": round(value * per_step / 1e9, 1),
         "frac": round(value * per_step / 1e9 / HBM_PEAK_GBS, 4),
         "this_build_bytes_per_particle_step": round(ab["step"] / N, 2),
-        "this_build_GBps": round(ab["step"] * args.steps / wall / 1e9, 1),
+        "this_build_GBps": round(ab["step"] * steps / wall / 1e9, 1),
     }
-    out = {
+    return {
         "metric": "particle-steps/s",
         "value": value,
         "unit": "particle-steps/s",
         "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
+        "steps": steps,
+        "warmup": warmup,
         "ms_per_step": round(ms_per_step, 4),
         "higher_is_better": True,
         "scaling": "weak",
-        "vs_baseline": round(value / REFERENCE_PSTEPS, 3) if args.config == "c2" and not args.sample else None,
+        "vs_baseline": round(value / REFERENCE_PSTEPS, 3) if config == "c2" and not sample else None,
         "dtype": "f32",
         "data": "synthetic (seeded uniform positions, Maxwell-Boltzmann velocities)",
-        "config": {"workload": cfg["workload"], "cells": [int(v) for v in p.volume_size], "particles": N,
+        "config": {"workload": cfg["workload"], "cells": volume, "particles": N,
                    "algorithm": cfg["algorithm"], "n": cfg["n"], "delta_t": 0.02, "drag": 0.001,
-                   "tiling": list(h.tiling()), "parallelism": "single GPU",
-                   "sampling": "a mean-field sample every step (step_and_sample)" if args.sample else "none (step)"},
+                   "tiling": tiles, "parallelism": "single GPU",
+                   "sampling": "a mean-field sample every step (step_and_sample)" if sample else "none (step)"},
         "roofline": roofline,
         "step_roofline": step_roofline,
         "device_ms": {"collide_total": round(collide_ms, 3), "step_total": round(total_ms, 3), "launches": launches},
     }
-    if not args.no_cpu:
-        print("[bench] cpu baseline", file=sys.stderr, flush=True)
-        out["cpu_baseline"] = cpu_baseline(cfg["n"])
-    print(json.dumps(out), flush=True)
 
 
 def bench_ensemble(args, cfg):

@@ -112,3 +112,87 @@ def test_ensemble_member_set_and_reset(pympcd):
     assert np.isfinite(v3).all()
     with pytest.raises(RuntimeError, match="out of range"):
         ens.get_particle_positions(5)
+
+
+def _member_oracle(p, system_id):
+    from helpers import stored_volume
+    from oracle_lib import Oracle
+
+    return Oracle(stored_volume(p), tuple(p.periodicity), n=p.n, drag=p.drag, delta_t=p.delta_t,
+                  thermal_velocity=float(np.sqrt(p.temperature)), rotation_deg=p.rotation_angle,
+                  algorithm=p.algorithm, rng="counter", seed=(p.seed, 0), system=system_id, distribute=False)
+
+
+def _step_members_against_oracle(ens, params, members, steps=2):
+    """One ensemble step at a time; each listed member's state after the step
+    equals the oracle's step from that member's state before it, keyed by the
+    member's (seed, system id), bit for bit.  The recorded permutation of the
+    flattened arrays maps each member's output slots onto its own input slots."""
+    h = Handle(ens)
+    h.record_permutation(True)
+    for _ in range(steps):
+        before = {k: (ens.get_particle_positions(k), ens.get_particle_velocities(k)) for k in members}
+        step = h.step_index
+        ens.step(1)
+        perm = h.last_permutation()
+        for k in members:
+            first, count, _, _ = h.system_info(k)
+            sub = perm[first:first + count].astype(np.int64) - first
+            assert sub.min() >= 0 and sub.max() < count, k
+            o = _member_oracle(params[k], params[k].system_id)
+            o.set_particles(*before[k])
+            o.set_step(step)
+            o.step(1)
+            opos, ovel = o.particles()
+            assert np.array_equal(ens.get_particle_positions(k), opos[sub]), k
+            assert np.array_equal(ens.get_particle_velocities(k), ovel[sub]), k
+
+
+@pytest.mark.parametrize("algorithm", ["srd", "extended"])
+def test_members_match_the_oracle_bitwise(pympcd, algorithm):
+    """Every member of a mixed ensemble (different grids, densities, walls,
+    drags, rotation angles, seeds and system ids) against the oracle."""
+    params = [_params(pympcd, m, algorithm) for m in MEMBERS]
+    ens = pympcd.Ensemble(params)
+    ens.step(3)
+    _step_members_against_oracle(ens, params, range(len(params)))
+
+
+def test_c4_channel_set(pympcd):
+    """The ensemble workload itself: 64 seeded SRD channels, cross-sections
+    32^2 / 64^2 / 128^2, wall gaps 20-200, n in {5, 10, 20}, drags 5e-4 to 2e-3
+    (about 5e8 particles).  Three steps, then one member of each density
+    stepped against the oracle, and three members of different density and
+    cross-section against standalone Simulations of the same parameters."""
+    from bench import c4_members, make_params as bench_params
+
+    spec = c4_members(0x5EED0004)       # the seed bench.py --config c4 uses
+    params = [bench_params(pympcd, m["fluid"], m["n"], "srd", m["drag"], m["seed"], m["system_id"]) for m in spec]
+    ens = pympcd.Ensemble(params)
+    assert ens.n_systems == 64
+    ens.step(3)
+    # the smallest member of each density for the oracle (a few 1e5 particles)
+    by_n = {}
+    for k, m in enumerate(spec):
+        size = m["fluid"][0] * m["fluid"][1] * m["fluid"][2] * m["n"]
+        if m["n"] not in by_n or size < by_n[m["n"]][1]:
+            by_n[m["n"]] = (k, size)
+    oracle_members = sorted(k for k, _ in by_n.values())
+    assert len(oracle_members) == 3
+    _step_members_against_oracle(ens, params, oracle_members, steps=1)
+    # standalone runs of three members of different density and cross-section,
+    # the smallest such ones
+    size = [m["fluid"][0] * m["fluid"][1] * m["fluid"][2] * m["n"] for m in spec]
+    picks = []
+    for k in sorted(range(len(spec)), key=lambda j: size[j]):
+        m = spec[k]
+        if all(spec[j]["n"] != m["n"] and spec[j]["fluid"][0] != m["fluid"][0] for j in picks):
+            picks.append(k)
+    assert len(picks) == 3
+    steps = Handle(ens).step_index
+    for k in picks:
+        s = pympcd.Simulation(params[k], "hip")
+        s.step(steps)
+        assert np.array_equal(ens.get_particle_positions(k), s.get_particle_positions()), k
+        assert np.array_equal(ens.get_particle_velocities(k), s.get_particle_velocities()), k
+        del s

@@ -60,7 +60,7 @@ def test_walled_cells_conserve_momentum(pympcd, algorithm):
     """Walls in z: every collision cell outside the two ghost layers conserves
     the momentum of its particles plus g dt each (simulation_functions.cpp:121,
     187, 384-391; the extended rule's thermal and angular terms sum to zero over
-    the cell), a lone particle keeps its velocity (no drag, Q1), and the ghost
+    the cell), a lone particle keeps its velocity (no drag, simulation_functions.cpp:169-189), and the ghost
     layers exchange momentum with the walls.  Device velocities after a step
     against the streamed velocities and shifted cells of the oracle's
     streaming half-step from the same state (bit-exact with the device's)."""
