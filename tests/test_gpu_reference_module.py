@@ -31,5 +31,7 @@ def test_reference_module_runs_hip_beside_cpu():
     for r in (hip, cpu):
         assert abs(r["fluid_density"] - want) < 0.02 * want
         assert abs(r["var_v"] - 1.0) < 0.08
-        assert r["max_abs_z"] <= 5.0
+        # inside the walls (z_wall = 5), or half a cell beyond for a particle
+        # left at its shifted position alone in a wall-layer cell
+        assert r["max_abs_z"] <= 5.5
         assert abs(r["mean_vx"]) < 0.1         # thermal noise of 3200 particles is 0.018
