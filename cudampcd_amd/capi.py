@@ -25,6 +25,46 @@ _vp = ctypes.c_void_p
 _lib = None
 
 
+class CParams(ctypes.Structure):
+    """mpcd_params_t of include/mpcd_hip.h."""
+    _fields_ = [("experiment", ctypes.c_int), ("algorithm", ctypes.c_int), ("collision_kernel", ctypes.c_int),
+                ("periodicity", ctypes.c_int * 3), ("volume_size", ctypes.c_float * 3), ("N", ctypes.c_uint64),
+                ("n", ctypes.c_uint32), ("device_id", ctypes.c_uint32), ("delta_t", ctypes.c_float),
+                ("temperature", ctypes.c_float), ("thermal_velocity", ctypes.c_float), ("drag", ctypes.c_float),
+                ("rotation_angle", ctypes.c_float), ("seed", ctypes.c_uint64), ("system_id", ctypes.c_uint32),
+                ("reserved", ctypes.c_uint32)]
+
+
+def c_params(p, seed=None) -> CParams:
+    """The C-ABI parameter block of Params `p` (seed: p.seed unless given)."""
+    c = CParams()
+    c.experiment = 0 if p.experiment == "standard" else 1
+    c.algorithm = 0 if p.algorithm == "srd" else 1
+    c.collision_kernel = 2
+    for a in range(3):
+        c.periodicity[a] = int(p.periodicity[a])
+        c.volume_size[a] = float(p.volume_size[a])
+    c.N = int(p.N)
+    c.n = int(p.n)
+    c.device_id = int(p.device_id)
+    c.delta_t = p.delta_t
+    c.temperature = p.temperature
+    c.thermal_velocity = p.thermal_velocity
+    c.drag = p.drag
+    c.rotation_angle = p.rotation_angle
+    c.seed = int(p.seed if seed is None else seed)
+    c.system_id = int(p.system_id)
+    return c
+
+
+def step_noalloc(params: CParams, step_index: int, n_steps: int, state_in: int, state_out: int,
+                 n_particles: int) -> None:
+    """mpcd_hip_step_noalloc on two device buffers (addresses of this library's HIP runtime)."""
+    if lib().mpcd_hip_step_noalloc(ctypes.byref(params), step_index, n_steps, ctypes.c_void_p(state_in),
+                                   ctypes.c_void_p(state_out), n_particles) != 0:
+        raise RuntimeError(lib().mpcd_hip_last_error(None).decode())
+
+
 def declared_symbols() -> list[str]:
     """Entry points declared in include/mpcd_hip.h."""
     text = HEADER.read_text()
@@ -76,6 +116,8 @@ def lib() -> ctypes.CDLL:
             "mpcd_hip_profile_loss_grad": [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_float, _f32p,
                                            _f64p, ctypes.c_size_t, _f64p, _f64p, _f64p, _f64p, _f32p],
             "mpcd_hip_tune_capacities": [ctypes.c_int, ctypes.c_uint32, _u32p, ctypes.c_uint32, _u32p],
+            "mpcd_hip_step_noalloc": [ctypes.POINTER(CParams), ctypes.c_uint64, ctypes.c_int, _vp, _vp,
+                                      ctypes.c_size_t],
             "mpcd_hip_bench_copy": [ctypes.c_size_t, ctypes.c_int, _f64p],
             "mpcd_hip_selftest_philox": [ctypes.c_uint32, ctypes.c_uint32, _u32p, _u32p, ctypes.c_size_t],
             "mpcd_hip_selftest_math": [_f32p, _f32p, ctypes.c_size_t, ctypes.c_int],

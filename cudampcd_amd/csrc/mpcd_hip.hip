@@ -101,7 +101,7 @@ struct Sys {
 // BIG_BLOCKS workgroups over the list.  The list holds one entry per
 // N / (smallest tile capacity + 1) + 1, so it cannot overflow.
 constexpr uint32_t BIG_BLOCKS = 256;
-inline uint64_t big_list_words(uint64_t n) { return 2 + 2 * (n / 257 + 2); }
+constexpr uint64_t big_list_words(uint64_t n) { return 2 + 2 * (n / 257 + 2); }
 
 // the 6 floats of particle i: x y z vx vy vz
 __host__ __device__ __forceinline__ uint64_t rec(uint64_t i) { return 6 * i; }
@@ -2057,6 +2057,34 @@ __global__ void k_selftest_math(const float* in, float* out, size_t n, int which
 
 }  // namespace mpcd_hip
 
+// Scratch of the allocation-free step (mpcd_hip_step_noalloc): the step's
+// cell-sort and per-cell arrays for a system of up to NOALLOC_MAX_PARTICLES
+// particles and NOALLOC_MAX_CELLS cells, 15.5 MiB per device, held from the
+// library's load until the process ends.  In host code `scratch` names a
+// host-side stand-in; its device address comes from hipGetSymbolAddress, once
+// per device (noalloc_scratch below).
+constexpr uint64_t NOALLOC_MAX_PARTICLES = uint64_t(1) << 20;
+constexpr uint64_t NOALLOC_MAX_CELLS = uint64_t(1) << 17;
+constexpr uint64_t NOALLOC_MAX_GHOST_FLOATS = uint64_t(1) << 20;   // SRD wall-ghost velocities: 3 n 2 X Y
+constexpr uint64_t noalloc_align(uint64_t b) { return (b + 255) / 256 * 256; }
+// byte offsets of the regions
+constexpr uint64_t NA_KP = 0;
+constexpr uint64_t NA_BLK = NA_KP + noalloc_align(sizeof(mpcd_hip::KParams));
+constexpr uint64_t NA_PERM = NA_BLK + noalloc_align(4 * 8);
+constexpr uint64_t NA_KEYS = NA_PERM + noalloc_align(4 * NOALLOC_MAX_PARTICLES);
+constexpr uint64_t NA_COUNT = NA_KEYS + noalloc_align(4 * NOALLOC_MAX_PARTICLES);
+constexpr uint64_t NA_OFFSET = NA_COUNT + noalloc_align(4 * NOALLOC_MAX_CELLS);
+constexpr uint64_t NA_CURSOR = NA_OFFSET + noalloc_align(4 * (NOALLOC_MAX_CELLS + 1));
+constexpr uint64_t NA_PARTIAL = NA_CURSOR + noalloc_align(4 * NOALLOC_MAX_CELLS);
+constexpr uint64_t NA_STATUS = NA_PARTIAL + noalloc_align(4 * (NOALLOC_MAX_CELLS / mpcd_hip::SCAN_CHUNK + 1));
+constexpr uint64_t NA_AXIS = NA_STATUS + noalloc_align(4 * (2 + 2 * (NOALLOC_MAX_PARTICLES / 257 + 2)));
+constexpr uint64_t NA_GCNT = NA_AXIS + noalloc_align(12 * NOALLOC_MAX_CELLS);
+constexpr uint64_t NA_GVEL = NA_GCNT + noalloc_align(4 * NOALLOC_MAX_CELLS);
+constexpr uint64_t NOALLOC_SCRATCH_BYTES = NA_GVEL + noalloc_align(4 * NOALLOC_MAX_GHOST_FLOATS);
+static_assert(NA_STATUS + 4 * mpcd_hip::big_list_words(NOALLOC_MAX_PARTICLES) <= NA_AXIS, "big-cell list fits");
+
+static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+
 // ============================================================== host side ===
 
 using namespace mpcd_hip;
@@ -3096,6 +3124,121 @@ void sample_mean_impl(mpcd_hip_t* h, uint64_t first, uint64_t n, float* d32, flo
     }
 }
 
+// ---------------------------------------------------- allocation-free step ---
+// mpcd_hip_step_noalloc: n_steps of one system whose state the caller holds in
+// device memory, with every array of the step in `scratch` instead of buffers
+// of a handle.  The launches are the handle's own (enqueue_step) on a
+// handle-shaped object on the stack whose pointers address the scratch
+// regions; nothing is allocated on the device.
+
+constexpr int NOALLOC_DEVICES = 64;
+std::mutex g_noalloc_mu;                       // one call at a time in this process
+unsigned char* g_noalloc_ptr[NOALLOC_DEVICES] = {};
+
+// device address of `scratch` on the current device (looked up on the first
+// call on that device; each device holds its own copy of the array)
+unsigned char* noalloc_scratch(int device) {
+    if (device < 0 || device >= NOALLOC_DEVICES) throw HipError{"device id above the allocation-free step's table"};
+    if (!g_noalloc_ptr[device]) {
+        void* ptr = nullptr;
+        ck(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+        if (!ptr) throw HipError{"hipGetSymbolAddress(scratch) returned no address"};
+        g_noalloc_ptr[device] = static_cast<unsigned char*>(ptr);
+    }
+    return g_noalloc_ptr[device];
+}
+
+void step_noalloc_impl(mpcd_params_t const& prm, uint64_t step_index, int n_steps, float* state_in,
+                       float* state_out, size_t n_particles) {
+    std::string const e = validate(prm);
+    if (!e.empty()) throw HipError{e};
+    if (n_steps < 1) throw HipError{"n_steps must be positive"};
+    if (!state_in || !state_out || state_in == state_out) throw HipError{"state_in and state_out must be two device buffers"};
+    if (n_particles != prm.N) throw HipError{"n_particles must equal params->N"};
+    uint64_t const C = static_cast<uint64_t>(prm.volume_size[0]) * static_cast<uint64_t>(prm.volume_size[1]) *
+                       static_cast<uint64_t>(prm.volume_size[2]);
+    uint64_t const walls = prm.periodicity[2] ? 0 : 2ull * static_cast<uint64_t>(prm.volume_size[0]) *
+                                                       static_cast<uint64_t>(prm.volume_size[1]);
+    if (prm.N > NOALLOC_MAX_PARTICLES || C > NOALLOC_MAX_CELLS)
+        throw HipError{"the allocation-free step takes at most " + std::to_string(NOALLOC_MAX_PARTICLES) +
+                       " particles and " + std::to_string(NOALLOC_MAX_CELLS) + " cells (got " + std::to_string(prm.N) +
+                       " and " + std::to_string(C) + ")"};
+    if (prm.algorithm == 0 && 3ull * prm.n * walls > NOALLOC_MAX_GHOST_FLOATS)
+        throw HipError{"the allocation-free step holds at most " + std::to_string(NOALLOC_MAX_GHOST_FLOATS / 3) +
+                       " wall-ghost slots (n times 2 X Y)"};
+    int ndev = 0;
+    ck(hipGetDeviceCount(&ndev), "hipGetDeviceCount");
+    if (static_cast<int>(prm.device_id) >= ndev) throw HipError{"device_id out of range"};
+    ck(hipSetDevice(static_cast<int>(prm.device_id)), "hipSetDevice");
+    unsigned char* const base = noalloc_scratch(static_cast<int>(prm.device_id));
+    if (prm.algorithm == 1) upload_axes(static_cast<int>(prm.device_id));
+
+    // the system table entry and the launch shape (choose_tiling / layout for one system)
+    KParams kp;
+    fill_geometry(prm, kp);
+    uint32_t const cap = pick_capacity(prm.algorithm, kp.n, 0);
+    kp.tile_cells = std::min<uint32_t>(std::max<uint32_t>(1, (cap + kp.n / 2) / kp.n),
+                                       static_cast<uint32_t>(tile_cell_max(static_cast<int>(cap), prm.algorithm)));
+    kp.step = static_cast<uint32_t>(step_index);
+    grid_shift(kp.rs, kp.step, kp.shift);
+    grid_shift(kp.rs, kp.step + 1, kp.shift_next);
+    auto blocks = [](uint64_t items) { return static_cast<uint32_t>(std::min<uint64_t>((items + WG - 1) / WG, 65536)); };
+    uint32_t const tiles = static_cast<uint32_t>((C + kp.tile_cells - 1) / kp.tile_cells);
+    uint32_t const blk[8] = {0u, blocks(prm.N), 0u, blocks(C), 0u, blocks(walls), 0u, tiles};
+
+    mpcd_hip_s g;   // pointers into `scratch`; owns nothing
+    g.ns = 1;
+    g.algorithm = prm.algorithm;
+    g.N = prm.N;
+    g.C = C;
+    g.W = walls;
+    g.GV = 3ull * prm.n * walls;
+    g.stream = nullptr;
+    g.lds_cap = cap;
+    g.blocks_part = blk[1];
+    g.blocks_cell = blk[3];
+    g.blocks_wall = blk[5];
+    g.ntiles = tiles;
+    g.nb = static_cast<uint32_t>((C + SCAN_CHUNK - 1) / SCAN_CHUNK);
+    g.perm = reinterpret_cast<uint32_t*>(base + NA_PERM);
+    g.keys = reinterpret_cast<uint32_t*>(base + NA_KEYS);
+    g.count = reinterpret_cast<uint32_t*>(base + NA_COUNT);
+    g.offset = reinterpret_cast<uint32_t*>(base + NA_OFFSET);
+    g.cursor = reinterpret_cast<uint32_t*>(base + NA_CURSOR);
+    g.partial = reinterpret_cast<uint32_t*>(base + NA_PARTIAL);
+    g.status = reinterpret_cast<uint32_t*>(base + NA_STATUS);
+    g.axis = reinterpret_cast<float*>(base + NA_AXIS);
+    g.wgcnt = reinterpret_cast<uint32_t*>(base + NA_GCNT);
+    g.wgvel = reinterpret_cast<float*>(base + NA_GVEL);
+    g.d_kp = reinterpret_cast<KParams*>(base + NA_KP);
+    g.d_blk = reinterpret_cast<uint32_t*>(base + NA_BLK);
+    g.sys = Sys{g.d_kp, g.d_blk, g.d_blk + 2, g.d_blk + 4, nullptr, g.d_blk + 6, 1u, static_cast<uint32_t>(C), prm.N};
+    // the result lands in state_out: for an even number of steps the input is
+    // first copied there and state_in is the other buffer
+    bool const even = (n_steps % 2) == 0;
+    g.st[0] = even ? state_out : state_in;
+    g.st[1] = even ? state_in : state_out;
+    g.cur = 0;
+
+    // set-up on the null stream; the host arrays live until the final synchronisation
+    ck(hipMemcpyAsync(g.d_kp, &kp, sizeof(KParams), hipMemcpyHostToDevice, nullptr), "upload system table");
+    ck(hipMemcpyAsync(g.d_blk, blk, sizeof(blk), hipMemcpyHostToDevice, nullptr), "upload block maps");
+    ck(hipMemsetAsync(g.status, 0, 8, nullptr), "clear status");   // error bits and the big-cell count
+    if (even) ck(hipMemcpyAsync(state_out, state_in, 24 * prm.N, hipMemcpyDeviceToDevice, nullptr), "copy state");
+    launch_keys(&g);
+    for (int s = 0; s < n_steps; ++s) {
+        uint32_t const io = (s > 0 ? 1u : 0u) | (s + 1 < n_steps ? 2u : 0u);
+        enqueue_step(&g, false, false, true, nullptr, nullptr, nullptr, io);
+        g.cur = 1 - g.cur;
+    }
+    ck(hipStreamSynchronize(nullptr), "allocation-free step");
+    uint32_t st = 0;
+    ck(hipMemcpy(&st, g.status, 4, hipMemcpyDeviceToHost), "status");
+    if (st & 1u)
+        throw HipError{"a cell holds more particles than one LDS tile (" + std::to_string(cap) +
+                       ") and its collision table (4n) does not fit the tile either"};
+}
+
 }  // namespace
 
 // ================================================================== C-ABI ===
@@ -3458,6 +3601,23 @@ int mpcd_hip_autotune(mpcd_hip_t* h, int steps_per_trial, uint32_t* cells_per_ti
         if (lds_capacity) *lds_capacity = h->lds_cap;
         if (ms_per_step) *ms_per_step = best / steps_per_trial;
     });
+}
+
+int mpcd_hip_step_noalloc(const mpcd_params_t* params, uint64_t step_index, int n_steps, float* state_in,
+                          float* state_out, size_t n_particles) {
+    std::lock_guard<std::mutex> lock(g_noalloc_mu);
+    try {
+        if (!params) throw HipError{"null params"};
+        step_noalloc_impl(*params, step_index, n_steps, state_in, state_out, n_particles);
+        return 0;
+    } catch (HipError const& e) {
+        std::lock_guard<std::mutex> g(g_err_mu);
+        g_create_err = e.msg;
+    } catch (std::exception const& e) {
+        std::lock_guard<std::mutex> g(g_err_mu);
+        g_create_err = e.what();
+    }
+    return 1;
 }
 
 int mpcd_hip_tune_capacities(int algorithm, uint32_t nmax, uint32_t* caps, uint32_t max_caps, uint32_t* n_caps) {

@@ -194,6 +194,31 @@ int mpcd_hip_autotune(mpcd_hip_t* h, int steps_per_trial, uint32_t* cells_per_ti
  * to max_caps of them and their number to *n_caps; needs no device. */
 int mpcd_hip_tune_capacities(int algorithm, uint32_t nmax, uint32_t* caps, uint32_t max_caps, uint32_t* n_caps);
 
+/* ---- allocation-free step (no reference counterpart) ----
+ *
+ * For callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc): n_steps of
+ * the step mpcd_hip_step runs, for one system described by *params, on a
+ * particle state the caller holds in device memory, with the same results
+ * bit for bit as a handle created from the same parameters, holding the same
+ * state at the same step index.  It calls no hipMalloc or hipFree: the step's
+ * sort and per-cell arrays live in a static device array of the library
+ * (15.5 MiB per device, held until the process ends).
+ *
+ * state_in / state_out: device buffers of n_particles records (x, y, z, vx,
+ * vy, vz as 6 floats, 24 bytes per particle), of the HIP runtime this library
+ * uses; the result is written to state_out in cell order (as a handle keeps
+ * it); state_in is overwritten when n_steps > 1.  step_index keys the random
+ * stream (mpcd_hip_get_step_index).  Limits: at most 2^20 particles and 2^17
+ * cells, and for the SRD rule with walls n * 2 X Y at most 2^20 / 3; larger
+ * systems are rejected with an error before any device work.  Runs on the
+ * null stream and returns after the device finished; calls from one process
+ * are serialised, each call leaves the array holding its own data and clears
+ * what the next needs cleared.  Errors: nonzero return, message in
+ * mpcd_hip_last_error(NULL). */
+int mpcd_hip_step_noalloc(const mpcd_params_t* params, uint64_t step_index, int n_steps, float* state_in,
+                          float* state_out, size_t n_particles);
+
 /* Copies `bytes` between two device buffers `reps` times with a plain float4
  * kernel and returns the mean time per copy: the practical HBM reference. */
 int mpcd_hip_bench_copy(size_t bytes, int reps, double* ms_per_copy);
