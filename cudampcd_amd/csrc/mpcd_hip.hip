@@ -73,6 +73,7 @@ struct KParams {
     float r2_init, r2_chan;  // its squared radius at initialisation (from X) and in streaming (from Z)
     float dt, drag_dt, tv, sin_a, cos_a;
     uint32_t n, cap;     // particles per cell, cell table capacity 4n
+    uint32_t mn[2];      // division magic for n (n > 1)
     Stream rs;
     uint32_t tile_cells; // cells per collide tile
     uint32_t N;          // particles of this system
@@ -809,14 +810,15 @@ constexpr int tile_cell_max(int cap, int algo) {
 // floats of the ghost region: extended ghosts (per coordinate), SRD staged
 // wall-cell ghost velocities (slightly less than half the capacity for the
 // 384-particle tile, to stay within 26 LDS granules).
-// Extended 512-particle tiles keep 112 ghost slots: a 48-cell batch rarely
+// Extended 512-particle tiles keep 100 ghost slots: a 48-cell batch rarely
 // holds more than a few wall cells, a batch whose ghosts would overflow ends
 // early, and the smaller region lets eight workgroups share a CU instead of
 // six (500x500x40 cells, n = 10: 256 slots 4.63 ms per step, 192 slots and
-// seven workgroups 4.16, 112 slots and eight 3.79).  A region must hold one
-// wall cell's n ghosts (choose_tiling).
+// seven workgroups 4.16, 112 slots and eight 3.79; 100 leave room for the
+// ghosts' thermal Gaussians at eight).  A region must hold one wall cell's n
+// ghosts (choose_tiling).
 constexpr int ghost_floats(int cap, int algo) {
-    return algo == 0 && cap == 384 ? cap / 2 - 32 : (algo == 1 && cap == 512 ? 112 : cap / 2);
+    return algo == 0 && cap == 384 ? cap / 2 - 32 : (algo == 1 && cap == 512 ? 100 : cap / 2);
 }
 
 // threads of the collide workgroup for an LDS capacity: more, smaller
@@ -841,20 +843,31 @@ struct Tile {
     uint32_t perm[CAP];              // source slot, arrival order; later scratch
     float cm[6][TC];                 // per cell: mean velocity xyz, axis xyz
     uint32_t off[TC + 1];            // cell offsets of the tile
-    uint32_t go[TC];                 // collision happened
+    uint8_t go[TC];                  // collision happened
     uint32_t cref[TC];               // reference cell index
     float cforce[ALGO == 0 ? TC : 1];  // SRD adjoint runs: force term f dt of each cell
     uint32_t gcnt[TC];               // ghost count
     uint32_t goff[TC];               // first ghost slot
     uint32_t wcell[TC];              // SRD: cell of each staged wall slot; extended: first ghost slot
     uint32_t wpre[ALGO == 1 ? TC + 1 : 1];  // extended: wall cells of the tile before each cell
-    uint32_t sw[4];                  // scan scratch
+    // extended: thermal draws (quads) of the batch before each cell (at most
+    // (CAP + GS) / 4 + TC: a byte up to the 512 tile)
+    std::conditional_t<(CAP <= 512), uint8_t, uint16_t> qb[ALGO == 1 ? TC + 1 : 1];
+    uint8_t wk[ALGO == 1 ? TC : 1];         // extended: cell of each wall cell of the batch, in order
     uint8_t cell[CAP];               // tile-local cell of each slot
     static constexpr int GS = ghost_floats(CAP, ALGO);
     float g[ALGO == 1 ? 6 : 1][GS];  // extended: ghosts, n slots per wall cell of the batch;
                                           // SRD: staged ghost velocities
+    float gg[ALGO == 1 ? GS : 1];    // extended: the ghosts' thermal Gaussians
     uint32_t nwall;                  // SRD: wall cells of the batch staged so far
 };
+// eight extended 512-particle workgroups share the 160 KiB of a CU (beside
+// the tile: the unsampled bins, 32 bytes, and the workgroup reduction of
+// __syncthreads_count, about 230 bytes)
+static_assert(sizeof(Tile<512, 1>) + 32 + 240 <= 20480, "extended 512 tile within a CU's eighth of the LDS");
+static_assert((512 + ghost_floats(512, 1)) / 4 + tile_cell_max(512, 1) < 256, "quad prefix of the 512 tile in a byte");
+// twelve SRD 384-particle workgroups (26 LDS granules of 512 bytes each)
+static_assert(sizeof(Tile<384, 0>) + 64 <= 13312, "SRD 384 tile within 26 LDS granules");
 
 // sampled steps: per-batch LDS bins of the mean-field accumulators (count and
 // the three fixed-point velocity sums of each cell, indexed by cell key)
@@ -998,14 +1011,18 @@ __device__ void srd_cells(KParams const& P, Tile<CAP, 0>& s, uint32_t ncell, uin
     }
 }
 
-// Extended rule (simulation_functions.cpp:207-406) in three stages so that
-// the draws run on all lanes and only the order-sensitive sums stay serial:
+// Extended rule (simulation_functions.cpp:207-406) in stages so that the
+// draws run on densely packed lanes and only the order-sensitive sums stay
+// serial:
 //   ext_cell_setup   one lane per cell: wall flags, ghost count (:137-146),
 //                    table size; stores ref, ghost count (bit 31: upper wall
 //                    layer) and the number of table particles kept
-//   ext_ghost        one lane per (cell, ghost): position and velocity (:147-167)
-//   ext_thermal      one lane per four table particles: their thermal Gaussians
-//                    (draw keyed by particle index / 4, as in the serial rule)
+//   ext_ghost        one lane per ghost slot of the batch's wall cells:
+//                    position and velocity (:147-167)
+//   thermal quads    one lane per draw of four Gaussians (draw q of a cell
+//                    covers its members 4q .. 4q + 3, table particles then
+//                    ghosts, as in the serial rule), for every member of each
+//                    cell that collides
 //   extended_cell    one lane per cell: the collision itself
 constexpr uint32_t UPPER_LAYER = 0x80000000u;
 
@@ -1173,17 +1190,9 @@ __device__ void extended_cell(KParams const& P, Tile<CAP, 1>& s, uint32_t k, uin
         float const t1 = proj0 * (static_cast<float>(size0) / static_cast<float>(size1));
         float mean0 = 0.0f, mean1 = 0.0f;
         Sym I = {0, 0, 0, 0, 0, 0};
-        float gq[4] = {0, 0, 0, 0};
         members(one{}, [&](float* base, uint32_t S, uint32_t li, uint32_t i, float const* x) {
-            float gi;
-            if (i < rr) {
-                gi = gauss[a + i];
-            } else {
-                // ghosts: the rest of the quad the last particles started, or a new one
-                if (i == rr || (i & 3u) == 0) thermal_quad(P, ref, i >> 2, gq);
-                uint32_t const qi = i & 3u;
-                gi = qi == 0 ? gq[0] : (qi == 1 ? gq[1] : (qi == 2 ? gq[2] : gq[3]));
-            }
+            // member i's Gaussian of quad i / 4 (drawn for the batch beforehand)
+            float const gi = i < rr ? gauss[a + li] : s.gg[gb + li];
             float const rx = x[0], ry = x[1], rz = x[2];
             float ux = x[3], uy = x[4], uz = x[5];
             bool const side = rx * ax + ry * ay + rz * az < 0;
@@ -1529,27 +1538,56 @@ __device__ __forceinline__ void collide_tile(Sys const& sys, const float* __rest
         } else {
             for (uint32_t k = tid; k < ncell; k += NT) {
                 ext_cell_setup<CAP>(P, s, k, s.off[kb + k + 1] - s.off[kb + k], c0 + kb + k);
-                s.wcell[k] = (s.wpre[kb + k] - s.wpre[kb]) * P.n;
+                uint32_t const w = s.wpre[kb + k] - s.wpre[kb];
+                s.wcell[k] = w * P.n;
+                if (s.wpre[kb + k + 1] != s.wpre[kb + k]) s.wk[w] = static_cast<uint8_t>(k);
             }
             __syncthreads();
-            // ghosts, one lane per (cell, ghost slot)
-            for (uint32_t t = tid; t < ncell * P.n; t += NT) {
-                uint32_t const k = t / P.n, g = t - k * P.n;
+            // thermal draws of the batch: quads of four Gaussians covering the
+            // members (table particles, then ghosts) of every cell that
+            // collides, counted before each cell (at most 64 cells: one wave)
+            static_assert(Tile<CAP, ALGO>::TC <= 64, "the quad prefix is one wave's scan");
+            if (tid < 64) {
+                uint32_t q = 0;
+                if (tid < ncell && s.go[tid]) q = (s.goff[tid] + (s.gcnt[tid] & ~UPPER_LAYER) + 3u) >> 2;
+#pragma unroll
+                for (int d = 1; d < 64; d <<= 1) {
+                    uint32_t const y = __shfl_up(q, d, 64);
+                    if (tid >= static_cast<uint32_t>(d)) q += y;
+                }
+                if (tid < ncell) s.qb[tid + 1] = static_cast<std::remove_reference_t<decltype(s.qb[0])>>(q);
+                if (tid == 0) s.qb[0] = 0;
+            }
+            __syncthreads();
+            // ghosts, one lane per ghost slot (n slots per wall cell of the batch)
+            uint32_t const nslot = (s.wpre[kb + ncell] - s.wpre[kb]) * P.n;
+            for (uint32_t t = tid; t < nslot; t += NT) {
+                uint32_t const w = P.n > 1 ? udiv(t, P.mn) : t;
+                uint32_t const g = t - w * P.n;
+                uint32_t const k = s.wk[w];
                 if (g < (s.gcnt[k] & ~UPPER_LAYER)) ext_ghost<CAP>(P, s, k, g, c0 + kb + k);
             }
-            // thermal Gaussians of the table particles, one lane per four
-            // (the permutation scratch is free until the histogram below)
+            // thermal Gaussians, one lane per quad: the table particles' into the
+            // permutation scratch by slot (free until the histogram below), the
+            // ghosts' by ghost slot
             float* const gauss = reinterpret_cast<float*>(s.perm);
-            for (uint32_t i = tid; i < m; i += NT) {
-                uint32_t const k = s.cell[i];
-                uint32_t const a = s.off[kb + k] - gb, li = i - a;
-                uint32_t const rr = s.goff[k];
-                if ((li & 3u) == 0 && li < rr && s.go[k]) {
-                    float gq[4];
-                    thermal_quad(P, s.cref[k], li >> 2, gq);
+            uint32_t const nq = s.qb[ncell];
+            for (uint32_t q = tid; q < nq; q += NT) {
+                uint32_t lo = 0, hi = ncell;   // the cell of quad q: the last k with qb[k] <= q
+                while (hi - lo > 1) {
+                    uint32_t const mid = (lo + hi) >> 1;
+                    if (s.qb[mid] <= q) lo = mid; else hi = mid;
+                }
+                uint32_t const k = lo, j = q - s.qb[k];
+                float gq[4];
+                thermal_quad(P, s.cref[k], j, gq);
+                uint32_t const a = s.off[kb + k] - gb, rr = s.goff[k];
+                uint32_t const tot = rr + (s.gcnt[k] & ~UPPER_LAYER);
 #pragma unroll
-                    for (uint32_t u = 0; u < 4; ++u)
-                        if (li + u < rr) gauss[i + u] = gq[u];
+                for (uint32_t u = 0; u < 4; ++u) {
+                    uint32_t const i = 4 * j + u;
+                    if (i < rr) gauss[a + i] = gq[u];
+                    else if (i < tot) s.gg[s.wcell[k] + (i - rr)] = gq[u];
                 }
             }
             __syncthreads();
@@ -2229,6 +2267,7 @@ void fill_geometry(mpcd_params_t const& p, KParams& k) {
     k.cos_a = cosf((M_PI * static_cast<double>(p.rotation_angle)) / 180);
     k.n = p.n;
     k.cap = 4 * p.n;
+    udiv_magic(p.n, k.mn);
     k.rs.k0 = static_cast<uint32_t>(p.seed);
     k.rs.k1 = static_cast<uint32_t>(p.seed >> 32);
     k.rs.sys = p.system_id;

@@ -231,7 +231,7 @@ def test_determinism_and_tiling_independence(pympcd):
 
 def test_extended_tiling_independence_and_ghost_bound(pympcd):
     """Extended rule, 5 cell layers (2 of them wall layers), n = 40: a 32-cell
-    tile holds about 13 wall cells (520 ghosts), more than the 112-slot ghost
+    tile holds about 13 wall cells (520 ghosts), more than the 100-slot ghost
     region of a 512 tile (room for two wall cells), so batches are cut by the
     ghost bound as well as by the particle capacity.  Every tiling gives the same bits, and one step
     with the smallest tiles matches the oracle bitwise."""

@@ -1,14 +1,19 @@
 """Physics checks of the HIP backend at sizes the oracle cannot reach.
 
 * Poiseuille flow at the reference's README geometry (10x10x100 fluid cells,
-  walls in z, n=10, T=1, g=1e-3, dt=0.02; README.md:52-102): viscosity within
-  2% of the SRD formula (37.50) and of the reference's measurement (37.27).
-  The flow is sampled once developed (slowest mode relaxes in about 13500
-  steps from rest).
+  walls in z, n=10, T=1, g=1e-3, dt=0.02; README.md:52-102): under the
+  reference's sampling protocols (1000 + 10,000 and 1000 + 100,000 steps) the
+  viscosity estimate matches the reference CPU backend's under the same
+  protocols; sampled once developed (the slowest mode relaxes in about 13500
+  steps from rest) it lies within 2% of the SRD formula (37.50) and of the
+  reference's printed measurement (37.27).
 * Conservation with periodic boundaries and no drag: total momentum (SRD and
   extended) and kinetic energy (SRD) to FP32 round-off; drag adds N g dt.
 """
 from __future__ import annotations
+
+import json
+from pathlib import Path
 
 import numpy as np
 import pytest
@@ -21,23 +26,64 @@ from oracle_lib import Oracle
 pytestmark = pytest.mark.gpu
 
 
+REFERENCE_C1 = json.loads((Path(__file__).parent / "golden" / "c1_reference_cpu.json").read_text())
+
+
+def _reference(protocol):
+    """The reference CPU backend's viscosity under a protocol: the mean over its seeds."""
+    r = REFERENCE_C1[f"reference_{protocol}"]
+    return r.get("eta_mean", r["eta"]), np.array(r.get("profile_x_mean", r["profile_x"]))
+
+
 def test_poiseuille_viscosity_within_2_percent(pympcd):
-    n, g, dt = 10, 0.001, 0.02
-    p = make_params(pympcd, (10, 10, 100), (1, 1, 0), n=n, seed=777, drag=g, delta_t=dt)
-    sim = pympcd.Simulation(p, "hip")
-    sim.step(1000)
-    sim.step_and_sample(10000)
-    _, vel_short = sim.get_mean_fields()
-    eta_short = poiseuille_viscosity(vel_short, n, g, 100)
-    sim.step(60000)
-    sim.step_and_sample(40000)
-    dens, vel = sim.get_mean_fields()
-    eta = poiseuille_viscosity(vel, n, g, 100)
+    """Sampled from the start of the flow, as the reference's protocols do, the
+    viscosity estimate carries the flow's development: the reference CPU
+    backend itself reads about 113 after 1000 + 10,000 steps and 42 after
+    1000 + 100,000 (the README's protocol, whose printed 37.27 it does not
+    reproduce; tests/golden/c1_reference_cpu.json).  One run's estimate
+    scatters by about 4% and 0.6% under these protocols, so eight seeds run as
+    one ensemble and their mean is compared with the reference's mean under
+    the same protocol; once the flow is developed the mean lies within 2% of
+    theory (37.50) and of the README's figure."""
+    n, g, dt, K = 10, 0.001, 0.02, 8
     theory = srd_viscosity(n, dt)
-    print(f"eta developed={eta:.3f} (1000+10000 protocol: {eta_short:.3f}) theory={theory:.3f} reference=37.27")
-    assert abs(eta - theory) / theory < 0.02
-    assert abs(eta - 37.27) / 37.27 < 0.02
-    assert abs(dens[:, :, 1:-1].mean() - n) < 0.05
+
+    def channels(seed0):
+        ps = []
+        for i in range(K):
+            p = make_params(pympcd, (10, 10, 100), (1, 1, 0), n=n, seed=seed0 + i, drag=g, delta_t=dt)
+            p.system_id = i
+            ps.append(p)
+        return pympcd.Ensemble(ps)
+
+    def etas(ens):
+        return np.array([poiseuille_viscosity(ens.get_mean_fields(i)[1], n, g, 100) for i in range(K)])
+
+    a = channels(777)
+    a.step(1000)
+    a.step_and_sample(10000)
+    eta_c1 = etas(a)
+    a.reset_sampling()
+    a.step(50000)
+    a.step_and_sample(40000)
+    eta_dev = etas(a)
+    dens = np.mean([a.get_mean_fields(i)[0][:, :, 1:-1].mean() for i in range(K)])
+    b = channels(1777)
+    b.step(1000)
+    b.step_and_sample(100000)
+    eta_readme = etas(b)
+    prof = np.mean([b.get_mean_fields(i)[1][..., 0].astype(np.float64).mean(axis=(0, 1)) for i in range(K)], axis=0)
+    ref_c1, _ = _reference("c1")
+    ref_readme, ref_prof = _reference("readme")
+    print(f"eta over {K} seeds: 1000+10000 {eta_c1.mean():.2f} +- {eta_c1.std(ddof=1):.2f} (reference CPU {ref_c1:.2f}); "
+          f"1000+100000 {eta_readme.mean():.3f} +- {eta_readme.std(ddof=1):.3f} (reference CPU {ref_readme:.3f}); "
+          f"developed {eta_dev.mean():.3f} +- {eta_dev.std(ddof=1):.3f}; theory {theory:.3f}; README 37.27")
+    assert abs(eta_c1.mean() - ref_c1) / ref_c1 < 0.05
+    assert abs(eta_readme.mean() - ref_readme) / ref_readme < 0.02
+    assert np.abs(prof[1:-1] - ref_prof[1:-1]).max() < 0.02 * ref_prof.max()
+    assert abs(eta_dev.mean() - theory) / theory < 0.02
+    assert abs(eta_dev.mean() - 37.27) / 37.27 < 0.02
+    assert abs(dens - n) < 0.05
 
 
 @pytest.mark.parametrize("algorithm", ["srd", "extended"])

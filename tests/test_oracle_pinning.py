@@ -521,3 +521,27 @@ def test_oracle_force_field_matches_drag():
     # force adds momentum n_collided * f dt per step in x
     dv = o2.particles()[1][:, 0].astype(np.float64).sum() - o1.particles()[1][:, 0].astype(np.float64).sum()
     assert dv > 0
+
+
+def test_reference_fp32_kahan_accumulation_error_is_stated():
+    """The reference accumulates the per-step mean fields in FP32 with Kahan
+    compensation (cpu_backend.cpp:85-107); this build in FP64.  On the
+    reference trajectory (the oracle in reference mode follows it bit for bit)
+    of the README Poiseuille case, the two differ by at most 3.1e-8 in any
+    cell's mean velocity after 100,000 samples and give the same viscosity to
+    7 digits (tests/golden/c1_reference_cpu.json, written by
+    tests/golden/make_c1_fixture.py); the oracle's FP32 path reproduces the
+    reference's own mean exactly."""
+    import json
+    from pathlib import Path
+
+    d = json.loads((Path(__file__).parent / "golden" / "c1_reference_cpu.json").read_text())
+    for k in ("c1", "readme"):
+        acc = d[f"accumulation_{k}"]
+        assert acc["eta_fp32_kahan"] == d[f"reference_{k}"]["eta"]
+        assert abs(acc["eta_fp64"] - acc["eta_fp32_kahan"]) < 1e-6 * acc["eta_fp64"]
+        assert acc["max_abs_velocity_diff"] < 5e-8
+        assert acc["max_abs_density_diff"] < 1e-6
+    # the protocols' own viscosity: the developing flow biases the estimate up
+    assert d["reference_c1"]["eta"] > 2.5 * d["theory_eta"]
+    assert 1.05 * d["theory_eta"] < d["reference_readme"]["eta"] < 1.2 * d["theory_eta"]
