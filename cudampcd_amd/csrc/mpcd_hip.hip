@@ -78,7 +78,7 @@ struct KParams {
     uint32_t tile_cells; // cells per collide tile
     uint32_t N;          // particles of this system
     uint32_t cb, pb, wb; // first cell key, particle slot, wall cell in the shared arrays
-    uint64_t gvb;        // first ghost-velocity float of this system
+    uint64_t gvb;        // first ghost float of this system (3 per slot SRD, 6 extended)
     // step state: written by the host at set-up, advanced on the device by k_advance
     uint32_t step;
     float shift[3], shift_next[3];
@@ -723,9 +723,12 @@ __device__ __forceinline__ void axes_block(Sys const& sys, float* __restrict__ a
 // A block takes WG wall cells at a time: one lane per cell draws the ghost
 // count, then the cells' n ghost slots are spread over all lanes (slot g of
 // cell k on lane k n + g), so the velocity draws run on full waves and
-// neighbouring lanes store neighbouring ghosts.
+// neighbouring lanes store neighbouring ghosts.  SRD: 3 floats per slot
+// (velocity); extended rule (ext): 6 (position in the cell, then velocity),
+// the positions' z in the part of the cell the reference's rejection loop
+// accepts (simulation_functions.cpp:147-167).
 __device__ __forceinline__ void ghosts_block(Sys const& sys, uint32_t* __restrict__ gcnt, float* __restrict__ gvel,
-                                             uint32_t blk) {
+                                             uint32_t blk, bool ext) {
     __shared__ uint32_t s_cnt[WG];
     uint32_t const s = find_sys(sys.blk_wall, sys.ns, blk);
     KParams const P = sys.P[s];
@@ -750,14 +753,34 @@ __device__ __forceinline__ void ghosts_block(Sys const& sys, uint32_t* __restric
         s_cnt[threadIdx.x] = added;
         __syncthreads();
         uint32_t const ncell = min(static_cast<uint32_t>(WG), 2 * XY - c0);
+        uint32_t const per = ext ? 6u : 3u;
         for (uint32_t t = threadIdx.x; t < ncell * P.n; t += WG) {
             uint32_t const k = t / P.n, g = t - k * P.n;
             if (g < s_cnt[k]) {
-                Words const v = P.rs.at(P_GHOST_VEL, g, ref_of_wall(c0 + k), P.step);
+                uint32_t const w = c0 + k, ref = ref_of_wall(w);
+                Words const v = P.rs.at(P_GHOST_VEL, g, ref, P.step);
                 float g0, g1, g2, g3;
                 box_muller(v.w[0], v.w[1], &g0, &g1);
                 box_muller(v.w[2], v.w[3], &g2, &g3);
-                float* const out = gvel + P.gvb + 3ull * P.n * c0 + 3ull * t;
+                float* out = gvel + P.gvb + static_cast<uint64_t>(per) * P.n * c0 + static_cast<uint64_t>(per) * t;
+                if (ext) {
+                    // ghost position: uniform in x and y, z in the accepted
+                    // part of the cell, about the cell centre (volume_container.hpp:86-91)
+                    bool const layer = w >= XY;
+                    uint32_t const wl = layer ? w - XY : w;
+                    uint32_t const iy = wl / P.dim[0], ix = wl - iy * P.dim[0];
+                    uint32_t const iz = layer ? iz_layer : static_cast<uint32_t>(sign);
+                    float const ox = (0.5f + static_cast<float>(ix)) - P.half[0];
+                    float const oy = (0.5f + static_cast<float>(iy)) - P.half[1];
+                    float const oz = (0.5f + static_cast<float>(iz)) - P.half[2];
+                    Words const r = P.rs.at(P_GHOST_POS, g, ref, P.step);
+                    float const uz = uniform(r.w[2]);
+                    float const z = (layer ^ sign) ? sh * uz : sh + (1.0f - sh) * uz;
+                    out[0] = (uniform(r.w[0]) - 0.5f) + ox;
+                    out[1] = (uniform(r.w[1]) - 0.5f) + oy;
+                    out[2] = (z - 0.5f) + oz;
+                    out += 3;
+                }
                 out[0] = g0 * P.tv;
                 out[1] = g1 * P.tv;
                 out[2] = g2 * P.tv;
@@ -767,12 +790,12 @@ __device__ __forceinline__ void ghosts_block(Sys const& sys, uint32_t* __restric
     }
 }
 
-// The SRD per-cell draws of a step in one launch: blocks below nbw draw the
-// wall cells' ghosts (fewer, compute-heavy blocks, dispatched first), the
-// rest the cells' axes (memory-heavy), so the two kinds overlap.
+// The per-cell draws of a step in one launch: blocks below nbw draw the wall
+// cells' ghosts (fewer, compute-heavy blocks, dispatched first), the rest (SRD
+// only) the cells' axes (memory-heavy), so the two kinds overlap.
 __global__ __launch_bounds__(WG) void k_cell_draws(Sys sys, float* __restrict__ axis, uint32_t* __restrict__ gcnt,
-                                                   float* __restrict__ gvel, uint32_t nbw) {
-    if (blockIdx.x < nbw) ghosts_block(sys, gcnt, gvel, blockIdx.x);
+                                                   float* __restrict__ gvel, uint32_t nbw, int ext) {
+    if (blockIdx.x < nbw) ghosts_block(sys, gcnt, gvel, blockIdx.x, ext != 0);
     else axes_block(sys, axis, blockIdx.x - nbw);
 }
 
@@ -1017,8 +1040,9 @@ __device__ void srd_cells(KParams const& P, Tile<CAP, 0>& s, uint32_t ncell, uin
 //   ext_cell_setup   one lane per cell: wall flags, ghost count (:137-146),
 //                    table size; stores ref, ghost count (bit 31: upper wall
 //                    layer) and the number of table particles kept
-//   ext_ghost        one lane per ghost slot of the batch's wall cells:
-//                    position and velocity (:147-167)
+//   ext_ghost        one lane per ghost slot of the batch's wall cells: its
+//                    position and velocity (:147-167), drawn for the step by
+//                    k_cell_draws like the ghost count
 //   thermal quads    one lane per draw of four Gaussians (draw q of a cell
 //                    covers its members 4q .. 4q + 3, table particles then
 //                    ghosts, as in the serial rule), for every member of each
@@ -1037,16 +1061,16 @@ __device__ __forceinline__ bool ghost_layer(KParams const& P, uint32_t iz, bool&
 }
 
 template <int CAP>
-__device__ void ext_cell_setup(KParams const& P, Tile<CAP, 1>& s, uint32_t k, uint32_t mc, uint32_t key) {
+__device__ void ext_cell_setup(KParams const& P, Tile<CAP, 1>& s, uint32_t k, uint32_t mc, uint32_t key,
+                               uint32_t const* __restrict__ gcnt) {
     uint32_t ix, iy, iz;
     decode_key(P, key, ix, iy, iz);
     uint32_t const ref = ref_of(P, ix, iy, iz);
-    float const sh = fabsf(P.shift[2]);
-    bool const sign = P.shift[2] > 0;
     bool layer;
     bool const ghosts = ghost_layer(P, iz, layer);
     uint32_t const reals_in = mc < P.cap ? mc : P.cap;
-    uint32_t const added = ghosts ? ghost_count(P, ref, sh, layer, sign) : 0u;
+    // ghost count drawn for the step by k_cell_draws (wall cell index w)
+    uint32_t const added = ghosts ? gcnt[P.wb + (layer ? P.dim[0] * P.dim[1] : 0u) + ix + P.dim[0] * iy] : 0u;
     uint32_t total = reals_in + added;
     if (total > P.cap) total = P.cap;
     s.cref[k] = ref;
@@ -1055,32 +1079,24 @@ __device__ void ext_cell_setup(KParams const& P, Tile<CAP, 1>& s, uint32_t k, ui
     s.go[k] = total > 1;
 }
 
+// ghost g of wall cell k of the batch, drawn for the step by k_cell_draws
+// (position and velocity, 6 floats), into its ghost slot
 template <int CAP>
-__device__ void ext_ghost(KParams const& P, Tile<CAP, 1>& s, uint32_t k, uint32_t g, uint32_t key) {
+__device__ __forceinline__ void ext_ghost(KParams const& P, Tile<CAP, 1>& s, uint32_t k, uint32_t g, uint32_t key,
+                                          float const* __restrict__ gvel) {
     uint32_t ix, iy, iz;
     decode_key(P, key, ix, iy, iz);
-    uint32_t const ref = s.cref[k];
-    float const sh = fabsf(P.shift[2]);
-    bool const sign = P.shift[2] > 0;
     bool const layer = (s.gcnt[k] & UPPER_LAYER) != 0u;
-    // cell centre (volume_container.hpp:86-91)
-    float const ox = (0.5f + static_cast<float>(ix)) - P.half[0];
-    float const oy = (0.5f + static_cast<float>(iy)) - P.half[1];
-    float const oz = (0.5f + static_cast<float>(iz)) - P.half[2];
-    uint32_t const o = s.wcell[k] + g;   // the cell's first ghost slot
-    Words const r = P.rs.at(P_GHOST_POS, g, ref, P.step);
-    float const uz = uniform(r.w[2]);
-    float const z = (layer ^ sign) ? sh * uz : sh + (1.0f - sh) * uz;
-    s.g[0][o] = (uniform(r.w[0]) - 0.5f) + ox;
-    s.g[1][o] = (uniform(r.w[1]) - 0.5f) + oy;
-    s.g[2][o] = (z - 0.5f) + oz;
-    Words const v = P.rs.at(P_GHOST_VEL, g, ref, P.step);
-    float g0, g1, g2, g3;
-    box_muller(v.w[0], v.w[1], &g0, &g1);
-    box_muller(v.w[2], v.w[3], &g2, &g3);
-    s.g[3][o] = g0 * P.tv;
-    s.g[4][o] = g1 * P.tv;
-    s.g[5][o] = g2 * P.tv;
+    uint32_t const w = (layer ? P.dim[0] * P.dim[1] : 0u) + ix + P.dim[0] * iy;
+    float2 const* const q = reinterpret_cast<float2 const*>(gvel + P.gvb + 6ull * (static_cast<uint64_t>(P.n) * w + g));
+    float2 const a = q[0], b = q[1], c = q[2];
+    uint32_t const o = s.wcell[k] + g;   // the cell's first ghost slot + g
+    s.g[0][o] = a.x;
+    s.g[1][o] = a.y;
+    s.g[2][o] = b.x;
+    s.g[3][o] = b.y;
+    s.g[4][o] = c.x;
+    s.g[5][o] = c.y;
 }
 
 // the four thermal Gaussians of draw `q` of cell `ref`
@@ -1537,7 +1553,7 @@ __device__ __forceinline__ void collide_tile(Sys const& sys, const float* __rest
             srd_cells<CAP, ADJ>(P, s, ncell, s.off + kb, gb, gvel, c0 + kb, T, CT);
         } else {
             for (uint32_t k = tid; k < ncell; k += NT) {
-                ext_cell_setup<CAP>(P, s, k, s.off[kb + k + 1] - s.off[kb + k], c0 + kb + k);
+                ext_cell_setup<CAP>(P, s, k, s.off[kb + k + 1] - s.off[kb + k], c0 + kb + k, gcnt);
                 uint32_t const w = s.wpre[kb + k] - s.wpre[kb];
                 s.wcell[k] = w * P.n;
                 if (s.wpre[kb + k + 1] != s.wpre[kb + k]) s.wk[w] = static_cast<uint8_t>(k);
@@ -1565,7 +1581,7 @@ __device__ __forceinline__ void collide_tile(Sys const& sys, const float* __rest
                 uint32_t const w = P.n > 1 ? udiv(t, P.mn) : t;
                 uint32_t const g = t - w * P.n;
                 uint32_t const k = s.wk[w];
-                if (g < (s.gcnt[k] & ~UPPER_LAYER)) ext_ghost<CAP>(P, s, k, g, c0 + kb + k);
+                if (g < (s.gcnt[k] & ~UPPER_LAYER)) ext_ghost<CAP>(P, s, k, g, c0 + kb + k, gvel);
             }
             // thermal Gaussians, one lane per quad: the table particles' into the
             // permutation scratch by slot (free until the histogram below), the
@@ -2103,7 +2119,7 @@ __global__ void k_selftest_math(const float* in, float* out, size_t n, int which
 // per device (noalloc_scratch below).
 constexpr uint64_t NOALLOC_MAX_PARTICLES = uint64_t(1) << 20;
 constexpr uint64_t NOALLOC_MAX_CELLS = uint64_t(1) << 17;
-constexpr uint64_t NOALLOC_MAX_GHOST_FLOATS = uint64_t(1) << 20;   // SRD wall-ghost velocities: 3 n 2 X Y
+constexpr uint64_t NOALLOC_MAX_GHOST_FLOATS = uint64_t(1) << 20;   // wall ghosts: (3 SRD, 6 extended) n 2 X Y
 constexpr uint64_t noalloc_align(uint64_t b) { return (b + 255) / 256 * 256; }
 // byte offsets of the regions
 constexpr uint64_t NA_KP = 0;
@@ -2166,8 +2182,8 @@ struct mpcd_hip_s {
     uint32_t* perm_rec = nullptr;
     bool record = false, recorded = false;
     float* axis = nullptr;           // SRD per-cell axes of the current step
-    uint32_t* wgcnt = nullptr;       // SRD wall-cell ghost counts
-    float* wgvel = nullptr;          // SRD wall-cell ghost velocities
+    uint32_t* wgcnt = nullptr;       // wall-cell ghost counts of the step
+    float* wgvel = nullptr;          // wall-cell ghosts of the step: velocities (SRD), positions and velocities (extended)
     // system table and block -> system maps on the device
     KParams* d_kp = nullptr;
     uint32_t* d_blk = nullptr;       // blk_part | blk_cell | blk_wall | tile0, ns + 1 entries each
@@ -2403,7 +2419,7 @@ void layout(mpcd_hip_s* h) {
         cb += k.C;
         pb += k.N;
         wb += walls;
-        gvb += 3ull * k.n * walls;
+        gvb += (h->algorithm == 1 ? 6ull : 3ull) * k.n * walls;   // floats per ghost slot
     }
     h->blocks_part = bp[ns];
     h->blocks_cell = bc[ns];
@@ -2458,10 +2474,11 @@ int guarded(mpcd_hip_s* h, F&& f) {
 
 // the SRD per-cell draws of the step in the system table (k_cell_draws)
 int launch_draws(mpcd_hip_s* h) {
-    uint32_t const nb = h->blocks_wall + h->blocks_cell;
+    // the extended rule draws its ghosts here and its axes in the collision
+    uint32_t const nb = h->blocks_wall + (h->algorithm == 0 ? h->blocks_cell : 0u);
     if (!nb) return 0;
     hipLaunchKernelGGL(k_cell_draws, dim3(nb), dim3(WG), 0, h->stream, h->sys, h->axis, h->wgcnt, h->wgvel,
-                       h->blocks_wall);
+                       h->blocks_wall, h->algorithm);
     ck(hipGetLastError(), "cell draws");
     return 1;
 }
@@ -2594,9 +2611,7 @@ int enqueue_step(mpcd_hip_s* h, bool sample, bool first, bool sort, uint32_t* re
                  uint32_t io) {
     int launches = 0;
     if (sort) launches += launch_sort(h);
-    if (h->algorithm == 0) {
-        launches += launch_draws(h);
-    }
+    launches += launch_draws(h);
     if (e0) ck(hipEventRecord(e0, h->stream), "event");
     if (h->algorithm == 0) {
         if (sample) launch_collide_cap<0, true>(h, rec, io); else launch_collide_cap<0, false>(h, rec, io);
@@ -3096,11 +3111,9 @@ int create_impl(const mpcd_params_t* params, uint32_t n_systems, mpcd_hip_t** ou
         ck(hipMalloc(&h->count, 4 * h->C), "alloc counts");
         ck(hipMalloc(&h->offset, 4 * (h->C + 1)), "alloc offsets");
         ck(hipMalloc(&h->cursor, 4 * h->C), "alloc cursors");
-        if (h->algorithm == 0) {
-            ck(hipMalloc(&h->axis, 12 * h->C), "alloc axes");
-            ck(hipMalloc(&h->wgcnt, 4 * std::max<uint64_t>(h->W, 1)), "alloc ghost counts");
-            ck(hipMalloc(&h->wgvel, 4 * std::max<uint64_t>(h->GV, 1)), "alloc ghost velocities");
-        }
+        if (h->algorithm == 0) ck(hipMalloc(&h->axis, 12 * h->C), "alloc axes");
+        ck(hipMalloc(&h->wgcnt, 4 * std::max<uint64_t>(h->W, 1)), "alloc ghost counts");
+        ck(hipMalloc(&h->wgvel, 4 * std::max<uint64_t>(h->GV, 1)), "alloc ghost draws");
         h->nb = static_cast<uint32_t>((h->C + SCAN_CHUNK - 1) / SCAN_CHUNK);
         ck(hipMalloc(&h->partial, 4 * std::max<uint32_t>(h->nb, 1)), "alloc scan partials");
         ck(hipMalloc(&h->status, 4 * big_list_words(h->N)), "alloc status and big-cell list");
@@ -3202,8 +3215,9 @@ void step_noalloc_impl(mpcd_params_t const& prm, uint64_t step_index, int n_step
         throw HipError{"the allocation-free step takes at most " + std::to_string(NOALLOC_MAX_PARTICLES) +
                        " particles and " + std::to_string(NOALLOC_MAX_CELLS) + " cells (got " + std::to_string(prm.N) +
                        " and " + std::to_string(C) + ")"};
-    if (prm.algorithm == 0 && 3ull * prm.n * walls > NOALLOC_MAX_GHOST_FLOATS)
-        throw HipError{"the allocation-free step holds at most " + std::to_string(NOALLOC_MAX_GHOST_FLOATS / 3) +
+    uint64_t const per_ghost = prm.algorithm == 1 ? 6 : 3;   // floats per ghost slot
+    if (per_ghost * prm.n * walls > NOALLOC_MAX_GHOST_FLOATS)
+        throw HipError{"the allocation-free step holds at most " + std::to_string(NOALLOC_MAX_GHOST_FLOATS / per_ghost) +
                        " wall-ghost slots (n times 2 X Y)"};
     int ndev = 0;
     ck(hipGetDeviceCount(&ndev), "hipGetDeviceCount");
@@ -3231,7 +3245,7 @@ void step_noalloc_impl(mpcd_params_t const& prm, uint64_t step_index, int n_step
     g.N = prm.N;
     g.C = C;
     g.W = walls;
-    g.GV = 3ull * prm.n * walls;
+    g.GV = per_ghost * prm.n * walls;
     g.stream = nullptr;
     g.lds_cap = cap;
     g.blocks_part = blk[1];

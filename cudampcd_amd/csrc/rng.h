@@ -50,11 +50,27 @@ MPCD_HD uint32_t mulhi32(uint32_t a, uint32_t b) {
     return static_cast<uint32_t>((static_cast<uint64_t>(a) * b) >> 32);
 }
 
+// full 64-bit product a * b.  On the GPU one v_mad_u64_u32 gives both halves
+// (the compiler otherwise issues v_mul_lo_u32 and v_mul_hi_u32, two
+// quarter-rate instructions); on the host the plain product.
+MPCD_HD uint64_t mul_wide(uint32_t a, uint32_t b) {
+#ifdef __HIP_DEVICE_COMPILE__
+    uint64_t r;
+    uint64_t carry;   // the instruction's carry-out (always clear: the addend is 0)
+    asm volatile("v_mad_u64_u32 %0, %1, %2, %3, 0" : "=v"(r), "=s"(carry) : "v"(a), "v"(b));
+    (void)carry;
+    return r;
+#else
+    return static_cast<uint64_t>(a) * b;
+#endif
+}
+
 MPCD_HD Words philox(uint32_t k0, uint32_t k1, uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3) {
 #pragma unroll
     for (int r = 0; r < 10; ++r) {
-        uint32_t const hi0 = mulhi32(0xD2511F53u, c0), lo0 = 0xD2511F53u * c0;
-        uint32_t const hi1 = mulhi32(0xCD9E8D57u, c2), lo1 = 0xCD9E8D57u * c2;
+        uint64_t const p0 = mul_wide(0xD2511F53u, c0), p1 = mul_wide(0xCD9E8D57u, c2);
+        uint32_t const hi0 = static_cast<uint32_t>(p0 >> 32), lo0 = static_cast<uint32_t>(p0);
+        uint32_t const hi1 = static_cast<uint32_t>(p1 >> 32), lo1 = static_cast<uint32_t>(p1);
         uint32_t const n0 = hi1 ^ c1 ^ k0;
         uint32_t const n2 = hi0 ^ c3 ^ k1;
         c0 = n0;

@@ -210,7 +210,7 @@ int mpcd_hip_tune_capacities(int algorithm, uint32_t nmax, uint32_t* caps, uint3
  * uses; the result is written to state_out in cell order (as a handle keeps
  * it); state_in is overwritten when n_steps > 1.  step_index keys the random
  * stream (mpcd_hip_get_step_index).  Limits: at most 2^20 particles and 2^17
- * cells, and for the SRD rule with walls n * 2 X Y at most 2^20 / 3; larger
+ * cells, and with walls n * 2 X Y at most 2^20 / 3 (SRD) or 2^20 / 6 (extended); larger
  * systems are rejected with an error before any device work.  Runs on the
  * null stream and returns after the device finished; calls from one process
  * are serialised, each call leaves the array holding its own data and clears
