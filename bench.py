@@ -329,7 +329,11 @@ def bench_ensemble(args, cfg):
     h = Handle(ens)
     N, C = h.N, h.C
     tc, cap, tune_ms = ens.autotune(4)   # steps per trial: two ranked the tile shapes unreliably
-    print(f"[bench] autotuned tiles: {tc} cells (system 0), LDS capacity {cap}, {tune_ms:.2f} ms/step", file=sys.stderr, flush=True)
+    shapes = {}
+    for i, m in enumerate(members):       # the shape each density class runs (cells per tile vary with the grid)
+        shapes.setdefault(f"n={m['n']}", sorted({tuple(h.system_tiling(j)) for j, q in enumerate(members)
+                                                 if q["n"] == m["n"]}))
+    print(f"[bench] autotuned tiles: {shapes}, {tune_ms:.2f} ms/step", file=sys.stderr, flush=True)
     ens.step(args.warmup)
     print(f"[bench] timing {args.steps} batched steps", file=sys.stderr, flush=True)
     t0 = time.perf_counter()
@@ -375,7 +379,8 @@ def bench_ensemble(args, cfg):
         "dtype": "f32",
         "data": "synthetic (seeded channel set, uniform positions, Maxwell-Boltzmann velocities)",
         "config": {"workload": cfg["workload"], "channels": len(members), "particles": N, "cells": C,
-                   "algorithm": cfg["algorithm"], "delta_t": 0.02, "tiling": [tc, cap], "parallelism": "single GPU"},
+                   "algorithm": cfg["algorithm"], "delta_t": 0.02,
+                   "tiling": {k: [list(x) for x in v] for k, v in shapes.items()}, "parallelism": "single GPU"},
         "plain_loop": {"value": loop_value, "ms_per_step": round(wall_loop * 1e3 / args.steps, 4),
                        "what": f"for each of {len(members)} Simulation objects (same seeds): step({args.steps})"},
         "batched_over_loop": round(value / loop_value, 4),

@@ -101,6 +101,7 @@ def lib() -> ctypes.CDLL:
             "mpcd_hip_last_timing": [_vp, _f64p, _f64p, ctypes.POINTER(ctypes.c_int)],
             "mpcd_hip_set_tiling": [_vp, ctypes.c_uint32, ctypes.c_uint32],
             "mpcd_hip_get_tiling": [_vp, _u32p, _u32p],
+            "mpcd_hip_system_tiling": [_vp, ctypes.c_uint32, _u32p, _u32p],
             "mpcd_hip_n_systems": [_vp, _u32p],
             "mpcd_hip_system_info": [_vp, ctypes.c_uint32, _u64p, _u64p, _u64p, _u64p],
             "mpcd_hip_system_get_particles": [_vp, ctypes.c_uint32, _f32p, _f32p, ctypes.c_size_t],
@@ -200,6 +201,12 @@ class Handle:
         b = ctypes.c_uint32()
         self._check(self.L.mpcd_hip_get_tiling(self.h, ctypes.byref(a), ctypes.byref(b)))
         return a.value, b.value
+
+    def system_tiling(self, s: int):
+        tc = ctypes.c_uint32()
+        cap = ctypes.c_uint32()
+        self._check(self.L.mpcd_hip_system_tiling(self.h, s, ctypes.byref(tc), ctypes.byref(cap)))
+        return int(tc.value), int(cap.value)
 
     def use_graph(self, on: bool = True):
         self._check(self.L.mpcd_hip_use_graph(self.h, int(on)))

@@ -76,6 +76,9 @@ struct KParams {
     uint32_t mn[2];      // division magic for n (n > 1)
     Stream rs;
     uint32_t tile_cells; // cells per collide tile
+    uint32_t lds_cap;    // LDS particle capacity of this system's collide tiles (host-side choice)
+    uint32_t tile_req;   // requested cells per tile (0: about one batch)
+    uint32_t gid;        // launch set: systems of one capacity and set share a collide launch
     uint32_t N;          // particles of this system
     uint32_t cb, pb, wb; // first cell key, particle slot, wall cell in the shared arrays
     uint64_t gvb;        // first ghost float of this system (3 per slot SRD, 6 extended)
@@ -1764,27 +1767,29 @@ __device__ __forceinline__ void collide_tile(Sys const& sys, const float* __rest
 // position, as in sampling), then over the steps of the run.
 
 constexpr int LAYERS_MAX = 1024;
+constexpr uint64_t LAYER_BLOCK_MAX = 65536;   // particles per k_layers block
 
 // mode 0: per-layer particle count and x-velocity sum of one step's state in
 // fixed point, so the sums do not depend on the order of the atomics: each
-// velocity is rounded to a multiple of 2^-40, q = rint(v 2^40), and split into
-// q >> 20 (sum[z], units of 2^-20) and q & (2^20 - 1) (sum[Z + z], units of
-// 2^-40), so neither 64-bit word overflows for |v| < 2^10 and fewer than 2^32
-// particles (one word at 2^-40 would overflow at |sum| = 2^23);
+// velocity is rounded to a multiple of 2^-40, q = rint(v 2^40).  A block sums
+// at most LAYER_BLOCK_MAX particles per layer in one 64-bit LDS word (exact
+// while |v| < 2^23 / LAYER_BLOCK_MAX = 128), and adds the block's sum Q to the
+// step's totals split as Q >> 20 (sum[z], units of 2^-20) and Q & (2^20 - 1)
+// (sum[Z + z], units of 2^-40), which cannot overflow for fewer than 2^32
+// particles (a single total at 2^-40 would overflow at |sum| = 2^23);
 // mode 1: loss seed dL/dv_x = w[layer] / count[layer] of every particle
 __global__ __launch_bounds__(WG) void k_layers(Sys sys, const float* __restrict__ st, int mode,
                                                uint32_t* __restrict__ cnt, unsigned long long* __restrict__ sum,
                                                const double* __restrict__ w, float* __restrict__ seed) {
     __shared__ uint32_t hc[LAYERS_MAX];
-    __shared__ unsigned long long hs[2][LAYERS_MAX];
+    __shared__ unsigned long long hs[LAYERS_MAX];
     KParams const P = sys.P[0];
     uint64_t const N = sys.NT;
     uint32_t const Z = P.dim[2];
     if (mode == 0) {
         for (uint32_t i = threadIdx.x; i < Z; i += WG) {
             hc[i] = 0u;
-            hs[0][i] = 0ull;
-            hs[1][i] = 0ull;
+            hs[i] = 0ull;
         }
         __syncthreads();
     }
@@ -1792,10 +1797,9 @@ __global__ __launch_bounds__(WG) void k_layers(Sys sys, const float* __restrict_
          j += static_cast<uint64_t>(gridDim.x) * WG) {
         uint32_t const iz = mod_dim(static_cast<uint32_t>(floorf(st[rec(j) + 2] + P.half[2])), Z);
         if (mode == 0) {
-            long long const q = static_cast<long long>(rint(static_cast<double>(st[rec(j) + 3]) * FIX_SCALE));
             atomicAdd(&hc[iz], 1u);
-            atomicAdd(&hs[0][iz], static_cast<unsigned long long>(q >> 20));
-            atomicAdd(&hs[1][iz], static_cast<unsigned long long>(q & 0xfffffll));
+            atomicAdd(&hs[iz], static_cast<unsigned long long>(static_cast<long long>(
+                                   rint(static_cast<double>(st[rec(j) + 3]) * FIX_SCALE))));
         } else {
             uint32_t const c = cnt[iz];
             seed[j] = c ? static_cast<float>(w[iz] / static_cast<double>(c)) : 0.0f;
@@ -1805,9 +1809,10 @@ __global__ __launch_bounds__(WG) void k_layers(Sys sys, const float* __restrict_
         __syncthreads();
         for (uint32_t i = threadIdx.x; i < Z; i += WG) {
             if (hc[i]) {
+                long long const q = static_cast<long long>(hs[i]);
                 atomicAdd(cnt + i, hc[i]);
-                atomicAdd(sum + i, hs[0][i]);
-                atomicAdd(sum + Z + i, hs[1][i]);
+                atomicAdd(sum + i, static_cast<unsigned long long>(q >> 20));
+                atomicAdd(sum + Z + i, static_cast<unsigned long long>(q & 0xfffffll));
             }
         }
     }
@@ -2200,7 +2205,20 @@ struct mpcd_hip_s {
     uint64_t step_index = 0;
     bool keys_valid = false, sorted_valid = false;
     bool streamed_pending = false;   // the state buffer holds an intermediate (streamed) state
-    uint32_t tile_req = 0, lds_cap = 0;  // requested cells per tile (0: largest that fits), LDS capacity
+    uint32_t lds_cap = 0;            // largest LDS capacity of the systems' tiles
+    // collide launches: systems sharing a compiled LDS capacity form a group
+    // whose tiles are contiguous, [t0, t1), and take one launch
+    struct Group {
+        uint32_t cap, t0, t1, gid;
+    };
+    static constexpr int MAX_GROUPS = 16;   // 7 capacities x 2 launch sets
+    Group grp[MAX_GROUPS] = {};
+    uint32_t ngrp = 0;
+    // autotuning: HIP events around each group's collide launch of eager steps,
+    // and the milliseconds of each group summed over the steps
+    bool group_timing = false;
+    hipEvent_t gev[2 * MAX_GROUPS] = {};
+    double group_ms[MAX_GROUPS] = {};
     // captured steps, by input buffer and sampling
     bool use_graph = true;
     hipGraphExec_t graph[2][2][4] = {};   // [input buffer][sampling][streamed in/out]
@@ -2326,11 +2344,17 @@ constexpr uint32_t TILE_CAP_MAX = TILE_CAPS[6];
 // default): the smallest at least as large that the rule can use (the
 // extended rule needs 512 or more and room for one wall cell's n ghosts in its
 // ghost region).
-uint32_t pick_capacity(int algorithm, uint32_t nmax, uint32_t cap) {
+uint32_t pick_capacity(int algorithm, uint32_t nmax, uint32_t cap, uint64_t cells = 0) {
     // default: SRD 384-particle batches on two-wave workgroups (twelve per
-    // CU), the extended rule 512; larger where the density makes single cells
-    // of several hundred particles likely
-    if (cap == 0) cap = std::max(algorithm == 0 ? 384u : 512u, 6 * nmax);
+    // CU); below 16 particles per cell in large grids 512 (more cells per
+    // batch: 1000x500x20 cells, n = 5: 64 cells of 512 1.20 ms per collide
+    // launch, 24 of 384 1.43; n = 10: 51 of 512 1.94, 24 of 384 2.02; small
+    // grids keep 384, whose more tiles fill the GPU better: 64x64x50 cells,
+    // n = 10, 21.2 ms per 200-step loss against 24.1 with 512); the extended
+    // rule 512; larger where the density makes single cells of several
+    // hundred particles likely
+    bool const sparse_large = nmax < 16 && cells >= (uint64_t(1) << 21);
+    if (cap == 0) cap = std::max(algorithm == 0 && !sparse_large ? 384u : 512u, 6 * nmax);
     for (uint32_t c : TILE_CAPS)
         if (c >= cap && (algorithm == 0 || (c >= 512 && static_cast<uint32_t>(ghost_floats(static_cast<int>(c), 1)) >= nmax)))
             return c;
@@ -2352,24 +2376,26 @@ std::vector<uint32_t> tune_capacities(int algorithm, uint32_t nmax, uint32_t cur
     return out;
 }
 
+// Tile shape of system s for the LDS capacity request `cap` (0: keep the
+// system's, or its density's default, pick_capacity) and cells-per-tile
+// request `tc` (0: about one batch).
+void choose_tiling_sys(mpcd_hip_s* h, uint32_t s, uint32_t tc, uint32_t cap) {
+    KParams& k = h->kp[s];
+    uint32_t const n = std::max<uint32_t>(1, k.n);
+    if (h->algorithm == 1 && TILE_CAP_MAX / 2 < n) throw HipError{"the extended rule supports n up to 1024"};
+    if (cap == 0) cap = k.lds_cap;
+    uint32_t const chosen = pick_capacity(h->algorithm, n, cap, k.C);
+    k.lds_cap = chosen;
+    k.tile_req = tc;
+    // cells per tile: about one batch of particles by default; a tile whose
+    // cells hold more (also on an explicit request) is processed in several batches
+    uint32_t const t = tc ? tc : std::max<uint32_t>(1, (chosen + n / 2) / n);
+    k.tile_cells = std::min<uint32_t>(t, static_cast<uint32_t>(tile_cell_max(static_cast<int>(chosen), h->algorithm)));
+}
+
+// the same request for every system
 void choose_tiling(mpcd_hip_s* h, uint32_t tc, uint32_t cap) {
-    uint32_t nmax = 1;
-    for (KParams const& k : h->kp) nmax = std::max(nmax, k.n);
-    // the extended rule needs room for the ghosts of one wall cell (n) in its
-    // ghost region
-    if (h->algorithm == 1 && TILE_CAP_MAX / 2 < nmax) throw HipError{"the extended rule supports n up to 1024"};
-    if (cap == 0 && h->lds_cap) cap = h->lds_cap;
-    uint32_t const chosen = pick_capacity(h->algorithm, nmax, cap);
-    h->lds_cap = chosen;
-    h->tile_req = tc;
-    for (KParams& k : h->kp) {
-        uint32_t const n = std::max<uint32_t>(1, k.n);
-        // cells per tile: about one batch of particles by default; a tile
-        // whose cells hold more (also on an explicit request) is processed in
-        // several batches
-        uint32_t const t = tc ? tc : std::max<uint32_t>(1, (chosen + n / 2) / n);
-        k.tile_cells = std::min<uint32_t>(t, static_cast<uint32_t>(tile_cell_max(static_cast<int>(chosen), h->algorithm)));
-    }
+    for (uint32_t s = 0; s < h->ns; ++s) choose_tiling_sys(h, s, tc, cap);
 }
 
 void drop_graphs(mpcd_hip_s* h) {
@@ -2402,7 +2428,7 @@ void layout(mpcd_hip_s* h) {
     uint32_t* bw = bc + ns + 1;
     uint32_t* bt = bw + ns + 1;
     auto blocks = [](uint64_t items) { return static_cast<uint32_t>(std::min<uint64_t>((items + WG - 1) / WG, 65536)); };
-    uint64_t cb = 0, pb = 0, wb = 0, gvb = 0, tiles = 0;
+    uint64_t cb = 0, pb = 0, wb = 0, gvb = 0;
     for (uint32_t s = 0; s < ns; ++s) {
         KParams& k = h->kp[s];
         k.cb = static_cast<uint32_t>(cb);
@@ -2413,14 +2439,32 @@ void layout(mpcd_hip_s* h) {
         bp[s + 1] = bp[s] + blocks(k.N);
         bc[s + 1] = bc[s] + blocks(k.C);
         bw[s + 1] = bw[s] + blocks(walls);
-        tiles += (k.C + k.tile_cells - 1) / k.tile_cells;
-        if (tiles >= 0xffffffffull) throw HipError{"too many collide tiles"};
-        bt[s + 1] = static_cast<uint32_t>(tiles);
         cb += k.C;
         pb += k.N;
         wb += walls;
         gvb += (h->algorithm == 1 ? 6ull : 3ull) * k.n * walls;   // floats per ghost slot
     }
+    // collide tiles: the systems of each compiled capacity together (one
+    // launch per capacity), bt[s] = first tile of system s, bt[ns] = all tiles
+    h->ngrp = 0;
+    h->lds_cap = 0;
+    uint64_t tiles = 0;
+    for (uint32_t gid = 0; gid < 2; ++gid)
+        for (uint32_t cap : TILE_CAPS) {
+            uint64_t const t0 = tiles;
+            for (uint32_t s = 0; s < ns; ++s) {
+                KParams const& k = h->kp[s];
+                if (k.lds_cap != cap || k.gid != gid) continue;
+                bt[s] = static_cast<uint32_t>(tiles);
+                tiles += (k.C + k.tile_cells - 1) / k.tile_cells;
+                if (tiles >= 0xffffffffull) throw HipError{"too many collide tiles"};
+            }
+            if (tiles > t0) {
+                h->grp[h->ngrp++] = mpcd_hip_s::Group{cap, static_cast<uint32_t>(t0), static_cast<uint32_t>(tiles), gid};
+                h->lds_cap = std::max(h->lds_cap, cap);
+            }
+        }
+    bt[ns] = static_cast<uint32_t>(tiles);
     h->blocks_part = bp[ns];
     h->blocks_cell = bc[ns];
     h->blocks_wall = bw[ns];
@@ -2432,8 +2476,11 @@ void layout(mpcd_hip_s* h) {
     ck(hipMemcpy(h->d_blk, blk.data(), 4 * blk.size(), hipMemcpyHostToDevice), "upload block maps");
     if (ns > 1) {
         std::vector<uint16_t> ts(h->ntiles);
-        for (uint32_t s = 0; s < ns; ++s)
-            for (uint32_t t = bt[s]; t < bt[s + 1]; ++t) ts[t] = static_cast<uint16_t>(s);
+        for (uint32_t s = 0; s < ns; ++s) {
+            KParams const& k = h->kp[s];
+            uint32_t const nt = (k.C + k.tile_cells - 1) / k.tile_cells;
+            for (uint32_t t = bt[s]; t < bt[s] + nt; ++t) ts[t] = static_cast<uint16_t>(s);
+        }
         if (h->tile_sys_cap < ts.size()) {
             if (h->d_tile_sys) ck(hipFree(h->d_tile_sys), "free tile map");
             h->d_tile_sys = nullptr;
@@ -2504,10 +2551,13 @@ void fit_capacity(mpcd_hip_s* h) {
     uint32_t m = 0;
     ck(hipMemcpyAsync(&m, h->cmax, 4, hipMemcpyDeviceToHost, h->stream), "count max");
     ck(hipStreamSynchronize(h->stream), "count max");
-    if (m > h->lds_cap && m <= TILE_CAP_MAX) {
-        choose_tiling(h, h->tile_req, m);
-        layout(h);
-    }
+    bool raised = false;
+    for (uint32_t s = 0; s < h->ns; ++s)
+        if (m > h->kp[s].lds_cap && m <= TILE_CAP_MAX) {
+            choose_tiling_sys(h, s, h->kp[s].tile_req, m);
+            raised = true;
+        }
+    if (raised) layout(h);
 }
 
 void launch_scan(mpcd_hip_s* h) {
@@ -2538,7 +2588,7 @@ __global__ __launch_bounds__(tile_threads(CAP), tile_blocks(CAP, ALGO)) void k_c
     Sys sys, const float* __restrict__ in, float* __restrict__ out, const uint32_t* __restrict__ perm,
     const uint32_t* __restrict__ offset, uint32_t* __restrict__ count_next, uint32_t* __restrict__ keys,
     uint32_t* __restrict__ perm_out, SampleBufs S, uint32_t* __restrict__ status, float const* __restrict__ axis,
-    uint32_t const* __restrict__ gcnt, float const* __restrict__ gvel, Tape T, uint32_t io) {
+    uint32_t const* __restrict__ gcnt, float const* __restrict__ gvel, Tape T, uint32_t io, uint32_t toff) {
     if constexpr (BIG) {
         uint32_t const n = status[1];
         for (uint32_t e = blockIdx.x; e < n; e += gridDim.x) {
@@ -2547,49 +2597,61 @@ __global__ __launch_bounds__(tile_threads(CAP), tile_blocks(CAP, ALGO)) void k_c
             __syncthreads();
         }
     } else {
+        // tiles [toff, toff + gridDim.x) of the handle's tile space
         collide_tile<ALGO, SAMPLE, CAP, ADJ, false>(sys, in, out, perm, offset, count_next, keys, perm_out, S, status,
-                                                    axis, gcnt, gvel, T, io, xcd_tile(blockIdx.x, gridDim.x));
+                                                    axis, gcnt, gvel, T, io, toff + xcd_tile(blockIdx.x, gridDim.x));
     }
 }
 
-template <int ALGO, bool SAMPLE, int CAP, bool ADJ = false>
-void launch_collide_t(mpcd_hip_s* h, uint32_t* rec, Tape const& T = Tape{}, uint32_t io = 0) {
+// one collide launch: tiles [t0, t1) (BIG: the big-cell launch, BIG_BLOCKS
+// workgroups over the cells listed as larger than their tile)
+template <int ALGO, bool SAMPLE, int CAP, bool ADJ = false, bool BIG = false>
+void launch_collide_t(mpcd_hip_s* h, uint32_t* rec, Tape const& T, uint32_t io, uint32_t t0, uint32_t t1) {
     SampleBufs S{h->scnt, h->ssum};
-    hipLaunchKernelGGL((k_collide<ALGO, SAMPLE, CAP, ADJ>), dim3(h->ntiles), dim3(tile_threads(CAP)), 0, h->stream, h->sys,
-                       h->st[h->cur], h->st[1 - h->cur], h->perm, h->offset, h->count, h->keys, rec, S, h->status,
-                       h->axis, h->wgcnt, h->wgvel, T, io);
-    if constexpr (!ADJ) {
-        // cells the first launch listed as larger than the tile (usually
-        // none: every workgroup returns at once)
-        hipLaunchKernelGGL((k_collide<ALGO, SAMPLE, CAP, false, true>), dim3(BIG_BLOCKS), dim3(tile_threads(CAP)), 0,
-                           h->stream, h->sys, h->st[h->cur], h->st[1 - h->cur], h->perm, h->offset, h->count, h->keys,
-                           rec, S, h->status, h->axis, h->wgcnt, h->wgvel, T, io);
-    }
+    uint32_t const grid = BIG ? BIG_BLOCKS : t1 - t0;
+    if (!grid) return;
+    hipLaunchKernelGGL((k_collide<ALGO, SAMPLE, CAP, ADJ, BIG>), dim3(grid), dim3(tile_threads(CAP)), 0, h->stream,
+                       h->sys, h->st[h->cur], h->st[1 - h->cur], h->perm, h->offset, h->count, h->keys, rec, S,
+                       h->status, h->axis, h->wgcnt, h->wgvel, T, io, t0);
 }
 
-template <int ALGO, bool SAMPLE>
-void launch_collide_cap(mpcd_hip_s* h, uint32_t* rec, uint32_t io) {
-    switch (h->lds_cap) {
+template <int ALGO, bool SAMPLE, bool BIG>
+void launch_collide_at(mpcd_hip_s* h, uint32_t cap, uint32_t* rec, uint32_t io, uint32_t t0, uint32_t t1) {
+    switch (cap) {
         case 256:
-            if constexpr (ALGO == 0) launch_collide_t<ALGO, SAMPLE, 256>(h, rec, Tape{}, io);
+            if constexpr (ALGO == 0) launch_collide_t<ALGO, SAMPLE, 256, false, BIG>(h, rec, Tape{}, io, t0, t1);
             else throw HipError{"the extended rule needs an LDS capacity of at least 512"};
             break;
         case 384:
-            if constexpr (ALGO == 0) launch_collide_t<ALGO, SAMPLE, 384>(h, rec, Tape{}, io);
+            if constexpr (ALGO == 0) launch_collide_t<ALGO, SAMPLE, 384, false, BIG>(h, rec, Tape{}, io, t0, t1);
             else throw HipError{"the extended rule needs an LDS capacity of at least 512"};
             break;
         case 448:
-            if constexpr (ALGO == 0) launch_collide_t<ALGO, SAMPLE, 448>(h, rec, Tape{}, io);
+            if constexpr (ALGO == 0) launch_collide_t<ALGO, SAMPLE, 448, false, BIG>(h, rec, Tape{}, io, t0, t1);
             else throw HipError{"the extended rule needs an LDS capacity of at least 512"};
             break;
-        case 512: launch_collide_t<ALGO, SAMPLE, 512>(h, rec, Tape{}, io); break;
+        case 512: launch_collide_t<ALGO, SAMPLE, 512, false, BIG>(h, rec, Tape{}, io, t0, t1); break;
         case 640:
-            if constexpr (ALGO == 0) launch_collide_t<ALGO, SAMPLE, 640>(h, rec, Tape{}, io);
-            else launch_collide_t<ALGO, SAMPLE, 1152>(h, rec, Tape{}, io);
+            if constexpr (ALGO == 0) launch_collide_t<ALGO, SAMPLE, 640, false, BIG>(h, rec, Tape{}, io, t0, t1);
+            else launch_collide_t<ALGO, SAMPLE, 1152, false, BIG>(h, rec, Tape{}, io, t0, t1);
             break;
-        case 1152: launch_collide_t<ALGO, SAMPLE, 1152>(h, rec, Tape{}, io); break;
-        default: launch_collide_t<ALGO, SAMPLE, 2048>(h, rec, Tape{}, io); break;
+        case 1152: launch_collide_t<ALGO, SAMPLE, 1152, false, BIG>(h, rec, Tape{}, io, t0, t1); break;
+        default: launch_collide_t<ALGO, SAMPLE, 2048, false, BIG>(h, rec, Tape{}, io, t0, t1); break;
     }
+}
+
+// The collide launches of a step: one per group of systems that share a
+// compiled LDS capacity, then the big-cell launch at the largest capacity
+// (cells the first launches listed as larger than their tile; usually none,
+// every workgroup returns at once)
+template <int ALGO, bool SAMPLE>
+void launch_collide_cap(mpcd_hip_s* h, uint32_t* rec, uint32_t io) {
+    for (uint32_t g = 0; g < h->ngrp; ++g) {
+        if (h->group_timing) ck(hipEventRecord(h->gev[2 * g], h->stream), "event");
+        launch_collide_at<ALGO, SAMPLE, false>(h, h->grp[g].cap, rec, io, h->grp[g].t0, h->grp[g].t1);
+        if (h->group_timing) ck(hipEventRecord(h->gev[2 * g + 1], h->stream), "event");
+    }
+    launch_collide_at<ALGO, SAMPLE, true>(h, h->lds_cap, rec, io, 0, 0);
 }
 
 void ensure_sampling(mpcd_hip_s* h) {
@@ -2679,7 +2741,7 @@ void run_steps(mpcd_hip_s* h, int n_steps, bool sample) {
             fit_capacity(h);
         }
         bool const first = sample && h->sampling_done;
-        bool const graphed = h->use_graph && !h->timing && !h->record && !first && !h->sorted_valid;
+        bool const graphed = h->use_graph && !h->timing && !h->record && !first && !h->sorted_valid && !h->group_timing;
         // inside one call the intermediate states stay streamed; the last step
         // writes the post-collision state the API reads
         uint32_t const io = (s > 0 ? 1u : 0u) | (s + 1 < n_steps ? 2u : 0u);
@@ -2691,6 +2753,14 @@ void run_steps(mpcd_hip_s* h, int n_steps, bool sample) {
             hipEvent_t const e0 = h->timing ? h->ev[2 * s] : nullptr;
             hipEvent_t const e1 = h->timing ? h->ev[2 * s + 1] : nullptr;
             h->launches += enqueue_step(h, sample, first, !h->sorted_valid, h->record ? h->perm_rec : nullptr, e0, e1, io);
+            if (h->group_timing) {
+                ck(hipStreamSynchronize(h->stream), "step");
+                for (uint32_t g = 0; g < h->ngrp; ++g) {
+                    float ms = 0;
+                    ck(hipEventElapsedTime(&ms, h->gev[2 * g], h->gev[2 * g + 1]), "elapsed");
+                    h->group_ms[g] += ms;
+                }
+            }
         }
         if (!(io & 2u)) h->streamed_pending = false;   // the post-collision state is written again
         if (sample) {
@@ -2859,13 +2929,13 @@ void adj_step(mpcd_hip_s* h, float const* force, int j) {
         rec = a.perm + uint64_t(j) * N;
     }
     switch (h->lds_cap) {
-        case 256: launch_collide_t<0, false, 256, true>(h, rec, T); break;
-        case 384: launch_collide_t<0, false, 384, true>(h, rec, T); break;
-        case 448: launch_collide_t<0, false, 448, true>(h, rec, T); break;
-        case 512: launch_collide_t<0, false, 512, true>(h, rec, T); break;
-        case 640: launch_collide_t<0, false, 640, true>(h, rec, T); break;
-        case 1152: launch_collide_t<0, false, 1152, true>(h, rec, T); break;
-        default: launch_collide_t<0, false, 2048, true>(h, rec, T); break;
+        case 256: launch_collide_t<0, false, 256, true>(h, rec, T, 0, 0, h->ntiles); break;
+        case 384: launch_collide_t<0, false, 384, true>(h, rec, T, 0, 0, h->ntiles); break;
+        case 448: launch_collide_t<0, false, 448, true>(h, rec, T, 0, 0, h->ntiles); break;
+        case 512: launch_collide_t<0, false, 512, true>(h, rec, T, 0, 0, h->ntiles); break;
+        case 640: launch_collide_t<0, false, 640, true>(h, rec, T, 0, 0, h->ntiles); break;
+        case 1152: launch_collide_t<0, false, 1152, true>(h, rec, T, 0, 0, h->ntiles); break;
+        default: launch_collide_t<0, false, 2048, true>(h, rec, T, 0, 0, h->ntiles); break;
     }
     ck(hipGetLastError(), "adjoint-run step");
     if (j >= 0) {
@@ -2879,12 +2949,15 @@ void adj_step(mpcd_hip_s* h, float const* force, int j) {
     h->sorted_valid = false;
 }
 
-// blocks of k_layers: each flushes its LDS histogram with two atomics per
+// blocks of k_layers: each flushes its LDS histogram with three atomics per
 // layer onto the same few addresses, so fewer, longer blocks (two per CU;
 // 64x64x52 cells, n = 10: 4096 blocks 64 us per launch, 1024 25 us, 512 a
-// little less)
+// little less), but no block sums more than LAYER_BLOCK_MAX particles (the
+// exactness bound of its fixed-point words)
 uint32_t layer_blocks(mpcd_hip_s* h) {
-    return static_cast<uint32_t>(std::max<uint64_t>(1, std::min<uint64_t>((h->N + WG - 1) / WG, 512)));
+    uint64_t const b = std::max<uint64_t>(std::min<uint64_t>((h->N + WG - 1) / WG, 512),
+                                          (h->N + LAYER_BLOCK_MAX - 1) / LAYER_BLOCK_MAX);
+    return static_cast<uint32_t>(std::max<uint64_t>(1, b));
 }
 
 struct ProfileResult {
@@ -3229,7 +3302,7 @@ void step_noalloc_impl(mpcd_params_t const& prm, uint64_t step_index, int n_step
     // the system table entry and the launch shape (choose_tiling / layout for one system)
     KParams kp;
     fill_geometry(prm, kp);
-    uint32_t const cap = pick_capacity(prm.algorithm, kp.n, 0);
+    uint32_t const cap = pick_capacity(prm.algorithm, kp.n, 0, C);
     kp.tile_cells = std::min<uint32_t>(std::max<uint32_t>(1, (cap + kp.n / 2) / kp.n),
                                        static_cast<uint32_t>(tile_cell_max(static_cast<int>(cap), prm.algorithm)));
     kp.step = static_cast<uint32_t>(step_index);
@@ -3248,6 +3321,8 @@ void step_noalloc_impl(mpcd_params_t const& prm, uint64_t step_index, int n_step
     g.GV = per_ghost * prm.n * walls;
     g.stream = nullptr;
     g.lds_cap = cap;
+    g.grp[0] = mpcd_hip_s::Group{cap, 0u, tiles, 0u};
+    g.ngrp = 1;
     g.blocks_part = blk[1];
     g.blocks_cell = blk[3];
     g.blocks_wall = blk[5];
@@ -3464,10 +3539,17 @@ int mpcd_hip_set_tiling(mpcd_hip_t* h, uint32_t cells_per_tile, uint32_t lds_cap
     });
 }
 
+int mpcd_hip_system_tiling(const mpcd_hip_t* h, uint32_t system, uint32_t* cells_per_tile, uint32_t* lds_capacity) {
+    if (!h || system >= h->ns) return 1;
+    if (cells_per_tile) *cells_per_tile = h->kp[system].tile_cells;
+    if (lds_capacity) *lds_capacity = h->kp[system].lds_cap;
+    return 0;
+}
+
 int mpcd_hip_get_tiling(const mpcd_hip_t* h, uint32_t* cells_per_tile, uint32_t* lds_capacity) {
     if (!h) return 1;
     if (cells_per_tile) *cells_per_tile = h->kp[0].tile_cells;
-    if (lds_capacity) *lds_capacity = h->lds_cap;
+    if (lds_capacity) *lds_capacity = h->kp[0].lds_cap;
     return 0;
 }
 
@@ -3600,59 +3682,109 @@ int mpcd_hip_autotune(mpcd_hip_t* h, int steps_per_trial, uint32_t* cells_per_ti
         struct Cand {
             uint32_t tc, cap;
         };
-        uint32_t nmax = 1;
-        for (KParams const& k : h->kp) nmax = std::max(nmax, k.n);
-        std::vector<Cand> cands;
-        for (uint32_t cap : tune_capacities(h->algorithm, nmax, h->lds_cap))
-            for (uint32_t tc : {0u, 48u, 32u, 16u, 8u}) cands.push_back({tc, cap});
+        // Density classes (the systems of one n), the most particles first.  A
+        // class's tile shape changes only its own collide work, so each class
+        // is tuned on the time of its own collide launches: while it is tuned
+        // its systems form launches of their own (launch set 1), timed with
+        // events over eager steps, the other classes at their current shapes.
+        std::vector<uint32_t> dens;
+        std::vector<uint64_t> mass;
+        for (KParams const& k : h->kp) {
+            auto const it = std::find(dens.begin(), dens.end(), k.n);
+            if (it == dens.end()) {
+                dens.push_back(k.n);
+                mass.push_back(k.N);
+            } else {
+                mass[it - dens.begin()] += k.N;
+            }
+        }
+        std::vector<size_t> order(dens.size());
+        for (size_t i = 0; i < order.size(); ++i) order[i] = i;
+        std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) { return mass[a] > mass[b]; });
+        std::vector<Cand> before(h->ns);
+        for (uint32_t s = 0; s < h->ns; ++s) before[s] = {h->kp[s].tile_req, h->kp[s].lds_cap};
         bool const timing = h->timing;
         h->timing = false;
-        hipEvent_t e0 = nullptr, e1 = nullptr;
-        ck(hipEventCreate(&e0), "event");
-        ck(hipEventCreate(&e1), "event");
-        double best = 1e300;
-        Cand const before = {h->tile_req, h->lds_cap};
-        Cand pick = before;
-        std::vector<std::pair<uint32_t, uint32_t>> seen;
-        try {
-            for (Cand const& c : cands) {
-                choose_tiling(h, c.tc, c.cap);
-                std::pair<uint32_t, uint32_t> const key{h->kp[0].tile_cells, h->lds_cap};
-                if (c.tc && std::find(seen.begin(), seen.end(), key) != seen.end()) continue;
-                seen.push_back(key);
-                layout(h);
-                run_steps(h, 1, false);  // captures the step graph
-                ck(hipEventRecord(e0, h->stream), "event");
-                run_steps(h, steps_per_trial, false);
-                ck(hipEventRecord(e1, h->stream), "event");
-                ck(hipEventSynchronize(e1), "event");
-                float ms = 0;
-                ck(hipEventElapsedTime(&ms, e0, e1), "elapsed");
-                if (ms < best) {
-                    best = ms;
-                    pick = c;
-                }
+        for (hipEvent_t& e : h->gev) ck(hipEventCreate(&e), "event");
+        auto cleanup = [&]() {
+            h->group_timing = false;
+            for (hipEvent_t& e : h->gev) {
+                if (e) (void)hipEventDestroy(e);
+                e = nullptr;
             }
-        } catch (...) {
-            (void)hipEventDestroy(e0);
-            (void)hipEventDestroy(e1);
+            for (KParams& k : h->kp) k.gid = 0;
             h->timing = timing;
-            // the handle keeps the tile shape it had before tuning
+        };
+        // milliseconds per step of the collide launches of launch set 1
+        auto class_trial = [&]() {
+            layout(h);
+            h->group_timing = true;
+            run_steps(h, 1, false);
+            for (double& v : h->group_ms) v = 0.0;
+            run_steps(h, steps_per_trial, false);
+            h->group_timing = false;
+            double ms = 0;
+            for (uint32_t g = 0; g < h->ngrp; ++g)
+                if (h->grp[g].gid == 1) ms += h->group_ms[g];
+            return ms / steps_per_trial;
+        };
+        double step_ms = 0;
+        try {
+            for (size_t ci : order) {
+                uint32_t const n = dens[ci];
+                std::vector<uint32_t> members;
+                for (uint32_t s = 0; s < h->ns; ++s) h->kp[s].gid = h->kp[s].n == n ? 1u : 0u;
+                for (uint32_t s = 0; s < h->ns; ++s)
+                    if (h->kp[s].n == n) members.push_back(s);
+                uint32_t const m0 = members[0];
+                auto apply = [&](Cand const& c) {
+                    for (uint32_t s : members) choose_tiling_sys(h, s, c.tc, c.cap);
+                };
+                Cand pick = {h->kp[m0].tile_req, h->kp[m0].lds_cap};
+                double best = 1e300;
+                std::vector<std::pair<uint32_t, uint32_t>> seen;
+                for (uint32_t cap : tune_capacities(h->algorithm, n, h->kp[m0].lds_cap))
+                    for (uint32_t tc : {0u, 48u, 32u, 16u, 8u}) {
+                        Cand const c = {tc, cap};
+                        apply(c);
+                        std::pair<uint32_t, uint32_t> const key{h->kp[m0].tile_cells, h->kp[m0].lds_cap};
+                        if (std::find(seen.begin(), seen.end(), key) != seen.end()) continue;
+                        seen.push_back(key);
+                        double const ms = class_trial();
+                        if (ms < best) {
+                            best = ms;
+                            pick = c;
+                        }
+                    }
+                apply(pick);
+            }
+            // the whole step with the chosen shapes, replayed from its graph
+            for (KParams& k : h->kp) k.gid = 0;
+            layout(h);
+            hipEvent_t e0 = h->gev[0], e1 = h->gev[1];
+            run_steps(h, 1, false);   // captures the step graph
+            ck(hipEventRecord(e0, h->stream), "event");
+            run_steps(h, steps_per_trial, false);
+            ck(hipEventRecord(e1, h->stream), "event");
+            ck(hipEventSynchronize(e1), "event");
+            float ms = 0;
+            ck(hipEventElapsedTime(&ms, e0, e1), "elapsed");
+            step_ms = ms / steps_per_trial;
+        } catch (...) {
+            cleanup();
+            // the handle keeps the tile shapes it had before tuning
             try {
-                choose_tiling(h, before.tc, before.cap);
+                for (uint32_t s = 0; s < h->ns; ++s) choose_tiling_sys(h, s, before[s].tc, before[s].cap);
                 layout(h);
             } catch (...) {
             }
             throw;
         }
-        (void)hipEventDestroy(e0);
-        (void)hipEventDestroy(e1);
-        h->timing = timing;
-        choose_tiling(h, pick.tc, pick.cap);
+        cleanup();
         layout(h);
         if (cells_per_tile) *cells_per_tile = h->kp[0].tile_cells;
-        if (lds_capacity) *lds_capacity = h->lds_cap;
-        if (ms_per_step) *ms_per_step = best / steps_per_trial;
+        if (lds_capacity) *lds_capacity = h->kp[0].lds_cap;
+        if (ms_per_step) *ms_per_step = step_ms;
     });
 }
 

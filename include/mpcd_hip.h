@@ -120,6 +120,10 @@ int mpcd_hip_last_timing(const mpcd_hip_t* h, double* collide_ms, double* total_
  * one batch are processed in several batches.  Results do not depend on it. */
 int mpcd_hip_set_tiling(mpcd_hip_t* h, uint32_t cells_per_tile, uint32_t lds_capacity);
 int mpcd_hip_get_tiling(const mpcd_hip_t* h, uint32_t* cells_per_tile, uint32_t* lds_capacity);
+/* The tile shape of one system (mpcd_hip_get_tiling gives system 0's).  Systems
+ * of an ensemble may run different shapes: each system keeps its own, and
+ * the systems sharing an LDS capacity take one collide launch. */
+int mpcd_hip_system_tiling(const mpcd_hip_t* h, uint32_t system, uint32_t* cells_per_tile, uint32_t* lds_capacity);
 /* ---- pathwise adjoint (no reference counterpart) ----
  *
  * Loss of `steps` SRD steps of a single-system handle from its current state,
@@ -182,10 +186,13 @@ int mpcd_hip_set_sampling_state(mpcd_hip_t* h, const double* acc, size_t n_cells
  * eagerly).  The step state (index, grid shifts) lives in device memory and is
  * advanced by the last kernel of the step, so one graph serves every step. */
 int mpcd_hip_use_graph(mpcd_hip_t* h, int enable);
-/* Autotuner: runs steps_per_trial real steps (after one untimed step) with each
- * compiled LDS capacity and several cells-per-tile limits, keeps the fastest
- * and returns it (cells per tile of system 0) with its time per step.  The
- * systems advance by these steps; results do not depend on the tile shape. */
+/* Autotuner: for each density class (the systems of one n, the most particles
+ * first) tries each compiled LDS capacity that holds a whole collision table
+ * with several cells-per-tile limits, timing that class's own collide
+ * launches with HIP events over steps_per_trial real steps (after one untimed
+ * step), and keeps the fastest for the class.  Returns system 0's shape and the
+ * time per step with the chosen shapes.  The systems advance by these steps;
+ * results do not depend on the tile shapes. */
 int mpcd_hip_autotune(mpcd_hip_t* h, int steps_per_trial, uint32_t* cells_per_tile, uint32_t* lds_capacity,
                       double* ms_per_step);
 /* LDS capacities the autotuner tries for `algorithm` (0 srd, 1 extended) when
