@@ -171,6 +171,7 @@ def main():
     ap.add_argument("--config", default="c2", choices=sorted(CONFIGS))
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--seed", type=int, default=None, help="default 0x5EED0000 + config number")
+    ap.add_argument("--windows", type=int, default=32, help="adjoint accuracy study: windows (c5)")
     ap.add_argument("--tiling", default="",
                     help="cells_per_tile,lds_capacity for k_collide (results do not depend on it)")
     ap.add_argument("--sample", action="store_true",
@@ -411,8 +412,13 @@ def bench_ensemble(args, cfg):
 
 
 def bench_adjoint(args, cfg):
-    """Time per gradient of the profile loss (forward + recompute + reverse) against
-    central finite differences of the same loss (two forward runs per parameter)."""
+    """Time per gradient of the profile loss against one loss evaluation, with
+    the configuration's checkpoint interval (forward, recompute of every segment
+    but the last, reverse) and with the whole tape resident in HBM (forward and
+    reverse only); then the accuracy study: the drag gradient of consecutive
+    windows from the adjoint and from central differences of the loss (common
+    random numbers within a window), their means with standard errors, and the
+    cost of each estimator per unit of variance."""
     import numpy as np
 
     import cudampcd_amd as pympcd
@@ -427,16 +433,20 @@ def bench_adjoint(args, cfg):
     target = a * (1.0 - zc ** 2)
     print(f"[bench] {sim.n_particles} particles, warm-up {args.warmup * 10} steps", file=sys.stderr, flush=True)
     sim.step(args.warmup * 10)
-    adjoint.profile_loss_grad(sim, steps, K, p.drag, p.rotation_angle, target)   # allocates the tape
     reps = max(1, min(args.steps, 5))
-    t0 = time.perf_counter()
-    for _ in range(reps):
-        r = adjoint.profile_loss_grad(sim, steps, K, p.drag, p.rotation_angle, target)
-    t_grad = (time.perf_counter() - t0) / reps
-    t0 = time.perf_counter()
-    for _ in range(reps):
-        lr = adjoint.profile_loss(sim, steps, p.drag, p.rotation_angle, target)
-    t_loss = (time.perf_counter() - t0) / reps
+
+    def timed(f):
+        f()   # allocates the tape
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            out = f()
+        return out, (time.perf_counter() - t0) / reps
+
+    r, t_grad = timed(lambda: adjoint.profile_loss_grad(sim, steps, K, p.drag, p.rotation_angle, target))
+    r_res, t_res = timed(lambda: adjoint.profile_loss_grad(sim, steps, steps, p.drag, p.rotation_angle, target))
+    lr, t_loss = timed(lambda: adjoint.profile_loss(sim, steps, p.drag, p.rotation_angle, target))
+    assert r_res.grad_drag == r.grad_drag and np.array_equal(r_res.grad_force, r.grad_force)
+
     def fd_drag(nsteps, frac):
         h = frac * p.drag
         return (adjoint.profile_loss(sim, nsteps, p.drag + h, p.rotation_angle, target).loss -
@@ -444,8 +454,31 @@ def bench_adjoint(args, cfg):
 
     short = cfg["checkpoint"]
     r_short = adjoint.profile_loss_grad(sim, short, K, p.drag, p.rotation_angle, target, want_force_grad=False)
-    fd_long = {f"h={f:g}g": fd_drag(steps, f) for f in (0.01, 0.05, 0.2)}
     fd_short = {f"h={f:g}g": fd_drag(short, f) for f in (0.01, 0.05, 0.2)}
+    # accuracy study over consecutive windows (each starts where the previous ended)
+    windows, fracs = args.windows, (0.05, 0.2)
+    print(f"[bench] accuracy study: {windows} windows of {steps} steps", file=sys.stderr, flush=True)
+    adj_g, fd_g = [], {f: [] for f in fracs}
+    for _ in range(windows):
+        adj_g.append(adjoint.profile_loss_grad(sim, steps, steps, p.drag, p.rotation_angle, target,
+                                               want_force_grad=False).grad_drag)
+        for f in fracs:
+            fd_g[f].append(fd_drag(steps, f))
+        sim.step(steps)
+
+    def stats(x):
+        x = np.asarray(x, np.float64)
+        return float(x.mean()), float(x.std(ddof=1)), float(x.std(ddof=1) / np.sqrt(len(x)))
+
+    am, asd, ase = stats(adj_g)
+    study = {"windows": windows, "adjoint": {"mean": am, "std": asd, "se": ase}}
+    for f in fracs:
+        m, sd, se = stats(fd_g[f])
+        # GPU-ms per estimate of d/d(drag) with standard error eps: (sd / eps)^2 samples
+        study[f"central_fd_h={f:g}g"] = {
+            "mean": m, "std": sd, "se": se, "z_vs_adjoint": (m - am) / float(np.hypot(se, ase)),
+            "adjoint_speedup_at_equal_se_one_scalar": round((sd ** 2 * 2 * t_loss) / (asd ** 2 * t_res), 2),
+        }
     C = int(np.prod([int(v) for v in p.volume_size]))
     N = sim.n_particles
     out = {
@@ -465,6 +498,11 @@ def bench_adjoint(args, cfg):
                    "checkpoint_every": K, "parallelism": "single GPU"},
         "loss": lr.loss,
         "ms_per_loss": round(t_loss * 1e3, 3),
+        "loss_evaluations_per_gradient": round(t_grad / t_loss, 3),
+        "resident_tape": {"ms_per_gradient": round(t_res * 1e3, 3),
+                          "loss_evaluations_per_gradient": round(t_res / t_loss, 3),
+                          "what": f"checkpoint interval {steps}: the forward pass tapes all {steps} steps "
+                                  f"(bit-identical gradient), nothing is recomputed"},
         "gradient": {"d_drag": r.grad_drag, "d_angle_per_degree": r.grad_angle,
                      "d_force_norm": float(np.linalg.norm(r.grad_force))},
         "central_fd_cost": {
@@ -473,12 +511,8 @@ def bench_adjoint(args, cfg):
             "adjoint_speedup_two_scalars": round(4 * t_loss / t_grad, 3),
             "adjoint_speedup_full_field": round(2 * (2 + C) * t_loss / t_grad, 1),
         },
-        "central_fd_d_drag": {
-            f"{steps}_steps": {"adjoint": r.grad_drag, **fd_long},
-            f"{short}_steps": {"adjoint": r_short.grad_drag, **fd_short},
-            "note": "common random numbers; over 200 steps cell reassignments decorrelate the perturbed "
-                    "trajectories and the differences are noise-dominated, over 20 they agree with the adjoint",
-        },
+        "central_fd_d_drag_short_window": {f"{short}_steps": {"adjoint": r_short.grad_drag, **fd_short}},
+        "accuracy_study_d_drag": study,
         "particle_steps_per_s_forward": N * steps / t_loss,
     }
     print(json.dumps(out), flush=True)

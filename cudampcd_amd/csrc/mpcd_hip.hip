@@ -1768,51 +1768,76 @@ __device__ __forceinline__ void collide_tile(Sys const& sys, const float* __rest
 
 constexpr int LAYERS_MAX = 1024;
 constexpr uint64_t LAYER_BLOCK_MAX = 65536;   // particles per k_layers block
+constexpr uint32_t LAYER_SLOTS = 16;          // copies of each step's layer totals (by block)
+constexpr int LAYER_UNROLL = 8;               // particles per lane with their loads in flight together
+constexpr int LAYER_REP_WORDS = 2048;         // LDS histogram entries (layers times lane replicas)
 
-// mode 0: per-layer particle count and x-velocity sum of one step's state in
-// fixed point, so the sums do not depend on the order of the atomics: each
-// velocity is rounded to a multiple of 2^-40, q = rint(v 2^40).  A block sums
-// at most LAYER_BLOCK_MAX particles per layer in one 64-bit LDS word (exact
-// while |v| < 2^23 / LAYER_BLOCK_MAX = 128), and adds the block's sum Q to the
-// step's totals split as Q >> 20 (sum[z], units of 2^-20) and Q & (2^20 - 1)
-// (sum[Z + z], units of 2^-40), which cannot overflow for fewer than 2^32
-// particles (a single total at 2^-40 would overflow at |sum| = 2^23);
-// mode 1: loss seed dL/dv_x = w[layer] / count[layer] of every particle
-__global__ __launch_bounds__(WG) void k_layers(Sys sys, const float* __restrict__ st, int mode,
-                                               uint32_t* __restrict__ cnt, unsigned long long* __restrict__ sum,
-                                               const double* __restrict__ w, float* __restrict__ seed) {
-    __shared__ uint32_t hc[LAYERS_MAX];
-    __shared__ unsigned long long hs[LAYERS_MAX];
+// Per-layer statistics of one step's state, and with `lz` the layer of every
+// particle (taped steps).  Statistics: count and x-velocity sum of every z
+// layer in fixed point, so the sums do not depend on the order of the
+// atomics: each velocity is rounded to a multiple of 2^-40, q = rint(v 2^40).
+// A block sums at most LAYER_BLOCK_MAX particles per layer in 64-bit LDS words
+// (exact while |v| < 2^23 / LAYER_BLOCK_MAX = 128); lanes spread over R
+// replicas of the histogram (lane mod R), since neighbouring particles share
+// a cell and would otherwise queue on one LDS word.  The block adds its sum Q
+// to copy blockIdx mod LAYER_SLOTS of the step's totals, split as Q >> 20
+// (units of 2^-20) and Q & (2^20 - 1) (units of 2^-40), which cannot
+// overflow for fewer than 2^32 particles; the host adds the copies.
+// cnt: [LAYER_SLOTS][Z]; sum: [2][LAYER_SLOTS][Z] (hi, then lo).
+__global__ __launch_bounds__(WG) void k_layers(Sys sys, const float* __restrict__ st, uint32_t* __restrict__ cnt,
+                                               unsigned long long* __restrict__ sum, uint16_t* __restrict__ lz) {
+    __shared__ uint32_t hc[LAYER_REP_WORDS];
+    __shared__ unsigned long long hs[LAYER_REP_WORDS];
     KParams const P = sys.P[0];
     uint64_t const N = sys.NT;
     uint32_t const Z = P.dim[2];
-    if (mode == 0) {
-        for (uint32_t i = threadIdx.x; i < Z; i += WG) {
+    // replicas: the largest power of two up to 16 that fits (2 or more, as Z <= LAYERS_MAX)
+    uint32_t const R = 1u << (31 - __clz(min(16u, static_cast<uint32_t>(LAYER_REP_WORDS) / Z)));
+    uint32_t const rep = (threadIdx.x & (R - 1u)) * Z;
+    bool const stats = cnt != nullptr;
+    if (stats) {
+        for (uint32_t i = threadIdx.x; i < R * Z; i += WG) {
             hc[i] = 0u;
             hs[i] = 0ull;
         }
         __syncthreads();
     }
-    for (uint64_t j = blockIdx.x * static_cast<uint64_t>(WG) + threadIdx.x; j < N;
-         j += static_cast<uint64_t>(gridDim.x) * WG) {
-        uint32_t const iz = mod_dim(static_cast<uint32_t>(floorf(st[rec(j) + 2] + P.half[2])), Z);
-        if (mode == 0) {
-            atomicAdd(&hc[iz], 1u);
-            atomicAdd(&hs[iz], static_cast<unsigned long long>(static_cast<long long>(
-                                   rint(static_cast<double>(st[rec(j) + 3]) * FIX_SCALE))));
-        } else {
-            uint32_t const c = cnt[iz];
-            seed[j] = c ? static_cast<float>(w[iz] / static_cast<double>(c)) : 0.0f;
+    uint64_t const stride = static_cast<uint64_t>(gridDim.x) * WG * LAYER_UNROLL;
+    for (uint64_t j0 = blockIdx.x * static_cast<uint64_t>(WG) * LAYER_UNROLL + threadIdx.x; j0 < N; j0 += stride) {
+        float2 r[LAYER_UNROLL];
+#pragma unroll
+        for (int q = 0; q < LAYER_UNROLL; ++q) {
+            uint64_t const j = j0 + static_cast<uint64_t>(q) * WG;
+            r[q] = j < N ? *reinterpret_cast<float2 const*>(st + rec(j) + 2) : make_float2(0.0f, 0.0f);
+        }
+#pragma unroll
+        for (int q = 0; q < LAYER_UNROLL; ++q) {
+            uint64_t const j = j0 + static_cast<uint64_t>(q) * WG;
+            if (j < N) {
+                uint32_t const iz = mod_dim(static_cast<uint32_t>(floorf(r[q].x + P.half[2])), Z);
+                if (lz) lz[j] = static_cast<uint16_t>(iz);
+                if (stats) {
+                    atomicAdd(&hc[rep + iz], 1u);
+                    atomicAdd(&hs[rep + iz], static_cast<unsigned long long>(static_cast<long long>(
+                                                 rint(static_cast<double>(r[q].y) * FIX_SCALE))));
+                }
+            }
         }
     }
-    if (mode == 0) {
+    if (stats) {
         __syncthreads();
+        uint32_t const slot = (blockIdx.x % LAYER_SLOTS) * Z;
         for (uint32_t i = threadIdx.x; i < Z; i += WG) {
-            if (hc[i]) {
-                long long const q = static_cast<long long>(hs[i]);
-                atomicAdd(cnt + i, hc[i]);
-                atomicAdd(sum + i, static_cast<unsigned long long>(q >> 20));
-                atomicAdd(sum + Z + i, static_cast<unsigned long long>(q & 0xfffffll));
+            uint32_t c = 0;
+            long long q = 0;
+            for (uint32_t k = 0; k < R; ++k) {
+                c += hc[k * Z + i];
+                q += static_cast<long long>(hs[k * Z + i]);
+            }
+            if (c) {
+                atomicAdd(cnt + slot + i, c);
+                atomicAdd(sum + slot + i, static_cast<unsigned long long>(q >> 20));
+                atomicAdd(sum + LAYER_SLOTS * Z + slot + i, static_cast<unsigned long long>(q & 0xfffffll));
             }
         }
     }
@@ -1830,19 +1855,15 @@ __global__ __launch_bounds__(WG) void k_layers(Sys sys, const float* __restrict_
 // of this step added to its x plane; lam_out: dL/dv of the input slots.
 // part[b], part[G + b]: this block's running sums of dL/dg and dL/dalpha (radians);
 // gf: dL/df per reference cell.
-// Reverse of one SRD step with the cells, tables, bounces and axes of the
-// recorded forward step: a table particle of a colliding cell gets
-// mu = R^T lam + (I - R^T) (sum of the table's lam) / total, every other
-// particle keeps lam, and each is scattered back to its source slot (negated
-// after a bounce); dL/dg, dL/dalpha and dL/df accumulate in FP64.
-// ADJ_CELLS cells per block (a quarter of the lanes take a cell in the cell
-// stage, all of them a particle; about 13 KB of LDS, eight blocks per CU:
-// 43 us per step at 64x64x52 cells, n = 10, against 48 with 128 cells per
-// block and 60 with 256).  The block's particles' adjoint values are staged in
-// LDS with coalesced loads; one lane per cell forms the cell's table sums (in
-// slot order, FP64) and its rotation of the mean; then every lane updates one
-// particle.  A block whose cells hold more particles than the staging buffer
-// runs the same arithmetic one lane per cell, reading memory directly.
+// ADJ_CELLS cells per block.  The kernel is bound by the latency of its
+// dependent memory rounds (a launch is about one wave of blocks), so it has
+// two: first the cell offsets, the cells' tape (axis, mean, total), their
+// force-gradient entries and the step's loss-seed table; then every slot's
+// adjoint, layer, source slot and streamed velocity, held in registers.  One
+// lane per cell forms the cell's table sums (in slot order, FP64) from LDS and
+// its rotation of the mean; then every lane finishes its own slots and
+// scatters them.  A block whose cells hold more particles than the staging
+// buffer runs the same arithmetic one lane per cell, reading memory directly.
 constexpr uint32_t ADJ_CELLS = 64;     // cells per block
 constexpr uint32_t ADJ_STAGE = 768;     // particles staged per block (12 n cells' worth at n = 10)
 
@@ -1853,7 +1874,8 @@ struct AdjIn {
     const uint32_t* tot;
     const uint32_t* perm;
     const float* vp;
-    const float* seed;
+    const uint16_t* lz;     // z layer of every output slot
+    const float* stab;      // loss seed dL/dv_x of a particle in each layer after the step
     const float* lam_in;
     float* lam_out;
 };
@@ -1881,15 +1903,20 @@ __device__ __forceinline__ void adj_put(AdjIn const& A, uint64_t N, uint32_t o, 
 
 // dL/dalpha contribution of table particle o: l . dR/dalpha (v' - u), with
 // dR/dalpha (v' - u) = -sin q + cos (q x a), q the part of v' - u normal to a
-__device__ __forceinline__ double adj_dalpha(KParams const& P, AdjIn const& A, uint64_t N, uint32_t o, float const* l,
-                                             float axx, float axy, float axz, float ux, float uy, float uz) {
-    float const wx = A.vp[o] - ux, wy = A.vp[N + o] - uy, wz = A.vp[2 * N + o] - uz;
+__device__ __forceinline__ double adj_dalpha_w(KParams const& P, float const* v, float const* l, float axx, float axy,
+                                               float axz, float ux, float uy, float uz) {
+    float const wx = v[0] - ux, wy = v[1] - uy, wz = v[2] - uz;
     float const d = wx * axx + wy * axy + wz * axz;
     float const qx = wx - axx * d, qy = wy - axy * d, qz = wz - axz * d;
     float const cx = qy * axz - qz * axy, cy = qz * axx - qx * axz, cz = qx * axy - qy * axx;
     return static_cast<double>(l[0]) * (-P.sin_a * qx + P.cos_a * cx) +
            static_cast<double>(l[1]) * (-P.sin_a * qy + P.cos_a * cy) +
            static_cast<double>(l[2]) * (-P.sin_a * qz + P.cos_a * cz);
+}
+__device__ __forceinline__ double adj_dalpha(KParams const& P, AdjIn const& A, uint64_t N, uint32_t o, float const* l,
+                                             float axx, float axy, float axz, float ux, float uy, float uz) {
+    float const v[3] = {A.vp[o], A.vp[N + o], A.vp[2 * N + o]};
+    return adj_dalpha_w(P, v, l, axx, axy, axz, ux, uy, uz);
 }
 
 // the whole reverse step of cell `key` on one lane (blocks too large to stage)
@@ -1899,7 +1926,7 @@ __device__ void adjoint_cell_serial(KParams const& P, AdjIn const& A, uint64_t N
     uint32_t const total = A.tot[key];
     uint32_t const reals = min(e - a, P.cap);
     auto lam = [&](uint32_t o, float* l) {
-        l[0] = A.lam_in[o] + A.seed[o];
+        l[0] = A.lam_in[o] + A.stab[A.lz[o]];
         l[1] = A.lam_in[N + o];
         l[2] = A.lam_in[2 * N + o];
     };
@@ -1946,76 +1973,122 @@ __device__ void adjoint_cell_serial(KParams const& P, AdjIn const& A, uint64_t N
 
 __global__ __launch_bounds__(WG) void k_adjoint_step(Sys sys, AdjIn A, double* __restrict__ part,
                                                      double* __restrict__ gf) {
+    constexpr uint32_t PPT = ADJ_STAGE / WG;   // staged slots per lane
+    static_assert(PPT * WG == ADJ_STAGE, "whole slots per lane");
     __shared__ float sl[3][ADJ_STAGE];   // adjoint (plus loss seed) of the block's particles
     __shared__ uint8_t scell[ADJ_STAGE]; // block-local cell of each staged particle
     __shared__ uint32_t soff[ADJ_CELLS + 1];
     __shared__ uint32_t sreal[ADJ_CELLS];      // table particles of a colliding cell, else 0
     __shared__ float scell_v[9][ADJ_CELLS];    // axis, cell mean u, (I - R^T) applied to the table mean
+    __shared__ float sstab[LAYERS_MAX];        // the step's loss seed per layer
     __shared__ double red[2][WG / 64];
     KParams const P = sys.P[0];
     uint64_t const N = sys.NT, C = sys.CT;
     uint32_t const tid = threadIdx.x;
     static_assert(ADJ_CELLS <= WG && ADJ_CELLS <= 256, "one lane per cell, cell index in a byte");
-    uint32_t const k0 = blockIdx.x * ADJ_CELLS;
+    // neighbouring cell ranges scatter into the same source windows: keep them
+    // on one XCD so their partial cache lines meet in one L2
+    uint32_t const bt = xcd_tile(blockIdx.x, gridDim.x);
+    uint32_t const k0 = bt * ADJ_CELLS;
     uint32_t const ncell = min(ADJ_CELLS, P.C - k0);   // k0 < P.C for every launched block
+    // round 1
     for (uint32_t k = tid; k <= ncell; k += WG) soff[k] = A.off[k0 + k];
+    for (uint32_t z = tid; z < P.dim[2]; z += WG) sstab[z] = A.stab[z];
+    uint32_t ctot = 0, cref = 0;
+    float cax = 0, cay = 0, caz = 0, cux = 0, cuy = 0, cuz = 0;
+    double gf_old = 0.0;
+    if (tid < ncell) {
+        uint32_t const key = k0 + tid;
+        ctot = A.tot[key];
+        cax = A.ax[key]; cay = A.ax[C + key]; caz = A.ax[2 * C + key];
+        cux = A.u[key]; cuy = A.u[C + key]; cuz = A.u[2 * C + key];
+        uint32_t ix, iy, iz;
+        decode_key(P, key, ix, iy, iz);
+        cref = P.cb + ref_of(P, ix, iy, iz);
+        if (ctot > 1) gf_old = gf[cref];
+    }
     __syncthreads();
     uint32_t const pa = soff[0], m = soff[ncell] - pa;
     double dg = 0.0, da = 0.0;
     if (m <= ADJ_STAGE) {
-        for (uint32_t i = tid; i < m; i += WG) {
-            uint64_t const o = pa + i;
-            sl[0][i] = A.lam_in[o] + A.seed[o];
-            sl[1][i] = A.lam_in[N + o];
-            sl[2][i] = A.lam_in[2 * N + o];
+        // round 2
+        float l[PPT][3], w[PPT][3];
+        uint32_t pe[PPT];
+        uint16_t lzv[PPT];
+#pragma unroll
+        for (uint32_t q = 0; q < PPT; ++q) {
+            uint32_t const i = tid + q * WG;
+            if (i < m) {
+                uint64_t const o = pa + i;
+                l[q][0] = A.lam_in[o];
+                l[q][1] = A.lam_in[N + o];
+                l[q][2] = A.lam_in[2 * N + o];
+                lzv[q] = A.lz[o];
+                pe[q] = A.perm[o];
+                w[q][0] = A.vp[o];
+                w[q][1] = A.vp[N + o];
+                w[q][2] = A.vp[2 * N + o];
+            }
         }
         if (tid < ncell)
             for (uint32_t i = soff[tid] - pa; i < soff[tid + 1] - pa; ++i) scell[i] = static_cast<uint8_t>(tid);
+#pragma unroll
+        for (uint32_t q = 0; q < PPT; ++q) {
+            uint32_t const i = tid + q * WG;
+            if (i < m) {
+                l[q][0] += sstab[lzv[q]];
+                sl[0][i] = l[q][0];
+                sl[1][i] = l[q][1];
+                sl[2][i] = l[q][2];
+            }
+        }
         __syncthreads();
         if (tid < ncell) {
-            uint32_t const key = k0 + tid;
             uint32_t const a = soff[tid] - pa, e = soff[tid + 1] - pa;
-            uint32_t const total = A.tot[key];
             uint32_t const reals = min(e - a, P.cap);
             sreal[tid] = 0u;
-            if (total > 1) {
-                float const axx = A.ax[key], axy = A.ax[C + key], axz = A.ax[2 * C + key];
+            if (ctot > 1) {
                 double sx = 0, sy = 0, sz = 0;
                 for (uint32_t i = a; i < a + reals; ++i) {
                     sx += sl[0][i];
                     sy += sl[1][i];
                     sz += sl[2][i];
                 }
-                float const inv = static_cast<float>(1.0 / static_cast<double>(total));
+                float const inv = static_cast<float>(1.0 / static_cast<double>(ctot));
                 float const mx = static_cast<float>(sx) * inv, my = static_cast<float>(sy) * inv,
                             mz = static_cast<float>(sz) * inv;
                 float rm[3];
-                adj_rot_t(P, axx, axy, axz, mx, my, mz, rm);
-                scell_v[0][tid] = axx; scell_v[1][tid] = axy; scell_v[2][tid] = axz;
-                scell_v[3][tid] = A.u[key]; scell_v[4][tid] = A.u[C + key]; scell_v[5][tid] = A.u[2 * C + key];
+                adj_rot_t(P, cax, cay, caz, mx, my, mz, rm);
+                scell_v[0][tid] = cax; scell_v[1][tid] = cay; scell_v[2][tid] = caz;
+                scell_v[3][tid] = cux; scell_v[4][tid] = cuy; scell_v[5][tid] = cuz;
                 scell_v[6][tid] = mx - rm[0]; scell_v[7][tid] = my - rm[1]; scell_v[8][tid] = mz - rm[2];
                 sreal[tid] = reals;
                 double const gx = sx * static_cast<double>(P.dt);
                 dg = gx;
-                uint32_t ix, iy, iz;
-                decode_key(P, key, ix, iy, iz);
-                gf[P.cb + ref_of(P, ix, iy, iz)] += gx;
+                gf[cref] = gf_old + gx;
             }
         }
         __syncthreads();
-        for (uint32_t i = tid; i < m; i += WG) {
-            uint32_t const k = scell[i];
-            uint32_t const o = pa + i;
-            float const l[3] = {sl[0][i], sl[1][i], sl[2][i]};
-            if (i - (soff[k] - pa) < sreal[k]) {
-                float const axx = scell_v[0][k], axy = scell_v[1][k], axz = scell_v[2][k];
-                float rl[3];
-                adj_rot_t(P, axx, axy, axz, l[0], l[1], l[2], rl);
-                float const mm[3] = {rl[0] + scell_v[6][k], rl[1] + scell_v[7][k], rl[2] + scell_v[8][k]};
-                da += adj_dalpha(P, A, N, o, l, axx, axy, axz, scell_v[3][k], scell_v[4][k], scell_v[5][k]);
-                adj_put(A, N, o, mm);
-            } else {
-                adj_put(A, N, o, l);
+#pragma unroll
+        for (uint32_t q = 0; q < PPT; ++q) {
+            uint32_t const i = tid + q * WG;
+            if (i < m) {
+                uint32_t const k = scell[i];
+                float mm[3] = {l[q][0], l[q][1], l[q][2]};
+                if (i - (soff[k] - pa) < sreal[k]) {
+                    float const axx = scell_v[0][k], axy = scell_v[1][k], axz = scell_v[2][k];
+                    float rl[3];
+                    adj_rot_t(P, axx, axy, axz, l[q][0], l[q][1], l[q][2], rl);
+                    mm[0] = rl[0] + scell_v[6][k];
+                    mm[1] = rl[1] + scell_v[7][k];
+                    mm[2] = rl[2] + scell_v[8][k];
+                    da += adj_dalpha_w(P, w[q], l[q], axx, axy, axz, scell_v[3][k], scell_v[4][k], scell_v[5][k]);
+                }
+                uint32_t const src = pe[q] & ~BOUNCED;
+                float const sg = (pe[q] & BOUNCED) ? -1.0f : 1.0f;
+                A.lam_out[src] = sg * mm[0];
+                A.lam_out[N + src] = sg * mm[1];
+                A.lam_out[2 * N + src] = sg * mm[2];
             }
         }
     } else if (tid < ncell) {
@@ -2038,8 +2111,8 @@ __global__ __launch_bounds__(WG) void k_adjoint_step(Sys sys, AdjIn A, double* _
             tg += red[0][k];
             ta += red[1][k];
         }
-        part[blockIdx.x] += tg;
-        part[gridDim.x + blockIdx.x] += ta;
+        part[bt] += tg;
+        part[gridDim.x + bt] += ta;
     }
 }
 
@@ -2228,16 +2301,16 @@ struct mpcd_hip_s {
         uint32_t K = 0, nck = 0, steps = 0;
         uint32_t* perm = nullptr;    // [K][N] recorded permutation with the bounce flag
         float* vp = nullptr;         // [K][3N] streamed velocities
-        float* seed = nullptr;       // [K][N] loss seed
-        uint32_t* off = nullptr;     // [K][C+1] cell offsets
+        uint16_t* lz = nullptr;      // [K][N] z layer of every output slot
+        uint32_t* off = nullptr;     // [K][C+1] cell offsets (the step's sort writes them here)
         float* u = nullptr;          // [K][3C] cell means
-        float* ax = nullptr;         // [K][3C] cell axes
+        float* ax = nullptr;         // [K][3C] cell axes (the step's draws write them here)
         uint32_t* tot = nullptr;     // [K][C] cell totals
         float* ckpt = nullptr;       // [nck][6N] states at the checkpoints (first: the start state)
         float* lam[2] = {nullptr, nullptr};
-        uint32_t* lcnt = nullptr;    // [steps][Z] layer counts
-        unsigned long long* lsum = nullptr;  // [steps][2][Z] layer x-velocity sums, fixed point (k_layers)
-        double* w = nullptr;         // [Z] dL/d(layer mean) / steps
+        uint32_t* lcnt = nullptr;    // [steps][LAYER_SLOTS][Z] layer counts
+        unsigned long long* lsum = nullptr;  // [steps][2][LAYER_SLOTS][Z] layer x-velocity sums, fixed point
+        float* stab = nullptr;       // [steps][Z] loss seed of a particle in each layer after each step
         double* part = nullptr;      // [2][blocks] per-block gradient sums
         double* gf = nullptr;        // [C] dL/df
         float* force = nullptr;      // [C] body force of the run
@@ -2866,11 +2939,11 @@ void destroy(mpcd_hip_s* h) {
 
 void adj_free(mpcd_hip_s* h) {
     auto& a = h->adj;
-    for (void* p : {static_cast<void*>(a.perm), static_cast<void*>(a.vp), static_cast<void*>(a.seed),
+    for (void* p : {static_cast<void*>(a.perm), static_cast<void*>(a.vp), static_cast<void*>(a.lz),
                     static_cast<void*>(a.off), static_cast<void*>(a.u), static_cast<void*>(a.ax),
                     static_cast<void*>(a.tot), static_cast<void*>(a.ckpt), static_cast<void*>(a.lam[0]),
                     static_cast<void*>(a.lam[1]), static_cast<void*>(a.lcnt), static_cast<void*>(a.lsum),
-                    static_cast<void*>(a.w), static_cast<void*>(a.part), static_cast<void*>(a.gf),
+                    static_cast<void*>(a.stab), static_cast<void*>(a.part), static_cast<void*>(a.gf),
                     static_cast<void*>(a.force)})
         if (p) (void)hipFree(p);
     a = mpcd_hip_s::Adj{};
@@ -2881,7 +2954,9 @@ void dalloc(T*& p, uint64_t count, const char* what) {
     ck(hipMalloc(&p, sizeof(T) * std::max<uint64_t>(count, 1)), what);
 }
 
-// buffers for `steps` steps with checkpoints every K steps (K = 0: forward only)
+// buffers for `steps` steps with checkpoints every K steps (K = 0: forward
+// only): K taped steps, and the start state of every segment but the last
+// (which the forward pass tapes)
 void adj_reserve(mpcd_hip_s* h, uint32_t steps, uint32_t K) {
     auto& a = h->adj;
     uint32_t const nck = K ? (steps + K - 1) / K : 1;
@@ -2890,14 +2965,14 @@ void adj_reserve(mpcd_hip_s* h, uint32_t steps, uint32_t K) {
     adj_free(h);
     uint64_t const N = h->N, C = h->C, Z = h->kp[0].dim[2];
     dalloc(a.ckpt, 6 * N * nck, "alloc checkpoints");
-    dalloc(a.lcnt, uint64_t(steps) * Z, "alloc layer counts");
-    dalloc(a.lsum, uint64_t(steps) * 2 * Z, "alloc layer sums");
-    dalloc(a.w, Z, "alloc layer weights");
+    dalloc(a.lcnt, uint64_t(steps) * LAYER_SLOTS * Z, "alloc layer counts");
+    dalloc(a.lsum, uint64_t(steps) * 2 * LAYER_SLOTS * Z, "alloc layer sums");
     dalloc(a.force, C, "alloc force field");
     if (K) {
+        dalloc(a.stab, uint64_t(steps) * Z, "alloc loss seeds");
         dalloc(a.perm, uint64_t(K) * N, "alloc tape");
         dalloc(a.vp, uint64_t(K) * 3 * N, "alloc tape");
-        dalloc(a.seed, uint64_t(K) * N, "alloc tape");
+        dalloc(a.lz, uint64_t(K) * N, "alloc tape");
         dalloc(a.off, uint64_t(K) * (C + 1), "alloc tape");
         dalloc(a.u, uint64_t(K) * 3 * C, "alloc tape");
         dalloc(a.ax, uint64_t(K) * 3 * C, "alloc tape");
@@ -2912,52 +2987,67 @@ void adj_reserve(mpcd_hip_s* h, uint32_t steps, uint32_t K) {
     a.steps = steps;
 }
 
-// one SRD step with the body force; records the tape slot `j` when given
-void adj_step(mpcd_hip_s* h, float const* force, int j) {
+// blocks of k_layers: one pass of WG * LAYER_UNROLL particles each (the
+// kernel waits on its loads, so all of them go out at once) up to 1024 blocks,
+// each flushing its histogram once with three atomics per layer (spread over
+// LAYER_SLOTS copies: 64x64x54 cells, n = 10: 256 blocks of four passes
+// 14.4 us per launch); no block sums more than LAYER_BLOCK_MAX particles (the
+// exactness bound of its fixed-point words)
+uint32_t layer_blocks(mpcd_hip_s* h) {
+    uint64_t const b = std::max<uint64_t>(std::min<uint64_t>((h->N + WG * LAYER_UNROLL - 1) / (WG * LAYER_UNROLL), 1024),
+                                          (h->N + LAYER_BLOCK_MAX - 1) / LAYER_BLOCK_MAX);
+    return static_cast<uint32_t>(std::max<uint64_t>(1, b));
+}
+
+// one SRD step with the body force, then the layer statistics of its output
+// into (cnt, sum) when given; records tape slot `j` when j >= 0: the step's
+// sort writes its offsets and the draws their axes straight into the slot
+void adj_step(mpcd_hip_s* h, float const* force, int j, uint32_t* cnt, unsigned long long* sum) {
     auto& a = h->adj;
     uint64_t const N = h->N, C = h->C;
-    if (!h->keys_valid) launch_keys(h);
-    if (!h->sorted_valid) launch_sort(h);
-    launch_draws(h);
     Tape T{};
     T.force = force;
     uint32_t* rec = nullptr;
+    uint16_t* lz = nullptr;
+    uint32_t* const own_offset = h->offset;
+    float* const own_axis = h->axis;
     if (j >= 0) {
         T.vp = a.vp + uint64_t(j) * 3 * N;
         T.u = a.u + uint64_t(j) * 3 * C;
         T.tot = a.tot + uint64_t(j) * C;
         rec = a.perm + uint64_t(j) * N;
+        lz = a.lz + uint64_t(j) * N;
+        h->offset = a.off + uint64_t(j) * (C + 1);
+        h->axis = a.ax + uint64_t(j) * 3 * C;
     }
-    switch (h->lds_cap) {
-        case 256: launch_collide_t<0, false, 256, true>(h, rec, T, 0, 0, h->ntiles); break;
-        case 384: launch_collide_t<0, false, 384, true>(h, rec, T, 0, 0, h->ntiles); break;
-        case 448: launch_collide_t<0, false, 448, true>(h, rec, T, 0, 0, h->ntiles); break;
-        case 512: launch_collide_t<0, false, 512, true>(h, rec, T, 0, 0, h->ntiles); break;
-        case 640: launch_collide_t<0, false, 640, true>(h, rec, T, 0, 0, h->ntiles); break;
-        case 1152: launch_collide_t<0, false, 1152, true>(h, rec, T, 0, 0, h->ntiles); break;
-        default: launch_collide_t<0, false, 2048, true>(h, rec, T, 0, 0, h->ntiles); break;
+    try {
+        if (!h->keys_valid) launch_keys(h);
+        if (!h->sorted_valid) launch_sort(h);
+        launch_draws(h);
+        switch (h->lds_cap) {
+            case 256: launch_collide_t<0, false, 256, true>(h, rec, T, 0, 0, h->ntiles); break;
+            case 384: launch_collide_t<0, false, 384, true>(h, rec, T, 0, 0, h->ntiles); break;
+            case 448: launch_collide_t<0, false, 448, true>(h, rec, T, 0, 0, h->ntiles); break;
+            case 512: launch_collide_t<0, false, 512, true>(h, rec, T, 0, 0, h->ntiles); break;
+            case 640: launch_collide_t<0, false, 640, true>(h, rec, T, 0, 0, h->ntiles); break;
+            case 1152: launch_collide_t<0, false, 1152, true>(h, rec, T, 0, 0, h->ntiles); break;
+            default: launch_collide_t<0, false, 2048, true>(h, rec, T, 0, 0, h->ntiles); break;
+        }
+        ck(hipGetLastError(), "adjoint-run step");
+    } catch (...) {
+        h->offset = own_offset;
+        h->axis = own_axis;
+        throw;
     }
-    ck(hipGetLastError(), "adjoint-run step");
-    if (j >= 0) {
-        ck(hipMemcpyAsync(a.off + uint64_t(j) * (C + 1), h->offset, 4 * (C + 1), hipMemcpyDeviceToDevice, h->stream), "tape");
-        ck(hipMemcpyAsync(a.ax + uint64_t(j) * 3 * C, h->axis, 12 * C, hipMemcpyDeviceToDevice, h->stream), "tape");
-    }
+    h->offset = own_offset;
+    h->axis = own_axis;
     hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, h->stream, h->sys, h->status + 1);
     h->cur = 1 - h->cur;
     h->step_index++;
     h->keys_valid = true;
     h->sorted_valid = false;
-}
-
-// blocks of k_layers: each flushes its LDS histogram with three atomics per
-// layer onto the same few addresses, so fewer, longer blocks (two per CU;
-// 64x64x52 cells, n = 10: 4096 blocks 64 us per launch, 1024 25 us, 512 a
-// little less), but no block sums more than LAYER_BLOCK_MAX particles (the
-// exactness bound of its fixed-point words)
-uint32_t layer_blocks(mpcd_hip_s* h) {
-    uint64_t const b = std::max<uint64_t>(std::min<uint64_t>((h->N + WG - 1) / WG, 512),
-                                          (h->N + LAYER_BLOCK_MAX - 1) / LAYER_BLOCK_MAX);
-    return static_cast<uint32_t>(std::max<uint64_t>(1, b));
+    if (cnt || lz)
+        hipLaunchKernelGGL(k_layers, dim3(layer_blocks(h)), dim3(WG), 0, h->stream, h->sys, h->st[h->cur], cnt, sum, lz);
 }
 
 struct ProfileResult {
@@ -3016,21 +3106,24 @@ ProfileResult profile_run(mpcd_hip_s* h, int steps, int K, float drag, float ang
         h->keys_valid = false;
         h->sorted_valid = false;
         upload_table(h);
-        // forward pass: layer statistics of every step, checkpoints
-        ck(hipMemsetAsync(a.lcnt, 0, 4ull * T * Z, h->stream), "clear");
-        ck(hipMemsetAsync(a.lsum, 0, 16ull * T * Z, h->stream), "clear");
-        uint32_t const lb = layer_blocks(h);
+        // forward pass: layer statistics of every step, the start state of
+        // every segment but the last, and the tape of the last segment (the
+        // whole run when K >= steps: nothing is recomputed then)
+        uint64_t const LZ = uint64_t(LAYER_SLOTS) * Z;
+        ck(hipMemsetAsync(a.lcnt, 0, 4ull * T * LZ, h->stream), "clear");
+        ck(hipMemsetAsync(a.lsum, 0, 16ull * T * LZ, h->stream), "clear");
+        uint32_t const nseg = Kc ? (T + Kc - 1) / Kc : 1;
+        uint32_t const tlast = (nseg - 1) * Kc;   // steps before the last segment
         for (uint32_t t = 1; t <= T; ++t) {
-            adj_step(h, force, -1);
-            hipLaunchKernelGGL(k_layers, dim3(lb), dim3(WG), 0, h->stream, h->sys, h->st[h->cur], 0,
-                               a.lcnt + uint64_t(t - 1) * Z, a.lsum + uint64_t(t - 1) * 2 * Z, nullptr, nullptr);
-            if (Kc && t % Kc == 0 && t < T)
+            int const j = (Kc && t > tlast) ? static_cast<int>(t - tlast - 1) : -1;
+            adj_step(h, force, j, a.lcnt + uint64_t(t - 1) * LZ, a.lsum + uint64_t(t - 1) * 2 * LZ);
+            if (Kc && t % Kc == 0 && t < tlast)
                 ck(hipMemcpyAsync(a.ckpt + uint64_t(t / Kc) * 6 * N, h->st[h->cur], 24 * N, hipMemcpyDeviceToDevice,
                                   h->stream), "checkpoint");
         }
         ck(hipGetLastError(), "forward pass");
-        std::vector<uint32_t> cnt(uint64_t(T) * Z);
-        std::vector<unsigned long long> sum(uint64_t(T) * 2 * Z);
+        std::vector<uint32_t> cnt(uint64_t(T) * LZ);
+        std::vector<unsigned long long> sum(uint64_t(T) * 2 * LZ);
         ck(hipStreamSynchronize(h->stream), "forward pass");
         uint32_t st = 0;
         ck(hipMemcpy(&st, h->status, 4, hipMemcpyDeviceToHost), "status");
@@ -3038,16 +3131,25 @@ ProfileResult profile_run(mpcd_hip_s* h, int steps, int K, float drag, float ang
             ck(hipMemset(h->status, 0, 4), "status");
             throw HipError{"a cell holds more particles than one LDS tile"};
         }
-        ck(hipMemcpy(cnt.data(), a.lcnt, 4ull * T * Z, hipMemcpyDeviceToHost), "download layers");
-        ck(hipMemcpy(sum.data(), a.lsum, 16ull * T * Z, hipMemcpyDeviceToHost), "download layers");
+        ck(hipMemcpy(cnt.data(), a.lcnt, 4ull * T * LZ, hipMemcpyDeviceToHost), "download layers");
+        ck(hipMemcpy(sum.data(), a.lsum, 16ull * T * LZ, hipMemcpyDeviceToHost), "download layers");
+        // per step and layer: particle count, and the sum of rint(v 2^40) as
+        // hi 2^20 + lo (both exact integers; the copies add up exactly)
+        std::vector<uint32_t> ct(uint64_t(T) * Z, 0u);
         res.profile.assign(zhi - zlo, 0.0);
         for (uint32_t t = 0; t < T; ++t)
             for (uint32_t z = zlo; z < zhi; ++z) {
-                uint32_t const c = cnt[uint64_t(t) * Z + z];
-                // sum of rint(v 2^40) = hi 2^20 + lo, both exact integers
-                double const hi = static_cast<double>(static_cast<long long>(sum[uint64_t(t) * 2 * Z + z]));
-                double const lo = static_cast<double>(sum[uint64_t(t) * 2 * Z + Z + z]);
-                double const m = c ? ((hi * 1048576.0 + lo) / FIX_SCALE) / c : 0.0;
+                uint32_t c = 0;
+                long long hi = 0;
+                unsigned long long lo = 0;
+                for (uint32_t k = 0; k < LAYER_SLOTS; ++k) {
+                    uint64_t const i = uint64_t(t) * LZ + uint64_t(k) * Z + z;
+                    c += cnt[i];
+                    hi += static_cast<long long>(sum[2 * uint64_t(t) * LZ + uint64_t(k) * Z + z]);
+                    lo += sum[2 * uint64_t(t) * LZ + LZ + uint64_t(k) * Z + z];
+                }
+                ct[uint64_t(t) * Z + z] = c;
+                double const m = c ? ((static_cast<double>(hi) * 1048576.0 + static_cast<double>(lo)) / FIX_SCALE) / c : 0.0;
                 res.profile[z - zlo] += m / T;
             }
         std::vector<double> w(Z, 0.0);
@@ -3058,32 +3160,37 @@ ProfileResult profile_run(mpcd_hip_s* h, int steps, int K, float drag, float ang
             w[z] = 2.0 * d / nl / T;
         }
         if (Kc) {
-            // reverse pass, one checkpoint segment at a time (recomputed with the tape)
-            ck(hipMemcpy(a.w, w.data(), 8ull * Z, hipMemcpyHostToDevice), "upload layer weights");
+            // loss seed of a particle in layer z after step t: w[z] / count
+            std::vector<float> stab(uint64_t(T) * Z, 0.0f);
+            for (uint32_t t = 0; t < T; ++t)
+                for (uint32_t z = 0; z < Z; ++z) {
+                    uint32_t const c = ct[uint64_t(t) * Z + z];
+                    stab[uint64_t(t) * Z + z] = c ? static_cast<float>(w[z] / static_cast<double>(c)) : 0.0f;
+                }
+            ck(hipMemcpy(a.stab, stab.data(), 4ull * T * Z, hipMemcpyHostToDevice), "upload loss seeds");
+            // reverse pass, one segment at a time from the last; every
+            // segment but the last is first recomputed from its start state
+            // with the tape
             uint32_t const gb = static_cast<uint32_t>((C + ADJ_CELLS - 1) / ADJ_CELLS);
             ck(hipMemsetAsync(a.lam[0], 0, 12 * N, h->stream), "clear");
             ck(hipMemsetAsync(a.part, 0, 16ull * gb, h->stream), "clear");
             ck(hipMemsetAsync(a.gf, 0, 8 * C, h->stream), "clear");
             int li = 0;
-            uint32_t const nseg = (T + Kc - 1) / Kc;
             for (int seg = static_cast<int>(nseg) - 1; seg >= 0; --seg) {
                 uint32_t const t0 = static_cast<uint32_t>(seg) * Kc, t1 = std::min(T, t0 + Kc);
-                ck(hipMemcpyAsync(h->st[h->cur], a.ckpt + uint64_t(seg) * 6 * N, 24 * N, hipMemcpyDeviceToDevice,
-                                  h->stream), "restore checkpoint");
-                h->step_index = s0 + t0;
-                h->keys_valid = false;
-                h->sorted_valid = false;
-                upload_table(h);
-                for (uint32_t t = t0 + 1; t <= t1; ++t) {
-                    int const j = static_cast<int>(t - t0 - 1);
-                    adj_step(h, force, j);
-                    hipLaunchKernelGGL(k_layers, dim3(lb), dim3(WG), 0, h->stream, h->sys, h->st[h->cur], 1,
-                                       a.lcnt + uint64_t(t - 1) * Z, nullptr, a.w, a.seed + uint64_t(j) * N);
+                if (static_cast<uint32_t>(seg) + 1 < nseg) {
+                    ck(hipMemcpyAsync(h->st[h->cur], a.ckpt + uint64_t(seg) * 6 * N, 24 * N, hipMemcpyDeviceToDevice,
+                                      h->stream), "restore checkpoint");
+                    h->step_index = s0 + t0;
+                    h->keys_valid = false;
+                    h->sorted_valid = false;
+                    upload_table(h);
+                    for (uint32_t t = t0 + 1; t <= t1; ++t) adj_step(h, force, static_cast<int>(t - t0 - 1), nullptr, nullptr);
                 }
                 for (uint32_t t = t1; t > t0; --t) {
                     uint64_t const j = t - t0 - 1;
                     AdjIn const in{a.off + j * (C + 1), a.u + j * 3 * C, a.ax + j * 3 * C, a.tot + j * C, a.perm + j * N,
-                                   a.vp + j * 3 * N, a.seed + j * N, a.lam[li], a.lam[1 - li]};
+                                   a.vp + j * 3 * N, a.lz + j * N, a.stab + uint64_t(t - 1) * Z, a.lam[li], a.lam[1 - li]};
                     hipLaunchKernelGGL(k_adjoint_step, dim3(gb), dim3(WG), 0, h->stream, h->sys, in, a.part, a.gf);
                     li = 1 - li;
                 }
