@@ -226,8 +226,10 @@ int mpcd_hip_tune_capacities(int algorithm, uint32_t nmax, uint32_t* caps, uint3
 int mpcd_hip_step_noalloc(const mpcd_params_t* params, uint64_t step_index, int n_steps, float* state_in,
                           float* state_out, size_t n_particles);
 
-/* Copies `bytes` between two device buffers `reps` times with a plain float4
- * kernel and returns the mean time per copy: the practical HBM reference. */
+/* Copies `bytes` between two device buffers `reps` times with each of four
+ * copy kernels (4 or 8 float4 loads in flight per lane, plain or nontemporal
+ * stores) and returns the mean time per copy of the fastest: the practical
+ * HBM reference.  No counterpart in the reference (measurement only). */
 int mpcd_hip_bench_copy(size_t bytes, int reps, double* ms_per_copy);
 /* Device math of the random stream, for known-answer tests: Philox4x32-10 of
  * n counters (4 words each), and the float transforms. */
