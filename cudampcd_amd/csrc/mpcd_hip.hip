@@ -469,13 +469,24 @@ __global__ __launch_bounds__(1024) void k_scan_partials(uint32_t* __restrict__ p
     }
 }
 
+// fold: `partial` holds the blocks' totals and every block adds up those
+// before it (grids of at most SCAN_FOLD_MAX blocks, which then skip
+// k_scan_partials and its launch); else it holds their exclusive prefix
+constexpr uint32_t SCAN_FOLD_MAX = 1024;
 __global__ __launch_bounds__(WG) void k_scan_down(uint32_t* __restrict__ count, uint32_t C,
                                                   const uint32_t* __restrict__ partial, uint32_t* __restrict__ offset,
-                                                  uint32_t* __restrict__ cursor, uint64_t N) {
+                                                  uint32_t* __restrict__ cursor, uint64_t N, int fold) {
     __shared__ uint32_t s_v[SCAN_CHUNK + SCAN_CHUNK / 32];
     __shared__ uint32_t s_w[WG / 64];
+    __shared__ uint32_t s_f[WG / 64];
     uint64_t const base = static_cast<uint64_t>(blockIdx.x) * SCAN_CHUNK;
     auto pad = [](uint32_t i) { return i + (i >> 5); };
+    uint32_t before = 0;
+    if (fold) {
+        for (uint32_t i = threadIdx.x; i < blockIdx.x; i += WG) before += partial[i];
+        for (int dd = 32; dd >= 1; dd >>= 1) before += static_cast<uint32_t>(__shfl_xor(before, dd, 64));
+        if ((threadIdx.x & 63u) == 0) s_f[threadIdx.x >> 6] = before;
+    }
 #pragma unroll
     for (int j = 0; j < SCAN_ITEMS; ++j) {
         uint32_t const li = j * WG + threadIdx.x;
@@ -488,7 +499,15 @@ __global__ __launch_bounds__(WG) void k_scan_down(uint32_t* __restrict__ count, 
     for (int j = 0; j < SCAN_ITEMS; ++j) run += s_v[pad(threadIdx.x * SCAN_ITEMS + j)];
     uint32_t tot;
     uint32_t pre = block_exclusive<WG>(run, s_w, &tot);
-    pre += partial[blockIdx.x];
+    if (fold) {
+        // s_f was written before the block_exclusive barriers
+        uint32_t t = 0;
+#pragma unroll
+        for (int k = 0; k < WG / 64; ++k) t += s_f[k];
+        pre += t;
+    } else {
+        pre += partial[blockIdx.x];
+    }
 #pragma unroll
     for (int j = 0; j < SCAN_ITEMS; ++j) {
         uint32_t const li = pad(threadIdx.x * SCAN_ITEMS + j);
@@ -628,12 +647,7 @@ __device__ __forceinline__ void perm_block(Sys const& sys, const uint32_t* __res
     }
 }
 
-__global__ __launch_bounds__(WG) void k_perm(Sys sys, const uint32_t* __restrict__ keys, uint32_t* __restrict__ cursor,
-                                             uint32_t* __restrict__ perm, uint64_t N) {
-    // neighbouring blocks fill the same cells: keep them on one XCD so their
-    // partial cache lines meet in one L2
-    perm_block(sys, keys, cursor, perm, N, xcd_tile(blockIdx.x, gridDim.x));
-}
+
 
 // per-cell ghost count in a wall layer (simulation_functions.cpp:137-146)
 __device__ __forceinline__ uint32_t ghost_count(KParams const& P, uint32_t ref, float sh, bool layer, bool sign) {
@@ -802,10 +816,27 @@ __global__ __launch_bounds__(WG) void k_cell_draws(Sys sys, float* __restrict__ 
     else axes_block(sys, axis, blockIdx.x - nbw);
 }
 
+// the step's permutation (blocks [0, np)) and, in the same launch, its
+// per-cell draws (the blocks after them: the work of k_cell_draws), so a step
+// pays for one launch less; the two parts touch disjoint memory
+__global__ __launch_bounds__(WG) void k_perm(Sys sys, const uint32_t* __restrict__ keys, uint32_t* __restrict__ cursor,
+                                             uint32_t* __restrict__ perm, uint64_t N, uint32_t np,
+                                             float* __restrict__ axis, uint32_t* __restrict__ gcnt,
+                                             float* __restrict__ gvel, uint32_t nbw, int ext) {
+    if (blockIdx.x < np) {
+        // neighbouring blocks fill the same cells: keep them on one XCD so
+        // their partial cache lines meet in one L2
+        perm_block(sys, keys, cursor, perm, N, xcd_tile(blockIdx.x, np));
+    } else {
+        uint32_t const b = blockIdx.x - np;
+        if (b < nbw) ghosts_block(sys, gcnt, gvel, b, ext != 0);
+        else axes_block(sys, axis, b - nbw);
+    }
+}
+
 // one step forward of every system's step state: the shift of step t + 1 and
 // the next shift (the grid shift is a pure function of (seed, system, step))
-__global__ void k_advance(Sys sys, uint32_t* __restrict__ big) {
-    uint32_t const s = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void advance_system(Sys const& sys, uint32_t* __restrict__ big, uint32_t s) {
     if (s == 0) big[0] = 0u;   // the step's big cells are done
     if (s >= sys.ns) return;
     KParams* const P = sys.P + s;
@@ -817,6 +848,10 @@ __global__ void k_advance(Sys sys, uint32_t* __restrict__ big) {
         P->shift[a] = P->shift_next[a];
         P->shift_next[a] = nx[a];
     }
+}
+
+__global__ void k_advance(Sys sys, uint32_t* __restrict__ big) {
+    advance_system(sys, big, blockIdx.x * blockDim.x + threadIdx.x);
 }
 
 // LDS tile of the collide kernel: up to CAP particles of whole cells, up to
@@ -1784,13 +1819,19 @@ constexpr int LAYER_REP_WORDS = 2048;         // LDS histogram entries (layers t
 // (units of 2^-20) and Q & (2^20 - 1) (units of 2^-40), which cannot
 // overflow for fewer than 2^32 particles; the host adds the copies.
 // cnt: [LAYER_SLOTS][Z]; sum: [2][LAYER_SLOTS][Z] (hi, then lo).
+// With `big`, block 0 also moves the system to its next step (the work of
+// k_advance): the pass reads only the system's z extent and half-width, which
+// that does not change.
 __global__ __launch_bounds__(WG) void k_layers(Sys sys, const float* __restrict__ st, uint32_t* __restrict__ cnt,
-                                               unsigned long long* __restrict__ sum, uint16_t* __restrict__ lz) {
+                                               unsigned long long* __restrict__ sum, uint16_t* __restrict__ lz,
+                                               uint32_t* __restrict__ big) {
     __shared__ uint32_t hc[LAYER_REP_WORDS];
     __shared__ unsigned long long hs[LAYER_REP_WORDS];
-    KParams const P = sys.P[0];
     uint64_t const N = sys.NT;
-    uint32_t const Z = P.dim[2];
+    uint32_t const Z = sys.P[0].dim[2];
+    float const half_z = sys.P[0].half[2];
+    if (big && blockIdx.x == 0 && threadIdx.x < max(sys.ns, 1u)) advance_system(sys, big, threadIdx.x);
+    if (!cnt && !lz) return;
     // replicas: the largest power of two up to 16 that fits (2 or more, as Z <= LAYERS_MAX)
     uint32_t const R = 1u << (31 - __clz(min(16u, static_cast<uint32_t>(LAYER_REP_WORDS) / Z)));
     uint32_t const rep = (threadIdx.x & (R - 1u)) * Z;
@@ -1814,7 +1855,7 @@ __global__ __launch_bounds__(WG) void k_layers(Sys sys, const float* __restrict_
         for (int q = 0; q < LAYER_UNROLL; ++q) {
             uint64_t const j = j0 + static_cast<uint64_t>(q) * WG;
             if (j < N) {
-                uint32_t const iz = mod_dim(static_cast<uint32_t>(floorf(r[q].x + P.half[2])), Z);
+                uint32_t const iz = mod_dim(static_cast<uint32_t>(floorf(r[q].x + half_z)), Z);
                 if (lz) lz[j] = static_cast<uint16_t>(iz);
                 if (stats) {
                     atomicAdd(&hc[rep + iz], 1u);
@@ -2609,10 +2650,14 @@ int guarded(mpcd_hip_s* h, F&& f) {
     return 2;
 }
 
-// the SRD per-cell draws of the step in the system table (k_cell_draws)
+// blocks of the step's per-cell draws: wall ghosts, and the SRD axes (the
+// extended rule draws its axes in the collision)
+uint32_t draw_blocks(mpcd_hip_s const* h) { return h->blocks_wall + (h->algorithm == 0 ? h->blocks_cell : 0u); }
+
+// the per-cell draws of the step in the system table alone (k_cell_draws),
+// for a step that needs no sort
 int launch_draws(mpcd_hip_s* h) {
-    // the extended rule draws its ghosts here and its axes in the collision
-    uint32_t const nb = h->blocks_wall + (h->algorithm == 0 ? h->blocks_cell : 0u);
+    uint32_t const nb = draw_blocks(h);
     if (!nb) return 0;
     hipLaunchKernelGGL(k_cell_draws, dim3(nb), dim3(WG), 0, h->stream, h->sys, h->axis, h->wgcnt, h->wgvel,
                        h->blocks_wall, h->algorithm);
@@ -2652,19 +2697,25 @@ void fit_capacity(mpcd_hip_s* h) {
 
 void launch_scan(mpcd_hip_s* h) {
     uint32_t const C = static_cast<uint32_t>(h->C);
+    int const fold = h->nb <= SCAN_FOLD_MAX ? 1 : 0;
     hipLaunchKernelGGL(k_scan_reduce, dim3(h->nb), dim3(WG), 0, h->stream, h->count, C, h->partial);
-    hipLaunchKernelGGL(k_scan_partials, dim3(1), dim3(1024), 0, h->stream, h->partial, h->nb);
-    hipLaunchKernelGGL(k_scan_down, dim3(h->nb), dim3(WG), 0, h->stream, h->count, C, h->partial, h->offset, h->cursor, h->N);
+    if (!fold) hipLaunchKernelGGL(k_scan_partials, dim3(1), dim3(1024), 0, h->stream, h->partial, h->nb);
+    hipLaunchKernelGGL(k_scan_down, dim3(h->nb), dim3(WG), 0, h->stream, h->count, C, h->partial, h->offset, h->cursor,
+                       h->N, fold);
 }
 
 uint32_t perm_blocks(mpcd_hip_s const* h) {
     return static_cast<uint32_t>((h->N + WG * PERM_ITEMS - 1) / (WG * PERM_ITEMS));
 }
 
-int launch_sort(mpcd_hip_s* h) {
+// the sort of the step (scan and permutation); with `draws` the step's
+// per-cell draws ride in the permutation's launch
+int launch_sort(mpcd_hip_s* h, bool draws = false) {
     launch_scan(h);
-    uint32_t const blocks = perm_blocks(h);
-    if (blocks) hipLaunchKernelGGL(k_perm, dim3(blocks), dim3(WG), 0, h->stream, h->sys, h->keys, h->cursor, h->perm, h->N);
+    uint32_t const np = perm_blocks(h), nd = draws ? draw_blocks(h) : 0u;
+    if (np + nd)
+        hipLaunchKernelGGL(k_perm, dim3(np + nd), dim3(WG), 0, h->stream, h->sys, h->keys, h->cursor, h->perm, h->N, np,
+                           h->axis, h->wgcnt, h->wgvel, h->blocks_wall, h->algorithm);
     ck(hipGetLastError(), "sort");
     return 4;
 }
@@ -2762,8 +2813,8 @@ void ensure_sampling(mpcd_hip_s* h) {
 int enqueue_step(mpcd_hip_s* h, bool sample, bool first, bool sort, uint32_t* rec, hipEvent_t e0, hipEvent_t e1,
                  uint32_t io) {
     int launches = 0;
-    if (sort) launches += launch_sort(h);
-    launches += launch_draws(h);
+    if (sort) launches += launch_sort(h, true);
+    else launches += launch_draws(h);
     if (e0) ck(hipEventRecord(e0, h->stream), "event");
     if (h->algorithm == 0) {
         if (sample) launch_collide_cap<0, true>(h, rec, io); else launch_collide_cap<0, false>(h, rec, io);
@@ -3039,8 +3090,8 @@ void adj_step(mpcd_hip_s* h, float const* force, int j, uint32_t* cnt, unsigned 
     }
     try {
         if (!h->keys_valid) launch_keys(h);
-        if (!h->sorted_valid) launch_sort(h);
-        launch_draws(h);
+        if (!h->sorted_valid) launch_sort(h, true);
+        else launch_draws(h);
         switch (h->lds_cap) {
             case 256: launch_collide_t<0, false, 256, true>(h, rec, T, 0, 0, h->ntiles); break;
             case 384: launch_collide_t<0, false, 384, true>(h, rec, T, 0, 0, h->ntiles); break;
@@ -3058,13 +3109,13 @@ void adj_step(mpcd_hip_s* h, float const* force, int j, uint32_t* cnt, unsigned 
     }
     h->offset = own_offset;
     h->axis = own_axis;
-    hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, h->stream, h->sys, h->status + 1);
     h->cur = 1 - h->cur;
     h->step_index++;
     h->keys_valid = true;
     h->sorted_valid = false;
-    if (cnt || lz)
-        hipLaunchKernelGGL(k_layers, dim3(layer_blocks(h)), dim3(WG), 0, h->stream, h->sys, h->st[h->cur], cnt, sum, lz);
+    // the layer pass also moves the system to the next step
+    hipLaunchKernelGGL(k_layers, dim3(layer_blocks(h)), dim3(WG), 0, h->stream, h->sys, h->st[h->cur], cnt, sum, lz,
+                       h->status + 1);
 }
 
 struct ProfileResult {
