@@ -172,6 +172,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--seed", type=int, default=None, help="default 0x5EED0000 + config number")
     ap.add_argument("--windows", type=int, default=32, help="adjoint accuracy study: windows (c5)")
+    ap.add_argument("--no-loop", action="store_true", help="c4: skip the plain per-channel loop (profiling)")
     ap.add_argument("--tiling", default="",
                     help="cells_per_tile,lds_capacity for k_collide (results do not depend on it)")
     ap.add_argument("--sample", action="store_true",
@@ -348,20 +349,22 @@ def bench_ensemble(args, cfg):
     collide_avg_ms = collide_ms / 5
     del ens, h
 
-    print(f"[bench] plain loop over {len(params)} Simulation objects", file=sys.stderr, flush=True)
-    sims = [pympcd.Simulation(q, "hip") for q in params]
-    for sm in sims:
-        sm.step(args.warmup)
-    t0 = time.perf_counter()
-    for sm in sims:
-        sm.step(args.steps)
-    wall_loop = time.perf_counter() - t0
-    del sims
+    wall_loop = None
+    if not args.no_loop:
+        print(f"[bench] plain loop over {len(params)} Simulation objects", file=sys.stderr, flush=True)
+        sims = [pympcd.Simulation(q, "hip") for q in params]
+        for sm in sims:
+            sm.step(args.warmup)
+        t0 = time.perf_counter()
+        for sm in sims:
+            sm.step(args.steps)
+        wall_loop = time.perf_counter() - t0
+        del sims
 
     copy_bytes = 4 * 10 ** 9
     copy_gbps = 2 * copy_bytes / capi.bench_copy(copy_bytes, 5) / 1e6
     value = N * args.steps / wall
-    loop_value = N * args.steps / wall_loop
+    loop_value = N * args.steps / wall_loop if wall_loop else None
     ab = algorithmic_bytes(N, C)
     achieved = ab["collide"] / (collide_avg_ms * 1e-3) / 1e9
     n_mean = N / sum(m["fluid"][0] * m["fluid"][1] * m["fluid"][2] for m in members)
@@ -382,9 +385,10 @@ def bench_ensemble(args, cfg):
         "config": {"workload": cfg["workload"], "channels": len(members), "particles": N, "cells": C,
                    "algorithm": cfg["algorithm"], "delta_t": 0.02,
                    "tiling": {k: [list(x) for x in v] for k, v in shapes.items()}, "parallelism": "single GPU"},
-        "plain_loop": {"value": loop_value, "ms_per_step": round(wall_loop * 1e3 / args.steps, 4),
-                       "what": f"for each of {len(members)} Simulation objects (same seeds): step({args.steps})"},
-        "batched_over_loop": round(value / loop_value, 4),
+        "plain_loop": None if wall_loop is None else {
+            "value": loop_value, "ms_per_step": round(wall_loop * 1e3 / args.steps, 4),
+            "what": f"for each of {len(members)} Simulation objects (same seeds): step({args.steps})"},
+        "batched_over_loop": None if wall_loop is None else round(value / loop_value, 4),
         "equal_accuracy": "per-channel results are bit-identical to the loop's (same seeds), so the viscosity "
                           "samples and their standard errors are identical; GPU-seconds per sample scale as 1/value",
         "roofline": {

@@ -816,16 +816,23 @@ __global__ __launch_bounds__(WG) void k_cell_draws(Sys sys, float* __restrict__ 
     else axes_block(sys, axis, blockIdx.x - nbw);
 }
 
+__global__ __launch_bounds__(WG) void k_perm(Sys sys, const uint32_t* __restrict__ keys, uint32_t* __restrict__ cursor,
+                                             uint32_t* __restrict__ perm, uint64_t N) {
+    // neighbouring blocks fill the same cells: keep them on one XCD so their
+    // partial cache lines meet in one L2
+    perm_block(sys, keys, cursor, perm, N, xcd_tile(blockIdx.x, gridDim.x));
+}
+
 // the step's permutation (blocks [0, np)) and, in the same launch, its
 // per-cell draws (the blocks after them: the work of k_cell_draws), so a step
-// pays for one launch less; the two parts touch disjoint memory
-__global__ __launch_bounds__(WG) void k_perm(Sys sys, const uint32_t* __restrict__ keys, uint32_t* __restrict__ cursor,
-                                             uint32_t* __restrict__ perm, uint64_t N, uint32_t np,
-                                             float* __restrict__ axis, uint32_t* __restrict__ gcnt,
-                                             float* __restrict__ gvel, uint32_t nbw, int ext) {
+// pays for one launch less; the two parts touch disjoint memory.  A kernel of
+// its own: the draws' registers would lower k_perm's occupancy
+__global__ __launch_bounds__(WG) void k_perm_draws(Sys sys, const uint32_t* __restrict__ keys,
+                                                   uint32_t* __restrict__ cursor, uint32_t* __restrict__ perm,
+                                                   uint64_t N, uint32_t np, float* __restrict__ axis,
+                                                   uint32_t* __restrict__ gcnt, float* __restrict__ gvel,
+                                                   uint32_t nbw, int ext) {
     if (blockIdx.x < np) {
-        // neighbouring blocks fill the same cells: keep them on one XCD so
-        // their partial cache lines meet in one L2
         perm_block(sys, keys, cursor, perm, N, xcd_tile(blockIdx.x, np));
     } else {
         uint32_t const b = blockIdx.x - np;
@@ -2708,14 +2715,24 @@ uint32_t perm_blocks(mpcd_hip_s const* h) {
     return static_cast<uint32_t>((h->N + WG * PERM_ITEMS - 1) / (WG * PERM_ITEMS));
 }
 
+// The per-cell draws ride in the permutation's launch when they are few (a
+// launch less per step: 64x64x52 cells, n = 10: 16.8 us against 12.7 + 11.5);
+// many of them run better on their own, at the occupancy of their small LDS
+// footprint rather than the permutation's (1000x1000x22 cells: 1.49 ms
+// against 1.10 + 0.34)
+constexpr uint32_t DRAWS_FOLD_MAX = 8192;   // draw blocks
+bool fold_draws(mpcd_hip_s const* h) { return draw_blocks(h) <= DRAWS_FOLD_MAX; }
+
 // the sort of the step (scan and permutation); with `draws` the step's
 // per-cell draws ride in the permutation's launch
 int launch_sort(mpcd_hip_s* h, bool draws = false) {
     launch_scan(h);
     uint32_t const np = perm_blocks(h), nd = draws ? draw_blocks(h) : 0u;
-    if (np + nd)
-        hipLaunchKernelGGL(k_perm, dim3(np + nd), dim3(WG), 0, h->stream, h->sys, h->keys, h->cursor, h->perm, h->N, np,
-                           h->axis, h->wgcnt, h->wgvel, h->blocks_wall, h->algorithm);
+    if (nd)
+        hipLaunchKernelGGL(k_perm_draws, dim3(np + nd), dim3(WG), 0, h->stream, h->sys, h->keys, h->cursor, h->perm,
+                           h->N, np, h->axis, h->wgcnt, h->wgvel, h->blocks_wall, h->algorithm);
+    else if (np)
+        hipLaunchKernelGGL(k_perm, dim3(np), dim3(WG), 0, h->stream, h->sys, h->keys, h->cursor, h->perm, h->N);
     ck(hipGetLastError(), "sort");
     return 4;
 }
@@ -2813,8 +2830,9 @@ void ensure_sampling(mpcd_hip_s* h) {
 int enqueue_step(mpcd_hip_s* h, bool sample, bool first, bool sort, uint32_t* rec, hipEvent_t e0, hipEvent_t e1,
                  uint32_t io) {
     int launches = 0;
-    if (sort) launches += launch_sort(h, true);
-    else launches += launch_draws(h);
+    bool const fold = sort && fold_draws(h);
+    if (sort) launches += launch_sort(h, fold);
+    if (!fold) launches += launch_draws(h);
     if (e0) ck(hipEventRecord(e0, h->stream), "event");
     if (h->algorithm == 0) {
         if (sample) launch_collide_cap<0, true>(h, rec, io); else launch_collide_cap<0, false>(h, rec, io);
@@ -3090,8 +3108,9 @@ void adj_step(mpcd_hip_s* h, float const* force, int j, uint32_t* cnt, unsigned 
     }
     try {
         if (!h->keys_valid) launch_keys(h);
-        if (!h->sorted_valid) launch_sort(h, true);
-        else launch_draws(h);
+        bool const fold = !h->sorted_valid && fold_draws(h);
+        if (!h->sorted_valid) launch_sort(h, fold);
+        if (!fold) launch_draws(h);
         switch (h->lds_cap) {
             case 256: launch_collide_t<0, false, 256, true>(h, rec, T, 0, 0, h->ntiles); break;
             case 384: launch_collide_t<0, false, 384, true>(h, rec, T, 0, 0, h->ntiles); break;
