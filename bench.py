@@ -1,7 +1,7 @@
 #!/usr/bin/env python3
 """Throughput of the MPCD time step on one MI355X.
 
-    python bench.py [--steps K] [--warmup W] [--config c2|c3] [--no-cpu]
+    python bench.py [--steps K] [--warmup W] [--config c2|c3|c4|c5] [--no-cpu]
 
 Default workload (c2): plain SRD, 1000x1000x20 fluid cells with walls in z
 (1000x1000x22 stored cells), n=20 -> 4e8 particles, T=1, dt=0.02, g=1e-3,
@@ -13,9 +13,15 @@ cells, wall gap uniform in [20, 200], n in {5, 10, 20}, g uniform in
 [5e-4, 2e-3], distinct seeds) advanced as one Ensemble (autotuned tiles,
 captured step graph), timed against a plain loop over 64 Simulation objects
 with the same seeds (identical per-channel results, so equal accuracy).
+c5: pathwise adjoint gradient of a time-averaged velocity-profile loss (SRD
+64x64x50 cells, n=10, 200 steps, checkpoint every 20) against one loss
+evaluation, with the whole tape resident as well, and the accuracy study of
+the drag gradient against central differences.
 
 A step is one shift + stream + sort + collide pass over all particles, with
-the particle state resident in HBM.  Prints one JSON line.
+the particle state resident in HBM.  Prints one JSON line; the default (c2)
+line also carries the sampled c2 step, c3, c4 and c5 under "extra", so that
+every workload is timed in the same run.
 """
 from __future__ import annotations
 
@@ -203,9 +209,11 @@ def main():
     if args.seed is None:
         args.seed = 0x5EED0000 + int(args.config[1:])
     if args.config == "c4":
-        return bench_ensemble(args, cfg)
+        print(json.dumps(bench_ensemble(args, cfg)), flush=True)
+        return
     if args.config == "c5":
-        return bench_adjoint(args, cfg)
+        print(json.dumps(bench_adjoint(args, cfg)), flush=True)
+        return
     tiling = tuple(int(x) for x in args.tiling.split(",")) if args.tiling else None
     out = measure(args.config, args.seed, args.steps, args.warmup, args.sample, tiling)
     if args.config == "c2" and not args.sample and not args.single:
@@ -215,6 +223,14 @@ def main():
             "c2_sampled": summary(measure("c2", args.seed, args.steps, args.warmup, True, None)),
             "c3": summary(measure("c3", 0x5EED0003, args.steps, args.warmup, False, None)),
         }
+        # the ensemble against its plain loop, and the adjoint gradient against
+        # the loss, in the same run (their own configurations' step counts)
+        sub = argparse.Namespace(**vars(args))
+        sub.steps, sub.warmup, sub.seed, sub.no_loop, sub.no_cpu = 50, 5, 0x5EED0004, False, True
+        out["extra"]["c4"] = bench_ensemble(sub, CONFIGS["c4"])
+        sub = argparse.Namespace(**vars(args))
+        sub.steps, sub.warmup, sub.seed, sub.no_cpu = 5, 10, 0x5EED0005, True
+        out["extra"]["c5"] = bench_adjoint(sub, CONFIGS["c5"])
     if not args.no_cpu:
         print("[bench] cpu baseline", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(CONFIGS[args.config].get("n", 10))
@@ -412,7 +428,7 @@ def bench_ensemble(args, cfg):
     if not args.no_cpu:
         print("[bench] cpu baseline", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(10)
-    print(json.dumps(out), flush=True)
+    return out
 
 
 def bench_adjoint(args, cfg):
@@ -519,7 +535,8 @@ def bench_adjoint(args, cfg):
         "accuracy_study_d_drag": study,
         "particle_steps_per_s_forward": N * steps / t_loss,
     }
-    print(json.dumps(out), flush=True)
+    del sim
+    return out
 
 
 if __name__ == "__main__":
