@@ -1667,13 +1667,20 @@ __device__ __forceinline__ void collide_tile(Sys const& sys, const float* __rest
         // rows one step away along the middle axis, smid keys away), the rest
         // straight to memory.  Window bases may wrap below zero: distances are
         // taken modulo 2^32, and only bins of real keys are ever non-zero.
+        // Bins are 16 bits, two to a word (bin b and bin b + WH of a window
+        // share word b, so neighbouring keys fall on different words; a bin
+        // counts at most one batch), which makes each window twice as wide as
+        // its words: centred on the batch it covers moves of up to
+        // (W - span) / 2 keys, whole rows of the second-fastest axis for
+        // fastest axes up to about 110 cells with 384-particle batches
         uint32_t* const hist = s.perm;
-        constexpr uint32_t W = static_cast<uint32_t>(CAP) / 3;
+        constexpr uint32_t WH = static_cast<uint32_t>(CAP) / 3;   // words per window
+        constexpr uint32_t W = 2 * WH;                              // bins per window
         uint32_t const span = ke - kb;
         uint32_t const wb0 = g0 + kb - (span < W ? (W - span) / 2 : 0u);
         uint32_t const smid = mid_stride(P, c0 + kb);
         uint32_t const wbm = wb0 - smid, wbp = wb0 + smid;
-        for (uint32_t i = tid; i < 3 * W; i += NT) hist[i] = 0u;
+        for (uint32_t i = tid; i < 3 * WH; i += NT) hist[i] = 0u;
         // sampling: un-shifted cells lie in the batch's cells or one step back
         // along each axis the shift is positive on, one step ahead where it is
         // negative, so two windows (the batch's keys widened on the shift's side
@@ -1763,20 +1770,29 @@ __device__ __forceinline__ void collide_tile(Sys const& sys, const float* __rest
             }
             if (valid) {
                 keys[static_cast<uint64_t>(gb) + i] = knext;
-                uint32_t w = knext - wb0;
-                if (w < W) atomicAdd(hist + w, 1u);
-                else if ((w = knext - wbm) < W) atomicAdd(hist + W + w, 1u);
-                else if ((w = knext - wbp) < W) atomicAdd(hist + 2 * W + w, 1u);
+                uint32_t w = knext - wb0, j = 0;
+                if (w >= W) {
+                    j = 1;
+                    w = knext - wbm;
+                    if (w >= W) {
+                        j = 2;
+                        w = knext - wbp;
+                    }
+                }
+                if (w < W) atomicAdd(hist + j * WH + (w < WH ? w : w - WH), w < WH ? 1u : 0x10000u);
                 else atomicAdd(count_next + knext, 1u);
             }
         }
         __syncthreads();
-        for (uint32_t i = tid; i < 3 * W; i += NT) {
+        for (uint32_t i = tid; i < 3 * WH; i += NT) {
             uint32_t const c = hist[i];
             if (c) {
-                uint32_t const j = i < W ? 0u : (i < 2 * W ? 1u : 2u);
-                uint32_t const base = j == 0 ? wb0 : (j == 1 ? wbm : wbp);
-                atomicAdd(count_next + (base + (i - j * W)), c);
+                uint32_t const j = i < WH ? 0u : (i < 2 * WH ? 1u : 2u);
+                // keys modulo 2^32: a window base may lie below zero
+                uint32_t const key = (j == 0 ? wb0 : (j == 1 ? wbm : wbp)) + (i - j * WH);
+                uint32_t const key_hi = key + WH;
+                if (c & 0xffffu) atomicAdd(count_next + key, c & 0xffffu);
+                if (c >> 16) atomicAdd(count_next + key_hi, c >> 16);
             }
         }
         if constexpr (SAMPLE) {
@@ -3909,18 +3925,24 @@ int mpcd_hip_autotune(mpcd_hip_t* h, int steps_per_trial, uint32_t* cells_per_ti
             for (KParams& k : h->kp) k.gid = 0;
             h->timing = timing;
         };
-        // milliseconds per step of the collide launches of launch set 1
+        // milliseconds per step of the collide launches of launch set 1: the
+        // fastest of steps_per_trial single steps (after one untimed step), so
+        // that a step slowed by something else does not decide the ranking
         auto class_trial = [&]() {
             layout(h);
             h->group_timing = true;
             run_steps(h, 1, false);
-            for (double& v : h->group_ms) v = 0.0;
-            run_steps(h, steps_per_trial, false);
+            double best_ms = 1e300;
+            for (int t = 0; t < steps_per_trial; ++t) {
+                for (double& v : h->group_ms) v = 0.0;
+                run_steps(h, 1, false);
+                double ms = 0;
+                for (uint32_t g = 0; g < h->ngrp; ++g)
+                    if (h->grp[g].gid == 1) ms += h->group_ms[g];
+                best_ms = std::min(best_ms, ms);
+            }
             h->group_timing = false;
-            double ms = 0;
-            for (uint32_t g = 0; g < h->ngrp; ++g)
-                if (h->grp[g].gid == 1) ms += h->group_ms[g];
-            return ms / steps_per_trial;
+            return best_ms;
         };
         double step_ms = 0;
         try {
@@ -3934,8 +3956,7 @@ int mpcd_hip_autotune(mpcd_hip_t* h, int steps_per_trial, uint32_t* cells_per_ti
                 auto apply = [&](Cand const& c) {
                     for (uint32_t s : members) choose_tiling_sys(h, s, c.tc, c.cap);
                 };
-                Cand pick = {h->kp[m0].tile_req, h->kp[m0].lds_cap};
-                double best = 1e300;
+                std::vector<std::pair<double, Cand>> tried;
                 std::vector<std::pair<uint32_t, uint32_t>> seen;
                 for (uint32_t cap : tune_capacities(h->algorithm, n, h->kp[m0].lds_cap))
                     for (uint32_t tc : {0u, 48u, 32u, 16u, 8u}) {
@@ -3944,11 +3965,22 @@ int mpcd_hip_autotune(mpcd_hip_t* h, int steps_per_trial, uint32_t* cells_per_ti
                         std::pair<uint32_t, uint32_t> const key{h->kp[m0].tile_cells, h->kp[m0].lds_cap};
                         if (std::find(seen.begin(), seen.end(), key) != seen.end()) continue;
                         seen.push_back(key);
-                        double const ms = class_trial();
-                        if (ms < best) {
-                            best = ms;
-                            pick = c;
-                        }
+                        tried.push_back({class_trial(), c});
+                    }
+                // the three fastest once more, in reverse order; each keeps its better time
+                std::stable_sort(tried.begin(), tried.end(),
+                                 [](auto const& a, auto const& b) { return a.first < b.first; });
+                size_t const again = std::min<size_t>(3, tried.size());
+                for (size_t i = again; i-- > 0;) {
+                    apply(tried[i].second);
+                    tried[i].first = std::min(tried[i].first, class_trial());
+                }
+                Cand pick = {h->kp[m0].tile_req, h->kp[m0].lds_cap};
+                double best = 1e300;
+                for (auto const& t : tried)
+                    if (t.first < best) {
+                        best = t.first;
+                        pick = t.second;
                     }
                 apply(pick);
             }

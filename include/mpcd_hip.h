@@ -190,7 +190,8 @@ int mpcd_hip_use_graph(mpcd_hip_t* h, int enable);
  * first) tries each compiled LDS capacity that holds a whole collision table
  * with several cells-per-tile limits, timing that class's own collide
  * launches with HIP events over steps_per_trial real steps (after one untimed
- * step), and keeps the fastest for the class.  Returns system 0's shape and the
+ * step; a trial counts its fastest step), times the three fastest shapes once
+ * more, and keeps the fastest for the class.  Returns system 0's shape and the
  * time per step with the chosen shapes.  The systems advance by these steps;
  * results do not depend on the tile shapes. */
 int mpcd_hip_autotune(mpcd_hip_t* h, int steps_per_trial, uint32_t* cells_per_tile, uint32_t* lds_capacity,
