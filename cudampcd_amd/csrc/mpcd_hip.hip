@@ -1667,20 +1667,29 @@ __device__ __forceinline__ void collide_tile(Sys const& sys, const float* __rest
         // rows one step away along the middle axis, smid keys away), the rest
         // straight to memory.  Window bases may wrap below zero: distances are
         // taken modulo 2^32, and only bins of real keys are ever non-zero.
-        // Bins are 16 bits, two to a word (bin b and bin b + WH of a window
-        // share word b, so neighbouring keys fall on different words; a bin
-        // counts at most one batch), which makes each window twice as wide as
-        // its words: centred on the batch it covers moves of up to
-        // (W - span) / 2 keys, whole rows of the second-fastest axis for
-        // fastest axes up to about 110 cells with 384-particle batches
+        // Bins are 16 bits, two to a word (bin g < NW in the low half of word
+        // g, bin NW + g in its high half, so neighbouring keys fall on
+        // different words; a bin counts at most one batch): NB = 2 NW bins in
+        // all, the own window's first, then the two middle-axis windows'.  A
+        // window centred on the batch covers moves of up to (width - span) / 2
+        // keys; a move along the second-fastest axis is D0 keys.  Equal thirds
+        // cover every neighbour while span + 2 (D0 + 1) fits a third (D0 up to
+        // about 110 with 384-particle batches); beyond that the middle-axis
+        // windows shrink to the batch's keys and the own window takes the rest
+        // (D0 up to about 300), so that only moves along both slow axes at
+        // once go to memory.
         uint32_t* const hist = s.perm;
-        constexpr uint32_t WH = static_cast<uint32_t>(CAP) / 3;   // words per window
-        constexpr uint32_t W = 2 * WH;                              // bins per window
+        constexpr uint32_t NW = 3 * (static_cast<uint32_t>(CAP) / 3);   // words
+        constexpr uint32_t NB = 2 * NW;                                  // bins
         uint32_t const span = ke - kb;
-        uint32_t const wb0 = g0 + kb - (span < W ? (W - span) / 2 : 0u);
+        uint32_t W1 = NB / 3;
+        if (span + 2 * (P.D0 + 1) > W1) W1 = min(NB / 3, span + 4);
+        uint32_t const W0 = NB - 2 * W1;
+        uint32_t const wb0 = g0 + kb - (span < W0 ? (W0 - span) / 2 : 0u);
         uint32_t const smid = mid_stride(P, c0 + kb);
-        uint32_t const wbm = wb0 - smid, wbp = wb0 + smid;
-        for (uint32_t i = tid; i < 3 * WH; i += NT) hist[i] = 0u;
+        uint32_t const wb1 = g0 + kb - (span < W1 ? (W1 - span) / 2 : 0u);
+        uint32_t const wbm = wb1 - smid, wbp = wb1 + smid;
+        for (uint32_t i = tid; i < NW; i += NT) hist[i] = 0u;
         // sampling: un-shifted cells lie in the batch's cells or one step back
         // along each axis the shift is positive on, one step ahead where it is
         // negative, so two windows (the batch's keys widened on the shift's side
@@ -1770,30 +1779,29 @@ __device__ __forceinline__ void collide_tile(Sys const& sys, const float* __rest
             }
             if (valid) {
                 keys[static_cast<uint64_t>(gb) + i] = knext;
-                uint32_t w = knext - wb0, j = 0;
-                if (w >= W) {
-                    j = 1;
-                    w = knext - wbm;
-                    if (w >= W) {
-                        j = 2;
+                uint32_t bin = knext - wb0;
+                if (bin >= W0) {
+                    uint32_t w = knext - wbm;
+                    if (w < W1) {
+                        bin = W0 + w;
+                    } else {
                         w = knext - wbp;
+                        bin = w < W1 ? W0 + W1 + w : ~0u;
                     }
                 }
-                if (w < W) atomicAdd(hist + j * WH + (w < WH ? w : w - WH), w < WH ? 1u : 0x10000u);
+                if (bin < NB) atomicAdd(hist + (bin < NW ? bin : bin - NW), bin < NW ? 1u : 0x10000u);
                 else atomicAdd(count_next + knext, 1u);
             }
         }
         __syncthreads();
-        for (uint32_t i = tid; i < 3 * WH; i += NT) {
+        // key of bin g, modulo 2^32 (a window base may lie below zero)
+        auto bin_key = [&](uint32_t g) -> uint32_t {
+            return g < W0 ? wb0 + g : (g < W0 + W1 ? wbm + (g - W0) : wbp + (g - W0 - W1));
+        };
+        for (uint32_t i = tid; i < NW; i += NT) {
             uint32_t const c = hist[i];
-            if (c) {
-                uint32_t const j = i < WH ? 0u : (i < 2 * WH ? 1u : 2u);
-                // keys modulo 2^32: a window base may lie below zero
-                uint32_t const key = (j == 0 ? wb0 : (j == 1 ? wbm : wbp)) + (i - j * WH);
-                uint32_t const key_hi = key + WH;
-                if (c & 0xffffu) atomicAdd(count_next + key, c & 0xffffu);
-                if (c >> 16) atomicAdd(count_next + key_hi, c >> 16);
-            }
+            if (c & 0xffffu) atomicAdd(count_next + bin_key(i), c & 0xffffu);
+            if (c >> 16) atomicAdd(count_next + bin_key(NW + i), c >> 16);
         }
         if constexpr (SAMPLE) {
             for (uint32_t i = tid; i < 2 * WS; i += NT) {
