@@ -343,6 +343,42 @@ __device__ __forceinline__ void wave_count(uint32_t* count, uint32_t key, bool v
     if (valid && g.leader == static_cast<int>(__lane_id())) atomicAdd(count + key, g.size);
 }
 
+// Per-cell stages give each cell a group of G = 2^lg aligned lanes (G a
+// function of the particles per cell n only, so results do not depend on the
+// tiling): member i of the cell is added by lane i mod G, each lane in member
+// order, and group_sum combines the G partial sums pairwise,
+// ((p0 + p1) + (p2 + p3)) + ((p4 + p5) + (p6 + p7)).  Each butterfly stage is
+// one DPP move inside a quad (lanes i ^ 1, then i ^ 2) or a half row (lane
+// 7 - i, which after the quad stages holds the other quad's sum); both lanes
+// of a pair add the same two values, so every lane ends with the same bits.
+// All G lanes of a group must be active.
+__host__ __device__ constexpr uint32_t cell_lanes_log2(uint32_t n) {
+    return n < 6 ? 0u : (n < 12 ? 1u : (n < 24 ? 2u : 3u));
+}
+constexpr int DPP_QUAD_SWAP1 = 0xB1;    // quad_perm [1, 0, 3, 2]
+constexpr int DPP_QUAD_SWAP2 = 0x4E;    // quad_perm [2, 3, 0, 1]
+constexpr int DPP_HALF_MIRROR = 0x141;  // row_half_mirror: lane 7 - i of each 8
+template <int CTRL>
+__device__ __forceinline__ float dpp_f(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+template <int CTRL>
+__device__ __forceinline__ int dpp_i(int v) {
+    return __builtin_amdgcn_update_dpp(0, v, CTRL, 0xF, 0xF, false);
+}
+__device__ __forceinline__ float group_sum(float v, uint32_t lg) {
+    if (lg >= 1) v = v + dpp_f<DPP_QUAD_SWAP1>(v);
+    if (lg >= 2) v = v + dpp_f<DPP_QUAD_SWAP2>(v);
+    if (lg >= 3) v = v + dpp_f<DPP_HALF_MIRROR>(v);
+    return v;
+}
+__device__ __forceinline__ int group_sum(int v, uint32_t lg) {
+    if (lg >= 1) v = v + dpp_i<DPP_QUAD_SWAP1>(v);
+    if (lg >= 2) v = v + dpp_i<DPP_QUAD_SWAP2>(v);
+    if (lg >= 3) v = v + dpp_i<DPP_HALF_MIRROR>(v);
+    return v;
+}
+
 __device__ __forceinline__ uint32_t xcd_tile(uint32_t b, uint32_t G) {
     // workgroups are dispatched round-robin over the 8 XCDs; give each XCD a
     // contiguous run of tiles so neighbouring tiles share its L2
@@ -1013,60 +1049,65 @@ struct SampleBins {
     uint32_t cnt[NB];
 };
 
-// SRD per-cell stage of one batch (simulation_functions.cpp:133-182): one lane
-// per cell sums its particles in slot order and its ghosts in ghost order (the
-// reference's order).  Ghost counts and velocities of wall cells (drawn by
-// k_cell_draws) were staged into LDS while the particles were gathered:
-// s.gcnt[k] ghosts at s.g[0][s.goff[k] ...], or, when the staging region was
-// full (s.goff[k] == ~0u), read from memory at gvel + 3 n s.cref[k].
+// SRD per-cell stage of one batch (simulation_functions.cpp:133-182): the mean
+// velocity of every cell over its table particles (slot order) and ghosts
+// (ghost order), summed by the cell's lane group (cell_lanes_log2: lane j adds
+// members j, j + G, ..., group_sum combines the lanes).  Ghost counts and
+// velocities of wall cells (drawn by k_cell_draws) were staged into LDS while
+// the particles were gathered: s.gcnt[k] ghosts at s.g[0][s.goff[k] ...], or,
+// when the staging region was full (s.goff[k] == ~0u), read from memory at
+// gvel + 3 n s.cref[k].
 template <int CAP, bool ADJ>
 __device__ void srd_cells(KParams const& P, Tile<CAP, 0>& s, uint32_t ncell, uint32_t const* off, uint32_t gb,
                           float const* __restrict__ gvel, uint32_t key0, Tape const& T, uint64_t CT) {
     constexpr uint32_t NT = tile_threads(CAP);
-    // three lanes per cell, one per velocity component (the three sums are
-    // independent chains, each kept in the reference's order), packed into as
-    // few waves as possible
-    for (uint32_t j = threadIdx.x; j < 3 * ncell; j += NT) {
-        uint32_t const k = j / 3, c = j - 3 * k;
+    uint32_t const lg = cell_lanes_log2(P.n), G = 1u << lg;
+    // groups are aligned and NT is a multiple of 64, so a group's lanes are
+    // all active or all idle in every round
+    for (uint32_t t = threadIdx.x; t < (ncell << lg); t += NT) {
+        uint32_t const k = t >> lg, j = t & (G - 1u);
         uint32_t const a = off[k] - gb, mc = off[k + 1] - gb - a;
         uint32_t const reals = mc < P.cap ? mc : P.cap;
         uint32_t const added = s.gcnt[k];
         uint32_t const total = reals + added;
-        float mean = 0.0f;
-        if (total > 1) {
-            float const* const v = s.p[3 + c] + a;
-            float sum = 0.0f;
-            uint32_t i = 0;
-            // loads issued eight particles at a time, the sum kept in slot order
-            for (; i + 8 <= reals; i += 8) {
-                float x[8];
-#pragma unroll
-                for (int u = 0; u < 8; ++u) x[u] = v[i + u];
-#pragma unroll
-                for (int u = 0; u < 8; ++u) sum = sum + x[u];
+        float mx = 0.0f, my = 0.0f, mz = 0.0f;
+        if (total > 1) {   // uniform over the group
+            float const* const v = &s.p[3][a];
+            float sx = 0.0f, sy = 0.0f, sz = 0.0f;
+            uint32_t i = j;
+            // this lane's table particles, loads issued two members at a time
+            for (; i + G < reals; i += 2 * G) {
+                float const x0 = v[i], y0 = v[CAP + i], z0 = v[2 * CAP + i];
+                float const x1 = v[i + G], y1 = v[CAP + i + G], z1 = v[2 * CAP + i + G];
+                sx = sx + x0; sy = sy + y0; sz = sz + z0;
+                sx = sx + x1; sy = sy + y1; sz = sz + z1;
             }
-            for (; i < reals; ++i) sum = sum + v[i];
-            // ghosts after the particles, in ghost order
+            for (; i < reals; i += G) {
+                sx = sx + v[i]; sy = sy + v[CAP + i]; sz = sz + v[2 * CAP + i];
+            }
+            // then its ghosts (member reals + q, ghost order)
             uint32_t const go = s.goff[k];
-            if (go != ~0u) {
-                float const* const g = &s.g[0][go + c];
-                for (uint32_t q = 0; q < added; ++q) sum = sum + g[3 * q];
-            } else {
-                float const* const g = gvel + P.gvb + 3ull * P.n * s.cref[k] + c;
-                for (uint32_t q = 0; q < added; ++q) sum = sum + g[3 * q];
+            float const* const g = go != ~0u ? &s.g[0][go] : gvel + P.gvb + 3ull * P.n * s.cref[k];
+            for (uint32_t q = i - reals; q < added; q += G) {
+                sx = sx + g[3 * q]; sy = sy + g[3 * q + 1]; sz = sz + g[3 * q + 2];
             }
+            sx = group_sum(sx, lg);
+            sy = group_sum(sy, lg);
+            sz = group_sum(sz, lg);
             float const inv = 1.0f / static_cast<float>(total);
-            mean = sum * inv;
-            s.cm[c][k] = mean;
-        }
-        if (c == 0) s.go[k] = total > 1;
-        if constexpr (ADJ) {
-            uint64_t const gk = P.cb + key0 + k;
-            if (T.tot) {
-                if (c == 0) T.tot[gk] = total;
-                T.u[c * CT + gk] = mean;
+            mx = sx * inv; my = sy * inv; mz = sz * inv;
+            if (j == 0) {
+                s.cm[0][k] = mx; s.cm[1][k] = my; s.cm[2][k] = mz;
             }
-            if (c == 0) {
+        }
+        if (j == 0) s.go[k] = total > 1;
+        if constexpr (ADJ) {
+            if (j == 0) {
+                uint64_t const gk = P.cb + key0 + k;
+                if (T.tot) {
+                    T.tot[gk] = total;
+                    T.u[gk] = mx; T.u[CT + gk] = my; T.u[2 * CT + gk] = mz;
+                }
                 float f = 0.0f;
                 if (T.force) {
                     uint32_t ix, iy, iz;
@@ -1151,11 +1192,16 @@ __device__ __forceinline__ void thermal_quad(KParams const& P, uint32_t ref, uin
     box_muller(r.w[2], r.w[3], &gq[2], &gq[3]);
 }
 
-// extended collision of one cell by one lane; ghosts and the table particles'
-// thermal Gaussians (in `gauss`, by LDS slot) were drawn by the stages above
+// extended collision of one cell by its lane group (lane j of G = 2^lg: the
+// cell's members j, j + G, ..., table particles first, then ghosts; sums over
+// the members combined by group_sum, so every lane of the group holds the
+// cell's sums and takes the same decisions); ghosts and the table particles'
+// thermal Gaussians (in `gauss`, by LDS slot) were drawn by the stages above.
+// Lane 0 of the group leaves the cell's results for ext_finish.
 template <int CAP>
-__device__ void extended_cell(KParams const& P, Tile<CAP, 1>& s, uint32_t k, uint32_t a, uint32_t key,
-                              float const* gauss) {
+__device__ void extended_cell(KParams const& P, Tile<CAP, 1>& s, uint32_t k, uint32_t j, uint32_t lg, uint32_t a,
+                              uint32_t key, float const* gauss) {
+    uint32_t const G = 1u << lg;
     uint32_t ix, iy, iz;
     decode_key(P, key, ix, iy, iz);
     uint32_t const ref = s.cref[k];
@@ -1166,7 +1212,7 @@ __device__ void extended_cell(KParams const& P, Tile<CAP, 1>& s, uint32_t k, uin
     float const oy = (0.5f + static_cast<float>(iy)) - P.half[1];
     float const oz = (0.5f + static_cast<float>(iz)) - P.half[2];
     uint32_t const gb = s.wcell[k];
-    if (total <= 1) return;
+    if (total <= 1) return;   // uniform over the group
 
     Words const e = P.rs.at(P_EXT, 0, ref, P.step);
     int const select = static_cast<int>(e.w[0] / 2) % 13;
@@ -1174,43 +1220,38 @@ __device__ void extended_cell(KParams const& P, Tile<CAP, 1>& s, uint32_t k, uin
     float ax = c_axes[select][0], ay = c_axes[select][1], az = c_axes[select][2];
     if (flip) { ax = -ax; ay = -ay; az = -az; }
 
-    // members in order: the table particles (LDS slots a ...), then the
-    // ghosts (ghost slots gb ...); the loops read MB members at a time and
-    // keep every sum in member order
-    // (coordinate c of a member at base[c * S + i]: S = CAP for particles,
-    // GS for ghosts, compile-time so the six rows are immediate LDS offsets)
-    // (reads of MB members issued together: 2 where the pass keeps little
-    // state, 1 in the collision pass, whose sums leave no registers for more)
-    auto members = [&](auto mb, auto&& f) {
-        constexpr uint32_t MB = decltype(mb)::value;
-        auto part = [&](float* const base, uint32_t const cnt, uint32_t const first, auto stride) {
-            constexpr uint32_t S = decltype(stride)::value;
-            uint32_t i = 0;
-            for (; i + MB <= cnt; i += MB) {
-                float x[MB][6];
-#pragma unroll
-                for (int u = 0; u < MB; ++u)
-#pragma unroll
-                    for (int c = 0; c < 6; ++c) x[u][c] = base[c * S + i + u];
-#pragma unroll
-                for (int u = 0; u < MB; ++u) f(base, S, i + u, first + i + u, x[u]);
-            }
-            for (; i < cnt; ++i) {
+    // this lane's members in order: table particles (LDS slots a + i, i < rr),
+    // then ghosts (ghost slots gb + i - rr); f(base, stride, local index,
+    // member index, the member's six coordinates).  Coordinate c of a member
+    // sits at base[c * S + local]: S = CAP for particles, GS for ghosts,
+    // compile-time so the six rows are immediate LDS offsets.
+    auto members = [&](auto&& f) {
+        uint32_t i = j;
+        {
+            float* const base = &s.p[0][0] + a;
+            constexpr uint32_t S = static_cast<uint32_t>(CAP);
+            for (; i < rr; i += G) {
                 float x[6];
 #pragma unroll
                 for (int c = 0; c < 6; ++c) x[c] = base[c * S + i];
-                f(base, S, i, first + i, x);
+                f(base, S, i, i, x);
             }
-        };
-        // (first: member index of the part's first member)
-        part(&s.p[0][0] + a, rr, 0u, std::integral_constant<uint32_t, static_cast<uint32_t>(CAP)>{});
-        part(&s.g[0][0] + gb, added, rr, std::integral_constant<uint32_t, static_cast<uint32_t>(Tile<CAP, 1>::GS)>{});
+        }
+        {
+            float* const base = &s.g[0][0] + gb;
+            constexpr uint32_t S = static_cast<uint32_t>(Tile<CAP, 1>::GS);
+            for (; i < total; i += G) {
+                uint32_t const li = i - rr;
+                float x[6];
+#pragma unroll
+                for (int c = 0; c < 6; ++c) x[c] = base[c * S + li];
+                f(base, S, li, i, x);
+            }
+        }
     };
 
     float cx = 0, cy = 0, cz = 0, mx = 0, my = 0, mz = 0;
-    using two = std::integral_constant<uint32_t, 2>;
-    using one = std::integral_constant<uint32_t, 1>;
-    members(two{}, [&](float*, uint32_t, uint32_t, uint32_t, float const* x) {
+    members([&](float*, uint32_t, uint32_t, uint32_t, float const* x) {
         cx = cx + (x[0] - ox);
         cy = cy + (x[1] - oy);
         cz = cz + (x[2] - oz);
@@ -1218,6 +1259,8 @@ __device__ void extended_cell(KParams const& P, Tile<CAP, 1>& s, uint32_t k, uin
         my = my + x[4];
         mz = mz + x[5];
     });
+    cx = group_sum(cx, lg); cy = group_sum(cy, lg); cz = group_sum(cz, lg);
+    mx = group_sum(mx, lg); my = group_sum(my, lg); mz = group_sum(mz, lg);
     float const inv = 1.0f / static_cast<float>(total);
     cx = cx * inv + ox;
     cy = cy * inv + oy;
@@ -1228,7 +1271,7 @@ __device__ void extended_cell(KParams const& P, Tile<CAP, 1>& s, uint32_t k, uin
 
     float proj0 = 0.0f;
     int size0 = 0;
-    members(two{}, [&](float* base, uint32_t S, uint32_t i, uint32_t, float const* x) {
+    members([&](float* base, uint32_t S, uint32_t i, uint32_t, float const* x) {
         float const rx = x[0] - cx, ry = x[1] - cy, rz = x[2] - cz;
         float const ux = x[3] - mx, uy = x[4] - my, uz = x[5] - mz;
         base[i] = rx; base[S + i] = ry; base[2 * S + i] = rz;
@@ -1238,12 +1281,14 @@ __device__ void extended_cell(KParams const& P, Tile<CAP, 1>& s, uint32_t k, uin
             size0 += 1;
         }
     });
+    proj0 = group_sum(proj0, lg);
+    size0 = group_sum(size0, lg);
     int const size1 = static_cast<int>(total) - size0;
     float const proj1 = -proj0 / static_cast<float>(size1);
     proj0 = proj0 / static_cast<float>(size0);
     float const arg = 0.01f * (proj1 - proj0) * static_cast<float>(size0) * static_cast<float>(size1);
     float const probability = 1.0f - exp_f(arg);
-    bool const collide = uniform(e.w[2]) < probability;
+    bool const collide = uniform(e.w[2]) < probability;   // uniform over the group
 
     float Lx = 0, Ly = 0, Lz = 0;
     if (collide) {
@@ -1251,7 +1296,7 @@ __device__ void extended_cell(KParams const& P, Tile<CAP, 1>& s, uint32_t k, uin
         float const t1 = proj0 * (static_cast<float>(size0) / static_cast<float>(size1));
         float mean0 = 0.0f, mean1 = 0.0f;
         Sym I = {0, 0, 0, 0, 0, 0};
-        members(one{}, [&](float* base, uint32_t S, uint32_t li, uint32_t i, float const* x) {
+        members([&](float* base, uint32_t S, uint32_t li, uint32_t i, float const* x) {
             // member i's Gaussian of quad i / 4 (drawn for the batch beforehand)
             float const gi = i < rr ? gauss[a + li] : s.gg[gb + li];
             float const rx = x[0], ry = x[1], rz = x[2];
@@ -1275,10 +1320,12 @@ __device__ void extended_cell(KParams const& P, Tile<CAP, 1>& s, uint32_t k, uin
             I.xz += -rx * rz;
             I.yz += -ry * rz;
         });
+        mean0 = group_sum(mean0, lg);
+        mean1 = group_sum(mean1, lg);
         float const a0 = mean0 / static_cast<float>(size0), a1 = mean1 / static_cast<float>(size1);
         float const v0x = ax * a0, v0y = ay * a0, v0z = az * a0;
         float const v1x = ax * a1, v1y = ay * a1, v1z = az * a1;
-        members(two{}, [&](float* base, uint32_t S, uint32_t li, uint32_t, float const* x) {
+        members([&](float* base, uint32_t S, uint32_t li, uint32_t, float const* x) {
             float const rx = x[0], ry = x[1], rz = x[2];
             bool const side = rx * ax + ry * ay + rz * az < 0;
             float const ux = x[3] - (side ? v0x : v1x);
@@ -1289,6 +1336,9 @@ __device__ void extended_cell(KParams const& P, Tile<CAP, 1>& s, uint32_t k, uin
             Ly = Ly - (rz * ux - rx * uz);
             Lz = Lz - (rx * uy - ry * ux);
         });
+        Lx = group_sum(Lx, lg); Ly = group_sum(Ly, lg); Lz = group_sum(Lz, lg);
+        I.xx = group_sum(I.xx, lg); I.yy = group_sum(I.yy, lg); I.zz = group_sum(I.zz, lg);
+        I.xy = group_sum(I.xy, lg); I.xz = group_sum(I.xz, lg); I.yz = group_sum(I.yz, lg);
         Sym const J = sym_inverse(I, 1e-5f);
         float const nx = (J.xx * Lx) + (J.xy * Ly) + (J.xz * Lz);
         float const ny = (J.xy * Lx) + (J.yy * Ly) + (J.yz * Lz);
@@ -1298,11 +1348,13 @@ __device__ void extended_cell(KParams const& P, Tile<CAP, 1>& s, uint32_t k, uin
     // the way back to the lab frame is per particle: ext_finish runs it on all
     // lanes from these (this cell's ghost count, reference index and ghost
     // slot are not read again, so they carry the correction)
-    s.cm[0][k] = cx; s.cm[1][k] = cy; s.cm[2][k] = cz;
-    s.cm[3][k] = mx; s.cm[4][k] = my; s.cm[5][k] = mz;
-    s.gcnt[k] = __float_as_uint(Lx);
-    s.cref[k] = __float_as_uint(Ly);
-    s.wcell[k] = __float_as_uint(Lz);
+    if (j == 0) {
+        s.cm[0][k] = cx; s.cm[1][k] = cy; s.cm[2][k] = cz;
+        s.cm[3][k] = mx; s.cm[4][k] = my; s.cm[5][k] = mz;
+        s.gcnt[k] = __float_as_uint(Lx);
+        s.cref[k] = __float_as_uint(Ly);
+        s.wcell[k] = __float_as_uint(Lz);
+    }
 }
 
 // table particle at LDS slot i of cell k after extended_cell: velocity back
@@ -1652,8 +1704,16 @@ __device__ __forceinline__ void collide_tile(Sys const& sys, const float* __rest
                 }
             }
             __syncthreads();
-            for (uint32_t k = tid; k < ncell; k += NT)
-                extended_cell<CAP>(P, s, k, s.off[kb + k] - gb, c0 + kb + k, gauss);
+            {
+                // lane groups of cell_lanes_log2(n) lanes per cell; groups are
+                // aligned and NT is a multiple of 64, so a group's lanes are
+                // all active or all idle in every round
+                uint32_t const lg = cell_lanes_log2(P.n);
+                for (uint32_t t = tid; t < (ncell << lg); t += NT) {
+                    uint32_t const k = t >> lg;
+                    extended_cell<CAP>(P, s, k, t & ((1u << lg) - 1u), lg, s.off[kb + k] - gb, c0 + kb + k, gauss);
+                }
+            }
             __syncthreads();
             // table particles of cells with more than one member, one lane each
             for (uint32_t i = tid; i < m; i += NT) {

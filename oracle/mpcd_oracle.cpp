@@ -32,6 +32,18 @@
 // Every floating-point expression keeps the operation order of the reference
 // (vector_3d.hpp operator definitions), and the file is compiled with
 // -ffp-contract=off so no fused multiply-add changes a rounding.
+//
+// Per-cell sums come in two orders (orc_params::sum_order):
+//   * 0 "serial": the reference's, members added one after another in member
+//     order (table particles, then ghosts);
+//   * 1 "lane groups": the GPU's, whose per-cell stages give each cell a group
+//     of G = group_lanes(n) lanes: member i is added by lane i mod G, each lane
+//     in member order, and the G partial sums are combined pairwise,
+//     ((p0 + p1) + (p2 + p3)) + ((p4 + p5) + (p6 + p7)), as the GPU's butterfly
+//     of cross-lane moves does.  Every other operation is the serial order's.
+//     The two orders differ by float rounding only; tests/test_oracle_pinning.py
+//     checks order 1 against order 0 (per-particle velocities within 1e-5,
+//     cell momentum, SRD energy).
 
 #include <cfloat>
 #include <cmath>
@@ -355,6 +367,7 @@ struct orc_params {
     unsigned system;        // mode 1: ensemble member id
     int ghost_xy_order;     // mode 0 only
     int experiment;         // 0 standard, 1 channel (cylinder about the x axis)
+    int sum_order;          // per-cell sums: 0 serial (reference order), 1 lane groups (GPU order)
 };
 }
 
@@ -364,6 +377,20 @@ struct Particle {
     V3 pos, vel;
     uint32_t cell;
 };
+
+// Lanes per cell of the GPU's per-cell stages: a function of the particles per
+// cell only, so that results do not depend on how cells are tiled.
+static inline uint32_t group_lanes(uint32_t n) { return n < 6 ? 1u : (n < 12 ? 2u : (n < 24 ? 4u : 8u)); }
+
+// Partial sums of the G lanes of a cell group combined pairwise (G a power of
+// two; G = 1 returns the one serial sum).
+template <class T, class Add>
+static T pairwise(std::vector<T> v, Add add) {
+    for (size_t w = 1; w < v.size(); w *= 2)
+        for (size_t j = 0; j + w < v.size(); j += 2 * w) v[j] = add(v[j], v[j + w]);
+    return v[0];
+}
+static inline float addf(float a, float b) { return a + b; }
 
 struct Sample {
     float density;
@@ -525,6 +552,7 @@ struct Oracle {
         uint32_t const step = static_cast<uint32_t>(step_index);
         std::vector<V3> bp(cap + p.n + 1), bv(cap + p.n + 1);
         std::vector<int64_t> bidx(cap + p.n + 1);
+        uint32_t const G = p.sum_order ? group_lanes(p.n) : 1u;
         for (uint32_t c = 0; c < C; ++c) {
             uint32_t m = std::min(cap, counter[c]);
             bool layer, ghosts;
@@ -549,8 +577,9 @@ struct Oracle {
                 }
             }
             if (total > 1) {
-                V3 sum = {0, 0, 0};
-                for (uint32_t i = 0; i < total; ++i) sum = add(sum, bv[i]);
+                std::vector<V3> part(G, V3{0, 0, 0});
+                for (uint32_t i = 0; i < total; ++i) part[i % G] = add(part[i % G], bv[i]);
+                V3 const sum = pairwise(part, add);
                 float inv = 1.0f / static_cast<float>(total);
                 V3 mean = mul(sum, inv);
                 V3 axis = rng.srd_axis(step, c);
@@ -631,6 +660,7 @@ struct Oracle {
         std::vector<V3> bp(cap + p.n + 1), bv(cap + p.n + 1);
         std::vector<int64_t> bidx(cap + p.n + 1);
         std::vector<uint8_t> side(cap + p.n + 1);
+        uint32_t const G = p.sum_order ? group_lanes(p.n) : 1u;
         for (uint32_t c = 0; c < C; ++c) {
             uint32_t m = std::min(cap, counter[c]);
             bool layer, ghosts;
@@ -661,25 +691,29 @@ struct Oracle {
                 V3 axis = ext_direction(select);
                 if (flip) axis = neg(axis);
 
-                V3 com = {0, 0, 0}, mean = {0, 0, 0};
+                // per-lane partial sums (one lane: the serial order)
+                std::vector<V3> pcom(G, V3{0, 0, 0}), pmean(G, V3{0, 0, 0});
                 for (uint32_t i = 0; i < total; ++i) {
-                    com = add(com, sub(bp[i], offset));
-                    mean = add(mean, bv[i]);
+                    pcom[i % G] = add(pcom[i % G], sub(bp[i], offset));
+                    pmean[i % G] = add(pmean[i % G], bv[i]);
                 }
+                V3 com = pairwise(pcom, add), mean = pairwise(pmean, add);
                 com = add(mul(com, 1.0f / total), offset);
                 mean = mul(mean, 1.0f / total);
 
                 float proj0 = 0.0f, proj1 = 0.0f;
                 int size0 = 0;
+                std::vector<float> pproj(G, 0.0f);
                 for (uint32_t i = 0; i < total; ++i) {
                     bp[i] = sub(bp[i], com);
                     bv[i] = sub(bv[i], mean);
                     side[i] = dot(bp[i], axis) < 0;
                     if (side[i]) {
-                        proj0 += dot(bv[i], axis);
+                        pproj[i % G] += dot(bv[i], axis);
                         size0 += 1;
                     }
                 }
+                proj0 = pairwise(pproj, addf);
                 int size1 = static_cast<int>(total) - size0;
                 proj1 = -proj0 / size1;
                 proj0 = proj0 / size0;
@@ -690,15 +724,19 @@ struct Oracle {
                 bool collide = rng.ext_gate(step, c) < probability;
                 if (collide) {
                     float t0 = proj1 * (float(size1) / size0), t1 = proj0 * (float(size0) / size1);
-                    float mean0 = 0.0f, mean1 = 0.0f;
-                    Sym I;
+                    // per lane: the two side sums, the angular momentum (added
+                    // to here, subtracted from in the next pass) and the inertia
+                    std::vector<float> pm0(G, 0.0f), pm1(G, 0.0f);
+                    std::vector<V3> pL(G, V3{0, 0, 0});
+                    std::vector<Sym> pI(G);
                     for (uint32_t i = 0; i < total; ++i) {
                         bool s = side[i];
                         float vr = rng.ext_gauss(step, c, i) * tv;
-                        (s ? mean0 : mean1) += vr;
-                        dL = add(dL, cross(bp[i], bv[i]));
+                        (s ? pm0 : pm1)[i % G] += vr;
+                        pL[i % G] = add(pL[i % G], cross(bp[i], bv[i]));
                         bv[i] = add(bv[i], mul(axis, vr - dot(bv[i], axis) + (s ? t0 : t1)));
                         V3 q = {bp[i].x * bp[i].x, bp[i].y * bp[i].y, bp[i].z * bp[i].z};
+                        Sym& I = pI[i % G];
                         I.xx += q.y + q.z;
                         I.yy += q.x + q.z;
                         I.zz += q.x + q.y;
@@ -706,11 +744,19 @@ struct Oracle {
                         I.xz += -bp[i].x * bp[i].z;
                         I.yz += -bp[i].y * bp[i].z;
                     }
+                    float const mean0 = pairwise(pm0, addf), mean1 = pairwise(pm1, addf);
                     V3 vm0 = mul(axis, mean0 / size0), vm1 = mul(axis, mean1 / size1);
                     for (uint32_t i = 0; i < total; ++i) {
                         bv[i] = sub(bv[i], side[i] ? vm0 : vm1);
-                        dL = sub(dL, cross(bp[i], bv[i]));
+                        pL[i % G] = sub(pL[i % G], cross(bp[i], bv[i]));
                     }
+                    dL = pairwise(pL, add);
+                    Sym I = pairwise(pI, [](Sym a, Sym b) {
+                        Sym o;
+                        o.xx = a.xx + b.xx; o.yy = a.yy + b.yy; o.zz = a.zz + b.zz;
+                        o.xy = a.xy + b.xy; o.xz = a.xz + b.xz; o.yz = a.yz + b.yz;
+                        return o;
+                    });
                     I = inverse(I, 1e-5f);
                     dL = apply(I, dL);
                 }

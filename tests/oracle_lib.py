@@ -36,6 +36,7 @@ class OrcParams(ctypes.Structure):
         ("system", ctypes.c_uint),
         ("ghost_xy_order", ctypes.c_int),
         ("experiment", ctypes.c_int),
+        ("sum_order", ctypes.c_int),
     ]
 
 
@@ -153,11 +154,16 @@ def fluid_count(volume, periodic, n):
 
 
 class Oracle:
-    """One oracle simulation.  `volume` is the stored grid (wall layers included)."""
+    """One oracle simulation.  `volume` is the stored grid (wall layers included).
+
+    `order`: per-cell summation order, "serial" (the reference's) or "lanes"
+    (the GPU's lane groups); default "lanes" with the counter stream (the GPU's
+    draws) and "serial" with the reference stream."""
 
     def __init__(self, volume, periodic=(1, 1, 0), n=10, N=None, delta_t=0.02, drag=0.0,
                  thermal_velocity=1.0, rotation_deg=120.0, algorithm="srd", rng="counter",
-                 seed=(12345, 67890), system=0, ghost_xy_order=0, distribute=True, experiment="standard"):
+                 seed=(12345, 67890), system=0, ghost_xy_order=0, distribute=True, experiment="standard",
+                 order=None):
         self.lib = oracle_lib()
         p = OrcParams()
         for i in range(3):
@@ -175,6 +181,10 @@ class Oracle:
         p.system = system
         p.ghost_xy_order = ghost_xy_order
         p.experiment = 1 if experiment == "channel" else 0
+        if order is None:
+            order = "lanes" if rng == "counter" else "serial"
+        assert order in ("serial", "lanes"), order
+        p.sum_order = 1 if order == "lanes" else 0
         self.params = p
         self.h = self.lib.orc_create(ctypes.byref(p))
         self.N = int(p.N)

@@ -12,6 +12,10 @@
    translate known answers (tests/unit/cpp/test_simulation_functions.cpp:4-43),
    the SRD rotation matrix about the drawn axis, momentum and energy
    conservation, the 3x3 inverse against numpy, the 13 extended directions.
+4. The GPU's per-cell summation order (lane groups combined pairwise, the
+   order the GPU tests compare with bit for bit) against the reference's
+   serial order, pinned in 1.: the same step within 1e-5, momentum and SRD
+   energy conserved (3. runs in that order).
 """
 from __future__ import annotations
 
@@ -545,3 +549,45 @@ def test_reference_fp32_kahan_accumulation_error_is_stated():
     # the protocols' own viscosity: the developing flow biases the estimate up
     assert d["reference_c1"]["eta"] > 2.5 * d["theory_eta"]
     assert 1.05 * d["theory_eta"] < d["reference_readme"]["eta"] < 1.2 * d["theory_eta"]
+
+
+# ------------------------------------------------------ 4. lane-group order
+@pytest.mark.parametrize("algorithm", ["srd", "extended"])
+@pytest.mark.parametrize("volume,periodic,n", [
+    ((8, 7, 12), (1, 1, 0), 10),   # two lanes per cell
+    ((6, 6, 8), (1, 1, 0), 20),    # four
+    ((5, 5, 8), (1, 1, 0), 40),    # eight; extended cells of more than 64 members
+    ((10, 9, 14), (1, 1, 0), 16),  # four, uneven member counts per lane
+    ((6, 5, 10), (1, 1, 1), 3),    # one lane: the serial order itself
+])
+def test_lane_group_order_matches_serial_order(algorithm, volume, periodic, n):
+    """Per-cell sums in the GPU's lane-group order against the reference's
+    serial order, one step at a time from a common state (six steps, wall
+    ghosts included): per-particle velocities within 1e-5 of the velocity
+    scale and 1e-5 norm-wise, positions within FP32 rounding (SRD: bitwise,
+    they do not depend on the sums), bitwise equal with one lane per cell, and
+    not bitwise equal otherwise (the two orders do round differently)."""
+    kw = dict(n=n, drag=0.01, delta_t=0.02, algorithm=algorithm, rng="counter", seed=(77, 0))
+    serial = Oracle(volume, periodic, order="serial", **kw)
+    lanes = Oracle(volume, periodic, order="lanes", **kw)
+    differ = False
+    for st in range(6):
+        p, v = lanes.particles()
+        serial.set_particles(p, v)
+        serial.set_step(st)
+        lanes.set_step(st)
+        serial.step(1)
+        lanes.step(1)
+        ps, vs = serial.particles()
+        pl, vl = lanes.particles()
+        scale = np.abs(vs).max()
+        assert np.abs(vl - vs).max() <= 1e-5 * scale
+        assert np.linalg.norm(vl - vs) <= 1e-5 * np.linalg.norm(vs)
+        if algorithm == "srd":
+            assert np.array_equal(pl, ps)
+        else:
+            assert np.abs(pl - ps).max() <= 4 * np.finfo(np.float32).eps * max(volume)
+        if n < 6:
+            assert np.array_equal(vl, vs) and np.array_equal(pl, ps)
+        differ |= not np.array_equal(vl, vs)
+    assert differ == (n >= 6)
