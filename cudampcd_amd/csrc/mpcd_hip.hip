@@ -905,9 +905,10 @@ constexpr int TCM = 64;   // largest cells per tile of any tile
 // of a CU (allocated in 512-byte granules): SRD 384-particle tiles 24 cells
 // (twelve two-wave workgroups, six waves per SIMD), SRD 448 tiles 32 cells
 // (ten workgroups).  Extended 512-particle tiles take 48 cells (eight
-// workgroups with the 112-slot ghost region): its per-cell stage runs one
-// lane per cell, so more cells per batch keep more lanes busy (500x500x40
-// cells, n = 10, six workgroups: 4.63 ms per step against 4.90 with 32 cells)
+// workgroups with the 112-slot ghost region): fewer, fuller tiles win, each
+// tile's fixed costs (offsets, permutation and record round trips, barriers)
+// being the larger part of its life (500x500x40 cells, n = 10, lane groups of
+// two: k_collide 2.82 ms at 48 cells, 2.97 at 40, 3.31 at 32, 3.79 at 24)
 constexpr int tile_cell_max(int cap, int algo) {
     return algo == 0 ? (cap <= 384 ? 24 : (cap <= 448 ? 32 : TCM)) : (cap <= 512 ? 48 : TCM);
 }
@@ -1121,8 +1122,7 @@ __device__ void srd_cells(KParams const& P, Tile<CAP, 0>& s, uint32_t ncell, uin
 }
 
 // Extended rule (simulation_functions.cpp:207-406) in stages so that the
-// draws run on densely packed lanes and only the order-sensitive sums stay
-// serial:
+// draws run on densely packed lanes and the per-cell sums run on lane groups:
 //   ext_cell_setup   one lane per cell: wall flags, ghost count (:137-146),
 //                    table size; stores ref, ghost count (bit 31: upper wall
 //                    layer) and the number of table particles kept
@@ -1133,7 +1133,7 @@ __device__ void srd_cells(KParams const& P, Tile<CAP, 0>& s, uint32_t ncell, uin
 //                    covers its members 4q .. 4q + 3, table particles then
 //                    ghosts, as in the serial rule), for every member of each
 //                    cell that collides
-//   extended_cell    one lane per cell: the collision itself
+//   extended_cell    one lane group per cell (cell_lanes_log2): the collision itself
 constexpr uint32_t UPPER_LAYER = 0x80000000u;
 
 // whether cell layer iz receives ghosts this step: the z layers next to the
