@@ -1431,18 +1431,38 @@ __device__ __forceinline__ void collide_tile(Sys const& sys, const float* __rest
             }
         }
     }
-    __syncthreads();
+    // (s.off and s.wpre are first read after the barrier that follows the
+    // batch's permutation load)
 
     uint32_t kb = 0;
     while (kb < nc) {
         // batch = longest run of whole cells from kb that fits the LDS tile
-        // (offsets are non-decreasing, so the fitting cells form a prefix)
-        bool fits = (kb + 1 + tid <= nc) && (s.off[kb + 1 + tid] - s.off[kb] <= cap);
-        if constexpr (ALGO == 1) fits = fits && (s.wpre[kb + 1 + tid] - s.wpre[kb]) * P.n <= static_cast<uint32_t>(s.GS);
-        uint32_t const nfit = static_cast<uint32_t>(__syncthreads_count(fits));
+        // (offsets are non-decreasing, so the fitting cells form a prefix).
+        // Every wave finds it on its own from the offsets in memory (the same
+        // few words for all; at most 64 cells per tile, one per lane), so no
+        // barrier is needed: lane l tests cells [kb, kb + l]
+        uint32_t const lane = tid & 63u;
+        uint32_t const ob = offset[g0 + kb];
+        uint32_t const k1 = kb + 1u + lane;
+        uint32_t const ok = k1 <= nc ? offset[g0 + k1] : 0u;
+        bool fits = k1 <= nc && ok - ob <= cap;
+        if constexpr (ALGO == 1) {
+            // and the ghosts of its wall cells fit the ghost region
+            bool wall = false;
+            if (kb + lane < nc) {
+                uint32_t ix, iy, iz;
+                bool layer;
+                decode_key(P, c0 + kb + lane, ix, iy, iz);
+                wall = ghost_layer(P, iz, layer);
+            }
+            uint64_t const wm = __ballot(wall);
+            uint64_t const upto = lane == 63u ? ~0ull : ((2ull << lane) - 1ull);
+            fits = fits && static_cast<uint32_t>(__popcll(wm & upto)) * P.n <= static_cast<uint32_t>(s.GS);
+        }
+        uint32_t const nfit = static_cast<uint32_t>(__popcll(__ballot(fits)));
         uint32_t const ke = kb + (nfit > 0 ? nfit : 1u);
-        uint32_t const gb = s.off[kb];
-        uint32_t m = s.off[ke] - gb;
+        uint32_t const gb = ob;
+        uint32_t m = __builtin_amdgcn_readlane(ok, ke - kb - 1u) - gb;
         // particles [i_begin, i_end) of the batch's range, source slots at
         // srcs[gb + i], moved through without collision: streamed, un-shifted
         // (simulation_functions.cpp:90-93), written, sampled, streamed with the
@@ -1555,6 +1575,8 @@ __device__ __forceinline__ void collide_tile(Sys const& sys, const float* __rest
                 s.gcnt[k] = gc;
                 s.goff[k] = go;
             }
+        } else {
+            __syncthreads();
         }
         // tile-local cell of every slot, four lanes per cell
         for (uint32_t t = tid; t < 4 * ncell; t += NT) {
@@ -1645,8 +1667,24 @@ __device__ __forceinline__ void collide_tile(Sys const& sys, const float* __rest
             (void)gstage;
         }
         __syncthreads();
+        // next-step histogram bins (in the permutation scratch, free from here
+        // on for SRD; the extended rule keeps its draws there until after its
+        // per-cell stage) and sampling bins cleared
+        constexpr uint32_t NW = 3 * (static_cast<uint32_t>(CAP) / 3);   // histogram words
+        auto clear_bins = [&]() {
+            for (uint32_t i = tid; i < NW; i += NT) s.perm[i] = 0u;
+            if constexpr (SAMPLE) {
+                for (uint32_t i = tid; i < 2 * WS; i += NT) {
+                    sb.cnt[i] = 0u;
+                    sb.sum[0][i] = 0ull;
+                    sb.sum[1][i] = 0ull;
+                    sb.sum[2][i] = 0ull;
+                }
+            }
+        };
         // per-cell stage
         if constexpr (ALGO == 0) {
+            clear_bins();
             srd_cells<CAP, ADJ>(P, s, ncell, s.off + kb, gb, gvel, c0 + kb, T, CT);
         } else {
             for (uint32_t k = tid; k < ncell; k += NT) {
@@ -1739,7 +1777,6 @@ __device__ __forceinline__ void collide_tile(Sys const& sys, const float* __rest
         // (D0 up to about 300), so that only moves along both slow axes at
         // once go to memory.
         uint32_t* const hist = s.perm;
-        constexpr uint32_t NW = 3 * (static_cast<uint32_t>(CAP) / 3);   // words
         constexpr uint32_t NB = 2 * NW;                                  // bins
         uint32_t const span = ke - kb;
         uint32_t W1 = NB / 3;
@@ -1749,7 +1786,6 @@ __device__ __forceinline__ void collide_tile(Sys const& sys, const float* __rest
         uint32_t const smid = mid_stride(P, c0 + kb);
         uint32_t const wb1 = g0 + kb - (span < W1 ? (W1 - span) / 2 : 0u);
         uint32_t const wbm = wb1 - smid, wbp = wb1 + smid;
-        for (uint32_t i = tid; i < NW; i += NT) hist[i] = 0u;
         // sampling: un-shifted cells lie in the batch's cells or one step back
         // along each axis the shift is positive on, one step ahead where it is
         // negative, so two windows (the batch's keys widened on the shift's side
@@ -1763,15 +1799,10 @@ __device__ __forceinline__ void collide_tile(Sys const& sys, const float* __rest
         uint32_t const need = span + 1u + P.D0;
         uint32_t const ws0 = g0 + kb - lo_off - (need < WS ? (WS - need) / 2 : 0u);
         uint32_t const wss = shm > 0.0f ? ws0 - smid : ws0 + smid;
-        if constexpr (SAMPLE) {
-            for (uint32_t i = tid; i < 2 * WS; i += NT) {
-                sb.cnt[i] = 0u;
-                sb.sum[0][i] = 0ull;
-                sb.sum[1][i] = 0ull;
-                sb.sum[2][i] = 0ull;
-            }
+        if constexpr (ALGO == 1) {
+            clear_bins();
+            __syncthreads();
         }
-        __syncthreads();
         // per-particle finish, write-out, next-step cell count, sampling
         for (uint32_t i0 = 0; i0 < m; i0 += NT) {
             uint32_t const i = i0 + tid;
@@ -1876,7 +1907,9 @@ __device__ __forceinline__ void collide_tile(Sys const& sys, const float* __rest
                 }
             }
         }
-        __syncthreads();
+        // LDS is reused by a next batch or the moved-through tail (after the
+        // tile, the big-cell launch's loop has its own barrier)
+        if (ke < nc || (BIG && tail)) __syncthreads();
         if (BIG && tail) {
             pass_through(psrc, m, m + tail);
             __syncthreads();
