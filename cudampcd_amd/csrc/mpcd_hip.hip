@@ -1936,7 +1936,8 @@ constexpr int LAYER_REP_WORDS = 2048;         // LDS histogram entries (layers t
 // layer in fixed point, so the sums do not depend on the order of the
 // atomics: each velocity is rounded to a multiple of 2^-40, q = rint(v 2^40).
 // A block sums at most LAYER_BLOCK_MAX particles per layer in 64-bit LDS words
-// (exact while |v| < 2^23 / LAYER_BLOCK_MAX = 128); lanes spread over R
+// (exact while |v| < 2^23 / LAYER_BLOCK_MAX = 128: a velocity outside, or
+// not a number, sets bit 2 of the status word and the run fails); lanes spread over R
 // replicas of the histogram (lane mod R), since neighbouring particles share
 // a cell and would otherwise queue on one LDS word.  The block adds its sum Q
 // to copy blockIdx mod LAYER_SLOTS of the step's totals, split as Q >> 20
@@ -1946,9 +1947,10 @@ constexpr int LAYER_REP_WORDS = 2048;         // LDS histogram entries (layers t
 // With `big`, block 0 also moves the system to its next step (the work of
 // k_advance): the pass reads only the system's z extent and half-width, which
 // that does not change.
+constexpr uint32_t STATUS_LAYER_VELOCITY = 4u;
 __global__ __launch_bounds__(WG) void k_layers(Sys sys, const float* __restrict__ st, uint32_t* __restrict__ cnt,
                                                unsigned long long* __restrict__ sum, uint16_t* __restrict__ lz,
-                                               uint32_t* __restrict__ big) {
+                                               uint32_t* __restrict__ status, uint32_t* __restrict__ big) {
     __shared__ uint32_t hc[LAYER_REP_WORDS];
     __shared__ unsigned long long hs[LAYER_REP_WORDS];
     uint64_t const N = sys.NT;
@@ -1982,6 +1984,7 @@ __global__ __launch_bounds__(WG) void k_layers(Sys sys, const float* __restrict_
                 uint32_t const iz = mod_dim(static_cast<uint32_t>(floorf(r[q].x + half_z)), Z);
                 if (lz) lz[j] = static_cast<uint16_t>(iz);
                 if (stats) {
+                    if (!(fabsf(r[q].y) < 128.0f)) atomicOr(status, STATUS_LAYER_VELOCITY);
                     atomicAdd(&hc[rep + iz], 1u);
                     atomicAdd(&hs[rep + iz], static_cast<unsigned long long>(static_cast<long long>(
                                                  rint(static_cast<double>(r[q].y) * FIX_SCALE))));
@@ -3251,7 +3254,7 @@ void adj_step(mpcd_hip_s* h, float const* force, int j, uint32_t* cnt, unsigned 
     h->sorted_valid = false;
     // the layer pass also moves the system to the next step
     hipLaunchKernelGGL(k_layers, dim3(layer_blocks(h)), dim3(WG), 0, h->stream, h->sys, h->st[h->cur], cnt, sum, lz,
-                       h->status + 1);
+                       h->status, h->status + 1);
 }
 
 struct ProfileResult {
@@ -3334,6 +3337,11 @@ ProfileResult profile_run(mpcd_hip_s* h, int steps, int K, float drag, float ang
         if (st & 1u) {
             ck(hipMemset(h->status, 0, 4), "status");
             throw HipError{"a cell holds more particles than one LDS tile"};
+        }
+        if (st & STATUS_LAYER_VELOCITY) {
+            ck(hipMemset(h->status, 0, 4), "status");
+            throw HipError{"a particle's x velocity reached 128 (or is not a number): the profile loss sums the "
+                           "layers' velocities in fixed point, exact only below that"};
         }
         ck(hipMemcpy(cnt.data(), a.lcnt, 4ull * T * LZ, hipMemcpyDeviceToHost), "download layers");
         ck(hipMemcpy(sum.data(), a.lsum, 16ull * T * LZ, hipMemcpyDeviceToHost), "download layers");

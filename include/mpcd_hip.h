@@ -142,7 +142,9 @@ int mpcd_hip_system_tiling(const mpcd_hip_t* h, uint32_t system, uint32_t* cells
  * NULL: no force / no force gradient.  The handle's state,
  * step index and parameters are unchanged afterwards (the same call repeats
  * the same trajectory: common random numbers).  profile (n_layers doubles,
- * host) may be NULL. */
+ * host) may be NULL.  The layers' x-velocity sums are exact fixed-point sums
+ * (multiples of 2^-40) while every |v_x| < 128: a run that meets a larger
+ * velocity, or one that is not a number, fails with an error. */
 int mpcd_hip_profile_loss(mpcd_hip_t* h, int steps, float drag, float rotation_angle, const float* force,
                           const double* target, size_t n_layers, double* loss, double* profile);
 int mpcd_hip_profile_loss_grad(mpcd_hip_t* h, int steps, int checkpoint_every, float drag, float rotation_angle,

@@ -177,3 +177,20 @@ def test_adjoint_rejects_what_it_does_not_model(pympcd):
         adjoint.profile_loss(sim, 3, p.drag, p.rotation_angle, target[:-1])
     with pytest.raises(ValueError, match="cells"):
         adjoint.profile_loss(sim, 3, p.drag, p.rotation_angle, target, force=np.zeros(10, np.float32))
+
+
+def test_layer_sums_reject_velocities_beyond_their_exact_range(pympcd):
+    """The layers' x-velocity sums are fixed point (2^-40), exact while every
+    |v_x| < 128: a particle beyond that makes the loss fail with a clear error,
+    and the handle works again once the state is sane."""
+    p, sim, target = _setup(pympcd)
+    pos, vel = sim.get_particle_positions(), sim.get_particle_velocities()
+    bad = vel.copy()
+    bad[7, 0] = 200.0
+    sim.set_particle_velocities(bad)
+    with pytest.raises(RuntimeError, match="fixed point"):
+        adjoint.profile_loss(sim, 3, p.drag, p.rotation_angle, target)
+    sim.set_particle_positions(pos)
+    sim.set_particle_velocities(vel)
+    r = adjoint.profile_loss(sim, 3, p.drag, p.rotation_angle, target)
+    assert np.isfinite(r.loss)

@@ -35,6 +35,11 @@ CONFIGS = [
     ((5, 4, 7), (1, 1, 0), 3, "extended"),
     ((4, 4, 6), (1, 1, 0), 1, "extended"),
     ((20, 16, 12), (1, 1, 0), 10, "extended"),
+    # lane groups of four and eight lanes per cell in the per-cell sums
+    # (n = 13 and 30; wall cells add n ghosts to the members)
+    ((7, 6, 8), (1, 1, 0), 13, "extended"),
+    ((8, 6, 8), (1, 1, 0), 30, "srd"),
+    ((6, 5, 8), (1, 1, 0), 30, "extended"),
     # longest axis above 128 cells: the internal key cuts it into 128-row
     # blocks (the last one shorter), so particles cross block edges, whose
     # keys fall outside the permutation's and histogram's windows
