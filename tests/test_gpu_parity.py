@@ -4,8 +4,10 @@
     permutation: bit-exact against the oracle's translate_particles table
     (simulation_functions.cpp:84-94).
 (b) per-particle positions and velocities after one step: expected bit-exact
-    (same IEEE operations in the same order, no contraction); the asserted bound
-    is the norm-wise relative 1e-5 of the specification.
+    (same IEEE operations in the same order, no contraction; the oracle with
+    the counter stream sums each cell in the GPU's lane-group order, which is
+    pinned against the reference's serial order in test_oracle_pinning.py);
+    the asserted bound is the norm-wise relative 1e-5 of the specification.
 Also: device Philox known answers and float transforms bitwise against the
 oracle, seeded initial state, sampling accumulators against the oracle's FP64
 accumulation.
@@ -186,7 +188,9 @@ def test_step_matches_oracle(pympcd, cfg):
 # windows and send cells outside them (block edges, periodic wrap) to memory
 SAMPLING_CFGS = [((6, 5, 8), (1, 1, 0), 10, "srd"), ((6, 5, 8), (1, 1, 0), 10, "extended"),
                  ((6, 4, 300), (1, 1, 0), 5, "srd"), ((300, 5, 6), (1, 1, 1), 4, "srd"),
-                 ((5, 4, 260), (1, 1, 0), 5, "extended")]
+                 ((5, 4, 260), (1, 1, 0), 5, "extended"),
+                 # four and eight lanes per cell in the per-cell sums
+                 ((7, 6, 8), (1, 1, 0), 20, "srd"), ((6, 5, 8), (1, 1, 0), 30, "extended")]
 
 
 @pytest.mark.parametrize("cfg", SAMPLING_CFGS, ids=lambda c: f"{c[3]}-{'x'.join(map(str, c[0]))}")
@@ -373,6 +377,7 @@ CHANNELS = [
     ((8, 10, 10), (1, 1, 1), 8, "srd"),
     ((10, 8, 8), (1, 1, 1), 8, "extended"),
     ((12, 9, 7), (1, 1, 0), 6, "srd"),
+    ((10, 8, 8), (1, 1, 1), 20, "extended"),   # four lanes per cell
 ]
 CIDS = [f"{a}-{'x'.join(map(str, f))}-p{''.join(map(str, p))}" for f, p, n, a in CHANNELS]
 
