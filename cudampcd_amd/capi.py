@@ -82,6 +82,7 @@ def lib() -> ctypes.CDLL:
         sigs = {
             "mpcd_hip_step": [_vp, ctypes.c_int],
             "mpcd_hip_step_and_sample": [_vp, ctypes.c_int],
+            "mpcd_hip_step_injected": [_vp, _f32p, _f32p, ctypes.c_size_t, _u32p, _f32p, ctypes.c_size_t],
             "mpcd_hip_get_sample_mean": [_vp, _f32p, _f32p, ctypes.c_size_t],
             "mpcd_hip_get_sample_mean_f64": [_vp, _f64p, _f64p, ctypes.c_size_t],
             "mpcd_hip_n_particles": [_vp, _u64p],
@@ -162,6 +163,26 @@ class Handle:
     @step_index.setter
     def step_index(self, s: int):
         self._check(self.L.mpcd_hip_set_step_index(self.h, s))
+
+    def step_injected(self, shift, axes, ghost_counts=None, ghost_velocities=None):
+        """One SRD step with the caller's draws (see mpcd_hip_step_injected):
+        shift (3,), axes (C, 3) in reference cell order, and with walls the
+        ghost counts (2 X Y,) and velocities (2 X Y, n, 3) of the wall-layer
+        cells."""
+        sh = np.ascontiguousarray(shift, dtype=np.float32).reshape(3)
+        ax = np.ascontiguousarray(axes, dtype=np.float32).reshape(-1)
+        if ax.size != 3 * self.C:
+            raise ValueError("axes must hold 3 values per cell")
+        gc = None if ghost_counts is None else np.ascontiguousarray(ghost_counts, dtype=np.uint32).reshape(-1)
+        gv = None if ghost_velocities is None else np.ascontiguousarray(ghost_velocities, dtype=np.float32).reshape(-1)
+        if (gc is None) != (gv is None):
+            raise ValueError("ghost counts and velocities go together")
+        if gc is not None and gv.size != 3 * gc.size * (gv.size // max(3 * gc.size, 1)):
+            raise ValueError("ghost velocities: n velocities of 3 floats per wall-layer cell")
+        self._check(self.L.mpcd_hip_step_injected(self.h, _p(sh, _f32p), _p(ax, _f32p), self.C,
+                                                  None if gc is None else _p(gc, _u32p),
+                                                  None if gv is None else _p(gv, _f32p),
+                                                  0 if gc is None else gc.size))
 
     def grid_shift(self, step: int) -> np.ndarray:
         out = np.empty(3, np.float32)

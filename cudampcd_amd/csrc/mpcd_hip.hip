@@ -2447,6 +2447,16 @@ struct mpcd_hip_s {
     float* axis = nullptr;           // SRD per-cell axes of the current step
     uint32_t* wgcnt = nullptr;       // wall-cell ghost counts of the step
     float* wgvel = nullptr;          // wall-cell ghosts of the step: velocities (SRD), positions and velocities (extended)
+    // draws of the next step given by the caller (mpcd_hip_step_injected):
+    // axes by internal cell key (3 planes), wall-cell ghost counts and
+    // velocities in the layout of wgcnt / wgvel; null: the step draws its own
+    struct Injected {
+        float shift[3];
+        std::vector<float> axes;
+        std::vector<uint32_t> gcnt;
+        std::vector<float> gvel;
+    };
+    Injected const* inject = nullptr;
     // system table and block -> system maps on the device
     KParams* d_kp = nullptr;
     uint32_t* d_blk = nullptr;       // blk_part | blk_cell | blk_wall | tile0, ns + 1 entries each
@@ -2672,6 +2682,9 @@ void upload_table(mpcd_hip_s* h) {
         k.step = static_cast<uint32_t>(h->step_index);
         grid_shift(k.rs, k.step, k.shift);
         grid_shift(k.rs, k.step + 1, k.shift_next);
+        // a step with injected draws (single system) uses the caller's shift
+        if (h->inject)
+            for (int a = 0; a < 3; ++a) k.shift[a] = h->inject->shift[a];
     }
     ck(hipStreamSynchronize(h->stream), "sync");
     ck(hipMemcpy(h->d_kp, h->kp.data(), sizeof(KParams) * h->ns, hipMemcpyHostToDevice), "upload system table");
@@ -2950,9 +2963,22 @@ void ensure_sampling(mpcd_hip_s* h) {
 int enqueue_step(mpcd_hip_s* h, bool sample, bool first, bool sort, uint32_t* rec, hipEvent_t e0, hipEvent_t e1,
                  uint32_t io) {
     int launches = 0;
-    bool const fold = sort && fold_draws(h);
+    bool const fold = sort && !h->inject && fold_draws(h);
     if (sort) launches += launch_sort(h, fold);
-    if (!fold) launches += launch_draws(h);
+    if (h->inject) {
+        // the caller's draws in place of this step's (pageable host memory:
+        // the copies complete before the collide launch reads them)
+        auto const& d = *h->inject;
+        ck(hipMemcpyAsync(h->axis, d.axes.data(), 4 * d.axes.size(), hipMemcpyHostToDevice, h->stream), "injected axes");
+        if (!d.gcnt.empty()) {
+            ck(hipMemcpyAsync(h->wgcnt, d.gcnt.data(), 4 * d.gcnt.size(), hipMemcpyHostToDevice, h->stream),
+               "injected ghost counts");
+            ck(hipMemcpyAsync(h->wgvel, d.gvel.data(), 4 * d.gvel.size(), hipMemcpyHostToDevice, h->stream),
+               "injected ghosts");
+        }
+    } else if (!fold) {
+        launches += launch_draws(h);
+    }
     if (e0) ck(hipEventRecord(e0, h->stream), "event");
     if (h->algorithm == 0) {
         if (sample) launch_collide_cap<0, true>(h, rec, io); else launch_collide_cap<0, false>(h, rec, io);
@@ -3020,7 +3046,8 @@ void run_steps(mpcd_hip_s* h, int n_steps, bool sample) {
             fit_capacity(h);
         }
         bool const first = sample && h->sampling_done;
-        bool const graphed = h->use_graph && !h->timing && !h->record && !first && !h->sorted_valid && !h->group_timing;
+        bool const graphed = h->use_graph && !h->timing && !h->record && !first && !h->sorted_valid && !h->group_timing &&
+                             !h->inject;
         // inside one call the intermediate states stay streamed; the last step
         // writes the post-collision state the API reads
         uint32_t const io = (s > 0 ? 1u : 0u) | (s + 1 < n_steps ? 2u : 0u);
@@ -3719,6 +3746,54 @@ int mpcd_hip_step_and_sample(mpcd_hip_t* h, int n_steps) {
     return guarded(h, [&] {
         ck(hipSetDevice(h->device), "hipSetDevice");
         run_steps(h, n_steps, true);
+    });
+}
+
+int mpcd_hip_step_injected(mpcd_hip_t* h, const float* shift, const float* axes, size_t n_cells,
+                           const uint32_t* ghost_counts, const float* ghost_velocities, size_t n_wall_cells) {
+    return guarded(h, [&] {
+        if (!shift || !axes) throw HipError{"injected step: shift and axes are required"};
+        if (h->ns != 1 || h->algorithm != 0) throw HipError{"injected step: single-system SRD handles only"};
+        KParams& k = h->kp[0];
+        uint32_t const X = k.dim[0], Y = k.dim[1], Z = k.dim[2];
+        uint64_t const XY = uint64_t(X) * Y;
+        bool const walls = !k.perio_z;
+        if (n_cells != h->C) throw HipError{"injected step: axes for a different number of cells"};
+        if (n_wall_cells != (walls ? 2 * XY : 0))
+            throw HipError{"injected step: ghost arrays for a different number of wall-layer cells"};
+        if (walls && (!ghost_counts || !ghost_velocities))
+            throw HipError{"injected step: a system with walls needs ghost counts and velocities"};
+        ck(hipSetDevice(h->device), "hipSetDevice");
+        mpcd_hip_s::Injected d;
+        d.axes.assign(3 * h->C, 0.0f);
+        for (uint32_t iz = 0; iz < Z; ++iz)
+            for (uint32_t iy = 0; iy < Y; ++iy)
+                for (uint32_t ix = 0; ix < X; ++ix) {
+                    uint64_t const c = ix + X * (iy + uint64_t(Y) * iz);
+                    uint64_t const key = k.cb + key_of(k, ix, iy, iz);
+                    for (int a = 0; a < 3; ++a) d.axes[a * h->C + key] = axes[3 * c + a];
+                }
+        if (walls) {
+            d.gcnt.assign(ghost_counts, ghost_counts + 2 * XY);
+            for (uint32_t g : d.gcnt)
+                if (g > k.n) throw HipError{"injected step: a ghost count above n"};
+            d.gvel.assign(ghost_velocities, ghost_velocities + 2 * XY * 3 * k.n);
+        }
+        // the step's state with the injected shift, its cells counted from scratch
+        for (int a = 0; a < 3; ++a) d.shift[a] = shift[a];
+        h->inject = &d;
+        h->keys_valid = false;
+        h->sorted_valid = false;
+        try {
+            upload_table(h);
+            run_steps(h, 1, false);
+        } catch (...) {
+            h->inject = nullptr;
+            upload_table(h);
+            throw;
+        }
+        h->inject = nullptr;
+        upload_table(h);   // the host table follows the device's (the next step's shifts)
     });
 }
 

@@ -69,6 +69,22 @@ int mpcd_hip_step(mpcd_hip_t* h, int n_steps);
 /* Replaces Backend::stepAndAccumulateSample (backend.hpp:14;
  * cuda_backend_impl.cu:182-204, cpu_backend.cpp:72-109). Accumulates in FP64. */
 int mpcd_hip_step_and_sample(mpcd_hip_t* h, int n_steps);
+/* One SRD step of a single-system handle with the caller's draws in place of
+ * its own: the parity hook that injects another implementation's random
+ * numbers (no reference counterpart; the reference draws inside its step,
+ * simulation_functions.cpp:108-202), so that a step can be compared with that
+ * implementation's particle by particle.  shift[3]: the grid shift; axes[3 C]:
+ * the rotation axis of every cell (x, y, z; reference cell order c = x +
+ * X (y + Y z); read for the cells that collide); with walls in z, for the
+ * cells w = (upper ? X Y : 0) + x + X y of this shift's two ghost layers
+ * (simulation_functions.cpp:137-138: the lower at z = (shift_z > 0 ? 1 : 0),
+ * the upper at z = Z - (shift_z > 0 ? 1 : 2)), ghost_counts[2 X Y] (each at
+ * most n) and ghost_velocities[2 X Y n 3] (velocities with the thermal
+ * velocity applied; the first count of a cell's n are used), NULL without
+ * walls.  n_cells must be C, n_wall_cells 2 X Y with walls and 0 without.
+ * Host memory.  The step index advances by one; later steps draw their own. */
+int mpcd_hip_step_injected(mpcd_hip_t* h, const float* shift, const float* axes, size_t n_cells,
+                           const uint32_t* ghost_counts, const float* ghost_velocities, size_t n_wall_cells);
 /* Replaces Backend::getSampleMean (backend.hpp:15; cpu_backend.cpp:111-122):
  * density[n_cells], velocity[3*n_cells], scaled by 1/sample_counter as the
  * reference does, and ends the current average. */

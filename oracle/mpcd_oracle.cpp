@@ -414,6 +414,15 @@ struct Oracle {
     // x body force per cell, v_x += g dt + f dt in the SRD rule (empty: none;
     // the reference has no force field)
     std::vector<float> force;
+    // draws of the last SRD collision, when logging (for injecting them into
+    // another implementation): the shift, every colliding cell's axis (zero
+    // for the others), and per wall-layer cell w = (upper layer ? XY : 0) +
+    // x + X y its ghost count and the first `count` of its n ghost velocities
+    // (thermal velocity applied)
+    bool log_draws = false;
+    V3 log_shift = {0, 0, 0};
+    std::vector<float> log_axes, log_gvel;
+    std::vector<uint32_t> log_gcnt;
 
     // sampling, reference semantics (cpu_backend.cpp:72-122) in FP32 Kahan
     bool sampling_done = true;
@@ -553,6 +562,12 @@ struct Oracle {
         std::vector<V3> bp(cap + p.n + 1), bv(cap + p.n + 1);
         std::vector<int64_t> bidx(cap + p.n + 1);
         uint32_t const G = p.sum_order ? group_lanes(p.n) : 1u;
+        if (log_draws) {
+            log_shift = shift;
+            log_axes.assign(3 * static_cast<size_t>(C), 0.0f);
+            log_gcnt.assign(2 * static_cast<size_t>(XY), 0u);
+            log_gvel.assign(2 * static_cast<size_t>(XY) * p.n * 3, 0.0f);
+        }
         for (uint32_t c = 0; c < C; ++c) {
             uint32_t m = std::min(cap, counter[c]);
             bool layer, ghosts;
@@ -562,6 +577,8 @@ struct Oracle {
                 for (uint32_t i = 0; i < p.n; ++i)
                     if ((rng.ghost_count_u(step, c, i) > sh) ^ layer ^ sign) ++added;
             uint32_t total = m + added;
+            size_t const w = (layer ? XY : 0u) + c % XY;   // wall-layer cell (when ghosts)
+            if (log_draws && ghosts) log_gcnt[w] = added;
             for (uint32_t i = 0; i < m; ++i) {
                 bidx[i] = list[c + static_cast<size_t>(i) * C];
                 bp[i] = parts[bidx[i]].pos;
@@ -574,6 +591,10 @@ struct Oracle {
                     bidx[i] = -1;
                     bp[i] = add(V3{g.x - 0.5f, g.y - 0.5f, g.z - 0.5f}, centre);
                     bv[i] = mul(rng.ghost_vel(step, c, i - m), p.thermal_velocity);
+                    if (log_draws) {
+                        float* const gv = &log_gvel[(w * p.n + (i - m)) * 3];
+                        gv[0] = bv[i].x; gv[1] = bv[i].y; gv[2] = bv[i].z;
+                    }
                 }
             }
             if (total > 1) {
@@ -583,6 +604,9 @@ struct Oracle {
                 float inv = 1.0f / static_cast<float>(total);
                 V3 mean = mul(sum, inv);
                 V3 axis = rng.srd_axis(step, c);
+                if (log_draws) {
+                    log_axes[3 * c] = axis.x; log_axes[3 * c + 1] = axis.y; log_axes[3 * c + 2] = axis.z;
+                }
                 for (uint32_t i = 0; i < total; ++i) {
                     V3 v = sub(bv[i], mean);
                     V3 vpar = mul(axis, dot(v, axis));
@@ -1036,6 +1060,17 @@ unsigned long long orc_get_cells(void* h, unsigned* cell_of_particle, unsigned* 
 }
 void orc_step(void* h, int n) { static_cast<Oracle*>(h)->step(n); }
 // x body force per cell (n_cells floats; NULL removes it)
+// draws of the SRD collisions from now on are logged (on != 0); the last
+// collision's: shift[3], axes[3 C] (reference cell order), ghost counts[2 XY]
+// and ghost velocities[2 XY n 3] (wall-layer cell w = (upper ? XY : 0) + x + X y)
+void orc_log_draws(void* h, int on) { static_cast<Oracle*>(h)->log_draws = on != 0; }
+void orc_get_draws(void* h, float* shift, float* axes, unsigned* gcnt, float* gvel) {
+    auto* o = static_cast<Oracle*>(h);
+    shift[0] = o->log_shift.x; shift[1] = o->log_shift.y; shift[2] = o->log_shift.z;
+    std::copy(o->log_axes.begin(), o->log_axes.end(), axes);
+    std::copy(o->log_gcnt.begin(), o->log_gcnt.end(), gcnt);
+    std::copy(o->log_gvel.begin(), o->log_gvel.end(), gvel);
+}
 void orc_set_force(void* h, const float* f) {
     Oracle* o = static_cast<Oracle*>(h);
     if (f) o->force.assign(f, f + o->C); else o->force.clear();

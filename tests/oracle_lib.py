@@ -111,6 +111,8 @@ def oracle_lib() -> ctypes.CDLL:
                                            c_int_p, c_float_p]
         lib.orc_sym_inverse.argtypes = [c_float_p, ctypes.c_float, c_float_p]
         lib.orc_set_force.argtypes = [ctypes.c_void_p, c_float_p]
+        lib.orc_log_draws.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        lib.orc_get_draws.argtypes = [ctypes.c_void_p, c_float_p, c_float_p, c_uint_p, c_float_p]
         lib.orc_frozen_profile_loss.restype = ctypes.c_double
         lib.orc_frozen_profile_loss.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_double,
                                                 c_double_p, c_double_p, ctypes.c_uint, ctypes.c_uint, ctypes.c_int,
@@ -274,6 +276,25 @@ class Oracle:
         v = np.empty(3 * self.C, np.float64)
         self.lib.orc_get_sample_mean_f64(self.h, _ptr(d, c_double_p), _ptr(v, c_double_p))
         return d, v
+
+    def log_draws(self, on=True):
+        """Record the draws of the SRD collisions from now on (see last_draws)."""
+        self.lib.orc_log_draws(self.h, int(on))
+
+    def last_draws(self):
+        """Draws of the last logged SRD collision: shift (3,), axes (C, 3) in
+        reference cell order (zero where no collision), ghost counts (2 X Y,)
+        and ghost velocities (2 X Y, n, 3) of the wall-layer cells
+        w = (upper layer ? X Y : 0) + x + X y."""
+        X, Y = int(self.params.volume[0]), int(self.params.volume[1])
+        n = int(self.params.n)
+        shift = np.empty(3, np.float32)
+        axes = np.empty((self.C, 3), np.float32)
+        gcnt = np.empty(2 * X * Y, np.uint32)
+        gvel = np.empty((2 * X * Y, n, 3), np.float32)
+        self.lib.orc_get_draws(self.h, _ptr(shift, c_float_p), _ptr(axes, c_float_p), _ptr(gcnt, c_uint_p),
+                               _ptr(gvel, c_float_p))
+        return shift, axes, gcnt, gvel
 
     def set_force(self, force):
         """x body force per cell (reference cell order) applied in the SRD rule; None removes it."""
