@@ -82,7 +82,10 @@ def lib() -> ctypes.CDLL:
         sigs = {
             "mpcd_hip_step": [_vp, ctypes.c_int],
             "mpcd_hip_step_and_sample": [_vp, ctypes.c_int],
-            "mpcd_hip_step_injected": [_vp, _f32p, _f32p, ctypes.c_size_t, _u32p, _f32p, ctypes.c_size_t],
+            "mpcd_hip_step_injected": [_vp, _f32p, _f32p, ctypes.c_size_t, _u32p, _f32p, ctypes.c_size_t,
+                                       ctypes.c_size_t],
+            "mpcd_hip_step_injected_extended": [_vp, _f32p, _u32p, _f32p, ctypes.c_size_t, ctypes.c_size_t,
+                                                _u32p, _f32p, ctypes.c_size_t, ctypes.c_size_t],
             "mpcd_hip_get_sample_mean": [_vp, _f32p, _f32p, ctypes.c_size_t],
             "mpcd_hip_get_sample_mean_f64": [_vp, _f64p, _f64p, ctypes.c_size_t],
             "mpcd_hip_n_particles": [_vp, _u64p],
@@ -177,12 +180,31 @@ class Handle:
         gv = None if ghost_velocities is None else np.ascontiguousarray(ghost_velocities, dtype=np.float32).reshape(-1)
         if (gc is None) != (gv is None):
             raise ValueError("ghost counts and velocities go together")
-        if gc is not None and gv.size != 3 * gc.size * (gv.size // max(3 * gc.size, 1)):
-            raise ValueError("ghost velocities: n velocities of 3 floats per wall-layer cell")
         self._check(self.L.mpcd_hip_step_injected(self.h, _p(sh, _f32p), _p(ax, _f32p), self.C,
                                                   None if gc is None else _p(gc, _u32p),
                                                   None if gv is None else _p(gv, _f32p),
-                                                  0 if gc is None else gc.size))
+                                                  0 if gc is None else gc.size, 0 if gv is None else gv.size))
+
+    def step_injected_extended(self, shift, cell_draws, thermal, ghost_counts=None, ghosts=None):
+        """One extended-rule step with the caller's draws (see
+        mpcd_hip_step_injected_extended): shift (3,), per cell (direction
+        index, sign, gate uniform bits) (C, 3) uint32, thermal Gaussians
+        (C, 4 n), and with walls the ghost counts (2 X Y,) and ghost positions
+        and velocities (2 X Y, n, 6)."""
+        sh = np.ascontiguousarray(shift, dtype=np.float32).reshape(3)
+        cd = np.ascontiguousarray(cell_draws, dtype=np.uint32).reshape(-1)
+        th = np.ascontiguousarray(thermal, dtype=np.float32).reshape(-1)
+        if cd.size != 3 * self.C:
+            raise ValueError("cell draws: 3 words per cell")
+        gc = None if ghost_counts is None else np.ascontiguousarray(ghost_counts, dtype=np.uint32).reshape(-1)
+        gg = None if ghosts is None else np.ascontiguousarray(ghosts, dtype=np.float32).reshape(-1)
+        if (gc is None) != (gg is None):
+            raise ValueError("ghost counts and ghosts go together")
+        self._check(self.L.mpcd_hip_step_injected_extended(self.h, _p(sh, _f32p), _p(cd, _u32p), _p(th, _f32p),
+                                                           self.C, th.size, None if gc is None else _p(gc, _u32p),
+                                                           None if gg is None else _p(gg, _f32p),
+                                                           0 if gc is None else gc.size,
+                                                           0 if gg is None else gg.size))
 
     def grid_shift(self, step: int) -> np.ndarray:
         out = np.empty(3, np.float32)

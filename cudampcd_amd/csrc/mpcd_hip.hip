@@ -85,6 +85,11 @@ struct KParams {
     // step state: written by the host at set-up, advanced on the device by k_advance
     uint32_t step;
     float shift[3], shift_next[3];
+    // extended rule, a step with injected draws (mpcd_hip_step_injected_extended;
+    // null otherwise): per reference cell the direction index, sign and gate
+    // uniform (bits), and the members' thermal Gaussians, cap per cell
+    uint32_t const* inj_ext;
+    float const* inj_gauss;
 };
 
 // The system table and the block -> system maps of one handle.
@@ -1214,9 +1219,18 @@ __device__ void extended_cell(KParams const& P, Tile<CAP, 1>& s, uint32_t k, uin
     uint32_t const gb = s.wcell[k];
     if (total <= 1) return;   // uniform over the group
 
-    Words const e = P.rs.at(P_EXT, 0, ref, P.step);
-    int const select = static_cast<int>(e.w[0] / 2) % 13;
-    int const flip = static_cast<int>(e.w[1] / 2) % 2;
+    int select, flip;
+    float gate;
+    if (P.inj_ext) {
+        select = static_cast<int>(P.inj_ext[3 * ref]);
+        flip = static_cast<int>(P.inj_ext[3 * ref + 1]);
+        gate = __uint_as_float(P.inj_ext[3 * ref + 2]);
+    } else {
+        Words const e = P.rs.at(P_EXT, 0, ref, P.step);
+        select = static_cast<int>(e.w[0] / 2) % 13;
+        flip = static_cast<int>(e.w[1] / 2) % 2;
+        gate = uniform(e.w[2]);
+    }
     float ax = c_axes[select][0], ay = c_axes[select][1], az = c_axes[select][2];
     if (flip) { ax = -ax; ay = -ay; az = -az; }
 
@@ -1288,7 +1302,7 @@ __device__ void extended_cell(KParams const& P, Tile<CAP, 1>& s, uint32_t k, uin
     proj0 = proj0 / static_cast<float>(size0);
     float const arg = 0.01f * (proj1 - proj0) * static_cast<float>(size0) * static_cast<float>(size1);
     float const probability = 1.0f - exp_f(arg);
-    bool const collide = uniform(e.w[2]) < probability;   // uniform over the group
+    bool const collide = gate < probability;   // uniform over the group
 
     float Lx = 0, Ly = 0, Lz = 0;
     if (collide) {
@@ -1731,7 +1745,12 @@ __device__ __forceinline__ void collide_tile(Sys const& sys, const float* __rest
                 }
                 uint32_t const k = lo, j = q - s.qb[k];
                 float gq[4];
-                thermal_quad(P, s.cref[k], j, gq);
+                if (P.inj_gauss) {
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) gq[u] = P.inj_gauss[static_cast<uint64_t>(s.cref[k]) * P.cap + 4 * j + u];
+                } else {
+                    thermal_quad(P, s.cref[k], j, gq);
+                }
                 uint32_t const a = s.off[kb + k] - gb, rr = s.goff[k];
                 uint32_t const tot = rr + (s.gcnt[k] & ~UPPER_LAYER);
 #pragma unroll
@@ -2969,7 +2988,9 @@ int enqueue_step(mpcd_hip_s* h, bool sample, bool first, bool sort, uint32_t* re
         // the caller's draws in place of this step's (pageable host memory:
         // the copies complete before the collide launch reads them)
         auto const& d = *h->inject;
-        ck(hipMemcpyAsync(h->axis, d.axes.data(), 4 * d.axes.size(), hipMemcpyHostToDevice, h->stream), "injected axes");
+        if (!d.axes.empty())
+            ck(hipMemcpyAsync(h->axis, d.axes.data(), 4 * d.axes.size(), hipMemcpyHostToDevice, h->stream),
+               "injected axes");
         if (!d.gcnt.empty()) {
             ck(hipMemcpyAsync(h->wgcnt, d.gcnt.data(), 4 * d.gcnt.size(), hipMemcpyHostToDevice, h->stream),
                "injected ghost counts");
@@ -3750,7 +3771,8 @@ int mpcd_hip_step_and_sample(mpcd_hip_t* h, int n_steps) {
 }
 
 int mpcd_hip_step_injected(mpcd_hip_t* h, const float* shift, const float* axes, size_t n_cells,
-                           const uint32_t* ghost_counts, const float* ghost_velocities, size_t n_wall_cells) {
+                           const uint32_t* ghost_counts, const float* ghost_velocities, size_t n_wall_cells,
+                           size_t n_ghost_values) {
     return guarded(h, [&] {
         if (!shift || !axes) throw HipError{"injected step: shift and axes are required"};
         if (h->ns != 1 || h->algorithm != 0) throw HipError{"injected step: single-system SRD handles only"};
@@ -3763,6 +3785,8 @@ int mpcd_hip_step_injected(mpcd_hip_t* h, const float* shift, const float* axes,
             throw HipError{"injected step: ghost arrays for a different number of wall-layer cells"};
         if (walls && (!ghost_counts || !ghost_velocities))
             throw HipError{"injected step: a system with walls needs ghost counts and velocities"};
+        if (n_ghost_values != (walls ? 2 * XY * 3 * k.n : 0))
+            throw HipError{"injected step: ghost velocities must hold n slots of 3 values per wall-layer cell"};
         ck(hipSetDevice(h->device), "hipSetDevice");
         mpcd_hip_s::Injected d;
         d.axes.assign(3 * h->C, 0.0f);
@@ -3793,6 +3817,69 @@ int mpcd_hip_step_injected(mpcd_hip_t* h, const float* shift, const float* axes,
             throw;
         }
         h->inject = nullptr;
+        upload_table(h);   // the host table follows the device's (the next step's shifts)
+    });
+}
+
+int mpcd_hip_step_injected_extended(mpcd_hip_t* h, const float* shift, const uint32_t* cell_draws,
+                                    const float* thermal, size_t n_cells, size_t n_thermal,
+                                    const uint32_t* ghost_counts, const float* ghosts, size_t n_wall_cells,
+                                    size_t n_ghost_values) {
+    return guarded(h, [&] {
+        if (!shift || !cell_draws || !thermal)
+            throw HipError{"injected step: shift, cell draws and thermal draws are required"};
+        if (h->ns != 1 || h->algorithm != 1) throw HipError{"injected step: single-system extended-rule handles only"};
+        KParams& k = h->kp[0];
+        uint64_t const XY = uint64_t(k.dim[0]) * k.dim[1];
+        bool const walls = !k.perio_z;
+        if (n_cells != h->C) throw HipError{"injected step: cell draws for a different number of cells"};
+        if (n_wall_cells != (walls ? 2 * XY : 0))
+            throw HipError{"injected step: ghost arrays for a different number of wall-layer cells"};
+        if (walls && (!ghost_counts || !ghosts))
+            throw HipError{"injected step: a system with walls needs ghost counts, positions and velocities"};
+        if (n_thermal != h->C * k.cap) throw HipError{"injected step: thermal draws must hold 4 n values per cell"};
+        if (n_ghost_values != (walls ? 2 * XY * 6 * k.n : 0))
+            throw HipError{"injected step: ghosts must hold n slots of 6 values per wall-layer cell"};
+        for (uint64_t c = 0; c < h->C; ++c)
+            if (cell_draws[3 * c] > 12 || cell_draws[3 * c + 1] > 1)
+                throw HipError{"injected step: a direction index above 12 or a sign above 1"};
+        ck(hipSetDevice(h->device), "hipSetDevice");
+        mpcd_hip_s::Injected d;
+        for (int a = 0; a < 3; ++a) d.shift[a] = shift[a];
+        if (walls) {
+            d.gcnt.assign(ghost_counts, ghost_counts + 2 * XY);
+            for (uint32_t g : d.gcnt)
+                if (g > k.n) throw HipError{"injected step: a ghost count above n"};
+            d.gvel.assign(ghosts, ghosts + 2 * XY * 6 * k.n);
+        }
+        uint32_t* dev_ext = nullptr;
+        float* dev_gauss = nullptr;
+        auto release = [&] {
+            k.inj_ext = nullptr;
+            k.inj_gauss = nullptr;
+            h->inject = nullptr;
+            (void)hipFree(dev_ext);
+            (void)hipFree(dev_gauss);
+        };
+        try {
+            ck(hipMalloc(&dev_ext, 12 * h->C), "alloc injected cell draws");
+            ck(hipMalloc(&dev_gauss, 4 * h->C * k.cap), "alloc injected thermal draws");
+            ck(hipMemcpy(dev_ext, cell_draws, 12 * h->C, hipMemcpyHostToDevice), "injected cell draws");
+            ck(hipMemcpy(dev_gauss, thermal, 4 * h->C * k.cap, hipMemcpyHostToDevice), "injected thermal draws");
+            k.inj_ext = dev_ext;
+            k.inj_gauss = dev_gauss;
+            h->inject = &d;
+            h->keys_valid = false;
+            h->sorted_valid = false;
+            upload_table(h);
+            run_steps(h, 1, false);
+        } catch (...) {
+            ck(hipStreamSynchronize(h->stream), "sync");
+            release();
+            upload_table(h);
+            throw;
+        }
+        release();
         upload_table(h);   // the host table follows the device's (the next step's shifts)
     });
 }

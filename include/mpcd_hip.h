@@ -81,10 +81,27 @@ int mpcd_hip_step_and_sample(mpcd_hip_t* h, int n_steps);
  * the upper at z = Z - (shift_z > 0 ? 1 : 2)), ghost_counts[2 X Y] (each at
  * most n) and ghost_velocities[2 X Y n 3] (velocities with the thermal
  * velocity applied; the first count of a cell's n are used), NULL without
- * walls.  n_cells must be C, n_wall_cells 2 X Y with walls and 0 without.
+ * walls.  n_cells must be C, n_wall_cells 2 X Y with walls and 0 without,
+ * n_ghost_values the length of ghost_velocities (2 X Y n 3, or 0).
  * Host memory.  The step index advances by one; later steps draw their own. */
 int mpcd_hip_step_injected(mpcd_hip_t* h, const float* shift, const float* axes, size_t n_cells,
-                           const uint32_t* ghost_counts, const float* ghost_velocities, size_t n_wall_cells);
+                           const uint32_t* ghost_counts, const float* ghost_velocities, size_t n_wall_cells,
+                           size_t n_ghost_values);
+/* The same for one extended-rule step (simulation_functions.cpp:207-406):
+ * shift[3]; cell_draws[3 C] per cell (reference order): the direction index
+ * 0-12 and sign 0/1 of simulation_functions.cpp:272-297 and the bits of the
+ * gate uniform; thermal[C 4 n]: the thermal Gaussians of a cell's members
+ * (table particles in slot order, then ghosts; read for the cells that
+ * collide); with walls, ghost_counts[2 X Y] and ghosts[2 X Y n 6] (position
+ * x, y, z and velocity with the thermal velocity applied, n slots per
+ * wall-layer cell w as above), NULL without walls; n_thermal (C 4 n) and
+ * n_ghost_values (2 X Y n 6, or 0) are the two arrays' lengths.  The direction and gate
+ * draws of the collision and its thermal draws come from these arrays
+ * (copied to the device for the step), the rest of the step as usual. */
+int mpcd_hip_step_injected_extended(mpcd_hip_t* h, const float* shift, const uint32_t* cell_draws,
+                                    const float* thermal, size_t n_cells, size_t n_thermal,
+                                    const uint32_t* ghost_counts, const float* ghosts, size_t n_wall_cells,
+                                    size_t n_ghost_values);
 /* Replaces Backend::getSampleMean (backend.hpp:15; cpu_backend.cpp:111-122):
  * density[n_cells], velocity[3*n_cells], scaled by 1/sample_counter as the
  * reference does, and ends the current average. */

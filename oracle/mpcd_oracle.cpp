@@ -414,15 +414,18 @@ struct Oracle {
     // x body force per cell, v_x += g dt + f dt in the SRD rule (empty: none;
     // the reference has no force field)
     std::vector<float> force;
-    // draws of the last SRD collision, when logging (for injecting them into
-    // another implementation): the shift, every colliding cell's axis (zero
-    // for the others), and per wall-layer cell w = (upper layer ? XY : 0) +
-    // x + X y its ghost count and the first `count` of its n ghost velocities
-    // (thermal velocity applied)
+    // draws of the last collision, when logging (for injecting them into
+    // another implementation): the shift; SRD: every colliding cell's axis
+    // (zero for the others); extended: every colliding cell's direction
+    // index, sign and gate uniform (bits) and the thermal Gaussians of its
+    // members (table particles, then ghosts; 4n per cell, zero where unused);
+    // both: per wall-layer cell w = (upper layer ? XY : 0) + x + X y its ghost
+    // count and its n ghost slots, the first `count` drawn: velocity (SRD, 3
+    // floats, thermal velocity applied) or position and velocity (extended, 6)
     bool log_draws = false;
     V3 log_shift = {0, 0, 0};
-    std::vector<float> log_axes, log_gvel;
-    std::vector<uint32_t> log_gcnt;
+    std::vector<float> log_axes, log_gvel, log_gauss;
+    std::vector<uint32_t> log_gcnt, log_ext;
 
     // sampling, reference semantics (cpu_backend.cpp:72-122) in FP32 Kahan
     bool sampling_done = true;
@@ -685,6 +688,13 @@ struct Oracle {
         std::vector<int64_t> bidx(cap + p.n + 1);
         std::vector<uint8_t> side(cap + p.n + 1);
         uint32_t const G = p.sum_order ? group_lanes(p.n) : 1u;
+        if (log_draws) {
+            log_shift = shift;
+            log_ext.assign(3 * static_cast<size_t>(C), 0u);
+            log_gauss.assign(static_cast<size_t>(C) * cap, 0.0f);
+            log_gcnt.assign(2 * static_cast<size_t>(XY), 0u);
+            log_gvel.assign(2 * static_cast<size_t>(XY) * p.n * 6, 0.0f);
+        }
         for (uint32_t c = 0; c < C; ++c) {
             uint32_t m = std::min(cap, counter[c]);
             bool layer, ghosts;
@@ -693,6 +703,8 @@ struct Oracle {
             if (ghosts)
                 for (uint32_t i = 0; i < p.n; ++i)
                     if ((rng.ghost_count_u(step, c, i) > sh) ^ layer ^ sign) ++added;
+            size_t const w = (layer ? XY : 0u) + c % XY;   // wall-layer cell (when ghosts)
+            if (log_draws && ghosts) log_gcnt[w] = added;
             uint32_t total = std::min(m + added, cap);
             uint32_t reals = total - added;   // may drop table entries when ghosts fill the cap
             for (uint32_t i = 0; i < reals; ++i) {
@@ -707,11 +719,20 @@ struct Oracle {
                     bidx[i] = -1;
                     bp[i] = add(V3{g.x - 0.5f, g.y - 0.5f, g.z - 0.5f}, offset);
                     bv[i] = mul(rng.ghost_vel(step, c, i - reals), tv);
+                    if (log_draws) {
+                        float* const gv = &log_gvel[(w * p.n + (i - reals)) * 6];
+                        gv[0] = bp[i].x; gv[1] = bp[i].y; gv[2] = bp[i].z;
+                        gv[3] = bv[i].x; gv[4] = bv[i].y; gv[5] = bv[i].z;
+                    }
                 }
             }
             if (total > 1) {
                 int select, flip;
                 rng.ext_axis(step, c, &select, &flip);
+                if (log_draws) {
+                    log_ext[3 * c] = static_cast<uint32_t>(select);
+                    log_ext[3 * c + 1] = static_cast<uint32_t>(flip);
+                }
                 V3 axis = ext_direction(select);
                 if (flip) axis = neg(axis);
 
@@ -745,7 +766,9 @@ struct Oracle {
                 float probability = 1 - (p.rng_mode == 0 ? std::exp(arg) : p_expf(arg));
 
                 V3 dL = {0, 0, 0};
-                bool collide = rng.ext_gate(step, c) < probability;
+                float const gate = rng.ext_gate(step, c);
+                if (log_draws) log_ext[3 * c + 2] = f2u(gate);
+                bool collide = gate < probability;
                 if (collide) {
                     float t0 = proj1 * (float(size1) / size0), t1 = proj0 * (float(size0) / size1);
                     // per lane: the two side sums, the angular momentum (added
@@ -755,7 +778,9 @@ struct Oracle {
                     std::vector<Sym> pI(G);
                     for (uint32_t i = 0; i < total; ++i) {
                         bool s = side[i];
-                        float vr = rng.ext_gauss(step, c, i) * tv;
+                        float const xi = rng.ext_gauss(step, c, i);
+                        if (log_draws) log_gauss[static_cast<size_t>(c) * cap + i] = xi;
+                        float vr = xi * tv;
                         (s ? pm0 : pm1)[i % G] += vr;
                         pL[i % G] = add(pL[i % G], cross(bp[i], bv[i]));
                         bv[i] = add(bv[i], mul(axis, vr - dot(bv[i], axis) + (s ? t0 : t1)));
@@ -1068,6 +1093,16 @@ void orc_get_draws(void* h, float* shift, float* axes, unsigned* gcnt, float* gv
     auto* o = static_cast<Oracle*>(h);
     shift[0] = o->log_shift.x; shift[1] = o->log_shift.y; shift[2] = o->log_shift.z;
     std::copy(o->log_axes.begin(), o->log_axes.end(), axes);
+    std::copy(o->log_gcnt.begin(), o->log_gcnt.end(), gcnt);
+    std::copy(o->log_gvel.begin(), o->log_gvel.end(), gvel);
+}
+// extended rule: shift[3], per cell (direction index, sign, gate bits)[3 C],
+// thermal Gaussians[C 4n], ghost counts[2 XY], ghost positions and velocities[2 XY n 6]
+void orc_get_ext_draws(void* h, float* shift, unsigned* ext, float* gauss, unsigned* gcnt, float* gvel) {
+    auto* o = static_cast<Oracle*>(h);
+    shift[0] = o->log_shift.x; shift[1] = o->log_shift.y; shift[2] = o->log_shift.z;
+    std::copy(o->log_ext.begin(), o->log_ext.end(), ext);
+    std::copy(o->log_gauss.begin(), o->log_gauss.end(), gauss);
     std::copy(o->log_gcnt.begin(), o->log_gcnt.end(), gcnt);
     std::copy(o->log_gvel.begin(), o->log_gvel.end(), gvel);
 }

@@ -113,6 +113,7 @@ def oracle_lib() -> ctypes.CDLL:
         lib.orc_set_force.argtypes = [ctypes.c_void_p, c_float_p]
         lib.orc_log_draws.argtypes = [ctypes.c_void_p, ctypes.c_int]
         lib.orc_get_draws.argtypes = [ctypes.c_void_p, c_float_p, c_float_p, c_uint_p, c_float_p]
+        lib.orc_get_ext_draws.argtypes = [ctypes.c_void_p, c_float_p, c_uint_p, c_float_p, c_uint_p, c_float_p]
         lib.orc_frozen_profile_loss.restype = ctypes.c_double
         lib.orc_frozen_profile_loss.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_double,
                                                 c_double_p, c_double_p, ctypes.c_uint, ctypes.c_uint, ctypes.c_int,
@@ -295,6 +296,22 @@ class Oracle:
         self.lib.orc_get_draws(self.h, _ptr(shift, c_float_p), _ptr(axes, c_float_p), _ptr(gcnt, c_uint_p),
                                _ptr(gvel, c_float_p))
         return shift, axes, gcnt, gvel
+
+    def last_ext_draws(self):
+        """Draws of the last logged extended-rule collision: shift (3,), per
+        cell (direction index, sign, gate uniform as float32 bits) (C, 3), the
+        members' thermal Gaussians (C, 4n), ghost counts (2 X Y,) and ghost
+        positions and velocities (2 X Y, n, 6) of the wall-layer cells."""
+        X, Y = int(self.params.volume[0]), int(self.params.volume[1])
+        n = int(self.params.n)
+        shift = np.empty(3, np.float32)
+        ext = np.empty((self.C, 3), np.uint32)
+        gauss = np.empty((self.C, 4 * n), np.float32)
+        gcnt = np.empty(2 * X * Y, np.uint32)
+        gvel = np.empty((2 * X * Y, n, 6), np.float32)
+        self.lib.orc_get_ext_draws(self.h, _ptr(shift, c_float_p), _ptr(ext, c_uint_p), _ptr(gauss, c_float_p),
+                                   _ptr(gcnt, c_uint_p), _ptr(gvel, c_float_p))
+        return shift, ext, gauss, gcnt, gvel
 
     def set_force(self, force):
         """x body force per cell (reference cell order) applied in the SRD rule; None removes it."""
