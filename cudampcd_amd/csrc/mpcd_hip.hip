@@ -702,24 +702,20 @@ __device__ __forceinline__ uint32_t ghost_count(KParams const& P, uint32_t ref, 
     return added;
 }
 
-// Marsaglia unit vector, attempts keyed by index (random.hpp:237-253)
+// Random unit vector, uniform on the sphere, from one counter draw: z uniform
+// on [-1, 1] and the azimuth uniform on [0, 2 pi) (Archimedes: the projection
+// of the uniform sphere measure on an axis is uniform).  The reference draws
+// the same distribution by Marsaglia's rejection (random.hpp:237-253); this
+// form needs no loop, so the lanes of a wave never wait on a redraw.
 __device__ __forceinline__ void srd_axis(KParams const& P, uint32_t ref, float& ax, float& ay, float& az) {
-    for (uint32_t a = 0;; ++a) {
-        Words const r = P.rs.at(P_AXIS, a, ref, P.step);
-#pragma unroll
-        for (int h = 0; h < 2; ++h) {
-            float const rd1 = 1.0f - 2.0f * uniform(r.w[2 * h]);
-            float const rd2 = 1.0f - 2.0f * uniform(r.w[2 * h + 1]);
-            float const rsq = rd1 * rd1 + rd2 * rd2;
-            if (!(rsq > 1.0f)) {
-                float const rdh = 2.0f * sqrtf(1.0f - rsq);
-                ax = rd1 * rdh;
-                ay = rd2 * rdh;
-                az = (1.0f - 2.0f * rsq);
-                return;
-            }
-        }
-    }
+    Words const r = P.rs.at(P_AXIS, 0u, ref, P.step);
+    float const z = 1.0f - 2.0f * uniform(r.w[0]);
+    float sn, cs;
+    sincospi(2.0f * uniform(r.w[1]), &sn, &cs);
+    float const rho = sqrtf((1.0f - z) * (1.0f + z));
+    ax = rho * cs;
+    ay = rho * sn;
+    az = z;
 }
 
 struct Sym {
@@ -754,11 +750,15 @@ __device__ __forceinline__ Sym sym_inverse(Sym const& m, float tol) {
     return o;
 }
 
-// Per-cell draws of the SRD rule, computed once per step by fully populated
-// waves instead of inside the particle tiles:
-//   axis[a][key]   random rotation axis of every cell (random.hpp:237-253)
+// Per-cell draws computed by fully populated waves instead of inside the
+// particle tiles:
 //   gcnt[w]        ghost count of wall cell w (simulation_functions.cpp:141-146)
-//   gvel[w][g][a]  ghost velocities, Maxwell-Boltzmann times sqrt(T) (:153-166)
+//   gvel           SRD: gvel[w][a], the sum of the cell's ghost velocities
+//                  (one draw, below); extended rule: gvel[w][g][6], each
+//                  ghost's position and velocity (:147-167)
+//   axis[a][key]   SRD rotation axis of every cell (random.hpp:237-253), only
+//                  for steps that keep their axes in memory (taped adjoint
+//                  steps); other steps draw the axis in the collide tile
 // Wall cells: the two z layers next to the walls for this step's shift,
 // w = (upper ? X*Y : 0) + x + X*y.
 __device__ __forceinline__ void axes_block(Sys const& sys, float* __restrict__ axis, uint32_t blk) {
@@ -779,12 +779,17 @@ __device__ __forceinline__ void axes_block(Sys const& sys, float* __restrict__ a
 }
 
 // A block takes WG wall cells at a time: one lane per cell draws the ghost
-// count, then the cells' n ghost slots are spread over all lanes (slot g of
-// cell k on lane k n + g), so the velocity draws run on full waves and
-// neighbouring lanes store neighbouring ghosts.  SRD: 3 floats per slot
-// (velocity); extended rule (ext): 6 (position in the cell, then velocity),
-// the positions' z in the part of the cell the reference's rejection loop
-// accepts (simulation_functions.cpp:147-167).
+// count.  SRD: the ghosts enter the collision only through the sum of their
+// velocities (the cell mean, simulation_functions.cpp:168-171; getCorrection
+// returns the mean, mpc_cell.hpp:82-84), and the sum of `added` independent
+// thermal velocities is one Gaussian of `added` times their variance, so the
+// same lane draws that sum at once: sqrt(added) sqrt(T) (g0, g1, g2), 3 floats
+// per wall cell.  Extended rule (ext): every ghost's position and velocity
+// enter the collision; the cells' n ghost slots are spread over all lanes
+// (slot g of cell k on lane k n + g), so the draws run on full waves and
+// neighbouring lanes store neighbouring ghosts, 6 floats per slot (position
+// in the cell, then velocity), the positions' z in the part of the cell the
+// reference's rejection loop accepts (simulation_functions.cpp:147-167).
 __device__ __forceinline__ void ghosts_block(Sys const& sys, uint32_t* __restrict__ gcnt, float* __restrict__ gvel,
                                              uint32_t blk, bool ext) {
     __shared__ uint32_t s_cnt[WG];
@@ -807,11 +812,23 @@ __device__ __forceinline__ void ghosts_block(Sys const& sys, uint32_t* __restric
         if (w < 2 * XY) {
             added = ghost_count(P, ref_of_wall(w), sh, w >= XY, sign);
             gcnt[P.wb + w] = added;
+            if (!ext && added) {
+                Words const v = P.rs.at(P_GHOST_VEL, 0u, ref_of_wall(w), P.step);
+                float g0, g1, g2, g3;
+                box_muller(v.w[0], v.w[1], &g0, &g1);
+                box_muller(v.w[2], v.w[3], &g2, &g3);
+                float const sc = P.tv * sqrtf(static_cast<float>(added));
+                float* const out = gvel + P.gvb + 3ull * w;
+                out[0] = g0 * sc;
+                out[1] = g1 * sc;
+                out[2] = g2 * sc;
+            }
         }
+        if (!ext) continue;   // uniform over the block
         s_cnt[threadIdx.x] = added;
         __syncthreads();
         uint32_t const ncell = min(static_cast<uint32_t>(WG), 2 * XY - c0);
-        uint32_t const per = ext ? 6u : 3u;
+        constexpr uint32_t per = 6u;
         for (uint32_t t = threadIdx.x; t < ncell * P.n; t += WG) {
             uint32_t const k = t / P.n, g = t - k * P.n;
             if (g < s_cnt[k]) {
@@ -821,7 +838,7 @@ __device__ __forceinline__ void ghosts_block(Sys const& sys, uint32_t* __restric
                 box_muller(v.w[0], v.w[1], &g0, &g1);
                 box_muller(v.w[2], v.w[3], &g2, &g3);
                 float* out = gvel + P.gvb + static_cast<uint64_t>(per) * P.n * c0 + static_cast<uint64_t>(per) * t;
-                if (ext) {
+                {
                     // ghost position: uniform in x and y, z in the accepted
                     // part of the cell, about the cell centre (volume_container.hpp:86-91)
                     bool const layer = w >= XY;
@@ -917,9 +934,8 @@ constexpr int TCM = 64;   // largest cells per tile of any tile
 constexpr int tile_cell_max(int cap, int algo) {
     return algo == 0 ? (cap <= 384 ? 24 : (cap <= 448 ? 32 : TCM)) : (cap <= 512 ? 48 : TCM);
 }
-// floats of the ghost region: extended ghosts (per coordinate), SRD staged
-// wall-cell ghost velocities (slightly less than half the capacity for the
-// 384-particle tile, to stay within 26 LDS granules).
+// ghost slots of an extended tile's ghost region (the SRD rule keeps no ghosts
+// in the tile: a wall cell's ghosts enter it as one velocity sum).
 // Extended 512-particle tiles keep 100 ghost slots: a 48-cell batch rarely
 // holds more than a few wall cells, a batch whose ghosts would overflow ends
 // early, and the smaller region lets eight workgroups share a CU instead of
@@ -927,9 +943,7 @@ constexpr int tile_cell_max(int cap, int algo) {
 // seven workgroups 4.16, 112 slots and eight 3.79; 100 leave room for the
 // ghosts' thermal Gaussians at eight).  A region must hold one wall cell's n
 // ghosts (choose_tiling).
-constexpr int ghost_floats(int cap, int algo) {
-    return algo == 0 && cap == 384 ? cap / 2 - 32 : (algo == 1 && cap == 512 ? 100 : cap / 2);
-}
+constexpr int ghost_slots(int cap) { return cap == 512 ? 100 : cap / 2; }
 
 // threads of the collide workgroup for an LDS capacity: more, smaller
 // workgroups keep more independent tiles in flight per CU
@@ -954,28 +968,26 @@ struct Tile {
     float cm[6][TC];                 // per cell: mean velocity xyz, axis xyz
     uint32_t off[TC + 1];            // cell offsets of the tile
     uint8_t go[TC];                  // collision happened
-    uint32_t cref[TC];               // reference cell index
+    uint32_t cref[ALGO == 1 ? TC : 1];   // extended: reference cell index
     float cforce[ALGO == 0 ? TC : 1];  // SRD adjoint runs: force term f dt of each cell
     uint32_t gcnt[TC];               // ghost count
-    uint32_t goff[TC];               // first ghost slot
-    uint32_t wcell[TC];              // SRD: cell of each staged wall slot; extended: first ghost slot
+    uint32_t goff[ALGO == 1 ? TC : 1];   // extended: table particles kept
+    uint32_t wcell[ALGO == 1 ? TC : 1];  // extended: first ghost slot
     uint32_t wpre[ALGO == 1 ? TC + 1 : 1];  // extended: wall cells of the tile before each cell
     // extended: thermal draws (quads) of the batch before each cell (at most
     // (CAP + GS) / 4 + TC: a byte up to the 512 tile)
     std::conditional_t<(CAP <= 512), uint8_t, uint16_t> qb[ALGO == 1 ? TC + 1 : 1];
     uint8_t wk[ALGO == 1 ? TC : 1];         // extended: cell of each wall cell of the batch, in order
     uint8_t cell[CAP];               // tile-local cell of each slot
-    static constexpr int GS = ghost_floats(CAP, ALGO);
-    float g[ALGO == 1 ? 6 : 1][GS];  // extended: ghosts, n slots per wall cell of the batch;
-                                          // SRD: staged ghost velocities
-    float gg[ALGO == 1 ? GS : 1];    // extended: the ghosts' thermal Gaussians
-    uint32_t nwall;                  // SRD: wall cells of the batch staged so far
+    static constexpr int GS = ALGO == 1 ? ghost_slots(CAP) : 1;
+    float g[ALGO == 1 ? 6 : 1][GS];  // extended: ghosts, n slots per wall cell of the batch
+    float gg[GS];                    // extended: the ghosts' thermal Gaussians
 };
 // eight extended 512-particle workgroups share the 160 KiB of a CU (beside
 // the tile: the unsampled bins, 32 bytes, and the workgroup reduction of
 // __syncthreads_count, about 230 bytes)
 static_assert(sizeof(Tile<512, 1>) + 32 + 240 <= 20480, "extended 512 tile within a CU's eighth of the LDS");
-static_assert((512 + ghost_floats(512, 1)) / 4 + tile_cell_max(512, 1) < 256, "quad prefix of the 512 tile in a byte");
+static_assert((512 + ghost_slots(512)) / 4 + tile_cell_max(512, 1) < 256, "quad prefix of the 512 tile in a byte");
 // twelve SRD 384-particle workgroups (26 LDS granules of 512 bytes each)
 static_assert(sizeof(Tile<384, 0>) + 64 <= 13312, "SRD 384 tile within 26 LDS granules");
 
@@ -1056,16 +1068,14 @@ struct SampleBins {
 };
 
 // SRD per-cell stage of one batch (simulation_functions.cpp:133-182): the mean
-// velocity of every cell over its table particles (slot order) and ghosts
-// (ghost order), summed by the cell's lane group (cell_lanes_log2: lane j adds
-// members j, j + G, ..., group_sum combines the lanes).  Ghost counts and
-// velocities of wall cells (drawn by k_cell_draws) were staged into LDS while
-// the particles were gathered: s.gcnt[k] ghosts at s.g[0][s.goff[k] ...], or,
-// when the staging region was full (s.goff[k] == ~0u), read from memory at
-// gvel + 3 n s.cref[k].
+// velocity of every cell over its table particles (slot order), summed by the
+// cell's lane group (cell_lanes_log2: lane j adds members j, j + G, ...,
+// group_sum combines the lanes), plus, for a wall cell with s.gcnt[k] > 0
+// ghosts, the sum of their velocities, which the per-cell draws left in the
+// cell's mean slots s.cm[0..2][k]; divided by table particles plus ghosts.
 template <int CAP, bool ADJ>
 __device__ void srd_cells(KParams const& P, Tile<CAP, 0>& s, uint32_t ncell, uint32_t const* off, uint32_t gb,
-                          float const* __restrict__ gvel, uint32_t key0, Tape const& T, uint64_t CT) {
+                          uint32_t key0, Tape const& T, uint64_t CT) {
     constexpr uint32_t NT = tile_threads(CAP);
     uint32_t const lg = cell_lanes_log2(P.n), G = 1u << lg;
     // groups are aligned and NT is a multiple of 64, so a group's lanes are
@@ -1091,15 +1101,16 @@ __device__ void srd_cells(KParams const& P, Tile<CAP, 0>& s, uint32_t ncell, uin
             for (; i < reals; i += G) {
                 sx = sx + v[i]; sy = sy + v[CAP + i]; sz = sz + v[2 * CAP + i];
             }
-            // then its ghosts (member reals + q, ghost order)
-            uint32_t const go = s.goff[k];
-            float const* const g = go != ~0u ? &s.g[0][go] : gvel + P.gvb + 3ull * P.n * s.cref[k];
-            for (uint32_t q = i - reals; q < added; q += G) {
-                sx = sx + g[3 * q]; sy = sy + g[3 * q + 1]; sz = sz + g[3 * q + 2];
-            }
             sx = group_sum(sx, lg);
             sy = group_sum(sy, lg);
             sz = group_sum(sz, lg);
+            // the ghosts' sum (read by every lane of the group before lane 0
+            // overwrites the slots with the mean below, in this wave's order)
+            if (added) {
+                sx = sx + s.cm[0][k];
+                sy = sy + s.cm[1][k];
+                sz = sz + s.cm[2][k];
+            }
             float const inv = 1.0f / static_cast<float>(total);
             mx = sx * inv; my = sy * inv; mz = sz * inv;
             if (j == 0) {
@@ -1558,40 +1569,51 @@ __device__ __forceinline__ void collide_tile(Sys const& sys, const float* __rest
         uint32_t const ncell = ke - kb;
         for (uint32_t i = tid; i < m; i += NT) s.perm[i] = psrc[gb + i];
         if constexpr (ALGO == 0) {
-            // per-cell draws: axis, and for wall cells the ghost count and a
-            // slot in the staging region (simulation_functions.cpp:137-146)
-            if (tid == 0) s.nwall = 0u;
-            __syncthreads();
+            // per-cell draws while the permutation loads travel: the axis
+            // (drawn here, or from `axis` for steps that keep their axes in
+            // memory), and for wall cells the ghost count and the sum of the
+            // ghosts' velocities (simulation_functions.cpp:137-171), which
+            // waits in the cell's mean slots until the per-cell stage adds it
             bool const sign = P.shift[2] > 0;
             uint32_t const XY = P.dim[0] * P.dim[1];
             for (uint32_t k = tid; k < ncell; k += NT) {
                 uint32_t const key = c0 + kb + k;
-                uint64_t const gk = P.cb + key;
-                s.cm[3][k] = axis[gk];
-                s.cm[4][k] = axis[CT + gk];
-                s.cm[5][k] = axis[2 * CT + gk];
                 uint32_t ix, iy, iz;
                 decode_key(P, key, ix, iy, iz);
+                if (axis) {
+                    uint64_t const gk = P.cb + key;
+                    s.cm[3][k] = axis[gk];
+                    s.cm[4][k] = axis[CT + gk];
+                    s.cm[5][k] = axis[2 * CT + gk];
+                } else {
+                    float ax, ay, az;
+                    srd_axis(P, ref_of(P, ix, iy, iz), ax, ay, az);
+                    s.cm[3][k] = ax;
+                    s.cm[4][k] = ay;
+                    s.cm[5][k] = az;
+                }
                 float const zf = static_cast<float>(iz);
                 bool const layer = (zf == (P.L[2] - (sign ? 1.0f : 2.0f)));
                 bool const ghosts = (!P.perio_z) && ((zf == static_cast<float>(sign)) || layer);
-                uint32_t gc = 0u, go = ~0u;
+                uint32_t gc = 0u;
+                float gx = 0.0f, gy = 0.0f, gz = 0.0f;
                 if (ghosts) {
                     uint32_t const w = (layer ? XY : 0u) + ix + P.dim[0] * iy;
-                    s.cref[k] = w;
                     gc = gcnt[P.wb + w];
-                    uint32_t const slot = atomicAdd(&s.nwall, 1u);
-                    if ((slot + 1) * 3 * P.n <= static_cast<uint32_t>(s.GS)) {
-                        go = slot * 3 * P.n;
-                        s.wcell[slot] = k;
+                    if (gc) {
+                        float const* const g = gvel + P.gvb + 3ull * w;
+                        gx = g[0];
+                        gy = g[1];
+                        gz = g[2];
                     }
                 }
                 s.gcnt[k] = gc;
-                s.goff[k] = go;
+                s.cm[0][k] = gx;
+                s.cm[1][k] = gy;
+                s.cm[2][k] = gz;
             }
-        } else {
-            __syncthreads();
         }
+        __syncthreads();
         // tile-local cell of every slot, four lanes per cell
         for (uint32_t t = tid; t < 4 * ncell; t += NT) {
             uint32_t const k = t / 4;
@@ -1599,26 +1621,6 @@ __device__ __forceinline__ void collide_tile(Sys const& sys, const float* __rest
             for (uint32_t i = a + (t & 3u); i < e; i += 4) s.cell[i] = static_cast<uint8_t>(k);
         }
         __syncthreads();
-        // SRD: the staged wall cells' ghost velocities are loaded into registers
-        // now, so their latency overlaps the gather's, and land in LDS below
-        constexpr int GPT = (Tile<CAP, ALGO>::GS + NT - 1) / NT;
-        float gstage[ALGO == 0 ? GPT : 1];
-        uint32_t nstaged = 0;
-        if constexpr (ALGO == 0) {
-            uint32_t const n3 = 3 * P.n;
-            uint32_t const nslot = min(s.nwall, static_cast<uint32_t>(s.GS) / n3);
-            nstaged = nslot * n3;
-#pragma unroll
-            for (int q = 0; q < GPT; ++q) {
-                uint32_t const t = tid + q * NT;
-                gstage[q] = 0.0f;
-                if (t < nstaged) {
-                    uint32_t const slot = t / n3, f = t - slot * n3;
-                    uint32_t const k = s.wcell[slot];
-                    if (f < 3 * s.gcnt[k]) gstage[q] = gvel[P.gvb + static_cast<uint64_t>(n3) * s.cref[k] + f];
-                }
-            }
-        }
         // gather this thread's particles, GQ at a time with all their loads in
         // flight together, then order each cell by source slot and re-apply
         // this step's streaming
@@ -1670,16 +1672,6 @@ __device__ __forceinline__ void collide_tile(Sys const& sys, const float* __rest
                 }
             }
         }
-        if constexpr (ALGO == 0) {
-#pragma unroll
-            for (int q = 0; q < GPT; ++q) {
-                uint32_t const t = tid + q * NT;
-                if (t < nstaged) s.g[0][t] = gstage[q];
-            }
-        } else {
-            (void)nstaged;
-            (void)gstage;
-        }
         __syncthreads();
         // next-step histogram bins (in the permutation scratch, free from here
         // on for SRD; the extended rule keeps its draws there until after its
@@ -1699,7 +1691,7 @@ __device__ __forceinline__ void collide_tile(Sys const& sys, const float* __rest
         // per-cell stage
         if constexpr (ALGO == 0) {
             clear_bins();
-            srd_cells<CAP, ADJ>(P, s, ncell, s.off + kb, gb, gvel, c0 + kb, T, CT);
+            srd_cells<CAP, ADJ>(P, s, ncell, s.off + kb, gb, c0 + kb, T, CT);
         } else {
             for (uint32_t k = tid; k < ncell; k += NT) {
                 ext_cell_setup<CAP>(P, s, k, s.off[kb + k + 1] - s.off[kb + k], c0 + kb + k, gcnt);
@@ -2395,13 +2387,13 @@ __global__ void k_selftest_math(const float* in, float* out, size_t n, int which
 
 // Scratch of the allocation-free step (mpcd_hip_step_noalloc): the step's
 // cell-sort and per-cell arrays for a system of up to NOALLOC_MAX_PARTICLES
-// particles and NOALLOC_MAX_CELLS cells, 15.5 MiB per device, held from the
+// particles and NOALLOC_MAX_CELLS cells, 14 MiB per device, held from the
 // library's load until the process ends.  In host code `scratch` names a
 // host-side stand-in; its device address comes from hipGetSymbolAddress, once
 // per device (noalloc_scratch below).
 constexpr uint64_t NOALLOC_MAX_PARTICLES = uint64_t(1) << 20;
 constexpr uint64_t NOALLOC_MAX_CELLS = uint64_t(1) << 17;
-constexpr uint64_t NOALLOC_MAX_GHOST_FLOATS = uint64_t(1) << 20;   // wall ghosts: (3 SRD, 6 extended) n 2 X Y
+constexpr uint64_t NOALLOC_MAX_GHOST_FLOATS = uint64_t(1) << 20;   // wall ghosts: (3 SRD, 6 n extended) 2 X Y
 constexpr uint64_t noalloc_align(uint64_t b) { return (b + 255) / 256 * 256; }
 // byte offsets of the regions
 constexpr uint64_t NA_KP = 0;
@@ -2413,11 +2405,10 @@ constexpr uint64_t NA_OFFSET = NA_COUNT + noalloc_align(4 * NOALLOC_MAX_CELLS);
 constexpr uint64_t NA_CURSOR = NA_OFFSET + noalloc_align(4 * (NOALLOC_MAX_CELLS + 1));
 constexpr uint64_t NA_PARTIAL = NA_CURSOR + noalloc_align(4 * NOALLOC_MAX_CELLS);
 constexpr uint64_t NA_STATUS = NA_PARTIAL + noalloc_align(4 * (NOALLOC_MAX_CELLS / mpcd_hip::SCAN_CHUNK + 1));
-constexpr uint64_t NA_AXIS = NA_STATUS + noalloc_align(4 * (2 + 2 * (NOALLOC_MAX_PARTICLES / 257 + 2)));
-constexpr uint64_t NA_GCNT = NA_AXIS + noalloc_align(12 * NOALLOC_MAX_CELLS);
+constexpr uint64_t NA_GCNT = NA_STATUS + noalloc_align(4 * (2 + 2 * (NOALLOC_MAX_PARTICLES / 257 + 2)));
 constexpr uint64_t NA_GVEL = NA_GCNT + noalloc_align(4 * NOALLOC_MAX_CELLS);
 constexpr uint64_t NOALLOC_SCRATCH_BYTES = NA_GVEL + noalloc_align(4 * NOALLOC_MAX_GHOST_FLOATS);
-static_assert(NA_STATUS + 4 * mpcd_hip::big_list_words(NOALLOC_MAX_PARTICLES) <= NA_AXIS, "big-cell list fits");
+static_assert(NA_STATUS + 4 * mpcd_hip::big_list_words(NOALLOC_MAX_PARTICLES) <= NA_GCNT, "big-cell list fits");
 
 static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
 
@@ -2463,12 +2454,16 @@ struct mpcd_hip_s {
     uint32_t* cmax = nullptr;        // largest per-cell count (k_count_max)
     uint32_t* perm_rec = nullptr;
     bool record = false, recorded = false;
-    float* axis = nullptr;           // SRD per-cell axes of the current step
+    float* axis = nullptr;           // SRD per-cell axes of the current step, when kept in memory
+    // the step's SRD axes go through `axis` (drawn by k_cell_draws or given by
+    // the caller) instead of being drawn in the collide tiles: taped adjoint
+    // steps (their reverse pass reads them) and injected steps
+    bool axes_mem = false;
     uint32_t* wgcnt = nullptr;       // wall-cell ghost counts of the step
-    float* wgvel = nullptr;          // wall-cell ghosts of the step: velocities (SRD), positions and velocities (extended)
+    float* wgvel = nullptr;          // wall-cell ghosts of the step: velocity sums (SRD), positions and velocities (extended)
     // draws of the next step given by the caller (mpcd_hip_step_injected):
     // axes by internal cell key (3 planes), wall-cell ghost counts and
-    // velocities in the layout of wgcnt / wgvel; null: the step draws its own
+    // ghosts in the layout of wgcnt / wgvel; null: the step draws its own
     struct Injected {
         float shift[3];
         std::vector<float> axes;
@@ -2643,7 +2638,7 @@ uint32_t pick_capacity(int algorithm, uint32_t nmax, uint32_t cap, uint64_t cell
     bool const sparse_large = nmax < 16 && cells >= (uint64_t(1) << 21);
     if (cap == 0) cap = std::max(algorithm == 0 && !sparse_large ? 384u : 512u, 6 * nmax);
     for (uint32_t c : TILE_CAPS)
-        if (c >= cap && (algorithm == 0 || (c >= 512 && static_cast<uint32_t>(ghost_floats(static_cast<int>(c), 1)) >= nmax)))
+        if (c >= cap && (algorithm == 0 || (c >= 512 && static_cast<uint32_t>(ghost_slots(static_cast<int>(c))) >= nmax)))
             return c;
     return TILE_CAP_MAX;
 }
@@ -2732,7 +2727,7 @@ void layout(mpcd_hip_s* h) {
         cb += k.C;
         pb += k.N;
         wb += walls;
-        gvb += (h->algorithm == 1 ? 6ull : 3ull) * k.n * walls;   // floats per ghost slot
+        gvb += (h->algorithm == 1 ? 6ull * k.n : 3ull) * walls;   // floats per wall cell
     }
     // collide tiles: the systems of each compiled capacity together (one
     // launch per capacity), bt[s] = first tile of system s, bt[ns] = all tiles
@@ -2809,9 +2804,12 @@ int guarded(mpcd_hip_s* h, F&& f) {
     return 2;
 }
 
-// blocks of the step's per-cell draws: wall ghosts, and the SRD axes (the
-// extended rule draws its axes in the collision)
-uint32_t draw_blocks(mpcd_hip_s const* h) { return h->blocks_wall + (h->algorithm == 0 ? h->blocks_cell : 0u); }
+// blocks of the step's per-cell draws: wall ghosts, and the SRD axes of a
+// step that keeps them in memory (other steps, and the extended rule, draw
+// their axes in the collision)
+uint32_t draw_blocks(mpcd_hip_s const* h) {
+    return h->blocks_wall + (h->algorithm == 0 && h->axes_mem ? h->blocks_cell : 0u);
+}
 
 // the per-cell draws of the step in the system table alone (k_cell_draws),
 // for a step that needs no sort
@@ -2868,11 +2866,12 @@ uint32_t perm_blocks(mpcd_hip_s const* h) {
 }
 
 // The per-cell draws ride in the permutation's launch when they are few (a
-// launch less per step: 64x64x52 cells, n = 10: 16.8 us against 12.7 + 11.5);
-// many of them run better on their own, at the occupancy of their small LDS
-// footprint rather than the permutation's (1000x1000x22 cells: 1.49 ms
-// against 1.10 + 0.34)
-constexpr uint32_t DRAWS_FOLD_MAX = 8192;   // draw blocks
+// launch less per step: 64x64x52 cells, n = 10, 16.8 us against 12.7 + 11.5
+// with the axes drawn there); many of them run better on their own, at the
+// occupancy of their small LDS footprint rather than the permutation's
+// (1000x1000x22 cells, SRD wall ghosts alone, 7813 blocks: 1.18 ms against
+// 1.06 + 0.03; the 64-channel ensemble 12.97 against 12.58 ms per step)
+constexpr uint32_t DRAWS_FOLD_MAX = 1024;   // draw blocks
 bool fold_draws(mpcd_hip_s const* h) { return draw_blocks(h) <= DRAWS_FOLD_MAX; }
 
 // the sort of the step (scan and permutation); with `draws` the step's
@@ -2922,7 +2921,7 @@ void launch_collide_t(mpcd_hip_s* h, uint32_t* rec, Tape const& T, uint32_t io, 
     if (!grid) return;
     hipLaunchKernelGGL((k_collide<ALGO, SAMPLE, CAP, ADJ, BIG>), dim3(grid), dim3(tile_threads(CAP)), 0, h->stream,
                        h->sys, h->st[h->cur], h->st[1 - h->cur], h->perm, h->offset, h->count, h->keys, rec, S,
-                       h->status, h->axis, h->wgcnt, h->wgvel, T, io, t0);
+                       h->status, h->axes_mem ? h->axis : nullptr, h->wgcnt, h->wgvel, T, io, t0);
 }
 
 template <int ALGO, bool SAMPLE, bool BIG>
@@ -3265,6 +3264,7 @@ void adj_step(mpcd_hip_s* h, float const* force, int j, uint32_t* cnt, unsigned 
     uint16_t* lz = nullptr;
     uint32_t* const own_offset = h->offset;
     float* const own_axis = h->axis;
+    h->axes_mem = true;
     if (j >= 0) {
         T.vp = a.vp + uint64_t(j) * 3 * N;
         T.u = a.u + uint64_t(j) * 3 * C;
@@ -3292,10 +3292,12 @@ void adj_step(mpcd_hip_s* h, float const* force, int j, uint32_t* cnt, unsigned 
     } catch (...) {
         h->offset = own_offset;
         h->axis = own_axis;
+        h->axes_mem = false;
         throw;
     }
     h->offset = own_offset;
     h->axis = own_axis;
+    h->axes_mem = false;
     h->cur = 1 - h->cur;
     h->step_index++;
     h->keys_valid = true;
@@ -3655,10 +3657,11 @@ void step_noalloc_impl(mpcd_params_t const& prm, uint64_t step_index, int n_step
         throw HipError{"the allocation-free step takes at most " + std::to_string(NOALLOC_MAX_PARTICLES) +
                        " particles and " + std::to_string(NOALLOC_MAX_CELLS) + " cells (got " + std::to_string(prm.N) +
                        " and " + std::to_string(C) + ")"};
-    uint64_t const per_ghost = prm.algorithm == 1 ? 6 : 3;   // floats per ghost slot
-    if (per_ghost * prm.n * walls > NOALLOC_MAX_GHOST_FLOATS)
-        throw HipError{"the allocation-free step holds at most " + std::to_string(NOALLOC_MAX_GHOST_FLOATS / per_ghost) +
-                       " wall-ghost slots (n times 2 X Y)"};
+    // floats per wall cell: the SRD ghosts' velocity sum, or n extended ghosts
+    uint64_t const per_wall = prm.algorithm == 1 ? 6ull * prm.n : 3ull;
+    if (per_wall * walls > NOALLOC_MAX_GHOST_FLOATS)
+        throw HipError{"the allocation-free step holds the ghosts of at most " +
+                       std::to_string(NOALLOC_MAX_GHOST_FLOATS / per_wall) + " wall-layer cells (2 X Y) at this n"};
     int ndev = 0;
     ck(hipGetDeviceCount(&ndev), "hipGetDeviceCount");
     if (static_cast<int>(prm.device_id) >= ndev) throw HipError{"device_id out of range"};
@@ -3685,7 +3688,7 @@ void step_noalloc_impl(mpcd_params_t const& prm, uint64_t step_index, int n_step
     g.N = prm.N;
     g.C = C;
     g.W = walls;
-    g.GV = per_ghost * prm.n * walls;
+    g.GV = per_wall * walls;
     g.stream = nullptr;
     g.lds_cap = cap;
     g.grp[0] = mpcd_hip_s::Group{cap, 0u, tiles, 0u};
@@ -3702,7 +3705,7 @@ void step_noalloc_impl(mpcd_params_t const& prm, uint64_t step_index, int n_step
     g.cursor = reinterpret_cast<uint32_t*>(base + NA_CURSOR);
     g.partial = reinterpret_cast<uint32_t*>(base + NA_PARTIAL);
     g.status = reinterpret_cast<uint32_t*>(base + NA_STATUS);
-    g.axis = reinterpret_cast<float*>(base + NA_AXIS);
+    g.axis = nullptr;   // the collide tiles draw the axes
     g.wgcnt = reinterpret_cast<uint32_t*>(base + NA_GCNT);
     g.wgvel = reinterpret_cast<float*>(base + NA_GVEL);
     g.d_kp = reinterpret_cast<KParams*>(base + NA_KP);
@@ -3801,22 +3804,31 @@ int mpcd_hip_step_injected(mpcd_hip_t* h, const float* shift, const float* axes,
             d.gcnt.assign(ghost_counts, ghost_counts + 2 * XY);
             for (uint32_t g : d.gcnt)
                 if (g > k.n) throw HipError{"injected step: a ghost count above n"};
-            d.gvel.assign(ghost_velocities, ghost_velocities + 2 * XY * 3 * k.n);
+            // the SRD collision takes a wall cell's ghosts as the sum of their
+            // velocities: summed here in ghost order
+            d.gvel.assign(3 * 2 * XY, 0.0f);
+            for (uint64_t w = 0; w < 2 * XY; ++w)
+                for (uint32_t g = 0; g < d.gcnt[w]; ++g)
+                    for (int a = 0; a < 3; ++a) d.gvel[3 * w + a] += ghost_velocities[(w * k.n + g) * 3 + a];
         }
         // the step's state with the injected shift, its cells counted from scratch
         for (int a = 0; a < 3; ++a) d.shift[a] = shift[a];
         h->inject = &d;
+        h->axes_mem = true;
         h->keys_valid = false;
         h->sorted_valid = false;
         try {
             upload_table(h);
             run_steps(h, 1, false);
         } catch (...) {
+            (void)hipStreamSynchronize(h->stream);
             h->inject = nullptr;
+            h->axes_mem = false;
             upload_table(h);
             throw;
         }
         h->inject = nullptr;
+        h->axes_mem = false;
         upload_table(h);   // the host table follows the device's (the next step's shifts)
     });
 }

@@ -246,15 +246,16 @@ int mpcd_hip_tune_capacities(int algorithm, uint32_t nmax, uint32_t* caps, uint3
  * bit for bit as a handle created from the same parameters, holding the same
  * state at the same step index.  It calls no hipMalloc or hipFree: the step's
  * sort and per-cell arrays live in a static device array of the library
- * (15.5 MiB per device, held until the process ends).
+ * (14 MiB per device, held until the process ends).
  *
  * state_in / state_out: device buffers of n_particles records (x, y, z, vx,
  * vy, vz as 6 floats, 24 bytes per particle), of the HIP runtime this library
  * uses; the result is written to state_out in cell order (as a handle keeps
  * it); state_in is overwritten when n_steps > 1.  step_index keys the random
  * stream (mpcd_hip_get_step_index).  Limits: at most 2^20 particles and 2^17
- * cells, and with walls n * 2 X Y at most 2^20 / 3 (SRD) or 2^20 / 6 (extended); larger
- * systems are rejected with an error before any device work.  Runs on the
+ * cells, and with walls 2 X Y at most 2^20 / 3 (SRD) or n * 2 X Y at most
+ * 2^20 / 6 (extended); larger systems are rejected with an error before any
+ * device work.  Runs on the
  * null stream and returns after the device finished; calls from one process
  * are serialised, each call leaves the array holding its own data and clears
  * what the next needs cleared.  Errors: nonzero return, message in

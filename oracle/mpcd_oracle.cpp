@@ -283,20 +283,26 @@ struct Draws {
         box_muller(r.v[2], r.v[3], &g2, &g3);
         return {g0, g1, g2};
     }
+    // mode 0: Marsaglia's rejection (random.hpp:237-253); mode 1: the same
+    // uniform distribution on the sphere from one counter draw, z uniform on
+    // [-1, 1] and the azimuth uniform on [0, 2 pi) (Archimedes)
     V3 srd_axis(uint32_t step, uint32_t cell) {
         if (mode == 0) return xo.unit_vector();
-        for (uint32_t a = 0;; ++a) {
-            U4 r = ctr(P_AXIS, a, cell, step);
-            for (int h = 0; h < 2; ++h) {
-                float rd1 = 1.0f - 2.0f * uf(r.v[2 * h]);
-                float rd2 = 1.0f - 2.0f * uf(r.v[2 * h + 1]);
-                float rsq = rd1 * rd1 + rd2 * rd2;
-                if (!(rsq > 1.0f)) {
-                    float rdh = 2.0f * sqrtf(1.0f - rsq);
-                    return {rd1 * rdh, rd2 * rdh, (1.0f - 2.0f * rsq)};
-                }
-            }
-        }
+        U4 r = ctr(P_AXIS, 0, cell, step);
+        float const z = 1.0f - 2.0f * uf(r.v[0]);
+        float s, c;
+        p_sincospi(2.0f * uf(r.v[1]), &s, &c);
+        float const rho = sqrtf((1.0f - z) * (1.0f + z));
+        return {rho * c, rho * s, z};
+    }
+    // mode 1: the standard normal triple whose multiple is the sum of a SRD
+    // wall cell's ghost velocities (Oracle::srd_ghost_sum)
+    V3 ghost_vel_sum(uint32_t step, uint32_t cell) {
+        U4 r = ctr(P_GHOST_VEL, 0, cell, step);
+        float g0, g1, g2, g3;
+        box_muller(r.v[0], r.v[1], &g0, &g1);
+        box_muller(r.v[2], r.v[3], &g2, &g3);
+        return {g0, g1, g2};
     }
     // extended: (discrete direction 0..12, sign bit) then gate uniform
     void ext_axis(uint32_t step, uint32_t cell, int* select, int* flip) {
@@ -556,6 +562,17 @@ struct Oracle {
         *ghosts = (!p.periodic[2]) && ((zidx(c) == static_cast<float>(sign)) || *layer);
     }
 
+    // Counter stream, SRD rule: the sum of a wall cell's `added` ghost
+    // velocities (each Maxwell-Boltzmann at the thermal velocity,
+    // simulation_functions.cpp:165) drawn at once, as a sum of independent
+    // Gaussians is one Gaussian of the summed variance: sqrt(added) times the
+    // thermal velocity times a standard normal triple.  The SRD collision uses
+    // ghosts only through this sum (the cell mean; getCorrection returns the
+    // mean, mpc_cell.hpp:82-84).
+    V3 srd_ghost_sum(uint32_t step, uint32_t c, uint32_t added) {
+        return mul(rng.ghost_vel_sum(step, c), p.thermal_velocity * sqrtf(static_cast<float>(added)));
+    }
+
     // simulation_functions.cpp:108-202
     void srd() {
         float const sh = fabsf(shift.z);
@@ -565,6 +582,7 @@ struct Oracle {
         std::vector<V3> bp(cap + p.n + 1), bv(cap + p.n + 1);
         std::vector<int64_t> bidx(cap + p.n + 1);
         uint32_t const G = p.sum_order ? group_lanes(p.n) : 1u;
+        bool const cstream = rng.mode == 1;   // counter stream
         if (log_draws) {
             log_shift = shift;
             log_axes.assign(3 * static_cast<size_t>(C), 0.0f);
@@ -587,7 +605,12 @@ struct Oracle {
                 bp[i] = parts[bidx[i]].pos;
                 bv[i] = parts[bidx[i]].vel;
             }
-            if (ghosts) {
+            // the counter stream keeps only the ghosts' velocity sum (they
+            // enter the collision through the cell mean alone); the reference
+            // stream draws every ghost, in the reference's order
+            V3 const gsum = cstream && added ? srd_ghost_sum(step, c, added) : V3{0, 0, 0};
+            uint32_t const members = cstream ? m : total;   // held in the buffers
+            if (ghosts && !cstream) {
                 V3 centre = cell_centre(c);
                 for (uint32_t i = m; i < total; ++i) {
                     V3 g = rng.ghost_pos(step, c, i - m, sh, layer, sign);
@@ -602,22 +625,23 @@ struct Oracle {
             }
             if (total > 1) {
                 std::vector<V3> part(G, V3{0, 0, 0});
-                for (uint32_t i = 0; i < total; ++i) part[i % G] = add(part[i % G], bv[i]);
-                V3 const sum = pairwise(part, add);
+                for (uint32_t i = 0; i < members; ++i) part[i % G] = add(part[i % G], bv[i]);
+                V3 sum = pairwise(part, add);
+                if (cstream && added) sum = add(sum, gsum);
                 float inv = 1.0f / static_cast<float>(total);
                 V3 mean = mul(sum, inv);
                 V3 axis = rng.srd_axis(step, c);
                 if (log_draws) {
                     log_axes[3 * c] = axis.x; log_axes[3 * c + 1] = axis.y; log_axes[3 * c + 2] = axis.z;
                 }
-                for (uint32_t i = 0; i < total; ++i) {
+                for (uint32_t i = 0; i < members; ++i) {
                     V3 v = sub(bv[i], mean);
                     V3 vpar = mul(axis, dot(v, axis));
                     V3 vperp = sub(v, vpar);
                     bv[i] = add(add(vpar, mul(vperp, cos_a)), mul(cross(vperp, axis), sin_a));
                 }
                 float const fdt = force.empty() ? 0.0f : force[c] * p.delta_t;
-                for (uint32_t i = 0; i < total; ++i) {
+                for (uint32_t i = 0; i < members; ++i) {
                     bv[i] = add(bv[i], mean);
                     bp[i] = wrap(sub(bp[i], shift));
                     if (force.empty()) bv[i].x += drag; else bv[i].x += drag + fdt;
@@ -954,8 +978,8 @@ struct Oracle {
                     size_t const q = list[c + static_cast<size_t>(i) * C];
                     for (int k = 0; k < 3; ++k) u[k] += sv[3 * q + k];
                 }
-                for (uint32_t i = 0; i < added; ++i) {
-                    V3 const gv = mul(rng.ghost_vel(step, c, i), p.thermal_velocity);
+                if (added) {
+                    V3 const gv = srd_ghost_sum(step, c, added);
                     u[0] += gv.x;
                     u[1] += gv.y;
                     u[2] += gv.z;
@@ -1148,6 +1172,26 @@ void orc_counter_draws(void* h, int purpose, unsigned step, unsigned cell, unsig
             oracle::V3 a = o->rng.srd_axis(step, cell + i);
             out[3 * i] = a.x; out[3 * i + 1] = a.y; out[3 * i + 2] = a.z;
         }
+    }
+}
+// what the SRD rule adds to a wall cell's velocity sum for `added` ghosts, for
+// `count` cells from `cell`: mode 1 the counter stream's one-draw sum, mode 0
+// the sum of `added` thermal velocities drawn one by one from the reference
+// stream (its Maxwell-Boltzmann triples times the thermal velocity, summed in
+// draw order), the reference's construction
+void orc_srd_ghost_sums(void* h, unsigned step, unsigned cell, unsigned count, unsigned added, float* out3) {
+    auto* o = static_cast<Oracle*>(h);
+    for (unsigned i = 0; i < count; ++i) {
+        oracle::V3 g{0, 0, 0};
+        if (o->rng.mode == 1) {
+            g = o->srd_ghost_sum(step, cell + i, added);
+        } else {
+            for (unsigned k = 0; k < added; ++k) g = oracle::add(g, oracle::mul(o->rng.ghost_vel(step, cell + i, k),
+                                                                          o->p.thermal_velocity));
+        }
+        out3[3 * i] = g.x;
+        out3[3 * i + 1] = g.y;
+        out3[3 * i + 2] = g.z;
     }
 }
 // `count` wall-ghost positions of one cell (x, y, z in [0, 1] each, before the

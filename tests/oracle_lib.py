@@ -106,6 +106,8 @@ def oracle_lib() -> ctypes.CDLL:
         lib.orc_ext_direction.argtypes = [ctypes.c_int, c_float_p]
         lib.orc_ghost_pos_draws.argtypes = [ctypes.c_void_p, ctypes.c_uint, ctypes.c_uint, ctypes.c_uint,
                                             ctypes.c_float, ctypes.c_int, ctypes.c_int, c_float_p]
+        lib.orc_srd_ghost_sums.argtypes = [ctypes.c_void_p, ctypes.c_uint, ctypes.c_uint, ctypes.c_uint, ctypes.c_uint,
+                                           c_float_p]
         c_int_p = ctypes.POINTER(ctypes.c_int)
         lib.orc_ext_axis_draws.argtypes = [ctypes.c_void_p, ctypes.c_uint, ctypes.c_uint, ctypes.c_uint, c_int_p,
                                            c_int_p, c_float_p]
@@ -242,6 +244,15 @@ class Oracle:
         out = np.empty((count, 3), np.float32)
         self.lib.orc_ghost_pos_draws(self.h, step, cell, count, float(shift), int(layer), int(sign),
                                      _ptr(out, c_float_p))
+        return out
+
+    def srd_ghost_sums(self, step, cell0, count, added):
+        """What the SRD rule adds to the velocity sum of wall cells [cell0,
+        cell0 + count) holding `added` ghosts each: counter mode the one-draw
+        sum, reference mode `added` thermal velocities drawn one by one and
+        summed (the reference's construction)."""
+        out = np.empty((count, 3), np.float32)
+        self.lib.orc_srd_ghost_sums(self.h, step, cell0, count, added, _ptr(out, c_float_p))
         return out
 
     def ext_axis_draws(self, step, cell0, count):

@@ -128,25 +128,40 @@ def test_checkpoint_interval_does_not_change_the_gradient(pympcd):
         assert np.array_equal(o[3], out[0][3])
 
 
-def test_gradient_against_full_model_finite_differences(pympcd):
+@pytest.mark.parametrize("seeds", [(5, 6, 7)])
+def test_gradient_against_full_model_finite_differences(pympcd, seeds):
     """Central differences of the GPU forward itself, common random numbers.
 
     Under the perturbation particles near cell faces change cells, which the
-    pathwise gradient leaves out by construction (that part is the
-    reassignment term, not rounding); measured at 16x16x20 cells, n = 10, 20
-    steps it is about 7% of dL/dg, so the check is agreement within 15% with
-    the same sign.  The angle derivative of this loss is too small at 20
-    steps for a relative comparison; the frozen-model test covers it."""
-    p, sim, target = _setup(pympcd, fluid=(16, 16, 20), n=10, seed=5, warm=20)
-    steps = 20
-    r = adjoint.profile_loss_grad(sim, steps, 7, p.drag, p.rotation_angle, target, want_force_grad=False)
-    hg = 0.05 * p.drag
-    lp = adjoint.profile_loss(sim, steps, p.drag + hg, p.rotation_angle, target).loss
-    lm = adjoint.profile_loss(sim, steps, p.drag - hg, p.rotation_angle, target).loss
-    fd_g = (lp - lm) / (2 * hg)
-    print(f"full model: dL/dg adjoint {r.grad_drag:.6e} fd {fd_g:.6e} ({r.grad_drag / fd_g - 1:+.2e})")
-    assert np.sign(r.grad_drag) == np.sign(fd_g)
-    assert abs(r.grad_drag - fd_g) <= 0.15 * abs(fd_g)
+    pathwise gradient leaves out by construction (the reassignment term).
+    Near the loss minimum the gradient is small and that term dominates a
+    single realisation (16x16x20 cells, n = 10, 20 steps, target between
+    -0.02 and 0.03: adjoint against differences -74% to +38% over six
+    seeds), so the target here lies away from the flow (0.05 in every
+    layer): the gradient is then carried by the momentum input g dt, exact in
+    the pathwise gradient, and the reassignment term is a correction (six
+    seeds: -9% to +7%, about -5% on average).  The loss is quadratic in g
+    (the profile is linear in it), so the central difference is exact at any
+    step: a step of g itself averages over many reassignments, where 5% of g
+    left differences scattered by up to 44%.  Checked: each seed within 15%
+    with the same sign, the seed mean within 10%."""
+    adj, fds = [], []
+    for seed in seeds:
+        p, sim, _ = _setup(pympcd, fluid=(16, 16, 20), n=10, seed=seed, warm=20)
+        steps = 20
+        Z = stored_volume(p)[2]
+        target = np.full(Z - 2, 0.05)
+        r = adjoint.profile_loss_grad(sim, steps, 7, p.drag, p.rotation_angle, target, want_force_grad=False)
+        hg = p.drag
+        lp = adjoint.profile_loss(sim, steps, p.drag + hg, p.rotation_angle, target).loss
+        lm = adjoint.profile_loss(sim, steps, p.drag - hg, p.rotation_angle, target).loss
+        fd_g = (lp - lm) / (2 * hg)
+        print(f"seed {seed}: dL/dg adjoint {r.grad_drag:.6e} fd {fd_g:.6e} ({r.grad_drag / fd_g - 1:+.2e})")
+        assert np.sign(r.grad_drag) == np.sign(fd_g)
+        assert abs(r.grad_drag - fd_g) <= 0.15 * abs(fd_g)
+        adj.append(r.grad_drag)
+        fds.append(fd_g)
+    assert abs(np.mean(adj) - np.mean(fds)) <= 0.10 * abs(np.mean(fds))
 
 
 def test_torch_autograd_function(pympcd):

@@ -105,6 +105,9 @@ def test_extended_step_with_reference_draws_matches_reference(pympcd, fluid, per
         orc.step(1)
         ref.step(1)
         shift, ext, gauss, gcnt, ghosts = orc.last_ext_draws()
+        # the reference keeps a cell's sides in one 64-bit word: its behaviour
+        # is defined for cells of at most 64 members (table and ghosts)
+        assert (np.count_nonzero(gauss, axis=1) <= 64).all()
         collided += int(np.count_nonzero(gauss))
         gated += int(np.count_nonzero(ext[:, 2]))
         if walls:

@@ -223,12 +223,51 @@ def test_reference_xoshiro_statistics():
 
 
 def test_counter_axes_are_uniform_unit_vectors():
+    """Uniform on the sphere, as the reference's Marsaglia draw
+    (random.hpp:237-253): each Cartesian component of a uniform unit vector is
+    uniform on [-1, 1] (Archimedes), and the azimuth about any axis is uniform
+    and independent of that component; a dropped square root, a wrong scale of
+    the azimuth or a swapped component fails one of these."""
     o = Oracle((4, 4, 4), (1, 1, 1), n=1, rng="counter", seed=(99, 0), distribute=False)
     ax = np.concatenate([o.draws(5, s, 0, 5000) for s in range(4)]).astype(np.float64)
     assert np.abs(np.linalg.norm(ax, axis=1) - 1).max() < 1e-6
     assert np.abs(ax.mean(axis=0)).max() < 0.02
     # isotropy: <a_i a_j> = delta_ij / 3
     assert np.abs(ax.T @ ax / len(ax) - np.eye(3) / 3).max() < 0.02
+    for k in range(3):
+        assert stats.kstest(ax[:, k], stats.uniform(-1, 2).cdf).pvalue > 0.01
+        i, j = [a for a in range(3) if a != k]
+        phi = np.arctan2(ax[:, j], ax[:, i])
+        assert stats.kstest(phi, stats.uniform(-np.pi, 2 * np.pi).cdf).pvalue > 0.01
+        # independence of the azimuth and the component (|component| split at 1/2)
+        lo, hi = phi[np.abs(ax[:, k]) < 0.5], phi[np.abs(ax[:, k]) >= 0.5]
+        assert stats.ks_2samp(lo, hi).pvalue > 0.01
+
+
+@pytest.mark.parametrize("added", [1, 3, 10, 20])
+def test_counter_srd_ghost_sum_matches_the_reference_construction(added):
+    """The SRD rule uses a wall cell's ghosts only through the sum of their
+    velocities (simulation_functions.cpp:168-171; getCorrection returns the
+    mean, mpc_cell.hpp:82-84).  The counter stream draws that sum at once;
+    the reference draws `added` Maxwell-Boltzmann velocities at the thermal
+    velocity and sums them.  Both must be N(0, added T) per component,
+    independent: each component is compared with the reference stream's
+    construction (two-sample KS) and with the closed form (KS against the
+    normal of variance added T); a missing square root, a missing thermal
+    velocity or correlated components fails."""
+    T = 0.7
+    kw = dict(n=20, thermal_velocity=math.sqrt(T), distribute=False)
+    oc = Oracle((4, 4, 4), (1, 1, 0), rng="counter", seed=(5, 6), **kw)
+    orf = Oracle((4, 4, 4), (1, 1, 0), rng="reference", seed=(5, 6), **kw)
+    cnt = 6000
+    c = np.concatenate([oc.srd_ghost_sums(s, 0, cnt // 3, added) for s in range(3)]).astype(np.float64)
+    r = orf.srd_ghost_sums(0, 0, cnt, added).astype(np.float64)
+    sd = math.sqrt(added * T)
+    for k in range(3):
+        assert stats.ks_2samp(c[:, k], r[:, k]).pvalue > 0.01
+        assert stats.kstest(c[:, k], stats.norm(0, sd).cdf).pvalue > 0.01
+    cov = np.cov(c.T) / (added * T)
+    assert np.abs(cov - np.eye(3)).max() < 0.08
 
 
 # ------------------------------------------------- 3. closed forms / invariants
