@@ -1162,9 +1162,10 @@ __device__ __forceinline__ bool ghost_layer(KParams const& P, uint32_t iz, bool&
     return (!P.perio_z) && ((zf == static_cast<float>(sign)) || layer);
 }
 
+// returns the cell's thermal quads: (members + 3) / 4 when it collides, else 0
 template <int CAP>
-__device__ void ext_cell_setup(KParams const& P, Tile<CAP, 1>& s, uint32_t k, uint32_t mc, uint32_t key,
-                               uint32_t const* __restrict__ gcnt) {
+__device__ uint32_t ext_cell_setup(KParams const& P, Tile<CAP, 1>& s, uint32_t k, uint32_t mc, uint32_t key,
+                                   uint32_t const* __restrict__ gcnt) {
     uint32_t ix, iy, iz;
     decode_key(P, key, ix, iy, iz);
     uint32_t const ref = ref_of(P, ix, iy, iz);
@@ -1179,6 +1180,7 @@ __device__ void ext_cell_setup(KParams const& P, Tile<CAP, 1>& s, uint32_t k, ui
     s.gcnt[k] = added | (layer ? UPPER_LAYER : 0u);
     s.goff[k] = total - added;   // table particles kept
     s.go[k] = total > 1;
+    return total > 1 ? (total + 3u) >> 2 : 0u;
 }
 
 // ghost g of wall cell k of the batch, drawn for the step by k_cell_draws
@@ -1382,24 +1384,27 @@ __device__ void extended_cell(KParams const& P, Tile<CAP, 1>& s, uint32_t k, uin
     }
 }
 
-// table particle at LDS slot i of cell k after extended_cell: velocity back
-// to the lab frame with drag and the angular-momentum correction, position
-// back from the centre of mass and un-shifted (simulation_functions.cpp:384-391)
+// table particle of cell k after extended_cell, its position and velocity
+// relative to the cell (p, v, as extended_cell left them in the tile) in
+// registers: velocity back to the lab frame with drag and the
+// angular-momentum correction, position back from the centre of mass and
+// un-shifted (simulation_functions.cpp:384-391)
 template <int CAP>
-__device__ __forceinline__ void ext_finish(KParams const& P, Tile<CAP, 1>& s, uint32_t k, uint32_t i) {
+__device__ __forceinline__ void ext_finish(KParams const& P, Tile<CAP, 1> const& s, uint32_t k, float& px,
+                                           float& py, float& pz, float& vx, float& vy, float& vz) {
     float const cx = s.cm[0][k], cy = s.cm[1][k], cz = s.cm[2][k];
     float const mx = s.cm[3][k], my = s.cm[4][k], mz = s.cm[5][k];
     float const Lx = __uint_as_float(s.gcnt[k]), Ly = __uint_as_float(s.cref[k]), Lz = __uint_as_float(s.wcell[k]);
-    float const rx = s.p[0][i], ry = s.p[1][i], rz = s.p[2][i];
-    float ux = s.p[3][i] + mx, uy = s.p[4][i] + my, uz = s.p[5][i] + mz;
+    float const rx = px, ry = py, rz = pz;
+    float ux = vx + mx, uy = vy + my, uz = vz + mz;
     ux += P.drag_dt;
     ux = ux - (ry * Lz - rz * Ly);
     uy = uy - (rz * Lx - rx * Lz);
     uz = uz - (rx * Ly - ry * Lx);
-    s.p[3][i] = ux; s.p[4][i] = uy; s.p[5][i] = uz;
-    s.p[0][i] = wrap1((cx + rx) - P.shift[0], P, 0);
-    s.p[1][i] = wrap1((cy + ry) - P.shift[1], P, 1);
-    s.p[2][i] = wrap1((cz + rz) - P.shift[2], P, 2);
+    vx = ux; vy = uy; vz = uz;
+    px = wrap1((cx + rx) - P.shift[0], P, 0);
+    py = wrap1((cy + ry) - P.shift[1], P, 1);
+    pz = wrap1((cz + rz) - P.shift[2], P, 2);
 }
 
 // particles gathered per thread per round (one for the 384-particle tile,
@@ -1693,20 +1698,21 @@ __device__ __forceinline__ void collide_tile(Sys const& sys, const float* __rest
             clear_bins();
             srd_cells<CAP, ADJ>(P, s, ncell, s.off + kb, gb, c0 + kb, T, CT);
         } else {
-            for (uint32_t k = tid; k < ncell; k += NT) {
-                ext_cell_setup<CAP>(P, s, k, s.off[kb + k + 1] - s.off[kb + k], c0 + kb + k, gcnt);
-                uint32_t const w = s.wpre[kb + k] - s.wpre[kb];
-                s.wcell[k] = w * P.n;
-                if (s.wpre[kb + k + 1] != s.wpre[kb + k]) s.wk[w] = static_cast<uint8_t>(k);
-            }
-            __syncthreads();
+            // per-cell set-up, one lane of the first wave per cell, and the
             // thermal draws of the batch: quads of four Gaussians covering the
             // members (table particles, then ghosts) of every cell that
-            // collides, counted before each cell (at most 64 cells: one wave)
-            static_assert(Tile<CAP, ALGO>::TC <= 64, "the quad prefix is one wave's scan");
+            // collides, counted before each cell by a scan of that wave (each
+            // lane holds its own cell's count, so no barrier between the two)
+            static_assert(Tile<CAP, ALGO>::TC <= 64, "one wave sets up the cells and scans their quads");
             if (tid < 64) {
                 uint32_t q = 0;
-                if (tid < ncell && s.go[tid]) q = (s.goff[tid] + (s.gcnt[tid] & ~UPPER_LAYER) + 3u) >> 2;
+                if (tid < ncell) {
+                    uint32_t const k = tid;
+                    q = ext_cell_setup<CAP>(P, s, k, s.off[kb + k + 1] - s.off[kb + k], c0 + kb + k, gcnt);
+                    uint32_t const w = s.wpre[kb + k] - s.wpre[kb];
+                    s.wcell[k] = w * P.n;
+                    if (s.wpre[kb + k + 1] != s.wpre[kb + k]) s.wk[w] = static_cast<uint8_t>(k);
+                }
 #pragma unroll
                 for (int d = 1; d < 64; d <<= 1) {
                     uint32_t const y = __shfl_up(q, d, 64);
@@ -1764,11 +1770,8 @@ __device__ __forceinline__ void collide_tile(Sys const& sys, const float* __rest
                 }
             }
             __syncthreads();
-            // table particles of cells with more than one member, one lane each
-            for (uint32_t i = tid; i < m; i += NT) {
-                uint32_t const k = s.cell[i];
-                if (s.go[k] && i - (s.off[kb + k] - gb) < s.goff[k]) ext_finish<CAP>(P, s, k, i);
-            }
+            // the permutation scratch held the thermal draws until here
+            clear_bins();
         }
         __syncthreads();
         // next step's cell counts: keys near this batch's cells go to three LDS
@@ -1810,10 +1813,6 @@ __device__ __forceinline__ void collide_tile(Sys const& sys, const float* __rest
         uint32_t const need = span + 1u + P.D0;
         uint32_t const ws0 = g0 + kb - lo_off - (need < WS ? (WS - need) / 2 : 0u);
         uint32_t const wss = shm > 0.0f ? ws0 - smid : ws0 + smid;
-        if constexpr (ALGO == 1) {
-            clear_bins();
-            __syncthreads();
-        }
         // per-particle finish, write-out, next-step cell count, sampling
         for (uint32_t i0 = 0; i0 < m; i0 += NT) {
             uint32_t const i = i0 + tid;
@@ -1829,7 +1828,10 @@ __device__ __forceinline__ void collide_tile(Sys const& sys, const float* __rest
                     px = wrap1(px - P.shift[0], P, 0);
                     py = wrap1(py - P.shift[1], P, 1);
                     pz = wrap1(pz - P.shift[2], P, 2);
-                } else if (ALGO == 0 && s.go[k]) {
+                } else if constexpr (ALGO == 1) {
+                    // table particles of cells with more than one member
+                    if (s.go[k] && slot < s.goff[k]) ext_finish<CAP>(P, s, k, px, py, pz, vx, vy, vz);
+                } else if (s.go[k]) {
                     // SRD rotation about the cell axis (simulation_functions.cpp:176-188)
                     float const mx = s.cm[0][k], my = s.cm[1][k], mz = s.cm[2][k];
                     float const ax = s.cm[3][k], ay = s.cm[4][k], az = s.cm[5][k];
