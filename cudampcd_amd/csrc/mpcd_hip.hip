@@ -925,9 +925,9 @@ __global__ void k_advance(Sys sys, uint32_t* __restrict__ big) {
 constexpr int TCM = 64;   // largest cells per tile of any tile
 // Small tiles keep fewer cells so that more workgroups fit the 160 KB of LDS
 // of a CU (allocated in 512-byte granules): SRD 384-particle tiles 24 cells
-// (twelve two-wave workgroups, six waves per SIMD), SRD 448 tiles 32 cells
-// (ten workgroups).  Extended 512-particle tiles take 48 cells (eight
-// workgroups with the 112-slot ghost region): fewer, fuller tiles win, each
+// (thirteen two-wave workgroups, six or seven waves per SIMD), SRD 448 tiles
+// 32 cells (ten workgroups).  Extended 512-particle tiles take 48 cells (eight
+// workgroups with the 100-slot ghost region): fewer, fuller tiles win, each
 // tile's fixed costs (offsets, permutation and record round trips, barriers)
 // being the larger part of its life (500x500x40 cells, n = 10, lane groups of
 // two: k_collide 2.82 ms at 48 cells, 2.97 at 40, 3.31 at 32, 3.79 at 24)
@@ -988,8 +988,9 @@ struct Tile {
 // __syncthreads_count, about 230 bytes)
 static_assert(sizeof(Tile<512, 1>) + 32 + 240 <= 20480, "extended 512 tile within a CU's eighth of the LDS");
 static_assert((512 + ghost_slots(512)) / 4 + tile_cell_max(512, 1) < 256, "quad prefix of the 512 tile in a byte");
-// twelve SRD 384-particle workgroups (26 LDS granules of 512 bytes each)
-static_assert(sizeof(Tile<384, 0>) + 64 <= 13312, "SRD 384 tile within 26 LDS granules");
+// thirteen SRD 384-particle workgroups share a CU (24 LDS granules of 512
+// bytes each; 68 VGPRs would allow fourteen)
+static_assert(sizeof(Tile<384, 0>) + 64 <= 12288, "SRD 384 tile within 24 LDS granules");
 
 // sampled steps: per-batch LDS bins of the mean-field accumulators (count and
 // the three fixed-point velocity sums of each cell, indexed by cell key)
@@ -2629,7 +2630,7 @@ constexpr uint32_t TILE_CAP_MAX = TILE_CAPS[6];
 // extended rule needs 512 or more and room for one wall cell's n ghosts in its
 // ghost region).
 uint32_t pick_capacity(int algorithm, uint32_t nmax, uint32_t cap, uint64_t cells = 0) {
-    // default: SRD 384-particle batches on two-wave workgroups (twelve per
+    // default: SRD 384-particle batches on two-wave workgroups (thirteen per
     // CU); below 16 particles per cell in large grids 512 (more cells per
     // batch: 1000x500x20 cells, n = 5: 64 cells of 512 1.20 ms per collide
     // launch, 24 of 384 1.43; n = 10: 51 of 512 1.94, 24 of 384 2.02; small
