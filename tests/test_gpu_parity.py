@@ -42,6 +42,10 @@ CONFIGS = [
     ((7, 6, 8), (1, 1, 0), 13, "extended"),
     ((8, 6, 8), (1, 1, 0), 30, "srd"),
     ((6, 5, 8), (1, 1, 0), 30, "extended"),
+    # dense walled cells: up to 100 ghosts a wall cell (the SRD rule's one
+    # velocity-sum draw scaled by sqrt(100)), 640- and 1152-particle tiles
+    ((4, 4, 5), (1, 1, 0), 100, "srd"),
+    ((4, 4, 5), (1, 1, 0), 100, "extended"),
     # longest axis above 128 cells: the internal key cuts it into 128-row
     # blocks (the last one shorter), so particles cross block edges, whose
     # keys fall outside the permutation's and histogram's windows
