@@ -1059,9 +1059,20 @@ __device__ void sort_slots(const uint32_t* src, uint32_t* tmp, uint32_t* dst, ui
     }
 }
 
-// keys per sampling window: the batch's cells and a margin for the step back
-// or ahead along the two faster internal axes (cells outside go to memory)
-__host__ __device__ constexpr uint32_t sample_window(int cap) { return static_cast<uint32_t>(cap) / 5; }
+// Sampling windows of a batch (cells outside them go to memory).  A particle's
+// un-shifted cell is its collision cell or the neighbour against the shift
+// along each axis, so its key lies in one of four ranges of span + 1 keys: the
+// batch's keys widened by one cell along the fastest axis on the shift's side,
+// that range one row (D0 keys) along the slowest axis, and both one
+// middle-axis step away; four windows of tile_cell_max + 1 keys cover them
+// for every grid.  The SRD 512-particle tile (up to 64 cells a batch) keeps two
+// windows of capacity / 5 keys each (the first two ranges as one contiguous
+// window, and the same keys one middle-axis step away): four windows of 65
+// keys would cost it a workgroup per CU.
+__host__ __device__ constexpr bool sample_four(int cap, int algo) { return !(algo == 0 && cap == 512); }
+__host__ __device__ constexpr uint32_t sample_window(int cap, int algo) {
+    return sample_four(cap, algo) ? static_cast<uint32_t>(tile_cell_max(cap, algo)) + 1u : static_cast<uint32_t>(cap) / 5;
+}
 template <uint32_t NB>
 struct SampleBins {
     unsigned long long sum[3][NB];
@@ -1428,8 +1439,9 @@ __device__ __forceinline__ void collide_tile(Sys const& sys, const float* __rest
     bool const in_streamed = (io & 1u) != 0, out_streamed = (io & 2u) != 0;
     constexpr uint32_t NT = tile_threads(CAP);
     __shared__ Tile<CAP, ALGO> s;
-    constexpr uint32_t WS = sample_window(CAP);
-    __shared__ SampleBins<SAMPLE ? 2 * WS : 1u> sb;
+    constexpr uint32_t WS = sample_window(CAP, ALGO);
+    constexpr uint32_t NWIN = sample_four(CAP, ALGO) ? 4u : 2u;
+    __shared__ SampleBins<SAMPLE ? NWIN * WS : 1u> sb;
     // BIG: `tile` is an entry of the big-cell list, the cell as a one-cell tile
     uint32_t* const big = status + 1;
     uint32_t const sy = BIG ? big[1 + 2 * tile] : (sys.ns > 1 ? sys.tile_sys[tile] : 0u);
@@ -1686,7 +1698,7 @@ __device__ __forceinline__ void collide_tile(Sys const& sys, const float* __rest
         auto clear_bins = [&]() {
             for (uint32_t i = tid; i < NW; i += NT) s.perm[i] = 0u;
             if constexpr (SAMPLE) {
-                for (uint32_t i = tid; i < 2 * WS; i += NT) {
+                for (uint32_t i = tid; i < NWIN * WS; i += NT) {
                     sb.cnt[i] = 0u;
                     sb.sum[0][i] = 0ull;
                     sb.sum[1][i] = 0ull;
@@ -1803,17 +1815,26 @@ __device__ __forceinline__ void collide_tile(Sys const& sys, const float* __rest
         uint32_t const wbm = wb1 - smid, wbp = wb1 + smid;
         // sampling: un-shifted cells lie in the batch's cells or one step back
         // along each axis the shift is positive on, one step ahead where it is
-        // negative, so two windows (the batch's keys widened on the shift's side
-        // by one cell along the fastest axis and one row (D0 keys) along the
-        // slowest, and the same keys one middle-axis step away on the shift's
-        // side) take their counts and fixed-point sums in LDS
+        // negative; the windows (sample_window) take their counts and
+        // fixed-point sums in LDS.  wq: first key of each window (modulo 2^32)
         float const sh0 = P.ord[0] == 0 ? P.shift[0] : (P.ord[0] == 1 ? P.shift[1] : P.shift[2]);
         float const shm = P.ord[1] == 0 ? P.shift[0] : (P.ord[1] == 1 ? P.shift[1] : P.shift[2]);
         float const sh2 = P.ord[2] == 0 ? P.shift[0] : (P.ord[2] == 1 ? P.shift[1] : P.shift[2]);
-        uint32_t const lo_off = (sh0 > 0.0f ? 1u : 0u) + (sh2 > 0.0f ? P.D0 : 0u);   // keys below the batch
-        uint32_t const need = span + 1u + P.D0;
-        uint32_t const ws0 = g0 + kb - lo_off - (need < WS ? (WS - need) / 2 : 0u);
-        uint32_t const wss = shm > 0.0f ? ws0 - smid : ws0 + smid;
+        uint32_t const dslow = sh2 > 0.0f ? 0u - P.D0 : P.D0;   // one row along the slowest axis
+        uint32_t const dmid = shm > 0.0f ? 0u - smid : smid;    // one middle-axis step
+        uint32_t wq[NWIN];
+        if constexpr (NWIN == 4) {
+            uint32_t const r0 = g0 + kb - (sh0 > 0.0f ? 1u : 0u);
+            wq[0] = r0;
+            wq[1] = r0 + dslow;
+            wq[2] = r0 + dmid;
+            wq[3] = r0 + dslow + dmid;
+        } else {
+            uint32_t const lo_off = (sh0 > 0.0f ? 1u : 0u) + (sh2 > 0.0f ? P.D0 : 0u);   // keys below the batch
+            uint32_t const need = span + 1u + P.D0;
+            wq[0] = g0 + kb - lo_off - (need < WS ? (WS - need) / 2 : 0u);
+            wq[1] = wq[0] + dmid;
+        }
         // per-particle finish, write-out, next-step cell count, sampling
         for (uint32_t i0 = 0; i0 < m; i0 += NT) {
             uint32_t const i = i0 + tid;
@@ -1862,8 +1883,10 @@ __device__ __forceinline__ void collide_tile(Sys const& sys, const float* __rest
                     unsigned long long const fx = static_cast<unsigned long long>(static_cast<long long>(rint(static_cast<double>(vx) * FIX_SCALE)));
                     unsigned long long const fy = static_cast<unsigned long long>(static_cast<long long>(rint(static_cast<double>(vy) * FIX_SCALE)));
                     unsigned long long const fz = static_cast<unsigned long long>(static_cast<long long>(rint(static_cast<double>(vz) * FIX_SCALE)));
-                    uint32_t b = ks - ws0;
-                    if (b >= WS) b = (ks - wss < WS) ? WS + (ks - wss) : ~0u;
+                    uint32_t b = ~0u;
+#pragma unroll
+                    for (int w = NWIN - 1; w >= 0; --w)
+                        if (ks - wq[w] < WS) b = static_cast<uint32_t>(w) * WS + (ks - wq[w]);
                     if (b != ~0u) {
                         atomicAdd(&sb.cnt[b], 1u);
                         atomicAdd(&sb.sum[0][b], fx);
@@ -1909,11 +1932,16 @@ __device__ __forceinline__ void collide_tile(Sys const& sys, const float* __rest
             if (c >> 16) atomicAdd(count_next + bin_key(NW + i), c >> 16);
         }
         if constexpr (SAMPLE) {
-            for (uint32_t i = tid; i < 2 * WS; i += NT) {
+            for (uint32_t i = tid; i < NWIN * WS; i += NT) {
                 uint32_t const c = sb.cnt[i];
                 if (c) {
                     // consecutive bins are consecutive keys: coalesced atomics
-                    uint64_t const rc = i < WS ? ws0 + i : wss + (i - WS);
+                    uint32_t const w = i / WS;
+                    uint32_t base = wq[0];   // wq[w] by selects (no indexed register array)
+#pragma unroll
+                    for (uint32_t v = 1; v < NWIN; ++v)
+                        if (w == v) base = wq[v];
+                    uint64_t const rc = base + (i - w * WS);
                     atomicAdd(S.cnt + rc, c);
                     atomicAdd(S.sum + 3ull * rc + 0, sb.sum[0][i]);
                     atomicAdd(S.sum + 3ull * rc + 1, sb.sum[1][i]);
