@@ -915,6 +915,22 @@ __device__ __forceinline__ void advance_system(Sys const& sys, uint32_t* __restr
     }
 }
 
+// a single system's step state at `step` (its shift and the next one), set
+// on the device so that a stream can restart a run without the host table
+__global__ void k_set_step(Sys sys, uint32_t step) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        KParams* const P = sys.P;
+        float s[3], nx[3];
+        grid_shift(P->rs, step, s);
+        grid_shift(P->rs, step + 1, nx);
+        P->step = step;
+        for (int a = 0; a < 3; ++a) {
+            P->shift[a] = s[a];
+            P->shift_next[a] = nx[a];
+        }
+    }
+}
+
 __global__ void k_advance(Sys sys, uint32_t* __restrict__ big) {
     advance_system(sys, big, blockIdx.x * blockDim.x + threadIdx.x);
 }
@@ -2536,9 +2552,13 @@ struct mpcd_hip_s {
     bool use_graph = true;
     hipGraphExec_t graph[2][2][4] = {};   // [input buffer][sampling][streamed in/out]
     int graph_launches[2] = {0, 0};
-    // adjoint runs: tape of one checkpoint segment, checkpoints, adjoint state
+    // adjoint runs: tape of one checkpoint segment (two, when segments are
+    // recomputed: one is recomputed while the other is run backwards),
+    // checkpoints, adjoint state
     struct Adj {
-        uint32_t K = 0, nck = 0, steps = 0;
+        uint32_t K = 0, nck = 0, steps = 0, halves = 0;
+        hipStream_t recompute = nullptr;   // second stream of the segment recomputes
+        hipEvent_t written[2] = {nullptr, nullptr}, read[2] = {nullptr, nullptr};   // per tape half
         uint32_t* perm = nullptr;    // [K][N] recorded permutation with the bounce flag
         float* vp = nullptr;         // [K][3N] streamed velocities
         uint16_t* lz = nullptr;      // [K][N] z layer of every output slot
@@ -3223,6 +3243,12 @@ void destroy(mpcd_hip_s* h) {
 
 void adj_free(mpcd_hip_s* h) {
     auto& a = h->adj;
+    if (a.recompute) {
+        (void)hipStreamSynchronize(a.recompute);
+        (void)hipStreamDestroy(a.recompute);
+    }
+    for (hipEvent_t e : {a.written[0], a.written[1], a.read[0], a.read[1]})
+        if (e) (void)hipEventDestroy(e);
     for (void* p : {static_cast<void*>(a.perm), static_cast<void*>(a.vp), static_cast<void*>(a.lz),
                     static_cast<void*>(a.off), static_cast<void*>(a.u), static_cast<void*>(a.ax),
                     static_cast<void*>(a.tot), static_cast<void*>(a.ckpt), static_cast<void*>(a.lam[0]),
@@ -3244,7 +3270,8 @@ void dalloc(T*& p, uint64_t count, const char* what) {
 void adj_reserve(mpcd_hip_s* h, uint32_t steps, uint32_t K) {
     auto& a = h->adj;
     uint32_t const nck = K ? (steps + K - 1) / K : 1;
-    if (a.ckpt && a.K >= K && a.nck >= nck && a.steps >= steps) return;
+    uint32_t const halves = nck > 1 ? 2u : 1u;
+    if (a.ckpt && a.K >= K && a.nck >= nck && a.steps >= steps && a.halves >= halves) return;
     ck(hipStreamSynchronize(h->stream), "sync");
     adj_free(h);
     uint64_t const N = h->N, C = h->C, Z = h->kp[0].dim[2];
@@ -3253,22 +3280,31 @@ void adj_reserve(mpcd_hip_s* h, uint32_t steps, uint32_t K) {
     dalloc(a.lsum, uint64_t(steps) * 2 * LAYER_SLOTS * Z, "alloc layer sums");
     dalloc(a.force, C, "alloc force field");
     if (K) {
+        uint64_t const S = uint64_t(K) * halves;   // tape slots
         dalloc(a.stab, uint64_t(steps) * Z, "alloc loss seeds");
-        dalloc(a.perm, uint64_t(K) * N, "alloc tape");
-        dalloc(a.vp, uint64_t(K) * 3 * N, "alloc tape");
-        dalloc(a.lz, uint64_t(K) * N, "alloc tape");
-        dalloc(a.off, uint64_t(K) * (C + 1), "alloc tape");
-        dalloc(a.u, uint64_t(K) * 3 * C, "alloc tape");
-        dalloc(a.ax, uint64_t(K) * 3 * C, "alloc tape");
-        dalloc(a.tot, uint64_t(K) * C, "alloc tape");
+        dalloc(a.perm, S * N, "alloc tape");
+        dalloc(a.vp, S * 3 * N, "alloc tape");
+        dalloc(a.lz, S * N, "alloc tape");
+        dalloc(a.off, S * (C + 1), "alloc tape");
+        dalloc(a.u, S * 3 * C, "alloc tape");
+        dalloc(a.ax, S * 3 * C, "alloc tape");
+        dalloc(a.tot, S * C, "alloc tape");
         dalloc(a.lam[0], 3 * N, "alloc adjoint");
         dalloc(a.lam[1], 3 * N, "alloc adjoint");
         dalloc(a.part, 2 * ((C + ADJ_CELLS - 1) / ADJ_CELLS), "alloc gradient partials");
         dalloc(a.gf, C, "alloc force gradient");
     }
+    if (halves > 1) {
+        ck(hipStreamCreateWithFlags(&a.recompute, hipStreamNonBlocking), "recompute stream");
+        for (int i = 0; i < 2; ++i) {
+            ck(hipEventCreateWithFlags(&a.written[i], hipEventDisableTiming), "event");
+            ck(hipEventCreateWithFlags(&a.read[i], hipEventDisableTiming), "event");
+        }
+    }
     a.K = K;
     a.nck = nck;
     a.steps = steps;
+    a.halves = halves;
 }
 
 // blocks of k_layers: one pass of WG * LAYER_UNROLL particles each (the
@@ -3469,25 +3505,56 @@ ProfileResult profile_run(mpcd_hip_s* h, int steps, int K, float drag, float ang
             ck(hipMemsetAsync(a.part, 0, 16ull * gb, h->stream), "clear");
             ck(hipMemsetAsync(a.gf, 0, 8 * C, h->stream), "clear");
             int li = 0;
+            // Segments are run backwards from the last, whose tape the forward
+            // pass wrote (tape half 0); every other one is recomputed from its
+            // checkpoint into the other half on a second stream while the
+            // segment after it runs backwards (the reverse step reads only its
+            // tape, the adjoint state and the system's fixed geometry; the
+            // recompute writes the state, the sort arrays and its own half).
+            hipStream_t const R = h->stream;
+            bool const two = nseg > 1;
+            auto half_of = [&](int seg) { return static_cast<uint32_t>((static_cast<int>(nseg) - 1 - seg) & 1); };
+            if (two) {
+                ck(hipEventRecord(a.written[0], R), "event");   // the forward pass and the clears above
+                ck(hipStreamWaitEvent(a.recompute, a.written[0], 0), "wait");
+            }
             for (int seg = static_cast<int>(nseg) - 1; seg >= 0; --seg) {
                 uint32_t const t0 = static_cast<uint32_t>(seg) * Kc, t1 = std::min(T, t0 + Kc);
-                if (static_cast<uint32_t>(seg) + 1 < nseg) {
-                    ck(hipMemcpyAsync(h->st[h->cur], a.ckpt + uint64_t(seg) * 6 * N, 24 * N, hipMemcpyDeviceToDevice,
-                                      h->stream), "restore checkpoint");
-                    h->step_index = s0 + t0;
-                    h->keys_valid = false;
-                    h->sorted_valid = false;
-                    upload_table(h);
-                    for (uint32_t t = t0 + 1; t <= t1; ++t) adj_step(h, force, static_cast<int>(t - t0 - 1), nullptr, nullptr);
+                if (seg >= 1) {
+                    // the segment before this one, into the other half, once the
+                    // reverse pass that last read that half (segment seg + 1) is done
+                    int const ps = seg - 1;
+                    uint32_t const hp = half_of(ps);
+                    uint32_t const p0 = static_cast<uint32_t>(ps) * Kc, p1 = std::min(T, p0 + Kc);
+                    if (ps + 2 <= static_cast<int>(nseg) - 1) ck(hipStreamWaitEvent(a.recompute, a.read[hp], 0), "wait");
+                    h->stream = a.recompute;
+                    try {
+                        ck(hipMemcpyAsync(h->st[h->cur], a.ckpt + uint64_t(ps) * 6 * N, 24 * N, hipMemcpyDeviceToDevice,
+                                          h->stream), "restore checkpoint");
+                        h->step_index = s0 + p0;
+                        h->keys_valid = false;
+                        h->sorted_valid = false;
+                        hipLaunchKernelGGL(k_set_step, dim3(1), dim3(64), 0, h->stream, h->sys, static_cast<uint32_t>(s0 + p0));
+                        for (uint32_t t = p0 + 1; t <= p1; ++t)
+                            adj_step(h, force, static_cast<int>(hp * Kc + (t - p0 - 1)), nullptr, nullptr);
+                        ck(hipEventRecord(a.written[hp], h->stream), "event");
+                    } catch (...) {
+                        h->stream = R;
+                        throw;
+                    }
+                    h->stream = R;
                 }
+                uint32_t const hs = half_of(seg);
+                if (two && seg < static_cast<int>(nseg) - 1) ck(hipStreamWaitEvent(R, a.written[hs], 0), "wait");
                 for (uint32_t t = t1; t > t0; --t) {
-                    uint64_t const j = t - t0 - 1;
+                    uint64_t const j = hs * Kc + (t - t0 - 1);
                     AdjIn const in{a.off + j * (C + 1), a.u + j * 3 * C, a.ax + j * 3 * C, a.tot + j * C, a.perm + j * N,
                                    a.vp + j * 3 * N, a.lz + j * N, a.stab + uint64_t(t - 1) * Z, a.lam[li], a.lam[1 - li]};
-                    hipLaunchKernelGGL(k_adjoint_step, dim3(gb), dim3(WG), 0, h->stream, h->sys, in, a.part, a.gf);
+                    hipLaunchKernelGGL(k_adjoint_step, dim3(gb), dim3(WG), 0, R, h->sys, in, a.part, a.gf);
                     li = 1 - li;
                 }
                 ck(hipGetLastError(), "reverse pass");
+                if (two) ck(hipEventRecord(a.read[hs], R), "event");
             }
             std::vector<double> part(2ull * gb), gf(C);
             ck(hipStreamSynchronize(h->stream), "reverse pass");
@@ -3505,6 +3572,8 @@ ProfileResult profile_run(mpcd_hip_s* h, int steps, int K, float drag, float ang
         restore();
         ck(hipStreamSynchronize(h->stream), "restore");
     } catch (...) {
+        // a segment recompute may still be writing the state
+        if (a.recompute) (void)hipStreamSynchronize(a.recompute);
         try {
             restore();
         } catch (...) {
