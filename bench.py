@@ -434,7 +434,7 @@ def bench_ensemble(args, cfg):
 def bench_adjoint(args, cfg):
     """Time per gradient of the profile loss against one loss evaluation, with
     the configuration's checkpoint interval (forward, recompute of every segment
-    but the last, reverse) and with the whole tape resident in HBM (forward and
+    but the last two, reverse) and with the whole tape resident in HBM (forward and
     reverse only); then the accuracy study: the drag gradient of consecutive
     windows from the adjoint and from central differences of the loss (common
     random numbers within a window), their means with standard errors, and the

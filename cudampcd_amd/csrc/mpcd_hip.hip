@@ -2552,9 +2552,9 @@ struct mpcd_hip_s {
     bool use_graph = true;
     hipGraphExec_t graph[2][2][4] = {};   // [input buffer][sampling][streamed in/out]
     int graph_launches[2] = {0, 0};
-    // adjoint runs: tape of one checkpoint segment (two, when segments are
-    // recomputed: one is recomputed while the other is run backwards),
-    // checkpoints, adjoint state
+    // adjoint runs: tape of one checkpoint segment (two when there are more
+    // segments: the forward pass tapes the last two, and later one is
+    // recomputed while the other is run backwards), checkpoints, adjoint state
     struct Adj {
         uint32_t K = 0, nck = 0, steps = 0, halves = 0;
         hipStream_t recompute = nullptr;   // second stream of the segment recomputes
@@ -3265,8 +3265,9 @@ void dalloc(T*& p, uint64_t count, const char* what) {
 }
 
 // buffers for `steps` steps with checkpoints every K steps (K = 0: forward
-// only): K taped steps, and the start state of every segment but the last
-// (which the forward pass tapes)
+// only): K taped steps (2 K with more than one segment), and the start state
+// of every segment (those of the last two, which the forward pass tapes, go
+// unused)
 void adj_reserve(mpcd_hip_s* h, uint32_t steps, uint32_t K) {
     auto& a = h->adj;
     uint32_t const nck = K ? (steps + K - 1) / K : 1;
@@ -3431,17 +3432,22 @@ ProfileResult profile_run(mpcd_hip_s* h, int steps, int K, float drag, float ang
         h->sorted_valid = false;
         upload_table(h);
         // forward pass: layer statistics of every step, the start state of
-        // every segment but the last, and the tape of the last segment (the
-        // whole run when K >= steps: nothing is recomputed then)
+        // every segment but the last two, and the tape of the last segment
+        // (half 0) and of the one before it (half 1), which the reverse pass
+        // then need not recompute (the whole run when K >= steps or two
+        // segments cover it: nothing is recomputed then)
         uint64_t const LZ = uint64_t(LAYER_SLOTS) * Z;
         ck(hipMemsetAsync(a.lcnt, 0, 4ull * T * LZ, h->stream), "clear");
         ck(hipMemsetAsync(a.lsum, 0, 16ull * T * LZ, h->stream), "clear");
         uint32_t const nseg = Kc ? (T + Kc - 1) / Kc : 1;
-        uint32_t const tlast = (nseg - 1) * Kc;   // steps before the last segment
+        uint32_t const tlast = (nseg - 1) * Kc;                   // steps before the last segment
+        uint32_t const ttape = nseg > 1 ? (nseg - 2) * Kc : tlast;   // steps before the forward pass's tape
         for (uint32_t t = 1; t <= T; ++t) {
-            int const j = (Kc && t > tlast) ? static_cast<int>(t - tlast - 1) : -1;
+            int j = -1;
+            if (Kc && t > tlast) j = static_cast<int>(t - tlast - 1);
+            else if (Kc && t > ttape) j = static_cast<int>(Kc + (t - ttape - 1));
             adj_step(h, force, j, a.lcnt + uint64_t(t - 1) * LZ, a.lsum + uint64_t(t - 1) * 2 * LZ);
-            if (Kc && t % Kc == 0 && t < tlast)
+            if (Kc && t % Kc == 0 && t < ttape)
                 ck(hipMemcpyAsync(a.ckpt + uint64_t(t / Kc) * 6 * N, h->st[h->cur], 24 * N, hipMemcpyDeviceToDevice,
                                   h->stream), "checkpoint");
         }
@@ -3498,19 +3504,21 @@ ProfileResult profile_run(mpcd_hip_s* h, int steps, int K, float drag, float ang
                 }
             ck(hipMemcpy(a.stab, stab.data(), 4ull * T * Z, hipMemcpyHostToDevice), "upload loss seeds");
             // reverse pass, one segment at a time from the last; every
-            // segment but the last is first recomputed from its start state
-            // with the tape
+            // segment but the last two is first recomputed from its start
+            // state with the tape
             uint32_t const gb = static_cast<uint32_t>((C + ADJ_CELLS - 1) / ADJ_CELLS);
             ck(hipMemsetAsync(a.lam[0], 0, 12 * N, h->stream), "clear");
             ck(hipMemsetAsync(a.part, 0, 16ull * gb, h->stream), "clear");
             ck(hipMemsetAsync(a.gf, 0, 8 * C, h->stream), "clear");
             int li = 0;
-            // Segments are run backwards from the last, whose tape the forward
-            // pass wrote (tape half 0); every other one is recomputed from its
-            // checkpoint into the other half on a second stream while the
-            // segment after it runs backwards (the reverse step reads only its
-            // tape, the adjoint state and the system's fixed geometry; the
-            // recompute writes the state, the sort arrays and its own half).
+            // Segments are run backwards from the last; the forward pass
+            // taped the last two (halves 0 and 1), and every earlier one is
+            // recomputed from its checkpoint into the half of the segment two
+            // after it on a second stream, once that segment has been run
+            // backwards, while the segment after it runs backwards (the
+            // reverse step reads only its tape, the adjoint state and the
+            // system's fixed geometry; the recompute writes the state, the
+            // sort arrays and its own half).
             hipStream_t const R = h->stream;
             bool const two = nseg > 1;
             auto half_of = [&](int seg) { return static_cast<uint32_t>((static_cast<int>(nseg) - 1 - seg) & 1); };
@@ -3520,13 +3528,13 @@ ProfileResult profile_run(mpcd_hip_s* h, int steps, int K, float drag, float ang
             }
             for (int seg = static_cast<int>(nseg) - 1; seg >= 0; --seg) {
                 uint32_t const t0 = static_cast<uint32_t>(seg) * Kc, t1 = std::min(T, t0 + Kc);
-                if (seg >= 1) {
+                if (seg >= 1 && seg + 1 < static_cast<int>(nseg)) {
                     // the segment before this one, into the other half, once the
                     // reverse pass that last read that half (segment seg + 1) is done
                     int const ps = seg - 1;
                     uint32_t const hp = half_of(ps);
                     uint32_t const p0 = static_cast<uint32_t>(ps) * Kc, p1 = std::min(T, p0 + Kc);
-                    if (ps + 2 <= static_cast<int>(nseg) - 1) ck(hipStreamWaitEvent(a.recompute, a.read[hp], 0), "wait");
+                    ck(hipStreamWaitEvent(a.recompute, a.read[hp], 0), "wait");
                     h->stream = a.recompute;
                     try {
                         ck(hipMemcpyAsync(h->st[h->cur], a.ckpt + uint64_t(ps) * 6 * N, 24 * N, hipMemcpyDeviceToDevice,
@@ -3545,7 +3553,7 @@ ProfileResult profile_run(mpcd_hip_s* h, int steps, int K, float drag, float ang
                     h->stream = R;
                 }
                 uint32_t const hs = half_of(seg);
-                if (two && seg < static_cast<int>(nseg) - 1) ck(hipStreamWaitEvent(R, a.written[hs], 0), "wait");
+                if (two && seg + 2 < static_cast<int>(nseg)) ck(hipStreamWaitEvent(R, a.written[hs], 0), "wait");
                 for (uint32_t t = t1; t > t0; --t) {
                     uint64_t const j = hs * Kc + (t - t0 - 1);
                     AdjIn const in{a.off + j * (C + 1), a.u + j * 3 * C, a.ax + j * 3 * C, a.tot + j * C, a.perm + j * N,

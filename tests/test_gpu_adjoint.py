@@ -120,7 +120,8 @@ def test_gradient_matches_frozen_model_finite_differences(pympcd, n):
 def test_checkpoint_interval_does_not_change_the_gradient(pympcd):
     p, sim, target = _setup(pympcd, seed=77)
     out = []
-    for K in (1, 5, STEPS, 1000):
+    # one segment, two (both taped by the forward pass), several with a short last one
+    for K in (1, 3, 5, 6, 7, STEPS, 1000):
         r = adjoint.profile_loss_grad(sim, STEPS, K, p.drag, p.rotation_angle, target)
         out.append((r.loss, r.grad_drag, r.grad_angle, r.grad_force))
     for o in out[1:]:
